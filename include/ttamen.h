@@ -1,0 +1,185 @@
+/*
+ * ttamen.h -- C ABI of libttamen, the MI355X-native AMEn linear solver for
+ * tensor-train systems A x = b.
+ *
+ * The drop-in boundary replaces the reference's native entry point
+ *   torchttcpp.amen_solve(A_cores, b_cores, x0_cores, N, rA, rb, r_x0, nswp, eps, rmax,
+ *                         max_full, kickrank, kick2, local_iterations, resets, verbose,
+ *                         preconditioner)
+ * (reference: cpp/cpp_ext.cpp:9, cpp/amen_solve.h:122-139, called from
+ * torchtt/solvers.py:283-285), and exposes the kernels it is built from so that each
+ * can be checked on its own.
+ *
+ * Conventions
+ *  - All array arguments named *_cores / device pointers point to GPU memory of the
+ *    current HIP device; the caller owns every buffer. No function allocates device
+ *    memory: scratch comes from the `workspace` argument, sized by the matching
+ *    *_workspace_size query.
+ *  - Cores are row-major: a vector core is r x n x r', an operator core is
+ *    r x m x n x r' (row index m, column index n), as torch/the reference store them.
+ *  - Every function returns 0 on success and a positive status otherwise
+ *    (1 invalid argument, 2 workspace too small, 3 HIP error, 4 unsupported,
+ *    5 non-finite result); ttamen_last_error() returns a message for the calling thread.
+ *  - `stream` is a hipStream_t (0 = the null stream). Calls are asynchronous unless
+ *    stated otherwise; amen_solve synchronises internally where its control flow
+ *    needs device results (rank decisions, convergence).
+ */
+#ifndef TTAMEN_H
+#define TTAMEN_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Library identification. */
+const char* ttamen_version(void);
+const char* ttamen_last_error(void);
+
+/* Statistics of one solve (host struct, filled by ttamen_amen_solve when non-NULL). */
+typedef struct ttamen_stats {
+  int64_t sweeps;            /* sweeps performed */
+  int64_t gmres_iters;       /* total GMRES iterations */
+  int64_t local_solves;      /* local problems solved */
+  int64_t direct_solves;     /* of which by the dense direct solver */
+  int64_t matvecs;           /* local-operator applications */
+  double matvec_flops;       /* algorithmic flops of those applications */
+  double matvec_bytes;       /* compulsory bytes of those applications */
+  double max_res_last;       /* max local residual of the last sweep */
+  double elapsed_s;          /* wall time inside the call */
+  int64_t max_rank;          /* largest TT rank reached */
+} ttamen_stats;
+
+/*
+ * AMEn solve (FP64 plain method).
+ * Replaces: torchttcpp.amen_solve, cpp/amen_solve.h:122-549.
+ *
+ *  d                 number of cores
+ *  N[d]              mode sizes; rA[d+1], rb[d+1] ranks of A and b (ends are 1)
+ *  A_cores[d]        operator cores rA[k] x N[k] x N[k] x rA[k+1] (double, device)
+ *  b_cores[d]        rhs cores rb[k] x N[k] x rb[k+1] (double, device)
+ *  x0_cores, rx0     initial guess cores/ranks, or NULL (then x0 = all-ones, rank 1)
+ *  z0_cores          initial residual-proxy cores rz[k] x N[k] x rz[k+1] with
+ *                    rz = (1, kickrank+kick2, ..., 1), normal random (device), or NULL
+ *                    to draw them on the device from `seed`
+ *  nswp, eps, rmax, max_full, kickrank, kick2, local_iterations, resets, verbose,
+ *  preconditioner    as in the reference (preconditioner: 0 none; 1, 2 unsupported)
+ *  precision         0 = FP64 everywhere (plain); 1 = FP32 local operator on MFMA with
+ *                    FP64 iterative refinement of every local solve
+ *  workspace         device scratch of ttamen_amen_solve_workspace_size(...) bytes
+ *  x_out             device buffer receiving the solution cores back to back
+ *  x_out_elems       its capacity in doubles (ttamen_amen_solve_output_elems(...))
+ *  rx_out[d+1]       host array receiving the solution ranks
+ *  offsets_out[d]    host array receiving each core's offset (in doubles) in x_out
+ */
+int ttamen_amen_solve(int d, const int64_t* N, const int64_t* rA, const int64_t* rb,
+                      const double* const* A_cores, const double* const* b_cores,
+                      const double* const* x0_cores, const int64_t* rx0,
+                      const double* const* z0_cores, uint64_t seed, int64_t nswp, double eps,
+                      int64_t rmax, int64_t max_full, int64_t kickrank, int64_t kick2,
+                      int64_t local_iterations, int64_t resets, int verbose, int preconditioner,
+                      int precision, void* workspace, size_t workspace_bytes, double* x_out,
+                      size_t x_out_elems, int64_t* rx_out, int64_t* offsets_out,
+                      ttamen_stats* stats, void* stream);
+
+/* Bytes of device workspace ttamen_amen_solve needs for these sizes (0 on bad input). */
+size_t ttamen_amen_solve_workspace_size(int d, const int64_t* N, const int64_t* rA,
+                                        const int64_t* rb, const int64_t* rx0, int64_t rmax,
+                                        int64_t kickrank, int64_t kick2,
+                                        int64_t local_iterations, int64_t max_full,
+                                        int precision);
+/* Doubles needed in x_out (solution cores at their rank bounds). */
+size_t ttamen_amen_solve_output_elems(int d, const int64_t* N, const int64_t* rx0, int64_t rmax,
+                                      int64_t kickrank, int64_t kick2);
+
+/*
+ * Local operator y[v] = Phi_L * A_k * Phi_R * x[v], v < nvec (FP64).
+ * Replaces: AMENsolveMV<T>::matvec, cpp/matvecs.h:196-212 and local_product,
+ * cpp/amen_solve.h:17-27 ('lsr,smnS,LSR,rnR->lmL').
+ * PhiL: l1 x s x r1, A: s x n x n x S, PhiR: l2 x S x r2, x: nvec x r1 x n x r2,
+ * y: nvec x l1 x n x l2.
+ */
+int ttamen_local_matvec(const double* PhiL, const double* A, const double* PhiR,
+                        const double* x, double* y, int64_t l1, int64_t s, int64_t r1,
+                        int64_t n, int64_t S, int64_t l2, int64_t r2, int64_t nvec,
+                        void* workspace, size_t workspace_bytes, void* stream);
+/* Same in FP32 (inputs/outputs float, FP32 accumulation on MFMA). */
+int ttamen_local_matvec_f32(const float* PhiL, const float* A, const float* PhiR, const float* x,
+                            float* y, int64_t l1, int64_t s, int64_t r1, int64_t n, int64_t S,
+                            int64_t l2, int64_t r2, int64_t nvec, void* workspace,
+                            size_t workspace_bytes, void* stream);
+size_t ttamen_local_matvec_workspace_size(int64_t l1, int64_t s, int64_t r1, int64_t n,
+                                          int64_t S, int64_t l2, int64_t r2, int64_t nvec);
+
+/*
+ * Interface updates (FP64). Replace compute_phi_fwd_A / compute_phi_bck_A /
+ * compute_phi_fwd_rhs / compute_phi_bck_rhs, cpp/amen_solve.h:38-98.
+ *  phi_fwd_A:  Pn[L,S,R] = sum Phi[l,s,r] xL[l,M,L] A[s,M,N,S] xR[r,N,R]
+ *  phi_bck_A:  Pp[l,s,r] = sum xL[l,M,L] Phi[L,S,R] A[s,M,N,S] xR[r,N,R]
+ *  phi_fwd_rhs: Pn[B,R] = sum Pb[b,r] bc[b,n,B] x[r,n,R]
+ *  phi_bck_rhs: Pp[b,r] = sum bc[b,n,B] Pb[B,R] x[r,n,R]
+ */
+int ttamen_phi_fwd_A(const double* Phi, const double* xL, const double* A, const double* xR,
+                     double* Pn, int64_t l, int64_t s, int64_t r, int64_t n, int64_t S,
+                     int64_t L, int64_t R, void* workspace, size_t workspace_bytes, void* stream);
+int ttamen_phi_bck_A(const double* Phi, const double* xL, const double* A, const double* xR,
+                     double* Pp, int64_t l, int64_t s, int64_t r, int64_t n, int64_t S,
+                     int64_t L, int64_t R, void* workspace, size_t workspace_bytes, void* stream);
+int ttamen_phi_fwd_rhs(const double* Pb, const double* bc, const double* x, double* Pn,
+                       int64_t b, int64_t r, int64_t n, int64_t B, int64_t R, void* workspace,
+                       size_t workspace_bytes, void* stream);
+int ttamen_phi_bck_rhs(const double* Pb, const double* bc, const double* x, double* Pp,
+                       int64_t b, int64_t r, int64_t n, int64_t B, int64_t R, void* workspace,
+                       size_t workspace_bytes, void* stream);
+size_t ttamen_phi_workspace_size(int64_t l, int64_t s, int64_t r, int64_t n, int64_t S,
+                                 int64_t L, int64_t R);
+
+/*
+ * Thin QR of a row-major m x n matrix (FP64): Q m x k (k = min(m,n)), R k x n, A = Q R,
+ * Q^T Q = I. Replaces at::linalg_qr in cpp/amen_solve.h:242,254,407,437,454 and
+ * cpp/ortho.h:59. R is not triangular in general (any orthonormal Q is accepted by the
+ * AMEn sweep); synchronous.
+ */
+int ttamen_qr(const double* A, int64_t m, int64_t n, double* Q, double* R, void* workspace,
+              size_t workspace_bytes, void* stream);
+/*
+ * Thin SVD of a row-major m x n matrix (FP64): U m x k, s k (descending), V n x k,
+ * A = U diag(s) V^T. Replaces at::linalg_svd in cpp/amen_solve.h:231,383,430 and
+ * cpp/compression.h:23.
+ */
+int ttamen_svd(const double* A, int64_t m, int64_t n, double* U, double* s, double* V,
+               void* workspace, size_t workspace_bytes, void* stream);
+size_t ttamen_dense_workspace_size(int64_t m, int64_t n);
+
+/* Dense solve B x = rhs, n < 2^15 (FP64, B overwritten). Replaces at::linalg_solve of the
+ * direct local solver, cpp/amen_solve.h:326. x overwrites rhs. */
+int ttamen_dense_solve(double* B, double* rhs, int64_t n, void* stream);
+
+/*
+ * Strided batched GEMM on the matrix cores (FP64), the building block of every
+ * contraction above: C[b](i,j) = alpha * sum_k A[b](i,k) B[b](k,j) + beta * C[b](i,j),
+ * element (i,k) of batch b at A + b*sab + i*sar + k*sac (likewise B, C).
+ */
+int ttamen_gemm_f64(int64_t M, int64_t N, int64_t K, int64_t batch, double alpha,
+                    const double* A, int64_t sar, int64_t sac, int64_t sab, const double* B,
+                    int64_t sbr, int64_t sbc, int64_t sbb, double beta, double* C, int64_t scr,
+                    int64_t scc, int64_t scb, void* workspace, size_t workspace_bytes,
+                    void* stream);
+
+/*
+ * Measurement hooks (no counterpart in the reference). When enabled, every full-size
+ * local-operator application of ttamen_amen_solve that is not speculative is bracketed
+ * by HIP events on the solver's stream. ttamen_profile_read fills out4 with
+ * {applications, total event time in ms, algorithmic flops, compulsory bytes}
+ * accumulated since ttamen_profile_enable. Process-global; not for concurrent solves.
+ */
+int ttamen_profile_enable(int enable);
+int ttamen_profile_read(double* out4);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TTAMEN_H */

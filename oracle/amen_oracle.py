@@ -1,0 +1,462 @@
+"""CPU oracle for the AMEn linear-solver path -- TEST INFRASTRUCTURE ONLY.
+
+A numpy FP64 restatement of the reference's native AMEn solver
+(cpp/amen_solve.h:122-549, cpp/gmres.h:15-161, cpp/ortho.h:16-73), written to be
+read line against line with it. Only tests/, __graft_entry__.smoke() and bench.py's
+cpu_baseline leg may import this module, and only as the checker / baseline; the
+product path (torchtt_amd) never does.
+
+Pinning (see tests/test_oracle.py):
+  * amen_solve: against the reference's own C++ build (oracle/_ref) on identical
+    seeded inputs, against the closed-form Laplacian solution (laplace_exact.py),
+    and by the TT residual ||Ax-b||/||b|| of its result;
+  * gmres: against numpy.linalg.solve on small dense systems, and the Arnoldi
+    relation A Q_k = Q_{k+1} Hbar_k;
+  * local_product / phi_*: against brute-force dense contractions and the
+    invariants phi_d = x^T A x and phib_d = <b, x>;
+  * rank_chop / round_tt: against the closed-form tail rule and dense SVD.
+The reference's Python twin (torchtt/solvers.py:290-688) differs in three places
+(GMRES iterations m+1, truncation start r = full-1, zero-norm guard 1.0); this
+module follows the C++ version.
+"""
+import numpy as np
+
+__all__ = ["local_product", "phi_fwd_A", "phi_bck_A", "phi_fwd_rhs", "phi_bck_rhs",
+           "gmres", "amen_solve", "rl_orthogonal", "rank_chop", "round_tt", "tt_full",
+           "tt_matvec", "tt_add", "tt_norm", "tt_residual"]
+
+
+# --------------------------------------------------------------------------- contractions
+def local_product(phi_right, phi_left, core_a, core):
+    """w[l,m,L] = sum Phi_L[l,s,r] A[s,m,n,S] Phi_R[L,S,R] x[r,n,R]  (amen_solve.h:17-27)."""
+    t1 = np.tensordot(core, phi_left, axes=([0], [2]))           # n R l s
+    t2 = np.tensordot(t1, core_a, axes=([0, 3], [2, 0]))         # R l m S
+    return np.tensordot(t2, phi_right, axes=([0, 3], [2, 1]))    # l m L
+
+
+def phi_bck_A(phi_now, core_left, core_a, core_right):
+    """Phi_prev[l,s,r] = sum xL[l,M,L] Phi[L,S,R] A[s,M,N,S] xR[r,N,R]  (amen_solve.h:38-47)."""
+    p = np.tensordot(core_left, phi_now, axes=([2], [0]))        # l M S R
+    p = np.tensordot(p, core_a, axes=([1, 2], [1, 3]))           # l R s N
+    return np.tensordot(p, core_right, axes=([1, 3], [2, 1]))    # l s r
+
+
+def phi_fwd_A(phi_now, core_left, core_a, core_right):
+    """Phi_next[L,S,R] = sum Phi[l,s,r] xL[l,M,L] A[s,M,N,S] xR[r,N,R]  (amen_solve.h:58-68)."""
+    p = np.tensordot(core_left, phi_now, axes=([0], [0]))        # M L s r
+    p = np.tensordot(p, core_a, axes=([0, 2], [1, 0]))           # L r N S
+    return np.tensordot(p, core_right, axes=([1, 2], [0, 1]))    # L S R
+
+
+def phi_bck_rhs(phi_now, core_b, core):
+    """Phib_prev[b,r] = sum b[b,n,B] Phib[B,R] x[r,n,R]  (amen_solve.h:78-83)."""
+    p = np.tensordot(core_b, phi_now, axes=([2], [0]))           # b n R
+    return np.tensordot(p, core, axes=([1, 2], [1, 2]))          # b r
+
+
+def phi_fwd_rhs(phi_now, core_b, core):
+    """Phib_next[B,R] = sum Phib[b,r] b[b,n,B] x[r,n,R]  (amen_solve.h:93-98)."""
+    p = np.tensordot(phi_now, core_b, axes=([0], [0]))           # r n B
+    return np.tensordot(p, core, axes=([0, 1], [0, 1]))          # B R
+
+
+# --------------------------------------------------------------------------- GMRES
+def _gmres_single(matvec, rhs, x0, iters, threshold):
+    """One GMRES(iters) cycle with MGS and Givens rotations (gmres.h:34-144)."""
+    r = rhs - matvec(x0)
+    b_norm = np.linalg.norm(rhs)
+    r_norm = np.linalg.norm(r)
+    if r_norm <= 0:
+        return x0.copy(), 1, 0
+    Q = [r / r_norm]
+    H = np.zeros((iters + 1, iters))
+    beta = np.zeros(iters + 1)
+    beta[0] = r_norm
+    cs = np.zeros(iters)
+    sn = np.zeros(iters)
+    flag = 0
+    k = 0
+    for k in range(iters):
+        q = matvec(Q[k])
+        for i in range(k + 1):                      # modified Gram-Schmidt (:98-101)
+            H[i, k] = q @ Q[i]
+            q = q - H[i, k] * Q[i]
+        h = np.linalg.norm(q)
+        with np.errstate(divide="ignore", invalid="ignore"):
+            q = q / h
+        H[k + 1, k] = h
+        Q.append(q)
+        hcol = H[:k + 2, k].copy()                  # Givens (:20-31, :113-116)
+        for i in range(k):
+            t = cs[i] * hcol[i] + sn[i] * hcol[i + 1]
+            hcol[i + 1] = -sn[i] * hcol[i] + cs[i] * hcol[i + 1]
+            hcol[i] = t
+        den = np.sqrt(hcol[k] ** 2 + hcol[k + 1] ** 2)
+        c, s = hcol[k] / den, hcol[k + 1] / den
+        hcol[k] = c * hcol[k] + s * hcol[k + 1]
+        hcol[k + 1] = 0.0
+        H[:k + 2, k] = hcol
+        cs[k], sn[k] = c, s
+        beta[k + 1] = -sn[k] * beta[k]              # (:120-122)
+        beta[k] = cs[k] * beta[k]
+        if abs(beta[k + 1]) / b_norm <= threshold:
+            flag = 1
+            break
+    k = k if k < iters else iters - 1               # (:131)
+    y = np.linalg.solve(H[:k + 1, :k + 1], beta[:k + 1])
+    sol = x0.copy()
+    for i in range(k + 1):
+        sol = sol + Q[i] * y[i]
+    return sol, flag, k + 1
+
+
+def gmres(matvec, rhs, x0, max_iters, threshold, resets):
+    """Restarted GMRES (gmres.h:146-161). Returns (solution, flag, iterations)."""
+    nit, flag = 0, 0
+    xs = x0
+    sol = x0
+    for _ in range(resets):
+        sol, flag, nowit = _gmres_single(matvec, rhs, xs, max_iters, threshold)
+        nit += nowit
+        if flag == 1:
+            break
+        xs = sol.copy()
+    return sol, flag, nit
+
+
+# --------------------------------------------------------------------------- QR / rank tools
+def _qr(m):
+    q, r = np.linalg.qr(m, mode="reduced")
+    return q, r
+
+
+def rl_orthogonal(cores):
+    """Right-to-left orthogonalisation in place (ortho.h:46-73). Returns the new ranks."""
+    d = len(cores)
+    for i in range(d - 1, 0, -1):
+        r0, n, r1 = cores[i].shape
+        q, r = _qr(cores[i].reshape(r0, n * r1).T)
+        rn = r.shape[0]
+        cores[i] = q.T.reshape(rn, n, r1)
+        rp, np_, _ = cores[i - 1].shape
+        cores[i - 1] = (cores[i - 1].reshape(-1, r0) @ r.T).reshape(rp, np_, rn)
+    return [c.shape[0] for c in cores] + [1]
+
+
+def rank_chop(s, eps):
+    """Smallest rank whose discarded tail has 2-norm below eps (ortho.h:16-44)."""
+    n = s.shape[0]
+    r = n - 1
+    if np.linalg.norm(s) == 0.0:
+        return 1
+    if eps <= 0.0:
+        return r
+    while r > 0:
+        if np.sum(s[r:] ** 2) >= eps * eps:
+            break
+        r -= 1
+    r += 1
+    return max(r, 1)
+
+
+def round_tt(cores, epsilon, rmax):
+    """TT rounding (compression.h:3-41): left-to-right QR then right-to-left SVD."""
+    cores = [c.copy() for c in cores]
+    d = len(cores)
+    if d <= 1:
+        return cores
+    for i in range(d - 1):                                  # lr_orthogonal (ortho.h:77-101)
+        r0, n, r1 = cores[i].shape
+        q, r = _qr(cores[i].reshape(r0 * n, r1))
+        cores[i] = q.reshape(r0, n, q.shape[1])
+        rn, nn, rr = cores[i + 1].shape
+        cores[i + 1] = (r @ cores[i + 1].reshape(rn, -1)).reshape(q.shape[1], nn, rr)
+    eps = epsilon / np.sqrt(d - 1.0)
+    core_now = cores[d - 1].reshape(cores[d - 1].shape[0], -1)
+    for i in range(d - 1, 0, -1):
+        r0, n, r1 = cores[i - 1].shape
+        core_next = cores[i - 1].reshape(r0 * n, r1)
+        u, s, vt = np.linalg.svd(core_now, full_matrices=False)
+        rc = rank_chop(s, eps * np.linalg.norm(s))
+        rnew = min(rmax, rc)
+        us = u[:, :rnew] @ np.diag(s[:rnew])
+        core_next = core_next @ us
+        cores[i] = vt[:rnew, :].reshape(rnew, cores[i].shape[1], cores[i].shape[2])
+        cores[i - 1] = core_next.reshape(r0, n, rnew)
+        core_now = core_next.reshape(r0, -1)
+    return cores
+
+
+# --------------------------------------------------------------------------- AMEn
+def amen_solve(A, b, x0=None, z0=None, nswp=22, eps=1e-10, rmax=32768, max_full=500,
+               kickrank=4, kick2=0, local_iterations=40, resets=2, rng=None, trace=None):
+    """AMEn solve following cpp/amen_solve.h:122-549 (FP64, no preconditioner).
+
+    A: operator cores rA x n x n x rA'; b: rhs cores; x0: optional initial cores;
+    z0: initial residual-proxy cores rz x n x rz' (normal random; the reference draws
+    them with torch.randn, so a test passes the same draws). `trace`, if a dict, receives
+    per-sweep ranks, max_res and GMRES iteration counts.
+    """
+    rng = np.random.default_rng(0) if rng is None else rng
+    d = len(A)
+    N = [a.shape[2] for a in A]
+    if x0 is None:
+        x = [np.ones((1, n, 1)) for n in N]                     # :161-164
+    else:
+        x = [c.copy() for c in x0]
+    rx = [1] + [c.shape[2] for c in x]
+    rz = [1] + [kickrank + kick2] * (d - 1) + [1]               # :169-173
+    if z0 is None:
+        z = [rng.standard_normal((rz[i], N[i], rz[i + 1])) for i in range(d)]
+    else:
+        z = [c.copy() for c in z0]
+    rz = rl_orthogonal(z)                                        # :178
+    rz = [1] + rz[1:-1] + [1]
+    one3, one2 = np.ones((1, 1, 1)), np.ones((1, 1))
+    Phiz = [one3] + [None] * (d - 1) + [one3]
+    Phiz_b = [one2] + [None] * (d - 1) + [one2]
+    Phis = [one3] + [None] * (d - 1) + [one3]
+    Phis_b = [one2] + [None] * (d - 1) + [one2]
+    normA = np.ones(max(d - 1, 0))
+    normb = np.ones(max(d - 1, 0))
+    normx = np.ones(max(d - 1, 0))
+    nrmsc = 1.0
+    damp = 2.0
+    last = False
+    if trace is not None:
+        trace.setdefault("ranks", [])
+        trace.setdefault("max_res", [])
+        trace.setdefault("gmres_iters", [])
+    swp = 0
+    for swp in range(nswp):
+        for k in range(d - 1, 0, -1):                            # :222-292
+            if not last:
+                if swp > 0:
+                    czA = local_product(Phiz[k + 1], Phiz[k], A[k], x[k])
+                    czy = np.tensordot(Phiz_b[k], b[k], axes=([0], [0]))
+                    czy = np.tensordot(czy, Phiz_b[k + 1], axes=([2], [0]))
+                    czy = czy * nrmsc - czA
+                    _, _, vt = np.linalg.svd(czy.reshape(czy.shape[0], -1), full_matrices=False)
+                    temp = min(kickrank, vt.shape[0])
+                    cz_new = vt[:temp, :].T
+                    if k < d - 1 and kick2 > 0:
+                        cz_new = np.concatenate([cz_new, rng.standard_normal((cz_new.shape[0], kick2))], 1)
+                else:
+                    cz_new = z[k].reshape(rz[k], -1).T
+                qz, _ = _qr(cz_new)
+                rz[k] = qz.shape[1]
+                z[k] = qz.T.reshape(rz[k], N[k], rz[k + 1])
+            if swp > 0:
+                nrmsc = nrmsc * normA[k - 1] * normx[k - 1] / normb[k - 1]
+            core = x[k].reshape(rx[k], N[k] * rx[k + 1]).T       # :252-267
+            q, r = _qr(core)
+            core_prev = np.tensordot(x[k - 1], r.T, axes=([2], [0]))
+            rx[k] = q.shape[1]
+            current_norm = np.linalg.norm(core_prev)
+            if current_norm > 0:
+                core_prev = core_prev / current_norm
+            else:
+                current_norm = 1.0
+            normx[k - 1] *= current_norm
+            x[k] = q.T.reshape(rx[k], N[k], rx[k + 1]).copy()
+            x[k - 1] = core_prev.copy()
+            Phis[k] = phi_bck_A(Phis[k + 1], x[k], A[k], x[k])   # :270-285
+            Phis_b[k] = phi_bck_rhs(Phis_b[k + 1], b[k], x[k])
+            nrm = np.linalg.norm(Phis[k])
+            nrm = nrm if nrm > 0 else 0.0
+            normA[k - 1] = nrm
+            Phis[k] = Phis[k] / nrm
+            nrm = np.linalg.norm(Phis_b[k])
+            nrm = nrm if nrm > 0 else 0.0
+            normb[k - 1] = nrm
+            Phis_b[k] = Phis_b[k] / nrm
+            nrmsc = nrmsc * normb[k - 1] / (normA[k - 1] * normx[k - 1])
+            if not last:                                          # :288-291
+                Phiz[k] = phi_bck_A(Phiz[k + 1], z[k], A[k], x[k]) / normA[k - 1]
+                Phiz_b[k] = phi_bck_rhs(Phiz_b[k + 1], b[k], z[k]) / normb[k - 1]
+        max_res = 0.0
+        gm_its = []
+        for k in range(d):                                        # :296-512
+            previous_solution = x[k].reshape(-1)
+            rhs = np.tensordot(Phis_b[k], b[k] * nrmsc, axes=([0], [0]))
+            rhs = np.tensordot(rhs, Phis_b[k + 1], axes=([2], [0])).reshape(-1)
+            norm_rhs = np.linalg.norm(rhs)
+            real_tol = (eps / np.sqrt(d)) / damp
+            shape_now = (rx[k], N[k], rx[k + 1])
+            use_full = rx[k] * N[k] * rx[k + 1] < max_full
+            pl, pr, ak = Phis[k], Phis[k + 1], A[k]
+
+            def op(v, pl=pl, pr=pr, ak=ak, shape_now=shape_now):
+                return local_product(pr, pl, ak, v.reshape(shape_now)).reshape(-1)
+
+            if use_full:                                          # :319-330
+                Bp = np.tensordot(ak, pr, axes=([3], [1]))        # s m n L R
+                Bp = np.tensordot(pl, Bp, axes=([1], [0]))        # l r m n L R
+                nl = rx[k] * N[k] * rx[k + 1]
+                B = Bp.transpose(0, 2, 4, 1, 3, 5).reshape(nl, nl)
+                solution_now = np.linalg.solve(B, rhs)
+                res_old = np.linalg.norm(B @ previous_solution - rhs) / norm_rhs
+                res_new = np.linalg.norm(B @ solution_now - rhs) / norm_rhs
+
+                def apply(v, B=B):
+                    return B @ v
+                gm_its.append(0)
+            else:                                                 # :331-365
+                eps_local = real_tol * norm_rhs
+                drhs = rhs - op(previous_solution)
+                eps_local /= np.linalg.norm(drhs)
+                ps = 0.0 * previous_solution
+                solution_now, flag, nit = gmres(op, drhs, ps, local_iterations, eps_local, resets)
+                gm_its.append(nit)
+                solution_now = solution_now + previous_solution
+                res_old = np.linalg.norm(op(previous_solution) - rhs) / norm_rhs
+                res_new = np.linalg.norm(op(solution_now) - rhs) / norm_rhs
+                apply = op
+            max_res = max(max_res, res_old)
+            solution_now = solution_now.reshape(rx[k] * N[k], rx[k + 1])
+            if k < d - 1:                                         # :382-404
+                u, s, vt = np.linalg.svd(solution_now, full_matrices=False)
+                v = vt
+                r = u.shape[1]
+                while r > 0:
+                    sol = (u[:, :r] * s[:r]) @ v[:r, :]
+                    res = np.linalg.norm(apply(sol.reshape(-1)) - rhs) / norm_rhs
+                    if res > max(res_new, real_tol * damp):
+                        break
+                    r -= 1
+                r += 1
+                r = r if (r < u.shape[1] and r < rmax) else min(u.shape[1], rmax)
+            else:                                                 # :406-410
+                u, v = _qr(solution_now)
+                r = u.shape[1]
+                s = np.ones(r)
+            u = u[:, :r]                                          # :413-416
+            v = (np.diag(s[:r]) @ v[:r, :]).T
+            if not last:                                          # :418-442
+                tmp = (u @ v.T).reshape(rx[k], N[k], rx[k + 1])
+                czA = local_product(Phiz[k + 1], Phiz[k], A[k], tmp)
+                czy = np.tensordot(Phiz_b[k], nrmsc * b[k], axes=([0], [0]))
+                czy = np.tensordot(czy, Phiz_b[k + 1], axes=([2], [0]))
+                tmp = (czy - czA).reshape(rz[k] * N[k], rz[k + 1])
+                uz, _, _ = np.linalg.svd(tmp, full_matrices=False)
+                rtmp = min(kickrank, uz.shape[1])
+                tmp = uz[:, :rtmp]
+                if k < d - 1 and kick2 > 0:
+                    tmp = np.concatenate([tmp, rng.standard_normal((tmp.shape[0], kick2))], 1)
+                qz, _ = _qr(tmp)
+                rz[k + 1] = qz.shape[1]
+                z[k] = qz.reshape(rz[k], N[k], rz[k + 1]).copy()
+            if k < d - 1:                                         # :443-504
+                if not last:
+                    tmp = (u @ v.T).reshape(rx[k], N[k], rx[k + 1])
+                    left_res = local_product(Phiz[k + 1], Phis[k], A[k], tmp)
+                    left_b = np.tensordot(Phis_b[k], b[k] * nrmsc, axes=([0], [0]))
+                    left_b = np.tensordot(left_b, Phiz_b[k + 1], axes=([2], [0]))
+                    uk = (left_b - left_res).reshape(u.shape[0], -1)
+                    uk = np.concatenate([u, uk], 1)
+                    r_add = left_res.shape[2]
+                    u, Rmat = _qr(uk)
+                    v = np.concatenate([v, np.zeros((rx[k + 1], r_add))], 1)
+                    v = v @ Rmat.T
+                r = u.shape[1]
+                v = np.tensordot(v, x[k + 1], axes=([0], [0]))
+                nrmsc = nrmsc * normA[k] * normx[k] / normb[k]
+                norm_now = np.linalg.norm(v)
+                if norm_now > 0:
+                    v = v / norm_now
+                else:
+                    norm_now = 1.0
+                normx[k] *= norm_now
+                x[k] = u.reshape(rx[k], N[k], r).copy()
+                x[k + 1] = v.reshape(r, N[k + 1], rx[k + 2]).copy()
+                rx[k + 1] = r
+                Phis[k + 1] = phi_fwd_A(Phis[k], x[k], A[k], x[k])
+                Phis_b[k + 1] = phi_fwd_rhs(Phis_b[k], b[k], x[k])
+                nrm = np.linalg.norm(Phis[k + 1])
+                nrm = nrm if nrm > 0 else 1.0
+                normA[k] = nrm
+                Phis[k + 1] = Phis[k + 1] / nrm
+                nrm = np.linalg.norm(Phis_b[k + 1])
+                nrm = nrm if nrm > 0 else 1.0
+                normb[k] = nrm
+                Phis_b[k + 1] = Phis_b[k + 1] / nrm
+                nrmsc = nrmsc * normb[k] / (normA[k] * normx[k])
+                if not last:
+                    Phiz[k + 1] = phi_fwd_A(Phiz[k], z[k], A[k], x[k]) / normA[k]
+                    Phiz_b[k + 1] = phi_fwd_rhs(Phiz_b[k], b[k], z[k]) / normb[k]
+            else:
+                usv = (u * s[:r]) @ v[:r, :].T
+                x[k] = usv.reshape(rx[k], N[k], rx[k + 1])
+        if trace is not None:
+            trace["ranks"].append(list(rx))
+            trace["max_res"].append(max_res)
+            trace["gmres_iters"].append(gm_its)
+        if last:
+            break
+        if max_res < eps:
+            last = True
+    if trace is not None:
+        trace["sweeps"] = swp + 1
+    norm_x = np.exp(np.sum(np.log(normx)) / d) if d > 1 else 1.0   # :534-541
+    return [c * norm_x for c in x]
+
+
+# --------------------------------------------------------------------------- TT helpers
+def tt_full(cores):
+    """Dense tensor of a TT vector (cores r x n x r') -- reference cpp/full.h:3-12."""
+    t = cores[0]
+    for c in cores[1:]:
+        t = np.tensordot(t, c, axes=([t.ndim - 1], [0]))
+    return t.reshape(t.shape[1:-1])
+
+
+def tt_matvec(A, x):
+    """Cores of A @ x for a TT operator A and TT vector x (rank rA*rx)."""
+    out = []
+    for a, c in zip(A, x):
+        ra, m, n, ra2 = a.shape
+        rc, _, rc2 = c.shape
+        w = np.einsum("amnb,cnd->acmbd", a, c)
+        out.append(w.reshape(ra * rc, m, ra2 * rc2))
+    return out
+
+
+def tt_add(x, y, alpha=1.0, beta=1.0):
+    """Cores of alpha*x + beta*y by block concatenation."""
+    d = len(x)
+    out = []
+    for k in range(d):
+        a, c = x[k], y[k]
+        if k == 0:
+            a = alpha * a
+            c = beta * c
+        ra, n, ra2 = a.shape
+        rc, _, rc2 = c.shape
+        if d == 1:
+            out.append(a + c)
+        elif k == 0:
+            out.append(np.concatenate([a, c], axis=2))
+        elif k == d - 1:
+            out.append(np.concatenate([a, c], axis=0))
+        else:
+            z = np.zeros((ra + rc, n, ra2 + rc2))
+            z[:ra, :, :ra2] = a
+            z[ra:, :, ra2:] = c
+            out.append(z)
+    return out
+
+
+def tt_norm(cores):
+    """Frobenius norm by a left-to-right QR sweep (no squaring of small quantities)."""
+    cur = cores[0]
+    for k in range(len(cores) - 1):
+        r0, n, r1 = cur.shape
+        q, r = _qr(cur.reshape(r0 * n, r1))
+        nxt = cores[k + 1]
+        cur = (r @ nxt.reshape(nxt.shape[0], -1)).reshape(q.shape[1], nxt.shape[1], nxt.shape[2])
+    return float(np.linalg.norm(cur))
+
+
+def tt_residual(A, x, b):
+    """||A x - b|| / ||b|| in TT arithmetic."""
+    return tt_norm(tt_add(tt_matvec(A, x), b, 1.0, -1.0)) / tt_norm(b)

@@ -1,0 +1,92 @@
+"""Write the golden fixtures of the AMEn path from the reference's own C++ solver.
+
+Runs in the build container only (it needs oracle/_ref, built from the reference sources
+by `make -C oracle`). For each case it stores the inputs (operator and rhs cores, the
+residual-proxy initial draws), the reference's solution cores and its trajectory summary
+(final ranks, sweep count), as .npz next to this script.
+
+    python tests/golden/make_golden.py
+"""
+import importlib.util
+import os
+import sys
+
+import numpy as np
+import torch
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+from torchtt_amd import workloads as W  # noqa: E402
+
+
+def load_ref():
+    import glob
+    so = glob.glob(os.path.join(ROOT, "oracle", "_ref", "torchttcpp*.so"))
+    if not so:
+        raise SystemExit("oracle/_ref is not built: run `make -C oracle` first")
+    spec = importlib.util.spec_from_file_location("torchttcpp", so[0])
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def run_case(ref, name, A, b, seed, nswp, eps, rmax, kickrank=4, kick2=0, x0=None):
+    d = len(A)
+    N = [a.shape[2] for a in A]
+    rA = [1] + [a.shape[3] for a in A]
+    rb = [1] + [c.shape[2] for c in b]
+    rz = [1] + [kickrank + kick2] * (d - 1) + [1]
+    torch.manual_seed(seed)
+    z0 = [torch.randn(rz[i], N[i], rz[i + 1], dtype=torch.float64) for i in range(d)]
+    torch.manual_seed(seed)
+    if x0 is None:
+        xc, rx = [], [1] * (d + 1)
+    else:
+        xc, rx = x0, [1] + [c.shape[2] for c in x0]
+    x = ref.amen_solve(A, b, xc, N, rA, rb, rx, nswp, eps, rmax, 500, kickrank, kick2, 40, 2, False, 0)
+    out = {"d": d, "seed": seed, "nswp": nswp, "eps": eps, "rmax": rmax, "kickrank": kickrank,
+           "kick2": kick2, "ranks": np.array([1] + [c.shape[2] for c in x])}
+    for k in range(d):
+        out["A%d" % k] = A[k].numpy()
+        out["b%d" % k] = b[k].numpy()
+        out["z%d" % k] = z0[k].numpy()
+        out["x%d" % k] = x[k].numpy()
+        if x0 is not None:
+            out["x0_%d" % k] = x0[k].numpy()
+    path = os.path.join(HERE, name + ".npz")
+    np.savez_compressed(path, **out)
+    print("wrote", path, "ranks", out["ranks"].tolist())
+
+
+def main():
+    torch.set_num_threads(8)
+    ref = load_ref()
+    # Laplacian d=6, n=16, h=1/17, b = ones, rank cap 20, 10 sweeps (two tolerances)
+    A, b = W.laplace_ones(6, 16)
+    run_case(ref, "laplace_d6_n16_eps1e-8", A, b, 0, 10, 1e-8, 20)
+    run_case(ref, "laplace_d6_n16_eps1e-10", A, b, 0, 10, 1e-10, 20)
+    # The reference test problem shape (tests/test_solvers.py:15-18): random operator
+    # [(4,4),(5,5),(6,6)] ranks [1,2,3,1], b = A x for a random x of ranks [1,2,3,1].
+    rng = np.random.default_rng(11)
+    Ar = [torch.from_numpy(rng.standard_normal(s)) for s in [(1, 4, 4, 2), (2, 5, 5, 3), (3, 6, 6, 1)]]
+    xr = [rng.standard_normal(s) for s in [(1, 4, 2), (2, 5, 3), (3, 6, 1)]]
+    br = []
+    for a, c in zip(Ar, xr):
+        w = np.einsum("amnb,cnd->acmbd", a.numpy(), c)
+        br.append(torch.from_numpy(w.reshape(a.shape[0] * c.shape[0], a.shape[1], a.shape[3] * c.shape[2])))
+    run_case(ref, "random_456", Ar, br, 3, 22, 1e-10, 32768)
+    # GMRES-sized local problems: Laplacian + potential d=4, n=16 with a rank-3 rhs
+    A, b = W.laplace_potential(4, 16, seed=5, rhs_rank=3)
+    run_case(ref, "potential_d4_n16", A, b, 1, 22, 1e-9, 60)
+    # warm start from x0 (a rank-2 random guess) on the Laplacian d=5, n=12
+    A, b = W.laplace_ones(5, 12)
+    rng = np.random.default_rng(2)
+    x0 = [torch.from_numpy(rng.standard_normal(s)) for s in [(1, 12, 2), (2, 12, 2), (2, 12, 2), (2, 12, 2), (2, 12, 1)]]
+    run_case(ref, "laplace_d5_n12_x0", A, b, 4, 22, 1e-9, 16, x0=x0)
+    # kickrank = 0 is not a fixture: the reference raises on it (rl_orthogonal_this on
+    # empty residual-proxy cores, cpp/amen_solve.h:169-178).
+
+
+if __name__ == "__main__":
+    main()

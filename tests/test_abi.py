@@ -1,0 +1,52 @@
+"""The C-ABI library loads and exports every entry point include/ttamen.h declares; the
+host-side size queries work without a GPU."""
+import ctypes
+import os
+import re
+
+import pytest
+
+from conftest import ROOT
+
+HEADER = os.path.join(ROOT, "include", "ttamen.h")
+
+
+def _declared():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(ttamen_[A-Za-z0-9_]+)\s*\(", text)))
+
+
+def test_header_and_binding_agree():
+    from torchtt_amd import _lib
+    assert _declared() == sorted(_lib.exported_names())
+
+
+def test_library_exports_every_declared_symbol():
+    from torchtt_amd import _lib
+    lib = _lib.load()
+    for name in _declared():
+        assert hasattr(lib, name), name
+    assert b"gfx950" in lib.ttamen_version()
+
+
+def test_workspace_queries_without_gpu():
+    from torchtt_amd import _lib
+    lib = _lib.load()
+    d = 10
+    N = _lib.i64_array([64] * d)
+    rA = _lib.i64_array([1] + [4] * (d - 1) + [1])
+    rb = _lib.i64_array([1] + [4] * (d - 1) + [1])
+    ws = lib.ttamen_amen_solve_workspace_size(d, N, rA, rb, None, 80, 4, 0, 40, 500, 0)
+    assert ws > 0
+    # solution buffer: cores at their rank bounds min(r_{k} n, rmax + kick, ...)
+    out = lib.ttamen_amen_solve_output_elems(d, N, None, 80, 4, 0)
+    R = [1]
+    for k in range(d):
+        right = 64 ** (d - k - 1)
+        R.append(1 if k == d - 1 else min(R[-1] * 64, 84, right + 4))
+    assert out == sum(R[k] * 64 * R[k + 1] for k in range(d))
+    # invalid input: size 0 and a message
+    bad = _lib.i64_array([1] + [4] * (d - 1) + [2])  # boundary rank must be 1
+    assert lib.ttamen_amen_solve_workspace_size(d, N, bad, rb, None, 80, 4, 0, 40, 500, 0) == 0
+    assert b"boundary" in lib.ttamen_last_error()

@@ -1,0 +1,139 @@
+"""End-to-end parity of the GPU AMEn solver (torchtt_amd.solvers.amen_solve).
+
+Bars: the solution agrees with the reference solver's solution on the same seeded inputs
+to a relative Frobenius difference <= 1e-8 and has the same final TT ranks; the relative
+TT residual ||Ax - b|| / ||b|| is checked for every result.
+"""
+import sys
+
+import numpy as np
+import pytest
+import torch
+
+import amen_oracle as O
+import laplace_exact as LE
+from conftest import GOLDEN_CASES, gpu_available, load_golden, load_ref
+
+pytestmark = [pytest.mark.gpu,
+              pytest.mark.skipif(not gpu_available(), reason="needs a ROCm GPU")]
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _tt(cores, dev="cuda"):
+    import torchtt_amd as tt
+    return tt.TT([torch.as_tensor(np.ascontiguousarray(c)).to(dev) for c in cores])
+
+
+def _rel_tt(x, y):
+    """||x - y|| / ||y|| in TT arithmetic (QR-based norm, no cancellation)."""
+    return float((x - y).norm() / y.norm())
+
+
+@pytest.mark.parametrize("name", GOLDEN_CASES)
+def test_amen_matches_reference_fixture(name):
+    import torchtt_amd as tt
+    c = load_golden(name)
+    _say("amen golden", name)
+    A, b = _tt(c["A"]), _tt(c["b"])
+    x0 = _tt(c["x0"]) if c["x0"] is not None else None
+    torch.manual_seed(c["seed"])
+    # the residual-proxy cores are drawn from the CPU generator, as the reference did
+    A_cpu = _tt(c["A"], "cpu")
+    z_expected = [torch.from_numpy(z) for z in c["z"]]
+    torch.manual_seed(c["seed"])
+    x = tt.solvers.amen_solve(A_cpu, _tt(c["b"], "cpu"), x0=_tt(c["x0"], "cpu") if x0 is not None else None,
+                              nswp=c["nswp"], eps=c["eps"], rmax=c["rmax"], kickrank=c["kickrank"], kick2=c["kick2"])
+    # check the draws we fed the solver are the fixture's
+    torch.manual_seed(c["seed"])
+    d = c["d"]
+    rz = [1] + [c["kickrank"] + c["kick2"]] * (d - 1) + [1]
+    for k in range(d):
+        zk = torch.randn((rz[k], A_cpu.N[k], rz[k + 1]), dtype=torch.float64)
+        assert torch.equal(zk, z_expected[k])
+    xr = _tt(c["x"], "cpu")
+    assert x.R == c["ranks"].tolist(), (x.R, c["ranks"].tolist())
+    diff = _rel_tt(x, xr)
+    res = tt.solvers.residual(A_cpu, x, _tt(c["b"], "cpu"))
+    _say("  rel diff vs reference %.3e, residual %.3e, stats %s" % (diff, res, tt.solvers.last_stats))
+    assert diff <= 1e-8
+    assert res <= 50 * c["eps"]
+
+
+def test_laplace_closed_form():
+    """d=6, n=16, h=1/17 Laplacian, b = ones, rank cap 20: error against the exact solution."""
+    import torchtt_amd as tt
+    from torchtt_amd import workloads as W
+    _say("amen laplace closed form")
+    A, b = W.laplace_ones(6, 16)
+    A, b = tt.TT([c.cuda() for c in A]), tt.TT([c.cuda() for c in b])
+    torch.manual_seed(0)
+    x = tt.solvers.amen_solve(A, b, nswp=10, eps=1e-10, rmax=20)
+    ex = LE.laplace_solve_full(6, 16, 1.0 / 17)
+    err = np.linalg.norm(x.full().cpu().numpy() - ex) / np.linalg.norm(ex)
+    res = tt.solvers.residual(A, x, b)
+    _say("  error vs closed form %.3e residual %.3e" % (err, res))
+    assert err < 1e-10
+    assert res < 1e-9
+
+
+def test_laplace_potential_d10_against_reference():
+    """d=10, n=64, Laplacian + potential (rank 4), random rank-4 rhs, rank cap 80."""
+    import torchtt_amd as tt
+    from torchtt_amd import workloads as W
+    ref = load_ref()
+    _say("amen d10 n64 vs", "reference build" if ref is not None else "oracle")
+    A, b = W.laplace_potential(10, 64, seed=0)
+    eps = 1e-10
+    torch.manual_seed(0)
+    x = tt.solvers.amen_solve(tt.TT(A), tt.TT(b), eps=eps, rmax=80)
+    stats = dict(tt.solvers.last_stats)
+    torch.manual_seed(0)
+    if ref is not None:
+        torch.set_num_threads(min(16, torch.get_num_threads()))
+        xr = ref.amen_solve(A, b, [], [64] * 10, tt.TT(A).R, tt.TT(b).R, [1] * 11, 22, eps, 80, 500, 4, 0, 40, 2,
+                            False, 0)
+        xr = tt.TT(list(xr))
+    else:
+        rz = [1] + [4] * 9 + [1]
+        z0 = [torch.randn((rz[k], 64, rz[k + 1]), dtype=torch.float64).numpy() for k in range(10)]
+        xr = tt.TT([torch.from_numpy(c) for c in O.amen_solve([a.numpy() for a in A], [c.numpy() for c in b],
+                                                              z0=z0, eps=eps, rmax=80)])
+    diff = _rel_tt(x, xr)
+    res = tt.solvers.residual(tt.TT(A), x, tt.TT(b))
+    _say("  ranks gpu %s ref %s" % (x.R, xr.R))
+    _say("  rel diff %.3e residual %.3e stats %s" % (diff, res, stats))
+    assert res < 1e-8
+    assert diff < 1e-8
+
+
+def test_edge_cases():
+    """d = 1, d = 2, no enrichment (kickrank = 0, which the reference rejects), warm start."""
+    import torchtt_amd as tt
+    rng = np.random.default_rng(9)
+    # d = 1: a single dense system solved by the direct local solver
+    A1 = [rng.standard_normal((1, 12, 12, 1)) + 12 * np.eye(12).reshape(1, 12, 12, 1)]
+    b1 = [rng.standard_normal((1, 12, 1))]
+    _say("amen d=1")
+    x1 = tt.solvers.amen_solve(_tt(A1), _tt(b1), eps=1e-12)
+    want = np.linalg.solve(A1[0][0, :, :, 0], b1[0][0, :, 0])
+    assert np.linalg.norm(x1.full().cpu().numpy() - want) / np.linalg.norm(want) < 1e-12
+    # d = 2
+    from torchtt_amd import workloads as W
+    A2, b2 = W.laplace_potential(2, 30, seed=3, rhs_rank=2)
+    _say("amen d=2")
+    x2 = tt.solvers.amen_solve(tt.TT(A2).cuda(), tt.TT(b2).cuda(), eps=1e-10)
+    assert tt.solvers.residual(tt.TT(A2).cuda(), x2, tt.TT(b2).cuda()) < 1e-9
+    # kickrank = 0 (plain ALS sweeps)
+    A3, b3 = W.laplace_ones(4, 10)
+    _say("amen kickrank=0")
+    x3 = tt.solvers.amen_solve(tt.TT(A3).cuda(), tt.TT(b3).cuda(), eps=1e-10, kickrank=0, nswp=30)
+    assert tt.solvers.residual(tt.TT(A3).cuda(), x3, tt.TT(b3).cuda()) < 1e-6
+    # input validation errors match the reference
+    with pytest.raises(tt.errors.InvalidArguments):
+        tt.solvers.amen_solve(tt.TT(A3), b3)
+    with pytest.raises(tt.errors.IncompatibleTypes):
+        tt.solvers.amen_solve(tt.TT(b3), tt.TT(b3))

@@ -1,0 +1,245 @@
+"""Kernel-level parity on the GPU: each C-ABI entry point against an FP64 CPU reference
+(the oracle's contractions, numpy linear algebra, or a torch fp32/fp64 product)."""
+import ctypes
+import sys
+
+import numpy as np
+import pytest
+import torch
+
+import amen_oracle as O
+from conftest import gpu_available
+
+pytestmark = [pytest.mark.gpu,
+              pytest.mark.skipif(not gpu_available(), reason="needs a ROCm GPU")]
+
+DEV = "cuda"
+
+
+def _lib():
+    from torchtt_amd import _lib as L
+    return L, L.load()
+
+
+def _p(t):
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def _stream():
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _ws(nbytes):
+    return torch.empty(int(nbytes), dtype=torch.uint8, device=DEV)
+
+
+def _rel(a, b):
+    a = np.asarray(a); b = np.asarray(b)
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+# ----------------------------------------------------------------- GEMM
+@pytest.mark.parametrize("M,N,K,batch,trans", [(64, 64, 64, 1, "nn"), (100, 37, 260, 3, "tn"),
+                                               (5, 7, 3, 2, "nt"), (84, 84, 5376, 1, "tn"),
+                                               (300, 20, 1000, 1, "tt"), (1, 1, 1, 1, "nn")])
+def test_gemm_f64(M, N, K, batch, trans):
+    L, lib = _lib()
+    _say("gemm", M, N, K, batch, trans)
+    g = torch.Generator().manual_seed(M * 7 + N)
+    A = torch.randn(batch, M, K, dtype=torch.float64, generator=g)
+    B = torch.randn(batch, K, N, dtype=torch.float64, generator=g)
+    C0 = torch.randn(batch, M, N, dtype=torch.float64, generator=g)
+    Ad = (A.transpose(1, 2).contiguous() if trans[0] == "t" else A).to(DEV)
+    Bd = (B.transpose(1, 2).contiguous() if trans[1] == "t" else B).to(DEV)
+    Cd = C0.clone().to(DEV)
+    sa = (1, M) if trans[0] == "t" else (K, 1)
+    sb = (1, K) if trans[1] == "t" else (N, 1)
+    ws = _ws(64 << 20)
+    code = lib.ttamen_gemm_f64(M, N, K, batch, 1.5, _p(Ad), sa[0], sa[1], M * K, _p(Bd), sb[0], sb[1], K * N,
+                               -0.5, _p(Cd), N, 1, M * N, _p(ws), ws.numel(), _stream())
+    L.check("gemm", code)
+    torch.cuda.synchronize()
+    want = 1.5 * torch.bmm(A, B) - 0.5 * C0
+    assert _rel(Cd.cpu().numpy(), want.numpy()) < 1e-13
+
+
+# ----------------------------------------------------------------- local operator
+SHAPES = [(84, 4, 84, 64, 4, 84, 84, 1), (7, 2, 5, 16, 2, 6, 4, 3), (4, 4, 20, 64, 4, 4, 24, 1),
+          (1, 1, 1, 16, 2, 5, 5, 2), (20, 2, 20, 8, 1, 1, 1, 1)]
+
+
+def _rand_local(shape, dtype=torch.float64, seed=0):
+    l1, s, r1, n, S, l2, r2, nv = shape
+    g = torch.Generator().manual_seed(seed)
+    pl = torch.randn(l1, s, r1, dtype=dtype, generator=g)
+    a = torch.randn(s, n, n, S, dtype=dtype, generator=g)
+    pr = torch.randn(l2, S, r2, dtype=dtype, generator=g)
+    x = torch.randn(nv, r1, n, r2, dtype=dtype, generator=g)
+    return pl, a, pr, x
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+def test_local_matvec_f64(shape):
+    L, lib = _lib()
+    l1, s, r1, n, S, l2, r2, nv = shape
+    _say("local_matvec f64", shape)
+    pl, a, pr, x = _rand_local(shape)
+    y = torch.empty(nv, l1, n, l2, dtype=torch.float64, device=DEV)
+    ws = _ws(lib.ttamen_local_matvec_workspace_size(l1, s, r1, n, S, l2, r2, nv))
+    t = [v.to(DEV) for v in (pl, a, pr, x)]
+    code = lib.ttamen_local_matvec(_p(t[0]), _p(t[1]), _p(t[2]), _p(t[3]), _p(y), l1, s, r1, n, S, l2, r2, nv,
+                                   _p(ws), ws.numel(), _stream())
+    L.check("local_matvec", code)
+    torch.cuda.synchronize()
+    for v in range(nv):
+        want = O.local_product(pr.numpy(), pl.numpy(), a.numpy(), x[v].numpy())
+        assert _rel(y[v].cpu().numpy(), want) < 1e-13, v
+
+
+@pytest.mark.parametrize("shape", SHAPES[:3])
+def test_local_matvec_f32(shape):
+    L, lib = _lib()
+    l1, s, r1, n, S, l2, r2, nv = shape
+    _say("local_matvec f32", shape)
+    pl, a, pr, x = _rand_local(shape, torch.float32, seed=1)
+    y = torch.empty(nv, l1, n, l2, dtype=torch.float32, device=DEV)
+    ws = _ws(lib.ttamen_local_matvec_workspace_size(l1, s, r1, n, S, l2, r2, nv) * 2)
+    t = [v.to(DEV) for v in (pl, a, pr, x)]
+    code = lib.ttamen_local_matvec_f32(_p(t[0]), _p(t[1]), _p(t[2]), _p(t[3]), _p(y), l1, s, r1, n, S, l2, r2,
+                                       nv, _p(ws), ws.numel(), _stream())
+    L.check("local_matvec_f32", code)
+    torch.cuda.synchronize()
+    for v in range(nv):
+        want = O.local_product(pr.double().numpy(), pl.double().numpy(), a.double().numpy(), x[v].double().numpy())
+        # FP32 accumulation over K = r1, s*n, S*r2 terms: relative error ~ sqrt(K) * 2^-24
+        assert _rel(y[v].cpu().double().numpy(), want) < 2e-5, v
+
+
+# ----------------------------------------------------------------- interfaces
+@pytest.mark.parametrize("dims", [(6, 2, 5, 16, 3, 7, 4), (84, 4, 84, 64, 4, 84, 84), (1, 1, 1, 9, 2, 3, 2)])
+def test_phi_A_updates(dims):
+    L, lib = _lib()
+    l, s, r, n, S, Lr, R = dims
+    _say("phi_A", dims)
+    g = torch.Generator().manual_seed(5)
+    a = torch.randn(s, n, n, S, dtype=torch.float64, generator=g)
+    xl = torch.randn(l, n, Lr, dtype=torch.float64, generator=g)
+    xr = torch.randn(r, n, R, dtype=torch.float64, generator=g)
+    phi_l = torch.randn(l, s, r, dtype=torch.float64, generator=g)
+    phi_r = torch.randn(Lr, S, R, dtype=torch.float64, generator=g)
+    ws = _ws(lib.ttamen_phi_workspace_size(l, s, r, n, S, Lr, R))
+    out_f = torch.empty(Lr, S, R, dtype=torch.float64, device=DEV)
+    out_b = torch.empty(l, s, r, dtype=torch.float64, device=DEV)
+    d = [v.to(DEV) for v in (phi_l, xl, a, xr, phi_r)]
+    L.check("phi_fwd_A", lib.ttamen_phi_fwd_A(_p(d[0]), _p(d[1]), _p(d[2]), _p(d[3]), _p(out_f), l, s, r, n, S, Lr, R,
+                                              _p(ws), ws.numel(), _stream()))
+    L.check("phi_bck_A", lib.ttamen_phi_bck_A(_p(d[4]), _p(d[1]), _p(d[2]), _p(d[3]), _p(out_b), l, s, r, n, S, Lr, R,
+                                              _p(ws), ws.numel(), _stream()))
+    torch.cuda.synchronize()
+    want_f = O.phi_fwd_A(phi_l.numpy(), xl.numpy(), a.numpy(), xr.numpy())
+    want_b = O.phi_bck_A(phi_r.numpy(), xl.numpy(), a.numpy(), xr.numpy())
+    assert _rel(out_f.cpu().numpy(), want_f) < 1e-13
+    assert _rel(out_b.cpu().numpy(), want_b) < 1e-13
+
+
+@pytest.mark.parametrize("dims", [(3, 7, 16, 4, 5), (4, 84, 64, 4, 84), (1, 1, 5, 1, 1)])
+def test_phi_rhs_updates(dims):
+    L, lib = _lib()
+    b, r, n, B, R = dims
+    _say("phi_rhs", dims)
+    g = torch.Generator().manual_seed(6)
+    bc = torch.randn(b, n, B, dtype=torch.float64, generator=g)
+    x = torch.randn(r, n, R, dtype=torch.float64, generator=g)
+    pl = torch.randn(b, r, dtype=torch.float64, generator=g)
+    pr = torch.randn(B, R, dtype=torch.float64, generator=g)
+    ws = _ws(64 << 20)
+    of = torch.empty(B, R, dtype=torch.float64, device=DEV)
+    ob = torch.empty(b, r, dtype=torch.float64, device=DEV)
+    d = [v.to(DEV) for v in (pl, bc, x, pr)]
+    L.check("phi_fwd_rhs", lib.ttamen_phi_fwd_rhs(_p(d[0]), _p(d[1]), _p(d[2]), _p(of), b, r, n, B, R, _p(ws), ws.numel(), _stream()))
+    L.check("phi_bck_rhs", lib.ttamen_phi_bck_rhs(_p(d[3]), _p(d[1]), _p(d[2]), _p(ob), b, r, n, B, R, _p(ws), ws.numel(), _stream()))
+    torch.cuda.synchronize()
+    assert _rel(of.cpu().numpy(), O.phi_fwd_rhs(pl.numpy(), bc.numpy(), x.numpy())) < 1e-13
+    assert _rel(ob.cpu().numpy(), O.phi_bck_rhs(pr.numpy(), bc.numpy(), x.numpy())) < 1e-13
+
+
+# ----------------------------------------------------------------- QR / SVD
+def _qr_case(kind, m, n, seed):
+    rng = np.random.default_rng(seed)
+    A = rng.standard_normal((m, n))
+    if kind == "zero_cols":
+        A[:, 1] = 0.0
+        A[:, n - 1] = 0.0
+    elif kind == "dependent":
+        A[:, n - 1] = A[:, 0] * 2.0 - A[:, 1]
+    elif kind == "graded":
+        A = A * np.logspace(0, -14, n)[None, :]
+    elif kind == "tiny_tail":
+        # an orthonormal block plus a tiny residual block (the enrichment QR shape)
+        q, _ = np.linalg.qr(rng.standard_normal((m, n - 4)))
+        A = np.concatenate([q, 1e-11 * rng.standard_normal((m, 4))], 1)
+    return A
+
+
+@pytest.mark.parametrize("kind,m,n", [("rand", 200, 30), ("rand", 5376, 84), ("rand", 30, 30), ("rand", 10, 25),
+                                      ("zero_cols", 100, 12), ("dependent", 80, 10), ("graded", 300, 40),
+                                      ("tiny_tail", 1000, 40), ("rand", 1, 1), ("rand", 256, 4),
+                                      ("rand", 8448, 136)])
+def test_qr(kind, m, n):
+    L, lib = _lib()
+    _say("qr", kind, m, n)
+    A = _qr_case(kind, m, n, m + n)
+    k = min(m, n)
+    Ad = torch.from_numpy(A).to(DEV)
+    Q = torch.empty(m, k, dtype=torch.float64, device=DEV)
+    R = torch.empty(k, n, dtype=torch.float64, device=DEV)
+    ws = _ws(lib.ttamen_dense_workspace_size(m, n))
+    L.check("qr", lib.ttamen_qr(_p(Ad), m, n, _p(Q), _p(R), _p(ws), ws.numel(), _stream()))
+    Qh, Rh = Q.cpu().numpy(), R.cpu().numpy()
+    assert np.linalg.norm(Qh.T @ Qh - np.eye(k)) < 1e-13 * max(1, np.sqrt(k)) * 10
+    assert np.linalg.norm(Qh @ Rh - A) <= 1e-14 * np.sqrt(m * n) * 10 * np.linalg.norm(A) + 1e-300
+
+
+@pytest.mark.parametrize("kind,m,n", [("rand", 200, 30), ("rand", 5376, 84), ("rand", 4, 256), ("rand", 30, 30),
+                                      ("graded", 300, 40), ("zero_cols", 60, 9), ("rand", 1, 7), ("rand", 9, 1),
+                                      ("rand", 1000, 136)])
+def test_svd(kind, m, n):
+    L, lib = _lib()
+    _say("svd", kind, m, n)
+    A = _qr_case(kind, m, n, 3 * m + n)
+    k = min(m, n)
+    Ad = torch.from_numpy(A).to(DEV)
+    U = torch.empty(m, k, dtype=torch.float64, device=DEV)
+    s = torch.empty(k, dtype=torch.float64, device=DEV)
+    V = torch.empty(n, k, dtype=torch.float64, device=DEV)
+    ws = _ws(lib.ttamen_dense_workspace_size(m, n))
+    L.check("svd", lib.ttamen_svd(_p(Ad), m, n, _p(U), _p(s), _p(V), _p(ws), ws.numel(), _stream()))
+    Uh, sh, Vh = U.cpu().numpy(), s.cpu().numpy(), V.cpu().numpy()
+    s_np = np.linalg.svd(A, compute_uv=False)
+    smax = max(s_np[0], 1e-300)
+    assert np.all(np.diff(sh) <= 1e-15 * smax)
+    assert np.max(np.abs(sh - s_np)) <= 1e-13 * smax * max(1, np.sqrt(k))
+    assert np.linalg.norm(Uh.T @ Uh - np.eye(k)) < 1e-12
+    assert np.linalg.norm(Vh.T @ Vh - np.eye(k)) < 1e-12
+    assert np.linalg.norm((Uh * sh) @ Vh.T - A) <= 1e-13 * np.sqrt(m * n) * np.linalg.norm(A)
+
+
+@pytest.mark.parametrize("n", [1, 7, 120, 499])
+def test_dense_solve(n):
+    L, lib = _lib()
+    _say("dense_solve", n)
+    rng = np.random.default_rng(n)
+    B = rng.standard_normal((n, n)) + n * 0.05 * np.eye(n)
+    rhs = rng.standard_normal(n)
+    Bd = torch.from_numpy(B.copy()).to(DEV)
+    rd = torch.from_numpy(rhs.copy()).to(DEV)
+    L.check("dense_solve", lib.ttamen_dense_solve(_p(Bd), _p(rd), n, _stream()))
+    torch.cuda.synchronize()
+    x = rd.cpu().numpy()
+    want = np.linalg.solve(B, rhs)
+    assert _rel(x, want) < 1e-11 * max(1.0, np.linalg.cond(B) / 1e3)

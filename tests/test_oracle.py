@@ -1,0 +1,171 @@
+"""Pin the CPU oracle (oracle/amen_oracle.py) before it is used to check the GPU path.
+
+Each check is independent of the formula it checks: reference fixtures written by the
+reference's own C++ solver, closed forms, brute-force dense assembly, invariants.
+"""
+import numpy as np
+import pytest
+
+import amen_oracle as O
+import laplace_exact as LE
+from conftest import GOLDEN_CASES, load_golden
+
+
+def _full(cores):
+    return O.tt_full(cores)
+
+
+# ----------------------------------------------------------------- against the reference
+@pytest.mark.parametrize("name", GOLDEN_CASES)
+def test_oracle_matches_reference_fixture(name):
+    c = load_golden(name)
+    x = O.amen_solve(c["A"], c["b"], x0=c["x0"], z0=c["z"], nswp=c["nswp"], eps=c["eps"], rmax=c["rmax"],
+                     kickrank=c["kickrank"], kick2=c["kick2"])
+    ranks = [1] + [core.shape[2] for core in x]
+    assert ranks == c["ranks"].tolist(), "trajectory (final ranks) differs from the reference"
+    fx, fr = _full(x), _full(c["x"])
+    rel = np.linalg.norm(fx - fr) / np.linalg.norm(fr)
+    assert rel <= 1e-8, rel
+
+
+def test_reference_fixture_residuals():
+    # the fixtures themselves solve their systems (guards against a stale/corrupt fixture)
+    for name in GOLDEN_CASES:
+        c = load_golden(name)
+        res = O.tt_residual(c["A"], c["x"], c["b"])
+        assert res < 50 * c["eps"], (name, res)
+
+
+# ----------------------------------------------------------------- closed forms
+def test_laplace_closed_form_is_exact():
+    # dense check of the fast-diagonalisation formula on a small grid
+    d, n, h = 3, 7, 1.0 / 8
+    T = LE.laplacian_dense_1d(n, h)
+    I = np.eye(n)
+    Afull = (np.kron(np.kron(T, I), I) + np.kron(np.kron(I, T), I) + np.kron(np.kron(I, I), T))
+    rng = np.random.default_rng(0)
+    bf = [rng.standard_normal(n) for _ in range(d)]
+    x = LE.laplace_solve_full(d, n, h, bf)
+    b = np.einsum("i,j,k->ijk", *bf).reshape(-1)
+    assert np.linalg.norm(Afull @ x.reshape(-1) - b) / np.linalg.norm(b) < 1e-13
+
+
+def test_oracle_laplace_against_closed_form():
+    c = load_golden("laplace_d6_n16_eps1e-10")
+    x = O.amen_solve(c["A"], c["b"], z0=c["z"], nswp=10, eps=1e-10, rmax=20)
+    ex = LE.laplace_solve_full(6, 16, 1.0 / 17)
+    assert np.linalg.norm(_full(x) - ex) / np.linalg.norm(ex) < 1e-10
+
+
+# ----------------------------------------------------------------- contractions
+def _dense_local_operator(phi_l, a, phi_r):
+    """B[(l,m,L),(r,n,R)] = sum_{s,S} phiL[l,s,r] A[s,m,n,S] phiR[L,S,R] via Kronecker products."""
+    l1, s, r1 = phi_l.shape
+    _, m, n, S = a.shape
+    l2, _, r2 = phi_r.shape
+    B = np.zeros((l1 * m * l2, r1 * n * r2))
+    for si in range(s):
+        for Si in range(S):
+            B += np.kron(np.kron(phi_l[:, si, :], a[si, :, :, Si]), phi_r[:, Si, :])
+    return B
+
+
+@pytest.mark.parametrize("shape", [(3, 2, 4, 5, 2, 3, 2), (1, 1, 1, 6, 1, 1, 1), (4, 3, 2, 3, 2, 5, 3)])
+def test_local_product_against_dense_operator(shape):
+    l1, s, r1, n, S, l2, r2 = shape
+    rng = np.random.default_rng(1)
+    pl = rng.standard_normal((l1, s, r1))
+    a = rng.standard_normal((s, n, n, S))
+    pr = rng.standard_normal((l2, S, r2))
+    x = rng.standard_normal((r1, n, r2))
+    w = O.local_product(pr, pl, a, x)
+    B = _dense_local_operator(pl, a, pr)
+    assert np.allclose(w.reshape(-1), B @ x.reshape(-1), rtol=1e-12, atol=1e-12)
+
+
+def _random_tt(rng, N, R):
+    return [rng.standard_normal((R[k], N[k], R[k + 1])) for k in range(len(N))]
+
+
+def test_phi_chains_give_bilinear_forms():
+    rng = np.random.default_rng(2)
+    N = [3, 4, 2, 3]
+    x = _random_tt(rng, N, [1, 2, 3, 2, 1])
+    y = _random_tt(rng, N, [1, 3, 2, 2, 1])
+    b = _random_tt(rng, N, [1, 2, 2, 3, 1])
+    A = [rng.standard_normal((ra, n, n, rb)) for ra, n, rb in zip([1, 2, 3, 2], N, [2, 3, 2, 1])]
+    Af = O.tt_full([a.reshape(a.shape[0], -1, a.shape[3]) for a in A])  # (m1 n1, m2 n2, ...)
+    Af = Af.reshape([v for n in N for v in (n, n)]).transpose(0, 2, 4, 6, 1, 3, 5, 7)
+    Amat = Af.reshape(int(np.prod(N)), -1)
+    xf, yf, bf = _full(x).reshape(-1), _full(y).reshape(-1), _full(b).reshape(-1)
+    # forward chains: Phi_d = y^T A x, Phib_d = <b, x>
+    phi = np.ones((1, 1, 1)); phib = np.ones((1, 1))
+    for k in range(4):
+        phi = O.phi_fwd_A(phi, y[k], A[k], x[k])
+        phib = O.phi_fwd_rhs(phib, b[k], x[k])
+    assert np.isclose(phi.item(), yf @ Amat @ xf, rtol=1e-12)
+    assert np.isclose(phib.item(), bf @ xf, rtol=1e-12)
+    # backward chains give the same numbers
+    phi = np.ones((1, 1, 1)); phib = np.ones((1, 1))
+    for k in range(3, -1, -1):
+        phi = O.phi_bck_A(phi, y[k], A[k], x[k])
+        phib = O.phi_bck_rhs(phib, b[k], x[k])
+    assert np.isclose(phi.item(), yf @ Amat @ xf, rtol=1e-12)
+    assert np.isclose(phib.item(), bf @ xf, rtol=1e-12)
+
+
+# ----------------------------------------------------------------- GMRES
+def test_gmres_solves_dense_system():
+    rng = np.random.default_rng(3)
+    n = 60
+    M = np.eye(n) * 4 + rng.standard_normal((n, n)) * 0.3
+    rhs = rng.standard_normal(n)
+    x, flag, nit = O.gmres(lambda v: M @ v, rhs, np.zeros(n), 60, 1e-12, 2)
+    assert flag == 1 and nit <= 60
+    assert np.linalg.norm(x - np.linalg.solve(M, rhs)) / np.linalg.norm(x) < 1e-10
+
+
+def test_gmres_restarts_on_laplacian():
+    # the standalone check of the reference (cpp/test_gmres.cpp:197-253) at a size where
+    # restarted GMRES(100) converges (the 1-D Laplacian's condition number grows like n^2)
+    n = 80
+    T = LE.laplacian_dense_1d(n, 1.0)
+    rhs = np.ones(n)
+    x, flag, nit = O.gmres(lambda v: T @ v, rhs, np.zeros(n), 100, 1e-8, 20)
+    assert flag == 1
+    assert np.linalg.norm(T @ x - rhs) / np.linalg.norm(rhs) <= 1.01e-8
+    # a cycle never returns a worse residual than its start (GMRES minimises the residual)
+    x1, _, _ = O.gmres(lambda v: T @ v, rhs, np.zeros(n), 5, 1e-30, 1)
+    x2, _, _ = O.gmres(lambda v: T @ v, rhs, np.zeros(n), 5, 1e-30, 2)
+    assert np.linalg.norm(T @ x2 - rhs) <= np.linalg.norm(T @ x1 - rhs) + 1e-12
+
+
+# ----------------------------------------------------------------- rank tools
+def test_rank_chop_rule():
+    s = np.array([10.0, 5.0, 1.0, 1e-3, 1e-6, 1e-9])
+    # brute force: smallest r with ||s[r:]|| < eps
+    for eps in [1e-10, 1e-8, 2e-3, 0.5, 2.0, 20.0]:
+        want = next((r for r in range(1, len(s) + 1) if np.linalg.norm(s[r:]) < eps), len(s))
+        assert O.rank_chop(s, eps) == max(want, 1), eps
+
+
+def test_round_tt_accuracy():
+    rng = np.random.default_rng(4)
+    x = _random_tt(rng, [4, 5, 3, 4], [1, 3, 4, 3, 1])
+    y = O.tt_add(x, x, 1.0, 0.5)  # rank doubles, content 1.5 x
+    z = O.round_tt(y, 1e-12, 100)
+    assert max(c.shape[2] for c in z) <= 4
+    fz, fx = _full(z), 1.5 * _full(x)
+    assert np.linalg.norm(fz - fx) / np.linalg.norm(fx) < 1e-11
+
+
+def test_tt_residual_matches_dense():
+    rng = np.random.default_rng(5)
+    N = [3, 4, 3]
+    A = [rng.standard_normal((ra, n, n, rb)) for ra, n, rb in zip([1, 2, 2], N, [2, 2, 1])]
+    x = _random_tt(rng, N, [1, 2, 2, 1])
+    b = _random_tt(rng, N, [1, 2, 1, 1])
+    Af = O.tt_full([a.reshape(a.shape[0], -1, a.shape[3]) for a in A])
+    Amat = Af.reshape([v for n in N for v in (n, n)]).transpose(0, 2, 4, 1, 3, 5).reshape(36, 36)
+    want = np.linalg.norm(Amat @ _full(x).reshape(-1) - _full(b).reshape(-1)) / np.linalg.norm(_full(b))
+    assert np.isclose(O.tt_residual(A, x, b), want, rtol=1e-10)

@@ -1,0 +1,227 @@
+// Tensor-train contractions of the AMEn sweep, each lowered to one to three
+// strided GEMMs on the MFMA kernel (no transposes in memory).
+//
+// Index conventions follow cpp/amen_solve.h of the reference:
+//   vector core        x[r, n, R]           (row-major)
+//   operator core      A[s, m, n, S]        (row m, column n)
+//   A-interface        Phi[l, s, r]         (l: test/output side, r: trial/x side)
+//   rhs interface      Phib[b, r]
+#include "ttamen_internal.h"
+
+namespace ttamen {
+
+// Local operator y[v] = PhiL * A * PhiR * x[v]   (reference: matvecs.h:196-212,
+// amen_solve.h:17-27, contraction 'lsr,smnS,LSR,rnR->lmL').
+//   stage 1 (batch v,s):  t1[v][l][s][n][R] = sum_r PhiL[l,s,r] x[v][r][n][R]
+//   stage 2 (batch v,l):  t2[v][l][m][S][R] = sum_{s,n} Apm[(m,S),(s,n)] t1[v][l][(s,n)][R]
+//   stage 3 (one GEMM):   y[(v,l,m)][L]     = sum_{S,R} t2[(v,l,m)][(S,R)] PhiR[L,(S,R)]
+template <typename T>
+static void matvec_impl(const T* PhiL, const T* Apm, const T* PhiR, int l1, int s, int r1, int n,
+                        int S, int l2, int r2, const T* x, T* y, int nvec, Arena& ws,
+                        hipStream_t st, const int* skip,
+                        void (*gemm)(const GemmDesc&, Arena&, hipStream_t)) {
+  const int64_t nx = (int64_t)r1 * n * r2;
+  const int64_t ny = (int64_t)l1 * n * l2;
+  const int64_t t1v = (int64_t)l1 * s * n * r2;  // per vector
+  const int64_t t2v = (int64_t)l1 * n * S * r2;
+  size_t mk = ws.mark();
+  T* t1 = ws.alloc<T>(t1v * nvec);
+  T* t2 = ws.alloc<T>(t2v * nvec);
+  {
+    GemmDesc g;
+    g.M = l1; g.N = n * r2; g.K = r1; g.B1 = nvec; g.B2 = s;
+    g.a.p = PhiL; g.a.r = (int64_t)s * r1; g.a.c = 1; g.a.b1 = 0; g.a.b2 = r1;
+    g.b.p = x; g.b.r = (int64_t)n * r2; g.b.c = 1; g.b.b1 = nx; g.b.b2 = 0;
+    g.c.p = t1; g.c.r = (int64_t)s * n * r2; g.c.c = 1; g.c.b1 = t1v; g.c.b2 = (int64_t)n * r2;
+    g.skip = skip;
+    gemm(g, ws, st);
+  }
+  {
+    GemmDesc g;
+    g.M = n * S; g.N = r2; g.K = s * n; g.B1 = nvec * l1; g.B2 = 1;
+    g.a.p = Apm; g.a.r = (int64_t)s * n; g.a.c = 1; g.a.b1 = 0;
+    g.b.p = t1; g.b.r = r2; g.b.c = 1; g.b.b1 = (int64_t)s * n * r2;
+    g.c.p = t2; g.c.r = r2; g.c.c = 1; g.c.b1 = (int64_t)n * S * r2;
+    g.skip = skip;
+    gemm(g, ws, st);
+  }
+  {
+    GemmDesc g;
+    g.M = nvec * l1 * n; g.N = l2; g.K = S * r2;
+    g.a.p = t2; g.a.r = (int64_t)S * r2; g.a.c = 1;
+    g.b.p = PhiR; g.b.r = 1; g.b.c = (int64_t)S * r2;
+    g.c.p = y; g.c.r = l2; g.c.c = 1;
+    g.skip = skip;
+    gemm(g, ws, st);
+  }
+  (void)ny;
+  ws.release(mk);
+}
+
+void local_matvec(const LocalOp& op, const double* x, double* y, int nvec, Arena& ws,
+                  hipStream_t st, const int* skip) {
+  matvec_impl<double>(op.PhiL, op.Apm, op.PhiR, op.l1, op.s, op.r1, op.n, op.S, op.l2, op.r2, x,
+                      y, nvec, ws, st, skip, gemm_f64);
+}
+
+void local_matvec_f32(const LocalOpF32& op, const float* x, float* y, int nvec, Arena& ws,
+                      hipStream_t st, const int* skip) {
+  matvec_impl<float>(op.PhiL, op.Apm, op.PhiR, op.l1, op.s, op.r1, op.n, op.S, op.l2, op.r2, x,
+                     y, nvec, ws, st, skip, gemm_f32);
+}
+
+size_t local_matvec_ws(const LocalOp& op, int nvec) {
+  Arena a;
+  a.alloc<double>((size_t)op.l1 * op.s * op.n * op.r2 * nvec);
+  a.alloc<double>((size_t)op.l1 * op.n * op.S * op.r2 * nvec);
+  return a.peak() + 64 * 1024 * 1024;  // generous split-K headroom
+}
+
+// Phi forward (reference amen_solve.h:58-68, 'lsr,lML,sMNS,rNR->LSR'):
+//   t2[l][(M,S)][R] = stages 1-2 of the local operator with PhiL = Phi, x = xR
+//   Pn[L][(S,R)]    = sum_{(l,M)} xL[(l,M)][L] t2[(l,M)][(S,R)]
+void phi_fwd_A(const double* Phi, const double* xL, const double* Apm, const double* xR,
+               double* Pn, int l, int s, int r, int n, int S, int L, int R, Arena& ws,
+               hipStream_t st) {
+  size_t mk = ws.mark();
+  double* t1 = ws.alloc<double>((size_t)l * s * n * R);
+  double* t2 = ws.alloc<double>((size_t)l * n * S * R);
+  {
+    GemmDesc g;
+    g.M = l; g.N = n * R; g.K = r; g.B1 = 1; g.B2 = s;
+    g.a.p = Phi; g.a.r = (int64_t)s * r; g.a.c = 1; g.a.b2 = r;
+    g.b.p = xR; g.b.r = (int64_t)n * R; g.b.c = 1;
+    g.c.p = t1; g.c.r = (int64_t)s * n * R; g.c.c = 1; g.c.b2 = (int64_t)n * R;
+    gemm_f64(g, ws, st);
+  }
+  {
+    GemmDesc g;
+    g.M = n * S; g.N = R; g.K = s * n; g.B1 = l;
+    g.a.p = Apm; g.a.r = (int64_t)s * n; g.a.c = 1;
+    g.b.p = t1; g.b.r = R; g.b.c = 1; g.b.b1 = (int64_t)s * n * R;
+    g.c.p = t2; g.c.r = R; g.c.c = 1; g.c.b1 = (int64_t)n * S * R;
+    gemm_f64(g, ws, st);
+  }
+  {
+    GemmDesc g;
+    g.M = L; g.N = S * R; g.K = l * n;
+    g.a.p = xL; g.a.r = 1; g.a.c = L;
+    g.b.p = t2; g.b.r = (int64_t)S * R; g.b.c = 1;
+    g.c.p = Pn; g.c.r = (int64_t)S * R; g.c.c = 1;
+    gemm_f64(g, ws, st);
+  }
+  ws.release(mk);
+}
+
+// Phi backward (reference amen_solve.h:38-47, 'LSR,lML,sMNS,rNR->lsr'):
+//   u1[(r,N)][S][L] = sum_R xR[(r,N)][R] Phi[L][S][R]            (batch S)
+//   u2[r][(s,M)][L] = sum_{(N,S)} A[(s,M)][(N,S)] u1[r][(N,S)][L] (batch r)
+//   Pp[l][s][r]     = sum_{(M,L)} xL[l][(M,L)] u2[r][s][(M,L)]    (batch s)
+void phi_bck_A(const double* Phi, const double* xL, const double* A, const double* xR,
+               double* Pp, int l, int s, int r, int n, int S, int L, int R, Arena& ws,
+               hipStream_t st) {
+  size_t mk = ws.mark();
+  double* u1 = ws.alloc<double>((size_t)r * n * S * L);
+  double* u2 = ws.alloc<double>((size_t)r * s * n * L);
+  {
+    GemmDesc g;
+    g.M = r * n; g.N = L; g.K = R; g.B2 = S;
+    g.a.p = xR; g.a.r = R; g.a.c = 1;
+    g.b.p = Phi; g.b.r = 1; g.b.c = (int64_t)S * R; g.b.b2 = R;
+    g.c.p = u1; g.c.r = (int64_t)S * L; g.c.c = 1; g.c.b2 = L;
+    gemm_f64(g, ws, st);
+  }
+  {
+    GemmDesc g;
+    g.M = s * n; g.N = L; g.K = n * S; g.B1 = r;
+    g.a.p = A; g.a.r = (int64_t)n * S; g.a.c = 1;
+    g.b.p = u1; g.b.r = L; g.b.c = 1; g.b.b1 = (int64_t)n * S * L;
+    g.c.p = u2; g.c.r = L; g.c.c = 1; g.c.b1 = (int64_t)s * n * L;
+    gemm_f64(g, ws, st);
+  }
+  {
+    GemmDesc g;
+    g.M = l; g.N = r; g.K = n * L; g.B2 = s;
+    g.a.p = xL; g.a.r = (int64_t)n * L; g.a.c = 1;
+    g.b.p = u2; g.b.r = 1; g.b.c = (int64_t)s * n * L; g.b.b2 = (int64_t)n * L;
+    g.c.p = Pp; g.c.r = (int64_t)s * r; g.c.c = 1; g.c.b2 = r;
+    gemm_f64(g, ws, st);
+  }
+  ws.release(mk);
+}
+
+// Phib forward (reference amen_solve.h:93-98, 'br,bnB,rnR->BR').
+void phi_fwd_rhs(const double* Pb, const double* bc, const double* x, double* Pn, int b, int r,
+                 int n, int B, int R, Arena& ws, hipStream_t st) {
+  size_t mk = ws.mark();
+  double* u = ws.alloc<double>((size_t)r * n * B);
+  {
+    GemmDesc g;  // u[r][(n,B)] = Pb^T[r][b] bc[b][(n,B)]
+    g.M = r; g.N = n * B; g.K = b;
+    g.a.p = Pb; g.a.r = 1; g.a.c = r;
+    g.b.p = bc; g.b.r = (int64_t)n * B; g.b.c = 1;
+    g.c.p = u; g.c.r = (int64_t)n * B; g.c.c = 1;
+    gemm_f64(g, ws, st);
+  }
+  {
+    GemmDesc g;  // Pn[B][R] = sum_{(r,n)} u[(r,n)][B] x[(r,n)][R]
+    g.M = B; g.N = R; g.K = r * n;
+    g.a.p = u; g.a.r = 1; g.a.c = B;
+    g.b.p = x; g.b.r = R; g.b.c = 1;
+    g.c.p = Pn; g.c.r = R; g.c.c = 1;
+    gemm_f64(g, ws, st);
+  }
+  ws.release(mk);
+}
+
+// Phib backward (reference amen_solve.h:78-83, 'BR,bnB,rnR->br').
+void phi_bck_rhs(const double* Pb, const double* bc, const double* x, double* Pp, int b, int r,
+                 int n, int B, int R, Arena& ws, hipStream_t st) {
+  size_t mk = ws.mark();
+  double* u = ws.alloc<double>((size_t)b * n * R);
+  {
+    GemmDesc g;  // u[(b,n)][R] = bc[(b,n)][B] Pb[B][R]
+    g.M = b * n; g.N = R; g.K = B;
+    g.a.p = bc; g.a.r = B; g.a.c = 1;
+    g.b.p = Pb; g.b.r = R; g.b.c = 1;
+    g.c.p = u; g.c.r = R; g.c.c = 1;
+    gemm_f64(g, ws, st);
+  }
+  {
+    GemmDesc g;  // Pp[b][r] = sum_{(n,R)} u[b][(n,R)] x[r][(n,R)]
+    g.M = b; g.N = r; g.K = n * R;
+    g.a.p = u; g.a.r = (int64_t)n * R; g.a.c = 1;
+    g.b.p = x; g.b.r = 1; g.b.c = (int64_t)n * R;
+    g.c.p = Pp; g.c.r = r; g.c.c = 1;
+    gemm_f64(g, ws, st);
+  }
+  ws.release(mk);
+}
+
+// Local right-hand side (reference amen_solve.h:303-304, 'br,bnB,BR->rnR'), scaled
+// by a device scalar (the sweep's norm correction nrmsc).
+void assemble_rhs(const double* PL, const double* bc, const double* PR, double* out, int b, int r,
+                  int n, int B, int R, const double* scale_ptr, Arena& ws, hipStream_t st) {
+  size_t mk = ws.mark();
+  double* u = ws.alloc<double>((size_t)r * n * B);
+  {
+    GemmDesc g;  // u[r][(n,B)] = PL^T[r][b] bc[b][(n,B)]
+    g.M = r; g.N = n * B; g.K = b;
+    g.a.p = PL; g.a.r = 1; g.a.c = r;
+    g.b.p = bc; g.b.r = (int64_t)n * B; g.b.c = 1;
+    g.c.p = u; g.c.r = (int64_t)n * B; g.c.c = 1;
+    gemm_f64(g, ws, st);
+  }
+  {
+    GemmDesc g;  // out[(r,n)][R] = u[(r,n)][B] PR[B][R]
+    g.M = r * n; g.N = R; g.K = B;
+    g.a.p = u; g.a.r = B; g.a.c = 1;
+    g.b.p = PR; g.b.r = R; g.b.c = 1;
+    g.c.p = out; g.c.r = R; g.c.c = 1;
+    if (scale_ptr) { g.scale_ptr = scale_ptr; g.scale_mode = 1; }
+    gemm_f64(g, ws, st);
+  }
+  ws.release(mk);
+}
+
+}  // namespace ttamen

@@ -1,0 +1,285 @@
+// Strided, batched GEMM on CDNA4 matrix cores (gfx950).
+//
+// Every tensor-train contraction of the AMEn sweep (local operator, interface
+// updates, right-hand-side assembly, Gram matrices of the QR) is lowered to this
+// one kernel by choosing strides, so the operands are never transposed in memory.
+//
+// Tile: BM x BN x BK per workgroup of 256 threads = 4 wave64s in a 2x2 grid.
+// Each wave owns a (BM/2) x (BN/2) sub-tile built from 16x16 MFMA blocks:
+//   FP64: v_mfma_f64_16x16x4_f64   (A: 1 double/lane, B: 1 double/lane, D: 4 doubles/lane)
+//   FP32: v_mfma_f32_16x16x4_f32   (same fragment shapes in float)
+// Fragment layout (16x16xK4 block): lane l holds A[i = l%16][k = l/16],
+// B[k = l/16][j = l%16], and D[i = 4*(l/16) + v][j = l%16] for v = 0..3.
+// Global -> register -> LDS double buffering: the next K tile is fetched while
+// the current one feeds the MFMAs. The global fetch walks whichever operand
+// axis has unit stride, so loads are coalesced for either orientation.
+// Split-K writes per-split partials that a second kernel sums in a fixed order
+// (deterministic results, no atomics).
+#include "ttamen_internal.h"
+#include <algorithm>
+
+namespace ttamen {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+template <typename T> struct AccT;
+template <> struct AccT<double> { typedef d4 type; };
+template <> struct AccT<float> { typedef f4 type; };
+
+__device__ __forceinline__ d4 mfma16(double a, double b, d4 c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ f4 mfma16(float a, float b, f4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
+template <typename T>
+struct GemmParams {
+  int M, N, K, B2, B;
+  int64_t kchunk;
+  const T* a;
+  int64_t ar, ac, ab1, ab2;
+  const T* b;
+  int64_t br, bc, bb1, bb2;
+  T* c;
+  int64_t cr, cc, cb1, cb2;
+  double alpha, beta;
+  const double* scale_ptr;
+  int scale_mode;
+  const int* skip;
+  T* partial;  // split-K scratch: [split][batch][M][N]
+};
+
+template <typename T>
+__device__ __forceinline__ double eff_alpha(const GemmParams<T>& p) {
+  double s = p.alpha;
+  if (p.scale_mode == 1) s *= *p.scale_ptr;
+  else if (p.scale_mode == 2) s /= *p.scale_ptr;
+  return s;
+}
+
+template <typename T, int BM, int BN, int BK>
+__global__ __launch_bounds__(256) void k_gemm(GemmParams<T> p) {
+  if (p.skip && *p.skip) return;
+  constexpr int MI = BM / 32, NI = BN / 32;  // 16x16 blocks per wave
+  constexpr int AE = BM * BK / 256, BE = BK * BN / 256;
+  constexpr int PA = 1, PB = 1;
+  __shared__ T As[2][BK][BM + PA];
+  __shared__ T Bs[2][BK][BN + PB];
+  typedef typename AccT<T>::type acc_t;
+
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+  const int tiles_n = (p.N + BN - 1) / BN;
+  const int tm = blockIdx.x / tiles_n, tn = blockIdx.x - tm * tiles_n;
+  const int bidx = blockIdx.y;
+  const int b1 = bidx / p.B2, b2 = bidx - b1 * p.B2;
+  const int64_t k0 = (int64_t)blockIdx.z * p.kchunk;
+  const int64_t k1 = min((int64_t)p.K, k0 + p.kchunk);
+  const T* A = p.a + b1 * p.ab1 + b2 * p.ab2;
+  const T* B = p.b + b1 * p.bb1 + b2 * p.bb2;
+  const int m0 = tm * BM, n0 = tn * BN;
+  const bool a_kfast = (p.ac == 1);
+  const bool b_nfast = (p.bc == 1);
+
+  T ra[AE], rb[BE];
+  auto loadA = [&](int64_t kt) {
+#pragma unroll
+    for (int e = 0; e < AE; ++e) {
+      int idx = tid + 256 * e, i, kk;
+      if (a_kfast) { i = idx / BK; kk = idx - i * BK; }
+      else { kk = idx / BM; i = idx - kk * BM; }
+      int64_t gi = m0 + i, gk = kt + kk;
+      ra[e] = (gi < p.M && gk < k1) ? A[gi * p.ar + gk * p.ac] : T(0);
+    }
+  };
+  auto storeA = [&](int buf) {
+#pragma unroll
+    for (int e = 0; e < AE; ++e) {
+      int idx = tid + 256 * e, i, kk;
+      if (a_kfast) { i = idx / BK; kk = idx - i * BK; }
+      else { kk = idx / BM; i = idx - kk * BM; }
+      As[buf][kk][i] = ra[e];
+    }
+  };
+  auto loadB = [&](int64_t kt) {
+#pragma unroll
+    for (int e = 0; e < BE; ++e) {
+      int idx = tid + 256 * e, j, kk;
+      if (b_nfast) { kk = idx / BN; j = idx - kk * BN; }
+      else { j = idx / BK; kk = idx - j * BK; }
+      int64_t gj = n0 + j, gk = kt + kk;
+      rb[e] = (gj < p.N && gk < k1) ? B[gk * p.br + gj * p.bc] : T(0);
+    }
+  };
+  auto storeB = [&](int buf) {
+#pragma unroll
+    for (int e = 0; e < BE; ++e) {
+      int idx = tid + 256 * e, j, kk;
+      if (b_nfast) { kk = idx / BN; j = idx - kk * BN; }
+      else { j = idx / BK; kk = idx - j * BK; }
+      Bs[buf][kk][j] = rb[e];
+    }
+  };
+
+  acc_t acc[MI][NI];
+#pragma unroll
+  for (int mi = 0; mi < MI; ++mi)
+#pragma unroll
+    for (int ni = 0; ni < NI; ++ni) acc[mi][ni] = acc_t{0, 0, 0, 0};
+
+  if (k0 < k1) {
+    loadA(k0);
+    loadB(k0);
+    storeA(0);
+    storeB(0);
+  }
+  __syncthreads();
+  int cur = 0;
+  const int lr = lane & 15, lk = lane >> 4;
+  for (int64_t kt = k0; kt < k1; kt += BK) {
+    const bool more = kt + BK < k1;
+    if (more) {
+      loadA(kt + BK);
+      loadB(kt + BK);
+    }
+#pragma unroll
+    for (int kk = 0; kk < BK; kk += 4) {
+      T af[MI], bf[NI];
+#pragma unroll
+      for (int mi = 0; mi < MI; ++mi) af[mi] = As[cur][kk + lk][wm * (BM / 2) + mi * 16 + lr];
+#pragma unroll
+      for (int ni = 0; ni < NI; ++ni) bf[ni] = Bs[cur][kk + lk][wn * (BN / 2) + ni * 16 + lr];
+#pragma unroll
+      for (int mi = 0; mi < MI; ++mi)
+#pragma unroll
+        for (int ni = 0; ni < NI; ++ni) acc[mi][ni] = mfma16(af[mi], bf[ni], acc[mi][ni]);
+    }
+    if (more) {
+      storeA(cur ^ 1);
+      storeB(cur ^ 1);
+    }
+    __syncthreads();
+    cur ^= 1;
+  }
+
+  if (p.partial) {
+    T* P = p.partial + ((int64_t)blockIdx.z * p.B + bidx) * (int64_t)p.M * p.N;
+#pragma unroll
+    for (int mi = 0; mi < MI; ++mi)
+#pragma unroll
+      for (int ni = 0; ni < NI; ++ni)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          int i = m0 + wm * (BM / 2) + mi * 16 + 4 * lk + v;
+          int j = n0 + wn * (BN / 2) + ni * 16 + lr;
+          if (i < p.M && j < p.N) P[(int64_t)i * p.N + j] = acc[mi][ni][v];
+        }
+    return;
+  }
+  const double al = eff_alpha(p);
+  T* C = p.c + b1 * p.cb1 + b2 * p.cb2;
+#pragma unroll
+  for (int mi = 0; mi < MI; ++mi)
+#pragma unroll
+    for (int ni = 0; ni < NI; ++ni)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        int i = m0 + wm * (BM / 2) + mi * 16 + 4 * lk + v;
+        int j = n0 + wn * (BN / 2) + ni * 16 + lr;
+        if (i < p.M && j < p.N) {
+          T* cp = C + (int64_t)i * p.cr + (int64_t)j * p.cc;
+          double val = al * (double)acc[mi][ni][v];
+          if (p.beta != 0.0) val += p.beta * (double)(*cp);
+          *cp = (T)val;
+        }
+      }
+}
+
+// Sum split-K partials in split order and apply alpha / beta.
+template <typename T>
+__global__ void k_gemm_reduce(GemmParams<T> p, int splits) {
+  if (p.skip && *p.skip) return;
+  const int64_t MN = (int64_t)p.M * p.N;
+  const int64_t total = MN * p.B;
+  const double al = eff_alpha(p);
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    int64_t bidx = t / MN, r = t - bidx * MN;
+    int i = (int)(r / p.N), j = (int)(r - (int64_t)i * p.N);
+    double s = 0.0;
+    for (int z = 0; z < splits; ++z) s += (double)p.partial[((int64_t)z * p.B + bidx) * MN + r];
+    int b1 = (int)(bidx / p.B2), b2 = (int)(bidx - (int64_t)b1 * p.B2);
+    T* cp = p.c + b1 * p.cb1 + b2 * p.cb2 + (int64_t)i * p.cr + (int64_t)j * p.cc;
+    double val = al * s;
+    if (p.beta != 0.0) val += p.beta * (double)(*cp);
+    *cp = (T)val;
+  }
+}
+
+template <typename T, int BM, int BN>
+static void launch_tile(GemmParams<T>& p, int splits, hipStream_t st) {
+  constexpr int BK = 16;
+  int tiles = ((p.M + BM - 1) / BM) * ((p.N + BN - 1) / BN);
+  dim3 grid(tiles, p.B, splits);
+  hipLaunchKernelGGL((k_gemm<T, BM, BN, BK>), grid, dim3(256), 0, st, p);
+  TT_LAUNCH_CHECK();
+}
+
+template <typename T>
+static void gemm_impl(const GemmDesc& g, Arena& ws, hipStream_t st) {
+  TT_CHECK(g.M >= 0 && g.N >= 0 && g.K >= 0 && g.B1 >= 1 && g.B2 >= 1, ERR_INVALID,
+           "gemm: bad sizes");
+  if (g.M == 0 || g.N == 0) return;
+  GemmParams<T> p;
+  p.M = g.M; p.N = g.N; p.K = g.K; p.B2 = g.B2; p.B = g.B1 * g.B2;
+  p.a = (const T*)g.a.p; p.ar = g.a.r; p.ac = g.a.c; p.ab1 = g.a.b1; p.ab2 = g.a.b2;
+  p.b = (const T*)g.b.p; p.br = g.b.r; p.bc = g.b.c; p.bb1 = g.b.b1; p.bb2 = g.b.b2;
+  p.c = (T*)g.c.p; p.cr = g.c.r; p.cc = g.c.c; p.cb1 = g.c.b1; p.cb2 = g.c.b2;
+  p.alpha = g.alpha; p.beta = g.beta; p.scale_ptr = g.scale_ptr; p.scale_mode = g.scale_mode;
+  p.skip = g.skip; p.partial = nullptr;
+  TT_CHECK(p.B <= 65535, ERR_INVALID, "gemm: batch too large");
+
+  const int BM = (g.M <= 32) ? 32 : 64;
+  const int BN = (g.N <= 32) ? 32 : 64;
+  int tiles = ((g.M + BM - 1) / BM) * ((g.N + BN - 1) / BN) * p.B;
+  // Split K when the output grid is too small to fill the 256 CUs and K is long.
+  int splits = 1;
+  if (g.K == 0) {
+    p.kchunk = 1;
+  } else {
+    if (tiles < 256 && g.K >= 512) {
+      splits = std::min<int64_t>((512 + tiles - 1) / tiles, g.K / 256);
+      splits = std::max(1, std::min(splits, 64));
+    }
+    int64_t kchunk = (g.K + splits - 1) / splits;
+    kchunk = (kchunk + 15) / 16 * 16;
+    splits = (int)((g.K + kchunk - 1) / kchunk);
+    p.kchunk = kchunk;
+  }
+  size_t mk = ws.mark();
+  if (splits > 1) p.partial = ws.alloc<T>((size_t)splits * p.B * g.M * g.N);
+  if (!ws.measuring()) {
+    if (g.K == 0) {
+      // C = beta * C: run the kernel with an empty K range.
+      splits = 1;
+    }
+    if (BM == 32 && BN == 32) launch_tile<T, 32, 32>(p, splits, st);
+    else if (BM == 32) launch_tile<T, 32, 64>(p, splits, st);
+    else if (BN == 32) launch_tile<T, 64, 32>(p, splits, st);
+    else launch_tile<T, 64, 64>(p, splits, st);
+    if (splits > 1) {
+      int64_t total = (int64_t)g.M * g.N * p.B;
+      int blocks = (int)std::min<int64_t>((total + 255) / 256, 4096);
+      hipLaunchKernelGGL((k_gemm_reduce<T>), dim3(blocks), dim3(256), 0, st, p, splits);
+      TT_LAUNCH_CHECK();
+    }
+  }
+  ws.release(mk);
+}
+
+void gemm_f64(const GemmDesc& g, Arena& ws, hipStream_t st) { gemm_impl<double>(g, ws, st); }
+void gemm_f32(const GemmDesc& g, Arena& ws, hipStream_t st) { gemm_impl<float>(g, ws, st); }
+
+}  // namespace ttamen

@@ -1,0 +1,82 @@
+// Device-resident restarted GMRES: state layout and host driver interface.
+#pragma once
+#include "ttamen_internal.h"
+#include <functional>
+
+namespace ttamen {
+
+constexpr int kGmresMaxCols = 128;    // (local_iterations + 1) upper bound
+constexpr int kGmresMaxBlocks = 256;  // blocks of the streaming kernels
+
+enum { GM_DONE = 0, GM_FLAG = 1, GM_KCOLS = 2, GM_ITERS = 3 };           // istate
+enum { GM_BNORM = 0, GM_THR = 1, GM_RNORM = 2, GM_ERR = 3 };             // dstate
+
+// Device buffers of one GMRES instance (all inside the caller's arena).
+struct GmresDev {
+  int m = 0;            // iterations per cycle
+  void* Q = nullptr;    // (m+1) x N basis, working precision
+  void* w = nullptr;    // N
+  double* H = nullptr;  // (m+1) x m, column k at H + k*(m+1)
+  double* beta = nullptr;
+  double* cs = nullptr;
+  double* sn = nullptr;
+  double* y = nullptr;
+  double* dstate = nullptr;
+  int* istate = nullptr;
+  double* partial = nullptr;  // 3 x kGmresMaxBlocks x kGmresMaxCols
+};
+
+template <typename T>
+struct GmresOp {
+  std::function<void(const T*, T*, const int*)> apply;          // y = Op x (skip flag)
+  std::function<void(double, const T*, double, T*, int64_t)> axpby;  // y = a x + b y
+  std::function<void(const T*, int64_t, double*)> norm;         // *slot = ||x||
+};
+
+struct GmresResult {
+  int flag = 0, iters = 0, cycles = 0;
+};
+
+// Small pinned-host mailbox used to poll device flags without stalling the queue.
+class HostSync {
+ public:
+  static constexpr int NS = 4;
+  void init() {
+    if (host_) return;
+    TT_HIP(hipHostMalloc((void**)&host_, sizeof(int) * (NS + 16), hipHostMallocDefault));
+    TT_HIP(hipHostMalloc((void**)&hostd_, sizeof(double) * 4096, hipHostMallocDefault));
+    for (int i = 0; i < NS; ++i) TT_HIP(hipEventCreateWithFlags(&ev_[i], hipEventDisableTiming));
+  }
+  void post(const int* dflag, int idx, hipStream_t st) {
+    int s = idx % NS;
+    TT_HIP(hipMemcpyAsync(host_ + s, dflag, sizeof(int), hipMemcpyDeviceToHost, st));
+    TT_HIP(hipEventRecord(ev_[s], st));
+  }
+  bool wait(int idx) {
+    if (idx < 0) return false;
+    int s = idx % NS;
+    TT_HIP(hipEventSynchronize(ev_[s]));
+    return host_[s] != 0;
+  }
+  int* host_int() { return host_ + NS; }
+  double* host_dbl() { return hostd_; }
+  ~HostSync() {
+    // Pinned buffers live for the process; freeing at exit can race the runtime teardown.
+  }
+
+ private:
+  int* host_ = nullptr;
+  double* hostd_ = nullptr;
+  hipEvent_t ev_[NS];
+};
+
+GmresDev gmres_alloc(Arena& ws, int m, int64_t N, int esize);
+
+// Solve Op dx = rhs from dx = 0 (x must be zeroed by the caller), restarted
+// `resets` times, stopping when |beta_{k+1}| / ||rhs|| <= real_tol*norm_rhs/dnorm.
+template <typename T>
+GmresResult gmres_run(GmresDev g, int64_t N, T* x, const T* rhs, const double* norm_rhs,
+                      const double* dnorm, double real_tol, int resets, const GmresOp<T>& op,
+                      RedScratch rs, HostSync& hs, hipStream_t st);
+
+}  // namespace ttamen

@@ -1,0 +1,323 @@
+// Device-resident restarted GMRES(m) for the AMEn local systems.
+//
+// Reference: cpp/gmres.h:15-161 (gmres_single / gmres / Givens rotation) and its
+// caller cpp/amen_solve.h:337-358. The reference keeps the Hessenberg matrix,
+// the Givens rotations and every dot product on the host (one device->host sync
+// per dot). Here the whole Arnoldi process stays on the device:
+//   * orthogonalisation is classical Gram-Schmidt applied twice (CGS2), each pass
+//     one streaming kernel over the Krylov basis with deterministic block partials;
+//   * the Givens update of the Hessenberg column, the residual estimate |beta_k+1|
+//     and the convergence test run in one thread of the normalisation kernel;
+//   * a device flag `done` makes every kernel of later iterations (including the
+//     local-operator GEMMs) return immediately, so the host can enqueue iteration
+//     k+1 before it has read the flag of iteration k (one sync per iteration,
+//     overlapped with the next iteration's work).
+// Vectors are in the working precision T (double, or float for the mixed-precision
+// inner solve); dot products and the Hessenberg are accumulated in double.
+#include "ttamen_internal.h"
+#include "gmres.h"
+#include <algorithm>
+#include <cmath>
+
+namespace ttamen {
+
+static constexpr int kGB = 256;  // threads per block
+
+static inline int gm_blocks(int64_t N) {
+  int64_t b = (N + kGB * 8 - 1) / (kGB * 8);
+  return (int)std::max<int64_t>(1, std::min<int64_t>(b, kGmresMaxBlocks));
+}
+
+__device__ __forceinline__ double gm_wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// partial[b * ldp + i] = sum over block b's slice of Q_i . w,   i < kc
+template <typename T>
+__global__ __launch_bounds__(kGB) void k_cgs_dots(const T* Q, int64_t N, int kc, const T* w,
+                                                  double* partial, int ldp, const int* done) {
+  if (*done) return;
+  __shared__ double sh[4][kGmresMaxCols];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int64_t chunk = (N + gridDim.x - 1) / gridDim.x;
+  const int64_t lo = blockIdx.x * chunk, hi = min(N, lo + chunk);
+  for (int i0 = 0; i0 < kc; i0 += 8) {
+    double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int64_t j = lo + threadIdx.x; j < hi; j += kGB) {
+      const double wj = (double)w[j];
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+        if (i0 + u < kc) acc[u] += (double)Q[(int64_t)(i0 + u) * N + j] * wj;
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      double v = gm_wave_sum(acc[u]);
+      if (lane == 0 && i0 + u < kc) sh[wv][i0 + u] = v;
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < kc; i += kGB)
+    partial[blockIdx.x * ldp + i] = sh[0][i] + sh[1][i] + sh[2][i] + sh[3][i];
+}
+
+// h_i = sum_b partial_in[b][i]; w -= sum_i h_i Q_i; H[:,k] (+)= h (block 0);
+// then either next-pass dots (mode 0) or ||w||^2 partials (mode 1) into partial_out.
+template <typename T>
+__global__ __launch_bounds__(kGB) void k_cgs_update(const T* Q, int64_t N, int kc, T* w,
+                                                    const double* partial_in, int nb_in, int ldp,
+                                                    double* partial_out, int mode, double* Hcol,
+                                                    int accumulate, const int* done) {
+  if (*done) return;
+  __shared__ double h[kGmresMaxCols];
+  __shared__ double sh[4][kGmresMaxCols];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  for (int i = threadIdx.x; i < kc; i += kGB) {
+    double s = 0.0;
+    for (int b = 0; b < nb_in; ++b) s += partial_in[b * ldp + i];
+    h[i] = s;
+    if (blockIdx.x == 0) Hcol[i] = accumulate ? Hcol[i] + s : s;
+  }
+  __syncthreads();
+  const int64_t chunk = (N + gridDim.x - 1) / gridDim.x;
+  const int64_t lo = blockIdx.x * chunk, hi = min(N, lo + chunk);
+  for (int64_t j = lo + threadIdx.x; j < hi; j += kGB) {
+    double v = (double)w[j];
+    for (int i = 0; i < kc; ++i) v -= h[i] * (double)Q[(int64_t)i * N + j];
+    w[j] = (T)v;
+  }
+  if (mode == 0) {
+    for (int i0 = 0; i0 < kc; i0 += 8) {
+      double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+      for (int64_t j = lo + threadIdx.x; j < hi; j += kGB) {
+        const double wj = (double)w[j];
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+          if (i0 + u < kc) acc[u] += (double)Q[(int64_t)(i0 + u) * N + j] * wj;
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        double v = gm_wave_sum(acc[u]);
+        if (lane == 0 && i0 + u < kc) sh[wv][i0 + u] = v;
+      }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < kc; i += kGB)
+      partial_out[blockIdx.x * ldp + i] = sh[0][i] + sh[1][i] + sh[2][i] + sh[3][i];
+  } else {
+    double acc = 0.0;
+    for (int64_t j = lo + threadIdx.x; j < hi; j += kGB) {
+      double v = (double)w[j];
+      acc += v * v;
+    }
+    acc = gm_wave_sum(acc);
+    if (lane == 0) sh[wv][0] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) partial_out[blockIdx.x * ldp] = sh[0][0] + sh[1][0] + sh[2][0] + sh[3][0];
+  }
+}
+
+// q_{k+1} = w / ||w||; block 0 / thread 0 finishes the Hessenberg column with Givens
+// rotations (reference gmres.h:20-31, 106-129) and tests convergence.
+template <typename T>
+__global__ __launch_bounds__(kGB) void k_gmres_close(const T* w, T* qn, int64_t N,
+                                                     const double* partial, int nb, int ldp,
+                                                     GmresDev g, int k, const int* done_in) {
+  if (*done_in) return;
+  double s = 0.0;
+  for (int b = 0; b < nb; ++b) s += partial[b * ldp];
+  const double hn = sqrt(s);
+  const double inv = (hn > 0.0) ? 1.0 / hn : 0.0;
+  for (int64_t j = blockIdx.x * (int64_t)kGB + threadIdx.x; j < N; j += (int64_t)gridDim.x * kGB)
+    qn[j] = (T)((double)w[j] * inv);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    const int ld = g.m + 1;
+    double* h = g.H + (int64_t)k * ld;
+    h[k + 1] = hn;
+    for (int i = 0; i < k; ++i) {
+      double t = g.cs[i] * h[i] + g.sn[i] * h[i + 1];
+      h[i + 1] = -g.sn[i] * h[i] + g.cs[i] * h[i + 1];
+      h[i] = t;
+    }
+    double den = sqrt(h[k] * h[k] + h[k + 1] * h[k + 1]);
+    double c = 1.0, sn = 0.0;
+    if (den > 0.0) { c = h[k] / den; sn = h[k + 1] / den; }
+    h[k] = c * h[k] + sn * h[k + 1];
+    h[k + 1] = 0.0;
+    g.cs[k] = c;
+    g.sn[k] = sn;
+    g.beta[k + 1] = -sn * g.beta[k];
+    g.beta[k] = c * g.beta[k];
+    const double err = fabs(g.beta[k + 1]) / g.dstate[GM_BNORM];
+    g.istate[GM_KCOLS] = k + 1;
+    g.istate[GM_ITERS] += 1;
+    g.dstate[GM_ERR] = err;
+    if (err <= g.dstate[GM_THR]) {
+      g.istate[GM_FLAG] = 1;
+      g.istate[GM_DONE] = 1;
+    } else if (k + 1 >= g.m) {
+      g.istate[GM_DONE] = 1;  // cycle exhausted (flag stays 0)
+    }
+  }
+}
+
+// Start of a cycle: r already holds rhs - Op(x0) and dstate[GM_RNORM] = ||r||.
+// q_0 = r / ||r||, beta = (||r||, 0, ...), done = (||r|| <= 0).
+template <typename T>
+__global__ __launch_bounds__(kGB) void k_gmres_begin(const T* r, T* q0, int64_t N, GmresDev g) {
+  const double rn = g.dstate[GM_RNORM];
+  const double inv = (rn > 0.0) ? 1.0 / rn : 0.0;
+  for (int64_t j = blockIdx.x * (int64_t)kGB + threadIdx.x; j < N; j += (int64_t)gridDim.x * kGB)
+    q0[j] = (T)((double)r[j] * inv);
+  if (blockIdx.x == 0) {
+    for (int i = threadIdx.x; i <= g.m; i += kGB) g.beta[i] = 0.0;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      g.beta[0] = rn;
+      g.istate[GM_KCOLS] = 0;
+      const int zero = !(rn > 0.0);
+      g.istate[GM_DONE] = zero;
+      if (zero) g.istate[GM_FLAG] = 1;
+    }
+  }
+}
+
+// thr = real_tol * norm_rhs / ||drhs||,  bnorm = ||drhs||  (reference amen_solve.h:340-343)
+__global__ void k_gmres_setup(GmresDev g, const double* norm_rhs, const double* dnorm,
+                              double real_tol) {
+  g.dstate[GM_BNORM] = *dnorm;
+  g.dstate[GM_THR] = real_tol * (*norm_rhs) / (*dnorm);
+  g.dstate[GM_RNORM] = *dnorm;
+  g.istate[GM_FLAG] = 0;
+  g.istate[GM_ITERS] = 0;
+  g.istate[GM_DONE] = 0;
+  g.istate[GM_KCOLS] = 0;
+}
+
+// y = H(0:kc,0:kc) \ beta(0:kc) (upper triangular), one thread.
+__global__ void k_gmres_y(GmresDev g) {
+  const int kc = g.istate[GM_KCOLS];
+  const int ld = g.m + 1;
+  for (int i = kc - 1; i >= 0; --i) {
+    double s = g.beta[i];
+    for (int j = i + 1; j < kc; ++j) s -= g.H[(int64_t)j * ld + i] * g.y[j];
+    const double d = g.H[(int64_t)i * ld + i];
+    g.y[i] = (d != 0.0) ? s / d : 0.0;
+  }
+}
+
+// x += sum_{i<kc} y_i Q_i
+template <typename T>
+__global__ __launch_bounds__(kGB) void k_gmres_xupd(const T* Q, int64_t N, T* x, GmresDev g) {
+  const int kc = g.istate[GM_KCOLS];
+  __shared__ double y[kGmresMaxCols];
+  for (int i = threadIdx.x; i < kc; i += kGB) y[i] = g.y[i];
+  __syncthreads();
+  for (int64_t j = blockIdx.x * (int64_t)kGB + threadIdx.x; j < N; j += (int64_t)gridDim.x * kGB) {
+    double v = (double)x[j];
+    for (int i = 0; i < kc; ++i) v += y[i] * (double)Q[(int64_t)i * N + j];
+    x[j] = (T)v;
+  }
+}
+
+__global__ void k_gmres_rearm(GmresDev g) { g.istate[GM_DONE] = 0; }
+
+// ---------------------------------------------------------------- host driver
+GmresDev gmres_alloc(Arena& ws, int m, int64_t N, int esize) {
+  GmresDev g;
+  g.m = m;
+  g.Q = ws.alloc<char>((size_t)(m + 1) * N * esize);
+  g.w = ws.alloc<char>((size_t)N * esize);
+  g.H = ws.alloc<double>((size_t)(m + 1) * m);
+  g.beta = ws.alloc<double>(m + 1);
+  g.cs = ws.alloc<double>(m);
+  g.sn = ws.alloc<double>(m);
+  g.y = ws.alloc<double>(m);
+  g.dstate = ws.alloc<double>(8);
+  g.istate = ws.alloc<int>(8);
+  g.partial = ws.alloc<double>((size_t)3 * kGmresMaxBlocks * kGmresMaxCols);
+  return g;
+}
+
+template <typename T>
+GmresResult gmres_run(GmresDev g, int64_t N, T* x, const T* rhs, const double* norm_rhs,
+                      const double* dnorm, double real_tol, int resets, const GmresOp<T>& op,
+                      RedScratch rs, HostSync& hs, hipStream_t st) {
+  TT_CHECK(g.m >= 1 && g.m + 1 <= kGmresMaxCols, ERR_UNSUPPORTED,
+           "gmres: local_iterations must be in [1, " + std::to_string(kGmresMaxCols - 1) + "]");
+  T* Q = (T*)g.Q;
+  T* w = (T*)g.w;
+  const int nb = gm_blocks(N);
+  const int ldp = kGmresMaxCols;
+  double* P1 = g.partial;
+  double* P2 = g.partial + (size_t)kGmresMaxBlocks * kGmresMaxCols;
+  double* P3 = g.partial + (size_t)2 * kGmresMaxBlocks * kGmresMaxCols;
+  const int* done = g.istate + GM_DONE;
+  GmresResult res;
+
+  hipLaunchKernelGGL(k_gmres_setup, dim3(1), dim3(1), 0, st, g, norm_rhs, dnorm, real_tol);
+  TT_LAUNCH_CHECK();
+  for (int cyc = 0; cyc < resets; ++cyc) {
+    // r = rhs - Op(x); first cycle starts from x = 0 so r = rhs.
+    const T* r = rhs;
+    if (cyc > 0) {
+      hipLaunchKernelGGL(k_gmres_rearm, dim3(1), dim3(1), 0, st, g);
+      TT_LAUNCH_CHECK();
+      op.apply(x, w, nullptr);
+      op.axpby(1.0, rhs, -1.0, w, N);  // w = rhs - Op(x)
+      op.norm(w, N, g.dstate + GM_RNORM);
+      r = w;
+    }
+    hipLaunchKernelGGL((k_gmres_begin<T>), dim3(nb), dim3(kGB), 0, st, r, Q, N, g);
+    TT_LAUNCH_CHECK();
+    int k = 0;
+    bool stop = false;
+    for (k = 0; k < g.m && !stop; ++k) {
+      T* qk = Q + (int64_t)k * N;
+      T* qn = Q + (int64_t)(k + 1) * N;
+      op.apply(qk, w, done);
+      double* Hcol = g.H + (int64_t)k * (g.m + 1);
+      hipLaunchKernelGGL((k_cgs_dots<T>), dim3(nb), dim3(kGB), 0, st, Q, N, k + 1, w, P1, ldp,
+                         done);
+      hipLaunchKernelGGL((k_cgs_update<T>), dim3(nb), dim3(kGB), 0, st, Q, N, k + 1, w, P1, nb,
+                         ldp, P2, 0, Hcol, 0, done);
+      hipLaunchKernelGGL((k_cgs_update<T>), dim3(nb), dim3(kGB), 0, st, Q, N, k + 1, w, P2, nb,
+                         ldp, P3, 1, Hcol, 1, done);
+      hipLaunchKernelGGL((k_gmres_close<T>), dim3(nb), dim3(kGB), 0, st, w, qn, N, P3, nb, ldp, g,
+                         k, done);
+      TT_LAUNCH_CHECK();
+      // Read the flag of the previous iteration while this one runs.
+      hs.post(done, k, st);
+      if (k >= 1 && hs.wait(k - 1)) stop = true;
+    }
+    // Flag of the last enqueued iteration.
+    bool converged_or_done = hs.wait(k - 1);
+    (void)converged_or_done;
+    hipLaunchKernelGGL(k_gmres_y, dim3(1), dim3(1), 0, st, g);
+    hipLaunchKernelGGL((k_gmres_xupd<T>), dim3(nb), dim3(kGB), 0, st, Q, N, x, g);
+    TT_LAUNCH_CHECK();
+    // Did this cycle converge?
+    int flag = 0;
+    TT_HIP(hipMemcpyAsync(hs.host_int(), g.istate + GM_FLAG, sizeof(int), hipMemcpyDeviceToHost,
+                          st));
+    TT_HIP(hipStreamSynchronize(st));
+    flag = *hs.host_int();
+    res.cycles = cyc + 1;
+    if (flag) break;
+  }
+  TT_HIP(hipMemcpyAsync(hs.host_int(), g.istate, sizeof(int) * 8, hipMemcpyDeviceToHost, st));
+  TT_HIP(hipStreamSynchronize(st));
+  res.flag = hs.host_int()[GM_FLAG];
+  res.iters = hs.host_int()[GM_ITERS];
+  return res;
+}
+
+template GmresResult gmres_run<double>(GmresDev, int64_t, double*, const double*, const double*,
+                                       const double*, double, int, const GmresOp<double>&,
+                                       RedScratch, HostSync&, hipStream_t);
+template GmresResult gmres_run<float>(GmresDev, int64_t, float*, const float*, const double*,
+                                      const double*, double, int, const GmresOp<float>&,
+                                      RedScratch, HostSync&, hipStream_t);
+
+}  // namespace ttamen

@@ -1,0 +1,583 @@
+// FP64 dense linear algebra for the AMEn sweep: tall-skinny QR, thin SVD and
+// the dense local solve. Replaces the at::linalg_qr / at::linalg_svd /
+// at::linalg_solve calls of the reference (cpp/amen_solve.h:231, 242, 254, 326,
+// 383, 407, 430, 437, 454; cpp/ortho.h:46-73).
+//
+// QR: shifted Cholesky-QR in three passes (Fukaya, Kannan, Nakatsukasa, Yamamoto,
+//   Yanagisawa, "Shifted Cholesky QR for computing the QR factorization of
+//   ill-conditioned matrices", SIAM J. Sci. Comput. 42 (2020)). Each pass is a Gram
+//   GEMM on the matrix cores, a one-workgroup Cholesky in LDS and a row-parallel
+//   triangular solve. Columns that the Cholesky finds dead (rank deficiency) are
+//   replaced by an orthonormal completion, and R is recomputed as Q^T A so that
+//   A = Q R holds to working precision whatever the rank.
+// SVD: QR first, then one-sided (Hestenes) Jacobi on the small square factor in
+//   one workgroup with the matrix resident in LDS, round-robin parallel ordering,
+//   Rutishauser's rotation formula. The rotations are logged and applied to the
+//   identity by a second, row-parallel kernel to form V.
+// LU: Gaussian elimination with partial pivoting in one workgroup (matrices of
+//   order < max_full, 500 by default).
+#include "ttamen_internal.h"
+#include <algorithm>
+#include <cmath>
+
+namespace ttamen {
+
+constexpr double kUnitRoundoff = 1.1102230246251565e-16;  // 2^-53
+constexpr int kJacobiMaxSweeps = 48;
+
+// ---------------------------------------------------------------- Cholesky
+// G (n x n, row-major, symmetric) + shift*I = R^T R, R upper triangular written
+// row-major to Rout. shift = shift_c * u * trace(G) when shift_c > 0.
+// A pivot <= dead_rel * max(diag) marks the column dead: R[j][j] = 1 and the rest
+// of row j is zero. dead[j] records it and *ndead counts them.
+__global__ __launch_bounds__(1024) void k_cholesky(const double* G, int n, double shift_c,
+                                                   double dead_rel, double* Rout, int* dead,
+                                                   int* ndead) {
+  extern __shared__ double W[];
+  __shared__ double s_piv, s_maxd, s_shift;
+  __shared__ int s_dead, s_nd;
+  const int tid = threadIdx.x, nt = blockDim.x;
+  for (int t = tid; t < n * n; t += nt) W[t] = G[t];
+  __syncthreads();
+  if (tid == 0) {
+    double tr = 0.0, md = 0.0;
+    for (int j = 0; j < n; ++j) {
+      tr += W[j * n + j];
+      md = fmax(md, W[j * n + j]);
+    }
+    double sh = (shift_c > 0.0) ? shift_c * kUnitRoundoff * tr : 0.0;
+    s_shift = sh;
+    s_maxd = md + sh;
+    s_nd = 0;
+  }
+  __syncthreads();
+  for (int j = tid; j < n; j += nt) W[j * n + j] += s_shift;
+  __syncthreads();
+  const double thr = dead_rel * s_maxd;
+  for (int j = 0; j < n; ++j) {
+    if (tid == 0) {
+      double d = W[j * n + j];
+      int isdead = !(d > thr) || !(d > 0.0);
+      s_dead = isdead;
+      s_piv = isdead ? 1.0 : sqrt(d);
+      W[j * n + j] = s_piv;
+      dead[j] = isdead;
+      if (isdead) s_nd += 1;
+    }
+    __syncthreads();
+    const double piv = s_piv;
+    const int isdead = s_dead;
+    for (int c = j + 1 + tid; c < n; c += nt) W[j * n + c] = isdead ? 0.0 : W[j * n + c] / piv;
+    __syncthreads();
+    if (!isdead) {
+      const int rem = n - j - 1;
+      for (int t = tid; t < rem * rem; t += nt) {
+        int a = j + 1 + t / rem, c = j + 1 + t % rem;
+        if (a <= c) W[a * n + c] -= W[j * n + a] * W[j * n + c];
+      }
+    }
+    __syncthreads();
+  }
+  for (int t = tid; t < n * n; t += nt) {
+    int a = t / n, c = t % n;
+    Rout[t] = (a <= c) ? W[t] : 0.0;
+  }
+  if (tid == 0) *ndead = s_nd;
+}
+
+// ---------------------------------------------------------------- triangular solve
+// X R = A for X (m x n row-major), R upper triangular (n x n row-major). One thread per
+// row; R is staged in LDS in packed upper-triangular form and read by broadcast.
+// In-place (X == A with unit column stride and row stride n) is allowed.
+__device__ __forceinline__ int packed_idx(int i, int j, int n) {
+  return i * n - (i * (i - 1)) / 2 + (j - i);
+}
+
+__global__ __launch_bounds__(256) void k_trsm_rows(const double* A, int64_t sr, int64_t sc,
+                                                   int64_t m, int n, const double* R, double* X) {
+  extern __shared__ double P[];
+  for (int t = threadIdx.x; t < n * n; t += blockDim.x) {
+    int i = t / n, j = t % n;
+    if (i <= j) P[packed_idx(i, j, n)] = R[t];
+  }
+  __syncthreads();
+  const int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (row >= m) return;
+  const double* a = A + row * sr;
+  double* x = X + row * n;
+  for (int J = 0; J < n; J += 16) {
+    const int jb = min(16, n - J);
+    double acc[16];
+#pragma unroll
+    for (int c = 0; c < 16; ++c) acc[c] = (c < jb) ? a[(int64_t)(J + c) * sc] : 0.0;
+    for (int i = 0; i < J; ++i) {
+      const double xi = x[i];
+      const int base = packed_idx(i, J, n);
+#pragma unroll
+      for (int c = 0; c < 16; ++c)
+        if (c < jb) acc[c] -= xi * P[base + c];
+    }
+#pragma unroll
+    for (int c = 0; c < 16; ++c) {
+      if (c < jb) {
+        const int base = packed_idx(J + c, J + c, n);
+        acc[c] /= P[base];
+#pragma unroll
+        for (int c2 = c + 1; c2 < 16; ++c2)
+          if (c2 < jb) acc[c2] -= acc[c] * P[base + (c2 - c)];
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < 16; ++c)
+      if (c < jb) x[J + c] = acc[c];
+  }
+}
+
+// ---------------------------------------------------------------- completion of dead columns
+// Replace the columns j with dead[j] != 0 of Q (m x n row-major, other columns
+// orthonormal) by unit vectors orthogonalised twice against all live columns.
+__device__ double block_sum_1024(double v, double* sh) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  __syncthreads();
+  if (lane == 0) sh[w] = v;
+  __syncthreads();
+  double r = 0.0;
+  for (int i = 0; i < nw; ++i) r += sh[i];
+  return r;
+}
+
+__global__ __launch_bounds__(1024) void k_complete_dead(double* Q, int64_t m, int n, int* dead,
+                                                        const int* ndead) {
+  if (*ndead == 0) return;
+  __shared__ double sh[16];
+  __shared__ int s_ok;
+  const int tid = threadIdx.x, nt = blockDim.x;
+  int probe = 0;
+  for (int j = 0; j < n; ++j) {
+    if (!dead[j]) continue;
+    for (int tries = 0; tries < (int)min<int64_t>(m, 4 * n + 8); ++tries) {
+      const int64_t p = (probe++ * 7919LL + j * 104729LL) % m;
+      for (int64_t i = tid; i < m; i += nt) Q[i * n + j] = (i == p) ? 1.0 : 0.0;
+      __syncthreads();
+      for (int pass = 0; pass < 2; ++pass) {
+        for (int c = 0; c < n; ++c) {
+          if (c == j || dead[c]) continue;
+          double part = 0.0;
+          for (int64_t i = tid; i < m; i += nt) part += Q[i * n + c] * Q[i * n + j];
+          double dot = block_sum_1024(part, sh);
+          for (int64_t i = tid; i < m; i += nt) Q[i * n + j] -= dot * Q[i * n + c];
+          __syncthreads();
+        }
+      }
+      double part = 0.0;
+      for (int64_t i = tid; i < m; i += nt) part += Q[i * n + j] * Q[i * n + j];
+      double nrm = sqrt(block_sum_1024(part, sh));
+      if (tid == 0) s_ok = (nrm > 0.5);
+      __syncthreads();
+      if (s_ok) {
+        for (int64_t i = tid; i < m; i += nt) Q[i * n + j] /= nrm;
+        __syncthreads();
+        if (tid == 0) dead[j] = 0;
+        __syncthreads();
+        break;
+      }
+      __syncthreads();
+    }
+  }
+}
+
+// ---------------------------------------------------------------- QR driver
+int qr(const double* A, int64_t sr, int64_t sc, int64_t m, int64_t n, double* Q, double* R,
+       Arena& ws, hipStream_t st) {
+  TT_CHECK(m >= 1 && n >= 1, ERR_INVALID, "qr: empty matrix");
+  if (m < n) {
+    if (!ws.measuring()) {
+      eye(Q, m, m, st);
+      copy2d(A, sr, sc, R, n, m, n, st);
+    }
+    return (int)m;
+  }
+  TT_CHECK(n <= kMaxSmall, ERR_UNSUPPORTED,
+           "qr: more than " + std::to_string(kMaxSmall) + " columns is not supported");
+  size_t mk = ws.mark();
+  double* G = ws.alloc<double>(n * n);
+  double* Rk = ws.alloc<double>(n * n);
+  int* dead = ws.alloc<int>(n + 1);
+  int* ndead = dead + n;
+  const size_t chol_lds = (size_t)n * n * sizeof(double);
+  const size_t trsm_lds = (size_t)n * (n + 1) / 2 * sizeof(double);
+  const int trsm_blocks = (int)((m + 255) / 256);
+
+  for (int pass = 0; pass < 3; ++pass) {
+    const double* src = (pass == 0) ? A : Q;
+    int64_t ssr = (pass == 0) ? sr : n, ssc = (pass == 0) ? sc : 1;
+    GemmDesc g;  // G = src^T src
+    g.M = (int)n; g.N = (int)n; g.K = (int)m;
+    g.a.p = src; g.a.r = ssc; g.a.c = ssr;
+    g.b.p = src; g.b.r = ssr; g.b.c = ssc;
+    g.c.p = G; g.c.r = n; g.c.c = 1;
+    gemm_f64(g, ws, st);
+    if (!ws.measuring()) {
+      const double shift_c = (pass == 0) ? 11.0 * ((double)m * n + (double)n * (n + 1)) : 0.0;
+      const double dead_rel = (pass == 0) ? 0.0 : 1e-12;
+      hipLaunchKernelGGL(k_cholesky, dim3(1), dim3(1024), chol_lds, st, G, (int)n, shift_c,
+                         dead_rel, Rk, dead, ndead);
+      TT_LAUNCH_CHECK();
+      hipLaunchKernelGGL(k_trsm_rows, dim3(trsm_blocks), dim3(256), trsm_lds, st, src, ssr, ssc,
+                         m, (int)n, Rk, Q);
+      TT_LAUNCH_CHECK();
+    }
+  }
+  if (!ws.measuring()) {
+    hipLaunchKernelGGL(k_complete_dead, dim3(1), dim3(1024), 0, st, Q, m, (int)n, dead, ndead);
+    TT_LAUNCH_CHECK();
+  }
+  {
+    GemmDesc g;  // R = Q^T A
+    g.M = (int)n; g.N = (int)n; g.K = (int)m;
+    g.a.p = Q; g.a.r = 1; g.a.c = n;
+    g.b.p = A; g.b.r = sr; g.b.c = sc;
+    g.c.p = R; g.c.r = n; g.c.c = 1;
+    gemm_f64(g, ws, st);
+  }
+  ws.release(mk);
+  return (int)n;
+}
+
+// ---------------------------------------------------------------- Jacobi SVD
+// Round-robin tournament on K (even) players: round t in [0, K-2], pair i in [0, K/2).
+__device__ __forceinline__ void rr_pair(int t, int i, int K, int& p, int& q) {
+  if (i == 0) {
+    p = t;
+    q = K - 1;
+  } else {
+    p = (t + i) % (K - 1);
+    q = (t - i + (K - 1)) % (K - 1);
+  }
+  if (p > q) { int tmp = p; p = q; q = tmp; }
+}
+
+// One-sided Jacobi on the columns of a k x k matrix M (element (i,j) at M[i*sr + j*sc]).
+// Outputs: s (k, descending), Ur (k x k row-major; column r = r-th left singular
+// vector), rank_of[j] (sorted position of original column j), log of rotations
+// (c, s) per (sweep, round, pair), *nsweeps.
+__global__ __launch_bounds__(1024) void k_jacobi(const double* M, int64_t sr, int64_t sc, int k,
+                                                 double* s_out, double* Ur, int* rank_of,
+                                                 double2* log, int* nsweeps) {
+  extern __shared__ double W[];  // column-major k x k, then sig[k]
+  double* sig = W + (size_t)k * k;
+  __shared__ int s_rot;
+  __shared__ double sh[16];
+  __shared__ int s_ok;
+  const int tid = threadIdx.x, nt = blockDim.x;
+  const int g = tid >> 4, gl = tid & 15;  // 64 groups of 16 lanes
+  const int K = (k % 2) ? k + 1 : k;
+  const int npairs = K / 2;
+  for (int t = tid; t < k * k; t += nt) {
+    int j = t / k, i = t % k;
+    W[t] = M[(int64_t)i * sr + (int64_t)j * sc];
+  }
+  const double tol = (double)k * kUnitRoundoff;
+  int sw = 0;
+  __syncthreads();
+  for (; sw < kJacobiMaxSweeps && k > 1; ++sw) {
+    if (tid == 0) s_rot = 0;
+    __syncthreads();
+    for (int t = 0; t < K - 1; ++t) {
+      for (int pi = g; pi < npairs; pi += 64) {
+        int p, q;
+        rr_pair(t, pi, K, p, q);
+        double c = 1.0, s = 0.0;
+        if (q < k) {
+          double* wp = W + (size_t)p * k;
+          double* wq = W + (size_t)q * k;
+          double al = 0.0, be = 0.0, ga = 0.0;
+          for (int i = gl; i < k; i += 16) {
+            double a = wp[i], b = wq[i];
+            al += a * a;
+            be += b * b;
+            ga += a * b;
+          }
+#pragma unroll
+          for (int o = 8; o > 0; o >>= 1) {
+            al += __shfl_xor(al, o, 16);
+            be += __shfl_xor(be, o, 16);
+            ga += __shfl_xor(ga, o, 16);
+          }
+          if (al > 0.0 && be > 0.0 && fabs(ga) > tol * sqrt(al) * sqrt(be)) {
+            double zeta = (be - al) / (2.0 * ga);
+            double t_;
+            if (fabs(zeta) > 1e150) t_ = 0.5 / zeta;
+            else t_ = copysign(1.0, zeta) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+            c = 1.0 / sqrt(1.0 + t_ * t_);
+            s = c * t_;
+            for (int i = gl; i < k; i += 16) {
+              double a = wp[i], b = wq[i];
+              wp[i] = c * a - s * b;
+              wq[i] = s * a + c * b;
+            }
+            if (gl == 0) atomicAdd(&s_rot, 1);
+          }
+        }
+        if (gl == 0) log[((size_t)sw * (K - 1) + t) * npairs + pi] = make_double2(c, s);
+      }
+      __syncthreads();
+    }
+    const int rot = s_rot;
+    __syncthreads();
+    if (rot == 0) {
+      ++sw;
+      break;
+    }
+  }
+  // singular values = column norms
+  for (int j = g; j < k; j += 64) {
+    double a = 0.0;
+    for (int i = gl; i < k; i += 16) a += W[(size_t)j * k + i] * W[(size_t)j * k + i];
+#pragma unroll
+    for (int o = 8; o > 0; o >>= 1) a += __shfl_xor(a, o, 16);
+    if (gl == 0) sig[j] = sqrt(a);
+  }
+  __syncthreads();
+  const double tiny = 1e-300;
+  for (int t = tid; t < k * k; t += nt) {
+    int j = t / k;
+    double sj = sig[j];
+    W[t] = (sj > tiny) ? W[t] / sj : 0.0;
+  }
+  __syncthreads();
+  // orthonormal completion for zero singular values (rare)
+  int probe = 0;
+  for (int j = 0; j < k; ++j) {
+    if (sig[j] > tiny) continue;
+    for (int tries = 0; tries < 4 * k + 8; ++tries) {
+      const int p = (probe++ * 37 + j * 101) % k;
+      for (int i = tid; i < k; i += nt) W[(size_t)j * k + i] = (i == p) ? 1.0 : 0.0;
+      __syncthreads();
+      for (int pass = 0; pass < 2; ++pass) {
+        for (int c = 0; c < k; ++c) {
+          if (c == j) continue;
+          bool live = sig[c] > tiny || c < j;
+          if (!live) continue;
+          double part = 0.0;
+          for (int i = tid; i < k; i += nt) part += W[(size_t)c * k + i] * W[(size_t)j * k + i];
+          double dot = block_sum_1024(part, sh);
+          for (int i = tid; i < k; i += nt) W[(size_t)j * k + i] -= dot * W[(size_t)c * k + i];
+          __syncthreads();
+        }
+      }
+      double part = 0.0;
+      for (int i = tid; i < k; i += nt) part += W[(size_t)j * k + i] * W[(size_t)j * k + i];
+      double nrm = sqrt(block_sum_1024(part, sh));
+      if (tid == 0) s_ok = nrm > 0.5;
+      __syncthreads();
+      if (s_ok) {
+        for (int i = tid; i < k; i += nt) W[(size_t)j * k + i] /= nrm;
+        __syncthreads();
+        break;
+      }
+      __syncthreads();
+    }
+  }
+  // sort descending (stable by original index)
+  for (int j = tid; j < k; j += nt) {
+    double sj = sig[j];
+    int r = 0;
+    for (int i = 0; i < k; ++i) {
+      double si = sig[i];
+      r += (si > sj) || (si == sj && i < j);
+    }
+    rank_of[j] = r;
+    s_out[r] = sj;
+  }
+  __syncthreads();
+  for (int t = tid; t < k * k; t += nt) {
+    int j = t / k, i = t % k;
+    Ur[(size_t)i * k + rank_of[j]] = W[t];
+  }
+  if (tid == 0) *nsweeps = sw;
+}
+
+// Row rho of V: apply the logged rotations to e_rho; write V[rho][rank_of[j]].
+__global__ __launch_bounds__(64) void k_jacobi_v(int k, const double2* log, const int* nsweeps,
+                                                 const int* rank_of, double* V, int64_t ldv) {
+  extern __shared__ double v[];
+  const int rho = blockIdx.x, lane = threadIdx.x;
+  const int K = (k % 2) ? k + 1 : k;
+  const int npairs = K / 2;
+  for (int j = lane; j < K; j += 64) v[j] = (j == rho) ? 1.0 : 0.0;
+  __syncthreads();
+  const int nsw = *nsweeps;
+  for (int sw = 0; sw < nsw; ++sw) {
+    for (int t = 0; t < K - 1; ++t) {
+      for (int pi = lane; pi < npairs; pi += 64) {
+        int p, q;
+        rr_pair(t, pi, K, p, q);
+        if (q < k) {
+          double2 cs = log[((size_t)sw * (K - 1) + t) * npairs + pi];
+          if (cs.y != 0.0) {
+            double a = v[p], b = v[q];
+            v[p] = cs.x * a - cs.y * b;
+            v[q] = cs.y * a + cs.x * b;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+  for (int j = lane; j < k; j += 64) V[(int64_t)rho * ldv + rank_of[j]] = v[j];
+}
+
+static void jacobi_svd_small(const double* M, int64_t sr, int64_t sc, int k, double* s,
+                             double* Ur, double* V, int64_t ldv, Arena& ws, hipStream_t st) {
+  TT_CHECK(k <= kMaxSmall, ERR_UNSUPPORTED,
+           "svd: rank above " + std::to_string(kMaxSmall) + " is not supported");
+  const int K = (k % 2) ? k + 1 : k;
+  size_t mk = ws.mark();
+  int* rank_of = ws.alloc<int>(k + 1);
+  int* nsw = rank_of + k;
+  double2* log = ws.alloc<double2>((size_t)kJacobiMaxSweeps * std::max(1, K - 1) * (K / 2 + 1));
+  if (!ws.measuring()) {
+    size_t lds = ((size_t)k * k + k) * sizeof(double);
+    hipLaunchKernelGGL(k_jacobi, dim3(1), dim3(1024), lds, st, M, sr, sc, k, s, Ur, rank_of, log,
+                       nsw);
+    TT_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_jacobi_v, dim3(k), dim3(64), (size_t)K * sizeof(double), st, k, log, nsw,
+                       rank_of, V, ldv);
+    TT_LAUNCH_CHECK();
+  }
+  ws.release(mk);
+}
+
+int svd(const double* Mx, int64_t sr, int64_t sc, int64_t m, int64_t n, double* U, double* s,
+        double* V, Arena& ws, hipStream_t st) {
+  TT_CHECK(m >= 1 && n >= 1, ERR_INVALID, "svd: empty matrix");
+  const int64_t k = std::min(m, n);
+  size_t mk = ws.mark();
+  if (m >= n) {
+    double* Q = ws.alloc<double>(m * n);
+    double* R = ws.alloc<double>(n * n);
+    double* Ur = ws.alloc<double>(n * n);
+    qr(Mx, sr, sc, m, n, Q, R, ws, st);
+    jacobi_svd_small(R, n, 1, (int)n, s, Ur, V, n, ws, st);
+    GemmDesc g;  // U = Q Ur
+    g.M = (int)m; g.N = (int)n; g.K = (int)n;
+    g.a = rm(Q, n); g.b = rm(Ur, n); g.c = orm(U, n);
+    gemm_f64(g, ws, st);
+  } else {
+    double* Q = ws.alloc<double>(n * m);
+    double* R = ws.alloc<double>(m * m);
+    double* Vr = ws.alloc<double>(m * m);
+    qr(Mx, sc, sr, n, m, Q, R, ws, st);             // M^T = Q R
+    jacobi_svd_small(R, 1, m, (int)m, s, U, Vr, m, ws, st);  // R^T = U S Vr^T
+    GemmDesc g;  // V = Q Vr
+    g.M = (int)n; g.N = (int)m; g.K = (int)m;
+    g.a = rm(Q, m); g.b = rm(Vr, m); g.c = orm(V, m);
+    gemm_f64(g, ws, st);
+  }
+  ws.release(mk);
+  return (int)k;
+}
+
+// ---------------------------------------------------------------- LU solve
+__device__ __forceinline__ void argmax_abs_1024(double v, int idx, double* shv, int* shi,
+                                                double& outv, int& outi) {
+  // wave reduction
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    double ov = __shfl_xor(v, o, 64);
+    int oi = __shfl_xor(idx, o, 64);
+    if (ov > v || (ov == v && oi < idx)) { v = ov; idx = oi; }
+  }
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  if (lane == 0) { shv[w] = v; shi[w] = idx; }
+  __syncthreads();
+  double bv = shv[0];
+  int bi = shi[0];
+  for (int i = 1; i < nw; ++i)
+    if (shv[i] > bv || (shv[i] == bv && shi[i] < bi)) { bv = shv[i]; bi = shi[i]; }
+  outv = bv;
+  outi = bi;
+  __syncthreads();
+}
+
+__global__ __launch_bounds__(1024) void k_lu_solve(double* B, double* rhs, int n) {
+  __shared__ double shv[16];
+  __shared__ int shi[16];
+  __shared__ double s_x;
+  const int tid = threadIdx.x, nt = blockDim.x;
+  for (int j = 0; j < n; ++j) {
+    double v = -1.0;
+    int idx = n;
+    for (int i = j + tid; i < n; i += nt) {
+      double a = fabs(B[(int64_t)i * n + j]);
+      if (a > v || (a == v && i < idx)) { v = a; idx = i; }
+    }
+    double pv;
+    int p;
+    argmax_abs_1024(v, idx, shv, shi, pv, p);
+    if (p != j && p < n) {
+      for (int c = j + tid; c < n; c += nt) {
+        double t = B[(int64_t)j * n + c];
+        B[(int64_t)j * n + c] = B[(int64_t)p * n + c];
+        B[(int64_t)p * n + c] = t;
+      }
+      if (tid == 0) {
+        double t = rhs[j];
+        rhs[j] = rhs[p];
+        rhs[p] = t;
+      }
+    }
+    __syncthreads();
+    const double piv = B[(int64_t)j * n + j];
+    const int rem = n - j - 1;
+    // multipliers stored in column j, then trailing update and rhs
+    for (int i = j + 1 + tid; i < n; i += nt) B[(int64_t)i * n + j] /= piv;
+    __syncthreads();
+    for (int64_t t = tid; t < (int64_t)rem * (rem + 1); t += nt) {
+      int i = j + 1 + (int)(t / (rem + 1));
+      int cc = (int)(t % (rem + 1));
+      double l = B[(int64_t)i * n + j];
+      if (cc < rem) {
+        int c = j + 1 + cc;
+        B[(int64_t)i * n + c] -= l * B[(int64_t)j * n + c];
+      } else {
+        rhs[i] -= l * rhs[j];
+      }
+    }
+    __syncthreads();
+  }
+  // back substitution, column oriented
+  for (int j = n - 1; j >= 0; --j) {
+    if (tid == 0) {
+      s_x = rhs[j] / B[(int64_t)j * n + j];
+      rhs[j] = s_x;
+    }
+    __syncthreads();
+    const double xj = s_x;
+    for (int i = tid; i < j; i += nt) rhs[i] -= B[(int64_t)i * n + j] * xj;
+    __syncthreads();
+  }
+}
+
+void lu_solve(double* B, double* rhs, int n, Arena& ws, hipStream_t st) {
+  (void)ws;
+  if (n <= 0 || ws.measuring()) return;
+  hipLaunchKernelGGL(k_lu_solve, dim3(1), dim3(1024), 0, st, B, rhs, n);
+  TT_LAUNCH_CHECK();
+}
+
+// Raise the dynamic-LDS limit of the kernels that stage a full block in LDS.
+void linalg_init() {
+  static bool done = false;
+  if (done) return;
+  const int lim = 160 * 1024;
+  TT_HIP(hipFuncSetAttribute((const void*)k_cholesky, hipFuncAttributeMaxDynamicSharedMemorySize, lim));
+  TT_HIP(hipFuncSetAttribute((const void*)k_jacobi, hipFuncAttributeMaxDynamicSharedMemorySize, lim));
+  TT_HIP(hipFuncSetAttribute((const void*)k_trsm_rows, hipFuncAttributeMaxDynamicSharedMemorySize, lim));
+  done = true;
+}
+
+}  // namespace ttamen

@@ -1,0 +1,230 @@
+// Internal declarations shared by the HIP translation units of libttamen.
+//
+// Everything here is device-side plumbing for the AMEn linear solver: a bump
+// arena over a caller-provided workspace, the strided batched GEMM descriptor
+// that every tensor-train contraction is lowered to, and the launchers of the
+// reduction / QR / SVD / GMRES kernels. Nothing in this header crosses the C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cstddef>
+#include <string>
+#include <stdexcept>
+#include <vector>
+
+namespace ttamen {
+
+// ---------------------------------------------------------------- errors
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+enum Status : int {
+  OK = 0,
+  ERR_INVALID = 1,      // bad argument (shape, size, null pointer)
+  ERR_WORKSPACE = 2,    // workspace too small
+  ERR_HIP = 3,          // HIP runtime error
+  ERR_UNSUPPORTED = 4,  // feature not built (e.g. preconditioner, rank above LDS limit)
+  ERR_NUMERIC = 5,      // non-finite value detected
+};
+
+void set_last_error(const std::string& msg);
+
+#define TT_HIP(call)                                                               \
+  do {                                                                             \
+    hipError_t e_ = (call);                                                        \
+    if (e_ != hipSuccess)                                                          \
+      throw ::ttamen::Error(::ttamen::ERR_HIP, std::string(#call) + ": " +         \
+                                                   hipGetErrorString(e_));         \
+  } while (0)
+
+#define TT_CHECK(cond, code, msg)                                                  \
+  do {                                                                             \
+    if (!(cond)) throw ::ttamen::Error((code), (msg));                             \
+  } while (0)
+
+#define TT_LAUNCH_CHECK() TT_HIP(hipGetLastError())
+
+// ---------------------------------------------------------------- arena
+// Bump allocator over one caller-owned device buffer. Reset() rewinds; Mark /
+// Release give scoped temporaries. Alignment is 256 bytes.
+class Arena {
+ public:
+  Arena() = default;
+  Arena(void* base, size_t bytes) : base_((char*)base), cap_(bytes), off_(0), peak_(0) {}
+  // When base is null the arena only measures: alloc() returns fake addresses and
+  // peak() reports the bytes a real run needs.
+  template <typename T>
+  T* alloc(size_t n) {
+    size_t b = (n * sizeof(T) + 255) & ~size_t(255);
+    if (b == 0) b = 256;
+    if (base_ && off_ + b > cap_)
+      throw Error(ERR_WORKSPACE, "workspace too small: need more than " +
+                                     std::to_string(off_ + b) + " bytes, have " +
+                                     std::to_string(cap_));
+    char* p = (base_ ? base_ : (char*)nullptr) + off_;
+    off_ += b;
+    if (off_ > peak_) peak_ = off_;
+    return (T*)p;
+  }
+  size_t mark() const { return off_; }
+  void release(size_t m) { off_ = m; }
+  size_t peak() const { return peak_; }
+  bool measuring() const { return base_ == nullptr; }
+
+ private:
+  char* base_ = nullptr;
+  size_t cap_ = 0, off_ = 0, peak_ = 0;
+};
+
+// ---------------------------------------------------------------- GEMM
+// C[b1,b2](i,j) = alpha * scale * sum_k A[b1,b2](i,k) B[b1,b2](k,j) + beta * C[b1,b2](i,j)
+// Element (i,k) of A for batch (b1,b2) lives at a + b1*a_b1 + b2*a_b2 + i*a_r + k*a_c.
+// `scale` is 1, or *scale_ptr, or 1 / *scale_ptr (scale_mode 0/1/2), read on the device.
+// When skip != nullptr and *skip != 0 the launch does nothing (used by GMRES to make
+// the iterations that follow convergence free without a host round trip).
+struct MatRef {
+  const void* p = nullptr;
+  int64_t r = 0, c = 0, b1 = 0, b2 = 0;
+};
+struct OutRef {
+  void* p = nullptr;
+  int64_t r = 0, c = 0, b1 = 0, b2 = 0;
+};
+
+enum class DType : int { F64 = 0, F32 = 1, BF16 = 2 };
+
+struct GemmDesc {
+  int M = 0, N = 0, K = 0;
+  int B1 = 1, B2 = 1;
+  MatRef a, b;
+  OutRef c;
+  double alpha = 1.0, beta = 0.0;
+  const double* scale_ptr = nullptr;
+  int scale_mode = 0;
+  const int* skip = nullptr;
+};
+
+// Launch a GEMM in FP64 (A, B, C all double). `ws` provides split-K scratch.
+void gemm_f64(const GemmDesc& g, Arena& ws, hipStream_t st);
+// FP32 inputs and outputs, FP32 accumulation on MFMA.
+void gemm_f32(const GemmDesc& g, Arena& ws, hipStream_t st);
+
+// Row-major helpers for building descriptors.
+inline MatRef rm(const void* p, int64_t ld) { MatRef m; m.p = p; m.r = ld; m.c = 1; return m; }
+inline MatRef rmT(const void* p, int64_t ld) { MatRef m; m.p = p; m.r = 1; m.c = ld; return m; }
+inline OutRef orm(void* p, int64_t ld) { OutRef m; m.p = p; m.r = ld; m.c = 1; return m; }
+
+// ---------------------------------------------------------------- BLAS-1 style
+// All reductions are deterministic: fixed block partition, per-block partials,
+// the last block to finish sums the partials in block order.
+struct RedScratch {
+  double* partial = nullptr;  // >= kRedBlocks * 64 doubles
+  unsigned* counter = nullptr;  // zero-initialised, restored to zero by the kernels
+};
+constexpr int kRedBlocks = 256;
+
+// out_slot = ||x||_2 (n doubles). guard: a non-positive result is stored as 1.0.
+// mul_slot (optional): *mul_slot *= result (after the guard).
+void norm2(const double* x, int64_t n, double* out_slot, RedScratch rs, hipStream_t st,
+           bool guard = false, double* mul_slot = nullptr, const int* skip = nullptr);
+void norm2_f32(const float* x, int64_t n, double* out_slot, RedScratch rs, hipStream_t st,
+               const int* skip = nullptr);
+// out_slot = ||x - y||_2 / (den_slot ? *den_slot : 1)
+void diffnorm(const double* x, const double* y, int64_t n, const double* den_slot, double* out_slot,
+              RedScratch rs, hipStream_t st);
+// x *= 1 / *slot  (slot holds an already-guarded positive number), optional extra scalar factor
+void scale_inv(double* x, int64_t n, const double* slot, hipStream_t st);
+void scale_by(double* x, int64_t n, const double* slot, hipStream_t st);
+// y = a*x + b*y with host scalars
+void axpby(double a, const double* x, double b, double* y, int64_t n, hipStream_t st,
+           const int* skip = nullptr);
+// z = x - y
+void sub(const double* x, const double* y, double* z, int64_t n, hipStream_t st);
+void copy(const double* x, double* y, int64_t n, hipStream_t st);
+void fill(double* x, int64_t n, double v, hipStream_t st);
+void eye(double* x, int64_t m, int64_t n, hipStream_t st);  // row-major m x n identity
+// Strided 2-D copy: dst(i,j) = src(i*sr + j*sc), dst row-major with leading dim ldd
+void copy2d(const double* src, int64_t sr, int64_t sc, double* dst, int64_t ldd, int64_t m,
+            int64_t n, hipStream_t st);
+void cast_f64_f32(const double* x, float* y, int64_t n, hipStream_t st);
+void cast_f32_f64(const float* x, double* y, int64_t n, hipStream_t st);
+// Permute an operator core A[s][m][n][S] -> P[m][S][s][n] (the layout the local
+// matvec's middle contraction reads as a row-major (n*S) x (s*n) matrix).
+void permute_core_A(const double* A, double* P, int s, int n, int S, hipStream_t st);
+void permute_core_A_f32(const double* A, float* P, int s, int n, int S, hipStream_t st);
+// Dense local operator B[(l,m,L),(r,n,R)] = sum_{s,S} PhiL[l,s,r] A[s,m,n,S] PhiR[L,S,R]
+void assemble_local(const double* PhiL, const double* A, const double* PhiR, double* B, int r1,
+                    int s, int n, int S, int r2, hipStream_t st);
+
+// ---------------------------------------------------------------- scalar program
+// Tiny single-thread kernels for the sweep's norm bookkeeping.
+// nrmsc = nrmsc * a * x / b   (mode 0)   or   nrmsc = nrmsc * b / (a * x)   (mode 1)
+void nrmsc_update(double* S, int i_nrmsc, int i_a, int i_x, int i_b, int mode, hipStream_t st);
+// S[dst] = max(S[dst], S[src] / S[den])
+void scalar_maxdiv(double* S, int dst, int src, int den, hipStream_t st);
+void scalar_set(double* S, int dst, double v, hipStream_t st);
+
+// ---------------------------------------------------------------- dense linear algebra
+constexpr int kMaxSmall = 140;   // largest square block kept in LDS (FP64)
+
+// Thin QR of a tall m x n (m >= n) matrix given by strides, via shifted
+// Cholesky-QR with three passes and triangular solves; Q is m x n row-major
+// (ld n), R = Q^T A is n x n row-major. If m < n: Q = I_m, R = A (m x n).
+// Returns the number of columns of Q (min(m, n)).
+int qr(const double* A, int64_t sr, int64_t sc, int64_t m, int64_t n, double* Q, double* R,
+       Arena& ws, hipStream_t st);
+// Thin SVD of an m x n matrix (strided): U (m x k row-major, ld k), s (k), V (n x k
+// row-major, ld k), k = min(m, n). Singular values descending.
+int svd(const double* M, int64_t sr, int64_t sc, int64_t m, int64_t n, double* U, double* s,
+        double* V, Arena& ws, hipStream_t st);
+// Solve B x = rhs for a dense n x n FP64 matrix (B is overwritten with its LU factors).
+void lu_solve(double* B, double* rhs, int n, Arena& ws, hipStream_t st);
+
+// ---------------------------------------------------------------- local operator
+// Local AMEn operator, y = PhiL x A x PhiR applied to nvec vectors:
+//   y[v][l,m,L] = sum PhiL[l,s,r] A[s,m,n,S] PhiR[L,S,R] x[v][r,n,R]
+// PhiL is l1 x s x r1, PhiR is l2 x S x r2, x is r1 x n x r2, y is l1 x n x l2.
+// Apm is the permuted core from permute_core_A.
+struct LocalOp {
+  const double* PhiL = nullptr;
+  const double* Apm = nullptr;
+  const double* PhiR = nullptr;
+  int l1 = 1, s = 1, r1 = 1, n = 1, S = 1, l2 = 1, r2 = 1;
+};
+void local_matvec(const LocalOp& op, const double* x, double* y, int nvec, Arena& ws,
+                  hipStream_t st, const int* skip = nullptr);
+size_t local_matvec_ws(const LocalOp& op, int nvec);
+
+struct LocalOpF32 {
+  const float* PhiL = nullptr;
+  const float* Apm = nullptr;
+  const float* PhiR = nullptr;
+  int l1 = 1, s = 1, r1 = 1, n = 1, S = 1, l2 = 1, r2 = 1;
+};
+void local_matvec_f32(const LocalOpF32& op, const float* x, float* y, int nvec, Arena& ws,
+                      hipStream_t st, const int* skip = nullptr);
+
+// Interface updates (FP64).
+// fwd: Pn[L,S,R] = sum Phi[l,s,r] xL[l,M,L] A[s,M,N,S] xR[r,N,R]
+void phi_fwd_A(const double* Phi, const double* xL, const double* Apm, const double* xR,
+               double* Pn, int l, int s, int r, int n, int S, int L, int R, Arena& ws,
+               hipStream_t st);
+// bck: Pp[l,s,r] = sum xL[l,M,L] Phi[L,S,R] A[s,M,N,S] xR[r,N,R]
+void phi_bck_A(const double* Phi, const double* xL, const double* A, const double* xR,
+               double* Pp, int l, int s, int r, int n, int S, int L, int R, Arena& ws,
+               hipStream_t st);
+// fwd rhs: Pn[B,R] = sum Pb[b,r] bc[b,n,B] x[r,n,R]
+void phi_fwd_rhs(const double* Pb, const double* bc, const double* x, double* Pn, int b, int r,
+                 int n, int B, int R, Arena& ws, hipStream_t st);
+// bck rhs: Pp[b,r] = sum bc[b,n,B] Pb[B,R] x[r,n,R]
+void phi_bck_rhs(const double* Pb, const double* bc, const double* x, double* Pp, int b, int r,
+                 int n, int B, int R, Arena& ws, hipStream_t st);
+// rhs[r,n,R] = scale * sum PL[b,r] bc[b,n,B] PR[B,R]   (scale = *scale_ptr, if given)
+void assemble_rhs(const double* PL, const double* bc, const double* PR, double* out, int b,
+                  int r, int n, int B, int R, const double* scale_ptr, Arena& ws,
+                  hipStream_t st);
+
+}  // namespace ttamen

@@ -1,0 +1,141 @@
+"""AMEn linear solver for TT systems A x = b -- drop-in for `torchtt.solvers.amen_solve`.
+
+Same signature, argument checks, errors and TT return as the reference
+(torchtt/solvers.py:211-287). The solve itself runs in libttamen's HIP kernels on the
+GPU (include/ttamen.h: ttamen_amen_solve, which replaces torchttcpp.amen_solve,
+cpp/amen_solve.h:122-549). There is no CPU fallback: without the library or a ROCm GPU
+the call raises.
+"""
+import ctypes
+import torch
+
+from ._tt import TT
+from .errors import InvalidArguments, IncompatibleTypes, ShapeMismatch
+from . import _lib
+
+#: statistics of the most recent solve (dict), see include/ttamen.h: ttamen_stats
+last_stats = None
+
+_PRECISIONS = {"fp64": 0, "fp32": 1}
+
+
+def cpp_enabled():
+    """True when the native solver library is built (reference solvers.py:26-33)."""
+    try:
+        _lib.load()
+        return True
+    except ImportError:
+        return False
+
+
+def _solver_device(A):
+    dev = A.cores[0].device
+    if dev.type == "cuda":
+        return dev
+    if not torch.cuda.is_available():
+        raise RuntimeError("torchtt_amd.solvers.amen_solve runs on a ROCm GPU; none is available.")
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+def amen_solve(A, b, nswp=22, x0=None, eps=1e-10, rmax=32768, max_full=500, kickrank=4, kick2=0,
+               trunc_norm='res', local_solver=1, local_iterations=40, resets=2, verbose=False,
+               preconditioner=None, use_cpp=True, band_diagonal=-1, use_single_precision=False,
+               precision=None, seed=None):
+    """Solve A x = b in the TT format with AMEn (Dolgov & Savostyanov, SIAM J. Sci. Comput. 36 (2014)).
+
+    Args follow the reference. `trunc_norm`, `local_solver`, `use_cpp` and `band_diagonal`
+    are accepted for compatibility; the native path always uses GMRES and the residual-based
+    truncation, as the reference's C++ path does. New arguments:
+        precision: 'fp64' (default) or 'fp32' (FP32 matrix-core local operator with FP64
+            iterative refinement); `use_single_precision=True` selects 'fp32'.
+        seed: seed for device-drawn random enrichment columns (only used when kick2 > 0).
+
+    Returns:
+        TT: the solution on the device and dtype of A.cores[0].
+    """
+    global last_stats
+    if not (isinstance(A, TT) and isinstance(b, TT)):
+        raise InvalidArguments('A and b must be TT instances.')
+    if not (A.is_ttm and not b.is_ttm):
+        raise IncompatibleTypes('A must be TT-matrix and b must be vector.')
+    if A.M != A.N:
+        raise ShapeMismatch('A is not quadratic.')
+    if A.N != b.N:
+        raise ShapeMismatch('Dimension mismatch.')
+    if preconditioner is None:
+        prec = 0
+    elif preconditioner == 'c':
+        prec = 1
+    elif preconditioner == 'r':
+        prec = 2
+    else:
+        raise InvalidArguments("Invalid preconditioner.")
+    if prec != 0:
+        raise NotImplementedError("Local preconditioners ('c', 'r') are not available in the GPU solver yet.")
+    if precision is None:
+        precision = 'fp32' if use_single_precision else 'fp64'
+    if precision not in _PRECISIONS:
+        raise InvalidArguments("precision must be one of %s." % sorted(_PRECISIONS))
+
+    lib = _lib.load()
+    out_device, out_dtype = A.cores[0].device, A.cores[0].dtype
+    dev = _solver_device(A)
+    N = list(b.N)
+    d = len(N)
+    rA, rb = A.R, b.R
+
+    # Residual-proxy initial cores: drawn like the reference draws them (torch.randn with
+    # the options of A, cpp/amen_solve.h:174-176), so a seeded run matches it.
+    rzc = kickrank + kick2
+    z_cores = None
+    if rzc > 0:
+        rz = [1] + [rzc] * (d - 1) + [1]
+        z_cores = [torch.randn((rz[i], N[i], rz[i + 1]), dtype=torch.float64, device=out_device)
+                   for i in range(d)]
+        z_cores = [z.to(dev).contiguous() for z in z_cores]
+
+    with torch.cuda.device(dev):
+        Ac = [c.to(device=dev, dtype=torch.float64).contiguous() for c in A.cores]
+        bc = [c.to(device=dev, dtype=torch.float64).contiguous() for c in b.cores]
+        if x0 is not None:
+            xc = [c.to(device=dev, dtype=torch.float64).contiguous() for c in x0.cores]
+            rx0 = x0.R
+        else:
+            xc, rx0 = None, None
+        Na, rAa, rba = _lib.i64_array(N), _lib.i64_array(rA), _lib.i64_array(rb)
+        rx0a = _lib.i64_array(rx0) if rx0 is not None else None
+        ws_bytes = lib.ttamen_amen_solve_workspace_size(d, Na, rAa, rba, rx0a, int(rmax), int(kickrank),
+                                                        int(kick2), int(local_iterations), int(max_full),
+                                                        _PRECISIONS[precision])
+        if ws_bytes == 0:
+            _lib.check("ttamen_amen_solve_workspace_size", 1)
+        out_elems = lib.ttamen_amen_solve_output_elems(d, Na, rx0a, int(rmax), int(kickrank), int(kick2))
+        ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dev)
+        xout = torch.empty(max(out_elems, 1), dtype=torch.float64, device=dev)
+        rx_out = (ctypes.c_int64 * (d + 1))()
+        off_out = (ctypes.c_int64 * d)()
+        stats = _lib.TTAmenStats()
+        code = lib.ttamen_amen_solve(
+            d, Na, rAa, rba, _lib.ptr_array(Ac), _lib.ptr_array(bc),
+            _lib.ptr_array(xc) if xc is not None else None, rx0a,
+            _lib.ptr_array(z_cores) if z_cores is not None else None,
+            int(seed) if seed is not None else 0, int(nswp), float(eps), int(rmax), int(max_full),
+            int(kickrank), int(kick2), int(local_iterations), int(resets), int(bool(verbose)), prec,
+            _PRECISIONS[precision], ctypes.c_void_p(ws.data_ptr()), ws_bytes,
+            ctypes.c_void_p(xout.data_ptr()), out_elems, rx_out, off_out, ctypes.byref(stats),
+            ctypes.c_void_p(_lib.stream_handle(dev)))
+        _lib.check("ttamen_amen_solve", code)
+        cores = []
+        for k in range(d):
+            n = rx_out[k] * N[k] * rx_out[k + 1]
+            c = xout[off_out[k]:off_out[k] + n].reshape(rx_out[k], N[k], rx_out[k + 1]).clone()
+            cores.append(c.to(device=out_device, dtype=out_dtype))
+        last_stats = stats.as_dict()
+        last_stats["workspace_bytes"] = int(ws_bytes)
+        del ws, xout
+    return TT(cores)
+
+
+def residual(A, x, b):
+    """Relative residual ||A x - b|| / ||b|| in TT arithmetic (the check every test applies)."""
+    return float((A @ x - b).norm() / b.norm())
