@@ -153,9 +153,11 @@ int ttamen_svd(const double* A, int64_t m, int64_t n, double* U, double* s, doub
                void* workspace, size_t workspace_bytes, void* stream);
 size_t ttamen_dense_workspace_size(int64_t m, int64_t n);
 
-/* Dense solve B x = rhs, n < 2^15 (FP64, B overwritten). Replaces at::linalg_solve of the
- * direct local solver, cpp/amen_solve.h:326. x overwrites rhs. */
-int ttamen_dense_solve(double* B, double* rhs, int64_t n, void* stream);
+/* Dense solve B x = rhs (FP64, B n x n row-major) by blocked LU with partial pivoting.
+ * Replaces at::linalg_solve of the direct local solver, cpp/amen_solve.h:326. */
+int ttamen_dense_solve(const double* B, const double* rhs, double* x, int64_t n, void* workspace,
+                       size_t workspace_bytes, void* stream);
+size_t ttamen_dense_solve_workspace_size(int64_t n);
 
 /*
  * Strided batched GEMM on the matrix cores (FP64), the building block of every

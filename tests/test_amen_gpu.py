@@ -127,11 +127,16 @@ def test_edge_cases():
     _say("amen d=2")
     x2 = tt.solvers.amen_solve(tt.TT(A2).cuda(), tt.TT(b2).cuda(), eps=1e-10)
     assert tt.solvers.residual(tt.TT(A2).cuda(), x2, tt.TT(b2).cuda()) < 1e-9
-    # kickrank = 0 (plain ALS sweeps)
+    # kickrank = 0: no enrichment, so ranks cannot grow past the start's (one-site ALS); the
+    # reference raises on this input. From a start of ranks (10, 16, 10) the sweeps converge.
     A3, b3 = W.laplace_ones(4, 10)
     _say("amen kickrank=0")
-    x3 = tt.solvers.amen_solve(tt.TT(A3).cuda(), tt.TT(b3).cuda(), eps=1e-10, kickrank=0, nswp=30)
-    assert tt.solvers.residual(tt.TT(A3).cuda(), x3, tt.TT(b3).cuda()) < 1e-6
+    g = torch.Generator().manual_seed(4)
+    x0 = tt.TT([torch.randn(r0, 10, r1, dtype=torch.float64, generator=g).cuda()
+                for r0, r1 in zip([1, 10, 16, 10], [10, 16, 10, 1])])
+    x3 = tt.solvers.amen_solve(tt.TT(A3).cuda(), tt.TT(b3).cuda(), x0=x0, eps=1e-10, kickrank=0, nswp=30)
+    assert max(x3.R) <= 16
+    assert tt.solvers.residual(tt.TT(A3).cuda(), x3, tt.TT(b3).cuda()) < 1e-8
     # input validation errors match the reference
     with pytest.raises(tt.errors.InvalidArguments):
         tt.solvers.amen_solve(tt.TT(A3), b3)

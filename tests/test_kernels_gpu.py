@@ -229,7 +229,7 @@ def test_svd(kind, m, n):
     assert np.linalg.norm((Uh * sh) @ Vh.T - A) <= 1e-13 * np.sqrt(m * n) * np.linalg.norm(A)
 
 
-@pytest.mark.parametrize("n", [1, 7, 120, 499])
+@pytest.mark.parametrize("n", [1, 7, 31, 32, 33, 120, 499])
 def test_dense_solve(n):
     L, lib = _lib()
     _say("dense_solve", n)
@@ -238,8 +238,10 @@ def test_dense_solve(n):
     rhs = rng.standard_normal(n)
     Bd = torch.from_numpy(B.copy()).to(DEV)
     rd = torch.from_numpy(rhs.copy()).to(DEV)
-    L.check("dense_solve", lib.ttamen_dense_solve(_p(Bd), _p(rd), n, _stream()))
+    xd = torch.empty(n, dtype=torch.float64, device=DEV)
+    ws = _ws(lib.ttamen_dense_solve_workspace_size(n))
+    L.check("dense_solve", lib.ttamen_dense_solve(_p(Bd), _p(rd), _p(xd), n, _p(ws), ws.numel(), _stream()))
     torch.cuda.synchronize()
-    x = rd.cpu().numpy()
+    x = xd.cpu().numpy()
     want = np.linalg.solve(B, rhs)
     assert _rel(x, want) < 1e-11 * max(1.0, np.linalg.cond(B) / 1e3)

@@ -56,7 +56,8 @@ _SIGNATURES = {
     "ttamen_qr": (ctypes.c_int, [c_dp, c_i64, c_i64, c_dp, c_dp, c_dp, c_size, c_dp]),
     "ttamen_svd": (ctypes.c_int, [c_dp, c_i64, c_i64, c_dp, c_dp, c_dp, c_dp, c_size, c_dp]),
     "ttamen_dense_workspace_size": (c_size, [c_i64, c_i64]),
-    "ttamen_dense_solve": (ctypes.c_int, [c_dp, c_dp, c_i64, c_dp]),
+    "ttamen_dense_solve": (ctypes.c_int, [c_dp, c_dp, c_dp, c_i64, c_dp, c_size, c_dp]),
+    "ttamen_dense_solve_workspace_size": (c_size, [c_i64]),
     "ttamen_profile_enable": (ctypes.c_int, [ctypes.c_int]),
     "ttamen_profile_read": (ctypes.c_int, [ctypes.POINTER(ctypes.c_double)]),
     "ttamen_gemm_f64": (ctypes.c_int, [c_i64, c_i64, c_i64, c_i64, ctypes.c_double,

@@ -574,10 +574,11 @@ class Solver {
     if (use_full) {
       stats_.direct_solves += 1;
       size_t mk2 = ws_.mark();
-      double* B = ws_.alloc<double>(Nl * Nl);
-      assemble_local(Phi_[k], A_[k], Phi_[k + 1], B, (int)r1, (int)s, (int)n, (int)S, (int)r2, st_);
-      copy(rhs, sol, Nl, st_);
-      lu_solve(B, sol, (int)Nl, ws_, st_);
+      double* B = ws_.alloc<double>(Nl * (Nl + 1));
+      assemble_local(Phi_[k], A_[k], Phi_[k + 1], B, Nl + 1, (int)r1, (int)s, (int)n, (int)S,
+                     (int)r2, st_);
+      copy2d(rhs, 1, 0, B + Nl, Nl + 1, Nl, 1, st_);
+      lu_solve_aug(B, (int)Nl, sol, ws_, st_);
       ws_.release(mk2);
       matvec(op, prev, e, 1);
       diffnorm(e, rhs, Nl, sc(TMP(T_NRHS)), sc(TMP(T_RESOLD)), rs_, st_);
@@ -877,7 +878,7 @@ static size_t solve_workspace_bound(const Plan& P, const Options& O) {
   const double cw = R + rz;
   double doubles = 0;
   doubles += 4 * Nl;                                   // rhs, sol, e, tmp
-  doubles += std::max((double)std::min<int64_t>(P.maxcore, O.max_full) * std::min<int64_t>(P.maxcore, O.max_full),
+  doubles += std::max((double)std::min<int64_t>(P.maxcore, O.max_full) * (std::min<int64_t>(P.maxcore, O.max_full) + 2),
                       Nl * (m + 4) + 3.0 * 256 * 128 + (m + 1) * m + 8 * m);  // dense B | GMRES
   doubles += 2 * R * s * n * R * 2;                    // local matvec t1, t2 (+ z products)
   doubles += Nl * 3 + 4 * R * R + 48.0 * (R + 2) * (R / 2 + 2) * 2;  // SVD: U, Q, R, Ur, V, log
@@ -959,6 +960,9 @@ int ttamen_amen_solve(int d, const int64_t* N, const int64_t* rA, const int64_t*
                  local_iterations >= 1,
              ERR_INVALID, "invalid solver parameters");
     linalg_init();
+    lu_init();
+    TT_CHECK(max_full <= (int64_t)lu_max_order() + 1, ERR_UNSUPPORTED,
+             "max_full above " + std::to_string(lu_max_order() + 1) + " is not supported");
     Plan P = make_plan(d, N, rA, rb, x0_cores ? rx0 : nullptr, rmax, kickrank, kick2);
     Options O;
     O.nswp = nswp; O.eps = eps; O.rmax = rmax; O.max_full = max_full; O.kickrank = kickrank;
@@ -1105,12 +1109,23 @@ int ttamen_svd(const double* A, int64_t m, int64_t n, double* U, double* s, doub
   });
 }
 
-int ttamen_dense_solve(double* B, double* rhs, int64_t n, void* stream) {
+int ttamen_dense_solve(const double* B, const double* rhs, double* x, int64_t n, void* workspace,
+                       size_t workspace_bytes, void* stream) {
   return guarded([&] {
-    TT_CHECK(n > 0 && n < 32768, ERR_INVALID, "dense_solve: bad order");
-    Arena ws((void*)1, 0);
-    lu_solve(B, rhs, (int)n, ws, (hipStream_t)stream);
+    TT_CHECK(B && rhs && x && workspace, ERR_INVALID, "null argument");
+    TT_CHECK(n > 0 && n <= lu_max_order(), ERR_INVALID, "dense_solve: bad order");
+    lu_init();
+    Arena ws(workspace, workspace_bytes);
+    hipStream_t st = (hipStream_t)stream;
+    double* aug = ws.alloc<double>(n * (n + 1));
+    copy2d(B, n, 1, aug, n + 1, n, n, st);
+    copy2d(rhs, 1, 0, aug + n, n + 1, n, 1, st);
+    lu_solve_aug(aug, (int)n, x, ws, st);
   });
+}
+
+size_t ttamen_dense_solve_workspace_size(int64_t n) {
+  return (size_t)(n * (n + 1) + n) * 8 + (size_t)n * 4 + (64ull << 20);
 }
 
 int ttamen_profile_enable(int enable) {

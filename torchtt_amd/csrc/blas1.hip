@@ -237,7 +237,7 @@ void permute_core_A_f32(const double* A, float* P, int s, int n, int S, hipStrea
 
 // B[(l,m,L),(r,n,R)] = sum_{s,S} PhiL[l,s,r] A[s,m,n,S] PhiR[L,S,R]
 __global__ void k_assemble(const double* PhiL, const double* A, const double* PhiR, double* B,
-                           int r1, int s, int n, int S, int r2) {
+                           int64_t ld, int r1, int s, int n, int S, int r2) {
   const int64_t N = (int64_t)r1 * n * r2;
   for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < N * N;
        t += (int64_t)gridDim.x * blockDim.x) {
@@ -251,14 +251,14 @@ __global__ void k_assemble(const double* PhiL, const double* A, const double* Ph
       for (int Si = 0; Si < S; ++Si)
         acc += pl * A[(((int64_t)si * n + m) * n + nn) * S + Si] * PhiR[((int64_t)L * S + Si) * r2 + R];
     }
-    B[t] = acc;
+    B[row * ld + col] = acc;
   }
 }
-void assemble_local(const double* PhiL, const double* A, const double* PhiR, double* B, int r1,
-                    int s, int n, int S, int r2, hipStream_t st) {
+void assemble_local(const double* PhiL, const double* A, const double* PhiR, double* B, int64_t ld,
+                    int r1, int s, int n, int S, int r2, hipStream_t st) {
   int64_t N = (int64_t)r1 * n * r2;
   hipLaunchKernelGGL(k_assemble, dim3(grid_for(N * N, 256, 4096)), dim3(256), 0, st, PhiL, A,
-                     PhiR, B, r1, s, n, S, r2);
+                     PhiR, B, ld, r1, s, n, S, r2);
   TT_LAUNCH_CHECK();
 }
 

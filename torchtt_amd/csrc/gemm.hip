@@ -8,8 +8,10 @@
 // Each wave owns a (BM/2) x (BN/2) sub-tile built from 16x16 MFMA blocks:
 //   FP64: v_mfma_f64_16x16x4_f64   (A: 1 double/lane, B: 1 double/lane, D: 4 doubles/lane)
 //   FP32: v_mfma_f32_16x16x4_f32   (same fragment shapes in float)
-// Fragment layout (16x16xK4 block): lane l holds A[i = l%16][k = l/16],
-// B[k = l/16][j = l%16], and D[i = 4*(l/16) + v][j = l%16] for v = 0..3.
+// Fragment layout (16x16xK4 block): lane l holds A[i = l%16][k = l/16] and
+// B[k = l/16][j = l%16]; result register v of lane l holds
+//   FP32: D[i = 4*(l/16) + v][j = l%16]      FP64: D[i = l/16 + 4*v][j = l%16]
+// (the FP64 layout was established on the device with known operands).
 // Global -> register -> LDS double buffering: the next K tile is fetched while
 // the current one feeds the MFMAs. The global fetch walks whichever operand
 // axis has unit stride, so loads are coalesced for either orientation.
@@ -26,6 +28,11 @@ typedef float f4 __attribute__((ext_vector_type(4)));
 template <typename T> struct AccT;
 template <> struct AccT<double> { typedef d4 type; };
 template <> struct AccT<float> { typedef f4 type; };
+
+// Row (within a 16x16 block) of result register v in a lane with lane/16 == lk.
+template <typename T> __device__ __forceinline__ int acc_row(int lk, int v);
+template <> __device__ __forceinline__ int acc_row<double>(int lk, int v) { return lk + 4 * v; }
+template <> __device__ __forceinline__ int acc_row<float>(int lk, int v) { return 4 * lk + v; }
 
 __device__ __forceinline__ d4 mfma16(double a, double b, d4 c) {
   return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
@@ -172,7 +179,7 @@ __global__ __launch_bounds__(256) void k_gemm(GemmParams<T> p) {
       for (int ni = 0; ni < NI; ++ni)
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
-          int i = m0 + wm * (BM / 2) + mi * 16 + 4 * lk + v;
+          int i = m0 + wm * (BM / 2) + mi * 16 + acc_row<T>(lk, v);
           int j = n0 + wn * (BN / 2) + ni * 16 + lr;
           if (i < p.M && j < p.N) P[(int64_t)i * p.N + j] = acc[mi][ni][v];
         }
@@ -186,7 +193,7 @@ __global__ __launch_bounds__(256) void k_gemm(GemmParams<T> p) {
     for (int ni = 0; ni < NI; ++ni)
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        int i = m0 + wm * (BM / 2) + mi * 16 + 4 * lk + v;
+        int i = m0 + wm * (BM / 2) + mi * 16 + acc_row<T>(lk, v);
         int j = n0 + wn * (BN / 2) + ni * 16 + lr;
         if (i < p.M && j < p.N) {
           T* cp = C + (int64_t)i * p.cr + (int64_t)j * p.cc;

@@ -6,10 +6,10 @@
 namespace ttamen {
 
 constexpr int kGmresMaxCols = 128;    // (local_iterations + 1) upper bound
-constexpr int kGmresMaxBlocks = 256;  // blocks of the streaming kernels
+constexpr int kGmresMaxBlocks = 256;   // blocks of the streaming kernels
 
-enum { GM_DONE = 0, GM_FLAG = 1, GM_KCOLS = 2, GM_ITERS = 3 };           // istate
-enum { GM_BNORM = 0, GM_THR = 1, GM_RNORM = 2, GM_ERR = 3 };             // dstate
+enum { GM_DONE = 0, GM_FLAG = 1, GM_KCOLS = 2, GM_ITERS = 3, GM_RSKIP = 4, GM_REORTHS = 5 };  // istate
+enum { GM_BNORM = 0, GM_THR = 1, GM_RNORM = 2, GM_ERR = 3, GM_WN2 = 4, GM_WNORM = 5 };      // dstate
 
 // Device buffers of one GMRES instance (all inside the caller's arena).
 struct GmresDev {
@@ -23,7 +23,7 @@ struct GmresDev {
   double* y = nullptr;
   double* dstate = nullptr;
   int* istate = nullptr;
-  double* partial = nullptr;  // 3 x kGmresMaxBlocks x kGmresMaxCols
+  double* partial = nullptr;  // per-block partials, row-major [row][block]
 };
 
 template <typename T>
@@ -34,7 +34,7 @@ struct GmresOp {
 };
 
 struct GmresResult {
-  int flag = 0, iters = 0, cycles = 0;
+  int flag = 0, iters = 0, cycles = 0, reorths = 0;
 };
 
 // Small pinned-host mailbox used to poll device flags without stalling the queue.

@@ -4,8 +4,10 @@
 // caller cpp/amen_solve.h:337-358. The reference keeps the Hessenberg matrix,
 // the Givens rotations and every dot product on the host (one device->host sync
 // per dot). Here the whole Arnoldi process stays on the device:
-//   * orthogonalisation is classical Gram-Schmidt applied twice (CGS2), each pass
-//     one streaming kernel over the Krylov basis with deterministic block partials;
+//   * orthogonalisation is classical Gram-Schmidt with selective re-orthogonalisation
+//     (a second pass only when the first cancels more than 1 - 1/sqrt(2) of ||w||);
+//     each pass is one dot kernel and one update kernel streaming the Krylov basis,
+//     with deterministic block partials reduced by the last block;
 //   * the Givens update of the Hessenberg column, the residual estimate |beta_k+1|
 //     and the convergence test run in one thread of the normalisation kernel;
 //   * a device flag `done` makes every kernel of later iterations (including the
@@ -21,10 +23,11 @@
 
 namespace ttamen {
 
-static constexpr int kGB = 256;  // threads per block
+static constexpr int kGB = 256;  // threads per block (kGB >= kGmresMaxBlocks)
+static_assert(kGB >= kGmresMaxBlocks, "one thread per partial in the block reductions");
 
 static inline int gm_blocks(int64_t N) {
-  int64_t b = (N + kGB * 8 - 1) / (kGB * 8);
+  int64_t b = (N + kGB * 4 - 1) / (kGB * 4);
   return (int)std::max<int64_t>(1, std::min<int64_t>(b, kGmresMaxBlocks));
 }
 
@@ -34,87 +37,92 @@ __device__ __forceinline__ double gm_wave_sum(double v) {
   return v;
 }
 
-// partial[b * ldp + i] = sum over block b's slice of Q_i . w,   i < kc
-template <typename T>
-__global__ __launch_bounds__(kGB) void k_cgs_dots(const T* Q, int64_t N, int kc, const T* w,
-                                                  double* partial, int ldp, const int* done) {
-  if (*done) return;
-  __shared__ double sh[4][kGmresMaxCols];
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int64_t chunk = (N + gridDim.x - 1) / gridDim.x;
-  const int64_t lo = blockIdx.x * chunk, hi = min(N, lo + chunk);
-  for (int i0 = 0; i0 < kc; i0 += 8) {
-    double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    for (int64_t j = lo + threadIdx.x; j < hi; j += kGB) {
-      const double wj = (double)w[j];
-#pragma unroll
-      for (int u = 0; u < 8; ++u)
-        if (i0 + u < kc) acc[u] += (double)Q[(int64_t)(i0 + u) * N + j] * wj;
-    }
-#pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      double v = gm_wave_sum(acc[u]);
-      if (lane == 0 && i0 + u < kc) sh[wv][i0 + u] = v;
-    }
+__device__ __forceinline__ bool gm_skip(const int* a) { return a && *a; }
+
+// out[i] = sum_b P[i * kGmresMaxBlocks + b] over b < nb, for i < rows; every thread of
+// the block gets the results in LDS. Fixed reduction tree: deterministic.
+__device__ void gm_reduce_rows(const double* P, int rows, int nb, double* out) {
+  __shared__ double red[kGB / 64][kGmresMaxCols + 1];
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  for (int i = 0; i < rows; ++i) {
+    double v = (t < nb) ? P[(int64_t)i * kGmresMaxBlocks + t] : 0.0;
+    v = gm_wave_sum(v);
+    if (lane == 0) red[wv][i] = v;
   }
   __syncthreads();
-  for (int i = threadIdx.x; i < kc; i += kGB)
-    partial[blockIdx.x * ldp + i] = sh[0][i] + sh[1][i] + sh[2][i] + sh[3][i];
+  for (int i = t; i < rows; i += kGB) {
+    double s = 0.0;
+    for (int w = 0; w < kGB / 64; ++w) s += red[w][i];
+    out[i] = s;
+  }
+  __syncthreads();
 }
 
-// h_i = sum_b partial_in[b][i]; w -= sum_i h_i Q_i; H[:,k] (+)= h (block 0);
-// then either next-pass dots (mode 0) or ||w||^2 partials (mode 1) into partial_out.
+// Whether the second (re-orthogonalisation) pass runs: the first pass cancelled more
+// than 1 - 1/sqrt(2) of ||w|| (Daniel, Gragg, Kaufman and Stewart, Math. Comp. 30 (1976)).
+// Pw: row kc of the first dot partials (w.w); Pn1: ||w'||^2 partials after pass 1.
+__device__ bool gm_need_reorth(const double* Pw, const double* Pn1, int nb) {
+  __shared__ double r2[2];
+  gm_reduce_rows(Pw, 1, nb, r2);
+  __shared__ double r1[2];
+  gm_reduce_rows(Pn1, 1, nb, r1);
+  return !(sqrt(r1[0]) > 0.7071067811865476 * sqrt(r2[0]));
+}
+
+// P[i][b] = Q_i . w over block b's chunk (i < kc), P[kc][b] = w . w. Each wave streams
+// whole rows over the chunk (coalesced); the chunk of w stays in L1.
 template <typename T>
-__global__ __launch_bounds__(kGB) void k_cgs_update(const T* Q, int64_t N, int kc, T* w,
-                                                    const double* partial_in, int nb_in, int ldp,
-                                                    double* partial_out, int mode, double* Hcol,
-                                                    int accumulate, const int* done) {
-  if (*done) return;
-  __shared__ double h[kGmresMaxCols];
-  __shared__ double sh[4][kGmresMaxCols];
+__global__ __launch_bounds__(kGB) void k_cgs_dots(const T* Q, int64_t N, int kc, const T* w,
+                                                  double* P, int with_ww, const int* skip,
+                                                  const double* Pw, const double* Pn1) {
+  if (gm_skip(skip)) return;
+  if (Pw && !gm_need_reorth(Pw, Pn1, gridDim.x)) return;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  for (int i = threadIdx.x; i < kc; i += kGB) {
-    double s = 0.0;
-    for (int b = 0; b < nb_in; ++b) s += partial_in[b * ldp + i];
-    h[i] = s;
-    if (blockIdx.x == 0) Hcol[i] = accumulate ? Hcol[i] + s : s;
-  }
-  __syncthreads();
   const int64_t chunk = (N + gridDim.x - 1) / gridDim.x;
   const int64_t lo = blockIdx.x * chunk, hi = min(N, lo + chunk);
+  const int rows = kc + (with_ww ? 1 : 0);
+  for (int i = wv; i < rows; i += kGB / 64) {
+    const T* row = (i < kc) ? Q + (int64_t)i * N : w;
+    double acc = 0.0;
+    for (int64_t j = lo + lane; j < hi; j += 64) acc += (double)row[j] * (double)w[j];
+    acc = gm_wave_sum(acc);
+    if (lane == 0) P[(int64_t)i * kGmresMaxBlocks + blockIdx.x] = acc;
+  }
+}
+
+// h = sum_b P[:, b]; w -= sum_i h_i Q_i over the block's chunk; Pn[b] = ||w chunk||^2.
+// Block 0 writes (pass 1) or adds (pass 2) h into the Hessenberg column.
+template <typename T>
+__global__ __launch_bounds__(kGB) void k_cgs_update(const T* Q, int64_t N, int kc, T* w,
+                                                    const double* P, double* Hcol, int accumulate,
+                                                    double* Pn, const int* skip, const double* Pw,
+                                                    const double* Pn1, int* reorth_count) {
+  if (gm_skip(skip)) return;
+  if (Pw && !gm_need_reorth(Pw, Pn1, gridDim.x)) return;
+  __shared__ double hs[kGmresMaxCols + 1];
+  __shared__ double sh[kGB / 64];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  gm_reduce_rows(P, kc, gridDim.x, hs);
+  if (blockIdx.x == 0) {
+    for (int i = threadIdx.x; i < kc; i += kGB) Hcol[i] = accumulate ? Hcol[i] + hs[i] : hs[i];
+    if (reorth_count && threadIdx.x == 0) *reorth_count += 1;
+  }
+  const int64_t chunk = (N + gridDim.x - 1) / gridDim.x;
+  const int64_t lo = blockIdx.x * chunk, hi = min(N, lo + chunk);
+  double acc = 0.0;
   for (int64_t j = lo + threadIdx.x; j < hi; j += kGB) {
     double v = (double)w[j];
-    for (int i = 0; i < kc; ++i) v -= h[i] * (double)Q[(int64_t)i * N + j];
+    for (int i = 0; i < kc; ++i) v -= hs[i] * (double)Q[(int64_t)i * N + j];
     w[j] = (T)v;
+    acc += v * v;
   }
-  if (mode == 0) {
-    for (int i0 = 0; i0 < kc; i0 += 8) {
-      double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-      for (int64_t j = lo + threadIdx.x; j < hi; j += kGB) {
-        const double wj = (double)w[j];
-#pragma unroll
-        for (int u = 0; u < 8; ++u)
-          if (i0 + u < kc) acc[u] += (double)Q[(int64_t)(i0 + u) * N + j] * wj;
-      }
-#pragma unroll
-      for (int u = 0; u < 8; ++u) {
-        double v = gm_wave_sum(acc[u]);
-        if (lane == 0 && i0 + u < kc) sh[wv][i0 + u] = v;
-      }
-    }
-    __syncthreads();
-    for (int i = threadIdx.x; i < kc; i += kGB)
-      partial_out[blockIdx.x * ldp + i] = sh[0][i] + sh[1][i] + sh[2][i] + sh[3][i];
-  } else {
-    double acc = 0.0;
-    for (int64_t j = lo + threadIdx.x; j < hi; j += kGB) {
-      double v = (double)w[j];
-      acc += v * v;
-    }
-    acc = gm_wave_sum(acc);
-    if (lane == 0) sh[wv][0] = acc;
-    __syncthreads();
-    if (threadIdx.x == 0) partial_out[blockIdx.x * ldp] = sh[0][0] + sh[1][0] + sh[2][0] + sh[3][0];
+  acc = gm_wave_sum(acc);
+  if (lane == 0) sh[wv] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s = 0.0;
+    for (int i = 0; i < kGB / 64; ++i) s += sh[i];
+    Pn[blockIdx.x] = s;
   }
 }
 
@@ -122,12 +130,14 @@ __global__ __launch_bounds__(kGB) void k_cgs_update(const T* Q, int64_t N, int k
 // rotations (reference gmres.h:20-31, 106-129) and tests convergence.
 template <typename T>
 __global__ __launch_bounds__(kGB) void k_gmres_close(const T* w, T* qn, int64_t N,
-                                                     const double* partial, int nb, int ldp,
-                                                     GmresDev g, int k, const int* done_in) {
+                                                     const double* Pw, const double* Pn1,
+                                                     const double* Pn2, GmresDev g, int k,
+                                                     const int* done_in) {
   if (*done_in) return;
-  double s = 0.0;
-  for (int b = 0; b < nb; ++b) s += partial[b * ldp];
-  const double hn = sqrt(s);
+  __shared__ double nn[2];
+  const bool reorth = gm_need_reorth(Pw, Pn1, gridDim.x);
+  gm_reduce_rows(reorth ? Pn2 : Pn1, 1, gridDim.x, nn);
+  const double hn = sqrt(nn[0]);
   const double inv = (hn > 0.0) ? 1.0 / hn : 0.0;
   for (int64_t j = blockIdx.x * (int64_t)kGB + threadIdx.x; j < N; j += (int64_t)gridDim.x * kGB)
     qn[j] = (T)((double)w[j] * inv);
@@ -193,6 +203,7 @@ __global__ void k_gmres_setup(GmresDev g, const double* norm_rhs, const double* 
   g.istate[GM_ITERS] = 0;
   g.istate[GM_DONE] = 0;
   g.istate[GM_KCOLS] = 0;
+  g.istate[GM_REORTHS] = 0;
 }
 
 // y = H(0:kc,0:kc) \ beta(0:kc) (upper triangular), one thread.
@@ -234,9 +245,9 @@ GmresDev gmres_alloc(Arena& ws, int m, int64_t N, int esize) {
   g.cs = ws.alloc<double>(m);
   g.sn = ws.alloc<double>(m);
   g.y = ws.alloc<double>(m);
-  g.dstate = ws.alloc<double>(8);
-  g.istate = ws.alloc<int>(8);
-  g.partial = ws.alloc<double>((size_t)3 * kGmresMaxBlocks * kGmresMaxCols);
+  g.dstate = ws.alloc<double>(16);
+  g.istate = ws.alloc<int>(16);
+  g.partial = ws.alloc<double>((size_t)kGmresMaxBlocks * (2 * kGmresMaxCols + 4));
   return g;
 }
 
@@ -249,11 +260,11 @@ GmresResult gmres_run(GmresDev g, int64_t N, T* x, const T* rhs, const double* n
   T* Q = (T*)g.Q;
   T* w = (T*)g.w;
   const int nb = gm_blocks(N);
-  const int ldp = kGmresMaxCols;
-  double* P1 = g.partial;
-  double* P2 = g.partial + (size_t)kGmresMaxBlocks * kGmresMaxCols;
-  double* P3 = g.partial + (size_t)2 * kGmresMaxBlocks * kGmresMaxCols;
   const int* done = g.istate + GM_DONE;
+  double* P1 = g.partial;                                        // (kc + 1) rows
+  double* P2 = P1 + (size_t)kGmresMaxBlocks * (kGmresMaxCols + 1);  // kc rows
+  double* Pn1 = P2 + (size_t)kGmresMaxBlocks * kGmresMaxCols;
+  double* Pn2 = Pn1 + kGmresMaxBlocks;
   GmresResult res;
 
   hipLaunchKernelGGL(k_gmres_setup, dim3(1), dim3(1), 0, st, g, norm_rhs, dnorm, real_tol);
@@ -278,31 +289,34 @@ GmresResult gmres_run(GmresDev g, int64_t N, T* x, const T* rhs, const double* n
       T* qn = Q + (int64_t)(k + 1) * N;
       op.apply(qk, w, done);
       double* Hcol = g.H + (int64_t)k * (g.m + 1);
-      hipLaunchKernelGGL((k_cgs_dots<T>), dim3(nb), dim3(kGB), 0, st, Q, N, k + 1, w, P1, ldp,
-                         done);
-      hipLaunchKernelGGL((k_cgs_update<T>), dim3(nb), dim3(kGB), 0, st, Q, N, k + 1, w, P1, nb,
-                         ldp, P2, 0, Hcol, 0, done);
-      hipLaunchKernelGGL((k_cgs_update<T>), dim3(nb), dim3(kGB), 0, st, Q, N, k + 1, w, P2, nb,
-                         ldp, P3, 1, Hcol, 1, done);
-      hipLaunchKernelGGL((k_gmres_close<T>), dim3(nb), dim3(kGB), 0, st, w, qn, N, P3, nb, ldp, g,
-                         k, done);
+      const double* Pw = P1 + (size_t)(k + 1) * kGmresMaxBlocks;  // row kc of P1: w.w
+      // pass 1: project out the basis
+      hipLaunchKernelGGL((k_cgs_dots<T>), dim3(nb), dim3(kGB), 0, st, Q, N, k + 1, w, P1, 1, done,
+                         (const double*)nullptr, (const double*)nullptr);
+      hipLaunchKernelGGL((k_cgs_update<T>), dim3(nb), dim3(kGB), 0, st, Q, N, k + 1, w, P1, Hcol, 0,
+                         Pn1, done, (const double*)nullptr, (const double*)nullptr, (int*)nullptr);
+      // pass 2 (re-orthogonalisation): each kernel re-derives the decision on the device
+      hipLaunchKernelGGL((k_cgs_dots<T>), dim3(nb), dim3(kGB), 0, st, Q, N, k + 1, w, P2, 0, done,
+                         Pw, (const double*)Pn1);
+      hipLaunchKernelGGL((k_cgs_update<T>), dim3(nb), dim3(kGB), 0, st, Q, N, k + 1, w, P2, Hcol, 1,
+                         Pn2, done, Pw, (const double*)Pn1, g.istate + GM_REORTHS);
+      hipLaunchKernelGGL((k_gmres_close<T>), dim3(nb), dim3(kGB), 0, st, w, qn, N, Pw,
+                         (const double*)Pn1, (const double*)Pn2, g, k, done);
       TT_LAUNCH_CHECK();
       // Read the flag of the previous iteration while this one runs.
       hs.post(done, k, st);
       if (k >= 1 && hs.wait(k - 1)) stop = true;
     }
     // Flag of the last enqueued iteration.
-    bool converged_or_done = hs.wait(k - 1);
-    (void)converged_or_done;
+    (void)hs.wait(k - 1);
     hipLaunchKernelGGL(k_gmres_y, dim3(1), dim3(1), 0, st, g);
     hipLaunchKernelGGL((k_gmres_xupd<T>), dim3(nb), dim3(kGB), 0, st, Q, N, x, g);
     TT_LAUNCH_CHECK();
     // Did this cycle converge?
-    int flag = 0;
     TT_HIP(hipMemcpyAsync(hs.host_int(), g.istate + GM_FLAG, sizeof(int), hipMemcpyDeviceToHost,
                           st));
     TT_HIP(hipStreamSynchronize(st));
-    flag = *hs.host_int();
+    const int flag = *hs.host_int();
     res.cycles = cyc + 1;
     if (flag) break;
   }
@@ -310,6 +324,7 @@ GmresResult gmres_run(GmresDev g, int64_t N, T* x, const T* rhs, const double* n
   TT_HIP(hipStreamSynchronize(st));
   res.flag = hs.host_int()[GM_FLAG];
   res.iters = hs.host_int()[GM_ITERS];
+  res.reorths = hs.host_int()[GM_REORTHS];
   return res;
 }
 

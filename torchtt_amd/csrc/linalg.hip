@@ -14,8 +14,6 @@
 //   one workgroup with the matrix resident in LDS, round-robin parallel ordering,
 //   Rutishauser's rotation formula. The rotations are logged and applied to the
 //   identity by a second, row-parallel kernel to form V.
-// LU: Gaussian elimination with partial pivoting in one workgroup (matrices of
-//   order < max_full, 500 by default).
 #include "ttamen_internal.h"
 #include <algorithm>
 #include <cmath>
@@ -24,31 +22,56 @@ namespace ttamen {
 
 constexpr double kUnitRoundoff = 1.1102230246251565e-16;  // 2^-53
 constexpr int kJacobiMaxSweeps = 48;
+constexpr int kQrPasses = 4;
 
 // ---------------------------------------------------------------- Cholesky
 // G (n x n, row-major, symmetric) + shift*I = R^T R, R upper triangular written
-// row-major to Rout. shift = shift_c * u * trace(G) when shift_c > 0.
-// A pivot <= dead_rel * max(diag) marks the column dead: R[j][j] = 1 and the rest
-// of row j is zero. dead[j] records it and *ndead counts them.
+// row-major to Rout.
+//   * If skip_in && *skip_in: an earlier pass already finished; only propagate the flag.
+//   * delta = max |G - I|. If delta < kNearIdentity the input is already orthonormal to
+//     that level: no shift, and this pass is the last (*final_out = 1).
+//   * Otherwise shift = shift_c * u * trace(G) when shift_c > 0 (shifted Cholesky-QR).
+//   * A pivot <= dead_rel * max(diag) marks the column dead: R[j][j] = 1 and the rest of
+//     row j is zero; dead[j] records it and *ndead counts them.
+constexpr double kNearIdentity = 1e-6;
+
 __global__ __launch_bounds__(1024) void k_cholesky(const double* G, int n, double shift_c,
-                                                   double dead_rel, double* Rout, int* dead,
-                                                   int* ndead) {
+                                                   double dead_rel, int last_pass, double* Rout,
+                                                   int* dead, int* ndead, const int* skip_in,
+                                                   int* final_out) {
   extern __shared__ double W[];
-  __shared__ double s_piv, s_maxd, s_shift;
-  __shared__ int s_dead, s_nd;
+  __shared__ double s_piv, s_maxd, s_shift, s_red[16];
+  __shared__ int s_dead, s_nd, s_final;
   const int tid = threadIdx.x, nt = blockDim.x;
-  for (int t = tid; t < n * n; t += nt) W[t] = G[t];
+  if (skip_in && *skip_in) {
+    if (tid == 0) *final_out = 1;
+    return;
+  }
+  double dl = 0.0;
+  for (int t = tid; t < n * n; t += nt) {
+    double g = G[t];
+    W[t] = g;
+    int a = t / n, c = t - a * n;
+    dl = fmax(dl, fabs(g - (a == c ? 1.0 : 0.0)));
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) dl = fmax(dl, __shfl_xor(dl, o, 64));
+  if ((tid & 63) == 0) s_red[tid >> 6] = dl;
   __syncthreads();
   if (tid == 0) {
+    double delta = 0.0;
+    for (int w = 0; w < (nt >> 6); ++w) delta = fmax(delta, s_red[w]);
     double tr = 0.0, md = 0.0;
     for (int j = 0; j < n; ++j) {
       tr += W[j * n + j];
       md = fmax(md, W[j * n + j]);
     }
-    double sh = (shift_c > 0.0) ? shift_c * kUnitRoundoff * tr : 0.0;
+    const int near_id = delta < kNearIdentity;
+    double sh = (shift_c > 0.0 && !near_id) ? shift_c * kUnitRoundoff * tr : 0.0;
     s_shift = sh;
     s_maxd = md + sh;
     s_nd = 0;
+    s_final = near_id || last_pass;
   }
   __syncthreads();
   for (int j = tid; j < n; j += nt) W[j * n + j] += s_shift;
@@ -82,7 +105,10 @@ __global__ __launch_bounds__(1024) void k_cholesky(const double* G, int n, doubl
     int a = t / n, c = t % n;
     Rout[t] = (a <= c) ? W[t] : 0.0;
   }
-  if (tid == 0) *ndead = s_nd;
+  if (tid == 0) {
+    *ndead = s_nd;
+    *final_out = s_final;
+  }
 }
 
 // ---------------------------------------------------------------- triangular solve
@@ -93,8 +119,10 @@ __device__ __forceinline__ int packed_idx(int i, int j, int n) {
   return i * n - (i * (i - 1)) / 2 + (j - i);
 }
 
-__global__ __launch_bounds__(256) void k_trsm_rows(const double* A, int64_t sr, int64_t sc,
-                                                   int64_t m, int n, const double* R, double* X) {
+__global__ __launch_bounds__(64) void k_trsm_rows(const double* A, int64_t sr, int64_t sc,
+                                                   int64_t m, int n, const double* R, double* X,
+                                                   const int* skip) {
+  if (skip && *skip) return;
   extern __shared__ double P[];
   for (int t = threadIdx.x; t < n * n; t += blockDim.x) {
     int i = t / n, j = t % n;
@@ -206,27 +234,35 @@ int qr(const double* A, int64_t sr, int64_t sc, int64_t m, int64_t n, double* Q,
   double* Rk = ws.alloc<double>(n * n);
   int* dead = ws.alloc<int>(n + 1);
   int* ndead = dead + n;
+  int* flags = ws.alloc<int>(kQrPasses + 1);  // flags[p]: a pass before p was the last one
   const size_t chol_lds = (size_t)n * n * sizeof(double);
   const size_t trsm_lds = (size_t)n * (n + 1) / 2 * sizeof(double);
-  const int trsm_blocks = (int)((m + 255) / 256);
+  const int trsm_blocks = (int)((m + 63) / 64);
 
-  for (int pass = 0; pass < 3; ++pass) {
+  // Passes: two shifted Cholesky-QR passes (each shrinks the condition number by about
+  // sqrt(11 (m n + n (n+1)) u)), then plain passes; a pass whose input Gram matrix is
+  // within 1e-6 of the identity is unshifted and the last one. Later passes see the flag
+  // and return at once.
+  for (int pass = 0; pass < kQrPasses; ++pass) {
     const double* src = (pass == 0) ? A : Q;
     int64_t ssr = (pass == 0) ? sr : n, ssc = (pass == 0) ? sc : 1;
+    const int* skip = (pass == 0) ? nullptr : flags + pass;
     GemmDesc g;  // G = src^T src
     g.M = (int)n; g.N = (int)n; g.K = (int)m;
     g.a.p = src; g.a.r = ssc; g.a.c = ssr;
     g.b.p = src; g.b.r = ssr; g.b.c = ssc;
     g.c.p = G; g.c.r = n; g.c.c = 1;
+    g.skip = skip;
     gemm_f64(g, ws, st);
     if (!ws.measuring()) {
-      const double shift_c = (pass == 0) ? 11.0 * ((double)m * n + (double)n * (n + 1)) : 0.0;
-      const double dead_rel = (pass == 0) ? 0.0 : 1e-12;
+      const double shift_c = (pass < 2) ? 11.0 * ((double)m * n + (double)n * (n + 1)) : 0.0;
+      const double dead_rel = (pass < 2) ? 0.0 : 1e-26;
       hipLaunchKernelGGL(k_cholesky, dim3(1), dim3(1024), chol_lds, st, G, (int)n, shift_c,
-                         dead_rel, Rk, dead, ndead);
+                         dead_rel, (int)(pass == kQrPasses - 1), Rk, dead, ndead, skip,
+                         flags + pass + 1);
       TT_LAUNCH_CHECK();
-      hipLaunchKernelGGL(k_trsm_rows, dim3(trsm_blocks), dim3(256), trsm_lds, st, src, ssr, ssc,
-                         m, (int)n, Rk, Q);
+      hipLaunchKernelGGL(k_trsm_rows, dim3(trsm_blocks), dim3(64), trsm_lds, st, src, ssr, ssc,
+                         m, (int)n, Rk, Q, skip);
       TT_LAUNCH_CHECK();
     }
   }
@@ -481,103 +517,35 @@ int svd(const double* Mx, int64_t sr, int64_t sc, int64_t m, int64_t n, double* 
   return (int)k;
 }
 
-// ---------------------------------------------------------------- LU solve
-__device__ __forceinline__ void argmax_abs_1024(double v, int idx, double* shv, int* shi,
-                                                double& outv, int& outi) {
-  // wave reduction
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) {
-    double ov = __shfl_xor(v, o, 64);
-    int oi = __shfl_xor(idx, o, 64);
-    if (ov > v || (ov == v && oi < idx)) { v = ov; idx = oi; }
-  }
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
-  if (lane == 0) { shv[w] = v; shi[w] = idx; }
-  __syncthreads();
-  double bv = shv[0];
-  int bi = shi[0];
-  for (int i = 1; i < nw; ++i)
-    if (shv[i] > bv || (shv[i] == bv && shi[i] < bi)) { bv = shv[i]; bi = shi[i]; }
-  outv = bv;
-  outi = bi;
-  __syncthreads();
-}
-
-__global__ __launch_bounds__(1024) void k_lu_solve(double* B, double* rhs, int n) {
-  __shared__ double shv[16];
-  __shared__ int shi[16];
-  __shared__ double s_x;
-  const int tid = threadIdx.x, nt = blockDim.x;
-  for (int j = 0; j < n; ++j) {
-    double v = -1.0;
-    int idx = n;
-    for (int i = j + tid; i < n; i += nt) {
-      double a = fabs(B[(int64_t)i * n + j]);
-      if (a > v || (a == v && i < idx)) { v = a; idx = i; }
-    }
-    double pv;
-    int p;
-    argmax_abs_1024(v, idx, shv, shi, pv, p);
-    if (p != j && p < n) {
-      for (int c = j + tid; c < n; c += nt) {
-        double t = B[(int64_t)j * n + c];
-        B[(int64_t)j * n + c] = B[(int64_t)p * n + c];
-        B[(int64_t)p * n + c] = t;
-      }
-      if (tid == 0) {
-        double t = rhs[j];
-        rhs[j] = rhs[p];
-        rhs[p] = t;
-      }
-    }
-    __syncthreads();
-    const double piv = B[(int64_t)j * n + j];
-    const int rem = n - j - 1;
-    // multipliers stored in column j, then trailing update and rhs
-    for (int i = j + 1 + tid; i < n; i += nt) B[(int64_t)i * n + j] /= piv;
-    __syncthreads();
-    for (int64_t t = tid; t < (int64_t)rem * (rem + 1); t += nt) {
-      int i = j + 1 + (int)(t / (rem + 1));
-      int cc = (int)(t % (rem + 1));
-      double l = B[(int64_t)i * n + j];
-      if (cc < rem) {
-        int c = j + 1 + cc;
-        B[(int64_t)i * n + c] -= l * B[(int64_t)j * n + c];
-      } else {
-        rhs[i] -= l * rhs[j];
-      }
-    }
-    __syncthreads();
-  }
-  // back substitution, column oriented
-  for (int j = n - 1; j >= 0; --j) {
-    if (tid == 0) {
-      s_x = rhs[j] / B[(int64_t)j * n + j];
-      rhs[j] = s_x;
-    }
-    __syncthreads();
-    const double xj = s_x;
-    for (int i = tid; i < j; i += nt) rhs[i] -= B[(int64_t)i * n + j] * xj;
-    __syncthreads();
-  }
-}
-
-void lu_solve(double* B, double* rhs, int n, Arena& ws, hipStream_t st) {
-  (void)ws;
-  if (n <= 0 || ws.measuring()) return;
-  hipLaunchKernelGGL(k_lu_solve, dim3(1), dim3(1024), 0, st, B, rhs, n);
-  TT_LAUNCH_CHECK();
-}
-
 // Raise the dynamic-LDS limit of the kernels that stage a full block in LDS.
+// The limit is the device's per-block LDS minus the kernel's own static LDS.
+static void raise_lds_limit(const void* fn, int dev_max) {
+  hipFuncAttributes fa;
+  TT_HIP(hipFuncGetAttributes(&fa, fn));
+  int lim = dev_max - (int)fa.sharedSizeBytes;
+  if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, lim) != hipSuccess) {
+    (void)hipGetLastError();  // keep the error out of later checks; launches report it
+  }
+}
+
 void linalg_init() {
-  static bool done = false;
-  if (done) return;
-  const int lim = 160 * 1024;
-  TT_HIP(hipFuncSetAttribute((const void*)k_cholesky, hipFuncAttributeMaxDynamicSharedMemorySize, lim));
-  TT_HIP(hipFuncSetAttribute((const void*)k_jacobi, hipFuncAttributeMaxDynamicSharedMemorySize, lim));
-  TT_HIP(hipFuncSetAttribute((const void*)k_trsm_rows, hipFuncAttributeMaxDynamicSharedMemorySize, lim));
-  done = true;
+  static int done_dev = -1;
+  int dev = 0;
+  TT_HIP(hipGetDevice(&dev));
+  if (done_dev == dev) return;
+  int dev_max = 0;
+  TT_HIP(hipDeviceGetAttribute(&dev_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+  raise_lds_limit((const void*)k_cholesky, dev_max);
+  raise_lds_limit((const void*)k_jacobi, dev_max);
+  raise_lds_limit((const void*)k_trsm_rows, dev_max);
+  done_dev = dev;
+}
+
+int lds_per_block_limit() {
+  int dev = 0, v = 0;
+  TT_HIP(hipGetDevice(&dev));
+  TT_HIP(hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+  return v;
 }
 
 }  // namespace ttamen

@@ -156,8 +156,9 @@ void cast_f32_f64(const float* x, double* y, int64_t n, hipStream_t st);
 void permute_core_A(const double* A, double* P, int s, int n, int S, hipStream_t st);
 void permute_core_A_f32(const double* A, float* P, int s, int n, int S, hipStream_t st);
 // Dense local operator B[(l,m,L),(r,n,R)] = sum_{s,S} PhiL[l,s,r] A[s,m,n,S] PhiR[L,S,R]
-void assemble_local(const double* PhiL, const double* A, const double* PhiR, double* B, int r1,
-                    int s, int n, int S, int r2, hipStream_t st);
+// (row-major with leading dimension ld >= r1*n*r2)
+void assemble_local(const double* PhiL, const double* A, const double* PhiR, double* B, int64_t ld,
+                    int r1, int s, int n, int S, int r2, hipStream_t st);
 
 // ---------------------------------------------------------------- scalar program
 // Tiny single-thread kernels for the sweep's norm bookkeeping.
@@ -180,8 +181,11 @@ int qr(const double* A, int64_t sr, int64_t sc, int64_t m, int64_t n, double* Q,
 // row-major, ld k), k = min(m, n). Singular values descending.
 int svd(const double* M, int64_t sr, int64_t sc, int64_t m, int64_t n, double* U, double* s,
         double* V, Arena& ws, hipStream_t st);
-// Solve B x = rhs for a dense n x n FP64 matrix (B is overwritten with its LU factors).
-void lu_solve(double* B, double* rhs, int n, Arena& ws, hipStream_t st);
+// Solve B x = rhs for the augmented n x (n+1) row-major matrix Aug = [B | rhs]
+// (overwritten) by blocked LU with partial pivoting; x receives the solution.
+void lu_solve_aug(double* Aug, int n, double* x, Arena& ws, hipStream_t st);
+void lu_init();
+int lu_max_order();
 
 // ---------------------------------------------------------------- local operator
 // Local AMEn operator, y = PhiL x A x PhiR applied to nvec vectors:
