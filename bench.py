@@ -1,80 +1,266 @@
-"""Benchmark of the AMEn solve on one MI355X.
+"""Benchmark of the AMEn linear solver on one MI355X.
 
-A step is one complete AMEn solve of the workload (all sweeps, to the solver's stopping
-rule) with the operator and right-hand side already resident in HBM. The default workload
-is the reference comparison case: d = 10, n = 64, A = Laplacian + potential (TT rank 4),
-b = random rank-4 TT with ||b|| = 1, rank cap 80, eps 1e-8, FP64.
+Workloads (--workload):
+  batch      (default) 256 independent anisotropic-diffusion systems: d ~ U{4..12},
+             n ~ U{8..64}, coefficients log-uniform in [0.1, 10] per mode, operator TT rank 2-4,
+             random rank-1..4 N(0,1) right-hand sides with ||b|| = 1, rank caps U{16..64},
+             eps 1e-8. A step is one pass over all 256 systems with the batched solver
+             (worker threads, one HIP stream each) in --precision (default fp32: FP32 local GMRES
+             on the matrix cores inside FP64 refinement). The plain FP64 solver, one system at a
+             time, is measured on a sample of the same systems for the comparison.
+  potential  one system: d=10, n=64, Laplacian + potential (TT rank 4), random rank-4 rhs,
+             rank cap 80, eps 1e-8; a step is one solve.
+  serve      Poisson arrivals at --rate requests/s of systems drawn as in `batch`, for
+             --duration seconds, served by the continuous-batching loop; reports p50/p99
+             latency (completion - arrival) and solves/s.
 
-Prints one JSON line with the throughput (solves/s), p50/p99 latency, the worst relative
-residual ||Ax - b|| / ||b|| of the timed solves, the roofline of the local operator (the
-solver's dominant kernel work, timed with HIP events inside the library on its stream) and
-the reference CPU solver's throughput on the same workload measured on this host.
-
-    python bench.py [--steps K] [--warmup W] [--precision fp64|fp32] [--no-cpu-baseline]
+`value` counts only solves whose relative residual ||Ax - b|| / ||b|| is <= 1e-8 (checked
+for every result in FP64 TT arithmetic), divided by the wall time of the timed steps, with
+inputs resident in HBM. The roofline is that of the local operator, timed with HIP events
+inside the library on the solver's streams (FP32 applications inside GMRES for fp32, FP64
+otherwise). The CPU baseline is the reference's own C++ solver on this host's cores.
 """
-import argparse
-import ctypes
-import glob
-import importlib.util
-import json
 import os
-import sys
-import time
 
-import numpy as np
-import torch
+os.environ.setdefault("GPU_MAX_HW_QUEUES", "16")  # concurrent streams of the batched solver
+
+import argparse  # noqa: E402
+import ctypes  # noqa: E402
+import glob  # noqa: E402
+import importlib.util  # noqa: E402
+import json  # noqa: E402
+import sys  # noqa: E402
+import time  # noqa: E402
+
+import numpy as np  # noqa: E402
+import torch  # noqa: E402
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "solves/sec at rel. residual ≤1e-8, mixed vs FP64 plain; p50/p99 solve latency"
 PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3}
-HBM_GBS = 8000.0
+RES_BAR = 1e-8
 
 
 def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
-    p.add_argument("--steps", type=int, default=5)
+    p.add_argument("--steps", type=int, default=1)
     p.add_argument("--warmup", type=int, default=1)
-    p.add_argument("--precision", default="fp64", choices=["fp64", "fp32"])
-    p.add_argument("--d", type=int, default=10)
-    p.add_argument("--n", type=int, default=64)
-    p.add_argument("--rmax", type=int, default=80)
+    p.add_argument("--workload", default="batch", choices=["batch", "potential", "serve"])
+    p.add_argument("--precision", default=None, choices=["fp64", "fp32"])
+    p.add_argument("--systems", type=int, default=256)
+    p.add_argument("--workers", type=int, default=8)
+    p.add_argument("--plain-sample", type=int, default=32)
     p.add_argument("--eps", type=float, default=1e-8)
+    p.add_argument("--rate", type=float, default=200.0)
+    p.add_argument("--duration", type=float, default=2.0)
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-threads", type=int, default=0)
-    return p.parse_args()
+    a = p.parse_args()
+    if a.precision is None:
+        a.precision = "fp64" if a.workload == "potential" else "fp32"
+    return a
 
 
-def cpu_baseline(A, b, args):
+def load_ref():
+    so = glob.glob(os.path.join(ROOT, "oracle", "_ref", "torchttcpp*.so"))
+    if not so:
+        return None
+    spec = importlib.util.spec_from_file_location("torchttcpp", so[0])
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def cpu_baseline(items, eps, args, budget_s=25.0):
     """The reference's own C++ AMEn (oracle/_ref, built from its sources) on this host's cores,
-    one full solve of the same workload; the CPU oracle restatement if that build is absent."""
+    solving the given (A, b, rmax) CPU systems one after another until `budget_s` is spent;
+    the CPU oracle restatement if that build is absent."""
     threads = args.cpu_threads or min(16, os.cpu_count() or 1)
     torch.set_num_threads(threads)
-    d = len(A)
-    so = glob.glob(os.path.join(ROOT, "oracle", "_ref", "torchttcpp*.so"))
-    rA = [1] + [a.shape[3] for a in A]
-    rb = [1] + [c.shape[2] for c in b]
-    torch.manual_seed(0)
-    if so:
-        spec = importlib.util.spec_from_file_location("torchttcpp", so[0])
-        ref = importlib.util.module_from_spec(spec)
-        spec.loader.exec_module(ref)
-        t0 = time.perf_counter()
-        ref.amen_solve(A, b, [], [a.shape[2] for a in A], rA, rb, [1] * (d + 1), 22, args.eps, args.rmax, 500,
-                       4, 0, 40, 2, False, 0)
-        dt = time.perf_counter() - t0
-        kind = "reference"
+    ref = load_ref()
+    done, t0 = 0, time.perf_counter()
+    for A, b, rmax in items:
+        d = len(A)
+        rA = [1] + [a.shape[3] for a in A]
+        rb = [1] + [c.shape[2] for c in b]
+        torch.manual_seed(0)
+        if ref is not None:
+            ref.amen_solve(A, b, [], [a.shape[2] for a in A], rA, rb, [1] * (d + 1), 22, eps, rmax, 500,
+                           4, 0, 40, 2, False, 0)
+        else:
+            sys.path.insert(0, os.path.join(ROOT, "oracle"))
+            import amen_oracle as O
+            O.amen_solve([a.numpy() for a in A], [c.numpy() for c in b], eps=eps, rmax=rmax)
+        done += 1
+        if time.perf_counter() - t0 > budget_s:
+            break
+    dt = time.perf_counter() - t0
+    return {"value": done / dt, "unit": "solves/s", "cores": threads,
+            "kind": "reference" if ref is not None else "port",
+            "sample": "first %d system(s) of the workload solved one after another (%.1f s)" % (done, dt)}
+
+
+def roofline(prof, precision):
+    n64, ms64, fl64, by64, n32, ms32, fl32, by32 = list(prof)
+    if precision == "fp32" and n32 > 0:
+        n, ms, fl, by, peak = n32, ms32, fl32, by32, PEAK_TFLOPS["fp32"]
+        kern = "local matvec FP32 (3 MFMA GEMM launches per application), Arnoldi steps that ran"
     else:
-        sys.path.insert(0, os.path.join(ROOT, "oracle"))
-        import amen_oracle as O
+        n, ms, fl, by, peak = n64, ms64, fl64, by64, PEAK_TFLOPS["fp64"]
+        kern = "local matvec FP64 (3 MFMA GEMM launches per application)"
+    avg_s = (ms / 1e3) / max(n, 1)
+    ach = (fl / max(n, 1)) / avg_s / 1e12 if n else 0.0
+    return {"kernel": kern, "bound": "mfma", "achieved": ach, "peak": peak, "unit": "TFLOP/s",
+            "frac": ach / peak, "traffic": None, "applications": int(n), "avg_us": avg_s * 1e6,
+            "flops_per_application": fl / max(n, 1), "compulsory_bytes_per_application": by / max(n, 1)}
+
+
+def base_record(args, value, total, config, dtype):
+    return {"metric": METRIC, "value": value, "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": total / max(args.steps, 1) * 1e3, "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": dtype, "data": "synthetic", "config": config}
+
+
+def dtype_of(p):
+    return "f64" if p == "fp64" else "f32-inner+f64-refinement"
+
+
+def run_batch(args, tt, W, lib, _lib):
+    dev = torch.device("cuda", 0)
+    systems = W.batch_systems(args.systems, seed=0)
+    items = [(tt.TT([c.to(dev) for c in s["A"]]), tt.TT([c.to(dev) for c in s["b"]]), s["rmax"]) for s in systems]
+    torch.manual_seed(0)
+    z0 = [tt.solvers._draw_z(it[1].N, 4, 0, "cpu") for it in items]
+    kw = dict(eps=args.eps, workers=args.workers)
+    for _ in range(args.warmup):
+        tt.solvers.amen_solve_batch(items[:16], precision=args.precision, z0=z0[:16], **kw)
+    torch.cuda.synchronize()
+    _lib.check("profile_enable", lib.ttamen_profile_enable(1))
+    t0 = time.perf_counter()
+    sols = None
+    for _ in range(args.steps):
+        sols = tt.solvers.amen_solve_batch(items, precision=args.precision, z0=z0, **kw)
+    torch.cuda.synchronize()
+    total = time.perf_counter() - t0
+    prof = (ctypes.c_double * 8)()
+    _lib.check("profile_read", lib.ttamen_profile_read(prof))
+    lib.ttamen_profile_enable(0)
+    stats = tt.solvers.last_batch_stats
+    res = np.array([tt.solvers.residual(it[0], x, it[1]) for it, x in zip(items, sols)])
+    ok = int(np.sum(res <= RES_BAR))
+    lat_ms = np.array([s["elapsed_s"] for s in stats]) * 1e3
+    value = ok * args.steps / total
+
+    # plain FP64 solver, one system at a time, on a sample of the same systems
+    ns = min(args.plain_sample, len(items))
+    t1 = time.perf_counter()
+    plain = [tt.solvers.amen_solve(A, b, rmax=rm, eps=args.eps, _z0=z) for (A, b, rm), z in zip(items[:ns], z0[:ns])]
+    torch.cuda.synchronize()
+    tplain = time.perf_counter() - t1
+    res_plain = np.array([tt.solvers.residual(it[0], x, it[1]) for it, x in zip(items[:ns], plain)])
+    ok_plain = int(np.sum(res_plain <= RES_BAR))
+    # same sample in the timed batched run: error of the batched result against the plain FP64 one
+    err = [float((x - y).norm() / y.norm()) for x, y in zip(sols[:ns], plain)]
+    ok_sample = int(np.sum(res[:ns] <= RES_BAR))
+
+    cfg = {"workload": "batched AMEn: %d anisotropic diffusion systems, d~U{4..12}, n~U{8..64}, op rank 2-4, "
+                       "rhs rank 1-4, rank caps U{16..64}, eps %g, %s, %d workers"
+                       % (len(items), args.eps, args.precision, args.workers),
+           "systems": len(items), "precision": args.precision, "workers": args.workers, "eps": args.eps}
+    out = base_record(args, value, total, cfg, dtype_of(args.precision))
+    out.update({
+        "systems_per_s_all": len(items) * args.steps / total,
+        "solved_le_1e-8": ok, "systems": len(items),
+        "p50_ms": float(np.percentile(lat_ms, 50)), "p99_ms": float(np.percentile(lat_ms, 99)),
+        "latency_note": "per-system solve time inside the batch (admission at t=0)",
+        "residual_max_of_solved": float(res[res <= RES_BAR].max()) if ok else None,
+        "plain_fp64_sequential": {"sample_systems": ns, "time_s": tplain,
+                                  "solves_per_s": ok_plain / tplain if ns else None,
+                                  "systems_per_s": ns / tplain if ns else None,
+                                  "solved_le_1e-8": ok_plain},
+        "speedup_vs_plain_fp64_sequential": value / (ok_plain / tplain) if ok_plain else None,
+        "same_sample_batched_solved_le_1e-8": ok_sample,
+        "error_vs_plain_fp64_max": max(err) if err else None,
+        "error_vs_plain_fp64_median": float(np.median(err)) if err else None,
+        "roofline": roofline(prof, args.precision),
+    })
+    if not args.no_cpu_baseline:
+        cpu_items = [(s["A"], s["b"], s["rmax"]) for s in systems]
+        out["cpu_baseline"] = cpu_baseline(cpu_items, args.eps, args)
+    return out
+
+
+def run_potential(args, tt, W, lib, _lib):
+    dev = torch.device("cuda", 0)
+    A, b = W.laplace_potential(10, 64, seed=0)
+    Ag = tt.TT([c.to(dev) for c in A])
+    bg = tt.TT([c.to(dev) for c in b])
+    for _ in range(args.warmup):
+        torch.manual_seed(0)
+        tt.solvers.amen_solve(Ag, bg, eps=args.eps, rmax=80, precision=args.precision)
+    torch.cuda.synchronize()
+    _lib.check("profile_enable", lib.ttamen_profile_enable(1))
+    lat, sols, stats = [], [], []
+    t_start = time.perf_counter()
+    for _ in range(args.steps):
+        torch.manual_seed(0)
         t0 = time.perf_counter()
-        O.amen_solve([a.numpy() for a in A], [c.numpy() for c in b], eps=args.eps, rmax=args.rmax)
-        dt = time.perf_counter() - t0
-        kind = "port"
-    return {"value": 1.0 / dt, "unit": "solves/s", "cores": threads, "kind": kind,
-            "sample": "1 full solve of the same workload (%.2f s)" % dt}
+        x = tt.solvers.amen_solve(Ag, bg, eps=args.eps, rmax=80, precision=args.precision)
+        torch.cuda.synchronize()
+        lat.append(time.perf_counter() - t0)
+        sols.append(x)
+        stats.append(dict(tt.solvers.last_stats))
+    torch.cuda.synchronize()
+    total = time.perf_counter() - t_start
+    prof = (ctypes.c_double * 8)()
+    _lib.check("profile_read", lib.ttamen_profile_read(prof))
+    lib.ttamen_profile_enable(0)
+    res = np.array([tt.solvers.residual(Ag, x, bg) for x in sols])
+    ok = int(np.sum(res <= RES_BAR))
+    lat_ms = np.array(lat) * 1e3
+    cfg = {"workload": "AMEn solve, Laplacian + potential, d=10, n=64, TT rank 4, random rank-4 rhs, "
+                       "rank cap 80, eps %g, %s" % (args.eps, args.precision),
+           "d": 10, "n": 64, "rmax": 80, "eps": args.eps, "precision": args.precision}
+    out = base_record(args, ok / total, total, cfg, dtype_of(args.precision))
+    out.update({"p50_ms": float(np.percentile(lat_ms, 50)), "p99_ms": float(np.percentile(lat_ms, 99)),
+                "residual_max": float(res.max()), "solved_le_1e-8": ok, "sweeps": stats[-1]["sweeps"],
+                "gmres_iters": stats[-1]["gmres_iters"], "max_rank": stats[-1]["max_rank"],
+                "roofline": roofline(prof, args.precision)})
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline([(A, b, 80)], args.eps, args, budget_s=0.0)
+    return out
+
+
+def run_serve(args, tt, W, lib, _lib):
+    dev = torch.device("cuda", 0)
+    arrivals = tt.serving.poisson_arrivals(args.rate, args.duration, seed=1)
+    systems = W.batch_systems(len(arrivals), seed=7)
+    items = [(tt.TT([c.to(dev) for c in s["A"]]), tt.TT([c.to(dev) for c in s["b"]]), s["rmax"]) for s in systems]
+    torch.manual_seed(0)
+    z0 = [tt.solvers._draw_z(it[1].N, 4, 0, "cpu") for it in items]
+    if args.warmup:
+        tt.solvers.amen_solve_batch(items[:8], eps=args.eps, precision=args.precision, z0=z0[:8], workers=4)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    sols, info = tt.serving.serve(items, arrivals, slots=args.workers, eps=args.eps, precision=args.precision,
+                                  z0=z0)
+    torch.cuda.synchronize()
+    total = time.perf_counter() - t0
+    res = np.array([tt.solvers.residual(it[0], x, it[1]) for it, x in zip(items, sols)])
+    ok = int(np.sum(res <= RES_BAR))
+    lat = info["latency_s"] * 1e3
+    cfg = {"workload": "served AMEn: Poisson arrivals at %g req/s for %g s (%d requests) of systems drawn as in "
+                       "the batched workload, %d slots, %s, eps %g"
+                       % (args.rate, args.duration, len(items), args.workers, args.precision, args.eps),
+           "rate": args.rate, "duration_s": args.duration, "requests": len(items), "slots": args.workers}
+    out = base_record(args, ok / total, total, cfg, dtype_of(args.precision))
+    out.update({"p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+                "solved_le_1e-8": ok, "requests": len(items), "makespan_s": total,
+                "queue_wait_p50_ms": float(np.percentile((info["start_s"] - info["arrival_s"]) * 1e3, 50))})
+    return out
 
 
 def main():
@@ -86,71 +272,9 @@ def main():
     import torchtt_amd as tt
     from torchtt_amd import workloads as W
     from torchtt_amd import _lib
-
     lib = _lib.load()
-    dev = torch.device("cuda", 0)
-    A, b = W.laplace_potential(args.d, args.n, seed=0)
-    Ag = tt.TT([c.to(dev) for c in A])
-    bg = tt.TT([c.to(dev) for c in b])
-    for _ in range(args.warmup):
-        torch.manual_seed(0)
-        tt.solvers.amen_solve(Ag, bg, eps=args.eps, rmax=args.rmax, precision=args.precision)
-    torch.cuda.synchronize()
-    _lib.check("profile_enable", lib.ttamen_profile_enable(1))
-    lat, sols, stats = [], [], []
-    torch.cuda.synchronize()
-    t_start = time.perf_counter()
-    for _ in range(args.steps):
-        torch.manual_seed(0)
-        t0 = time.perf_counter()
-        x = tt.solvers.amen_solve(Ag, bg, eps=args.eps, rmax=args.rmax, precision=args.precision)
-        torch.cuda.synchronize()
-        lat.append(time.perf_counter() - t0)
-        sols.append(x)
-        stats.append(dict(tt.solvers.last_stats))
-    torch.cuda.synchronize()
-    total = time.perf_counter() - t_start
-    prof = (ctypes.c_double * 4)()
-    _lib.check("profile_read", lib.ttamen_profile_read(prof))
-    lib.ttamen_profile_enable(0)
-    residuals = [tt.solvers.residual(Ag, x, bg) for x in sols]
-    n_mv, ms_mv, fl_mv, by_mv = prof[0], prof[1], prof[2], prof[3]
-    avg_s = (ms_mv / 1e3) / max(n_mv, 1)
-    achieved_tf = (fl_mv / max(n_mv, 1)) / avg_s / 1e12 if n_mv else 0.0
-    peak = PEAK_TFLOPS[args.precision]
-    lat_ms = np.array(lat) * 1e3
-    out = {
-        "metric": METRIC,
-        "value": args.steps / total,
-        "unit": "solves/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": total / args.steps * 1e3,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64" if args.precision == "fp64" else "f32+f64-refinement",
-        "data": "synthetic",
-        "config": {"workload": "AMEn solve, Laplacian + potential, d=%d, n=%d, TT rank 4, random rank-4 rhs, "
-                               "rank cap %d, eps %g, %s" % (args.d, args.n, args.rmax, args.eps, args.precision),
-                   "d": args.d, "n": args.n, "rmax": args.rmax, "eps": args.eps, "precision": args.precision},
-        "p50_ms": float(np.percentile(lat_ms, 50)),
-        "p99_ms": float(np.percentile(lat_ms, 99)),
-        "residual_max": max(residuals),
-        "all_residuals_le_1e-8": bool(max(residuals) <= 1e-8),
-        "sweeps": stats[-1]["sweeps"],
-        "gmres_iters": stats[-1]["gmres_iters"],
-        "max_rank": stats[-1]["max_rank"],
-        "roofline": {"kernel": "local_matvec (3 MFMA GEMM launches per application)", "bound": "mfma",
-                     "achieved": achieved_tf, "peak": peak, "unit": "TFLOP/s", "frac": achieved_tf / peak,
-                     "traffic": None, "applications": int(n_mv), "avg_us": avg_s * 1e6,
-                     "flops_per_application": fl_mv / max(n_mv, 1),
-                     "compulsory_bytes_per_application": by_mv / max(n_mv, 1)},
-    }
-    if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(A, b, args)
-    print(json.dumps(out), flush=True)
+    fn = {"batch": run_batch, "potential": run_potential, "serve": run_serve}[args.workload]
+    print(json.dumps(fn(args, tt, W, lib, _lib)), flush=True)
 
 
 if __name__ == "__main__":

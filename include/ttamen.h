@@ -50,6 +50,9 @@ typedef struct ttamen_stats {
   double max_res_last;       /* max local residual of the last sweep */
   double elapsed_s;          /* wall time inside the call */
   int64_t max_rank;          /* largest TT rank reached */
+  int64_t refinements;       /* FP64 refinement steps (precision 1) */
+  int64_t matvecs_f32;       /* FP32 local-operator applications (precision 1) */
+  double matvec_flops_f32;   /* their algorithmic flops */
 } ttamen_stats;
 
 /*
@@ -66,8 +69,10 @@ typedef struct ttamen_stats {
  *                    to draw them on the device from `seed`
  *  nswp, eps, rmax, max_full, kickrank, kick2, local_iterations, resets, verbose,
  *  preconditioner    as in the reference (preconditioner: 0 none; 1, 2 unsupported)
- *  precision         0 = FP64 everywhere (plain); 1 = FP32 local operator on MFMA with
- *                    FP64 iterative refinement of every local solve
+ *  precision         0 = FP64 everywhere (plain); 1 = local GMRES in FP32 on the FP32
+ *                    local operator (MFMA f32), inside FP64 iterative refinement of every
+ *                    local solve (FP64 residual rhs - B x each step); interfaces, QR/SVD,
+ *                    truncation and the direct solver stay FP64
  *  workspace         device scratch of ttamen_amen_solve_workspace_size(...) bytes
  *  x_out             device buffer receiving the solution cores back to back
  *  x_out_elems       its capacity in doubles (ttamen_amen_solve_output_elems(...))
@@ -83,6 +88,61 @@ int ttamen_amen_solve(int d, const int64_t* N, const int64_t* rA, const int64_t*
                       int precision, void* workspace, size_t workspace_bytes, double* x_out,
                       size_t x_out_elems, int64_t* rx_out, int64_t* offsets_out,
                       ttamen_stats* stats, void* stream);
+
+/*
+ * Batched solves of independent systems (no counterpart in the reference, which solves one
+ * system per call): `workers` host threads, each with its own HIP stream and its own slice
+ * of `workspace`, take systems from a shared queue and run the same solver as
+ * ttamen_amen_solve; results do not depend on how systems are distributed. Shared
+ * options apply to all systems; rmax and seed are per system. Each system's status is
+ * written to its `status` field; the call returns the first failure's status.
+ * The worker streams wait for the work already queued on `stream` before starting.
+ */
+typedef struct ttamen_system {
+  int d;
+  const int64_t* N;             /* d */
+  const int64_t* rA;            /* d+1 */
+  const int64_t* rb;            /* d+1 */
+  const double* const* A_cores; /* d device pointers */
+  const double* const* b_cores; /* d device pointers */
+  const double* const* x0_cores; /* d device pointers or NULL */
+  const int64_t* rx0;           /* d+1 or NULL */
+  const double* const* z0_cores; /* d device pointers or NULL */
+  uint64_t seed;
+  int64_t rmax;
+  double* x_out;                /* device */
+  size_t x_out_elems;
+  int64_t* rx_out;              /* host, d+1 */
+  int64_t* offsets_out;         /* host, d */
+  ttamen_stats* stats;          /* host, may be NULL */
+  int status;                   /* out */
+} ttamen_system;
+
+int ttamen_amen_solve_batch(ttamen_system* systems, int count, int64_t nswp, double eps,
+                            int64_t max_full, int64_t kickrank, int64_t kick2,
+                            int64_t local_iterations, int64_t resets, int precision, int workers,
+                            void* workspace, size_t workspace_bytes, void* stream);
+/* Workspace for ttamen_amen_solve_batch: workers x the largest single-system need. */
+size_t ttamen_amen_solve_batch_workspace_size(const ttamen_system* systems, int count,
+                                              int64_t kickrank, int64_t kick2,
+                                              int64_t local_iterations, int64_t max_full,
+                                              int precision, int workers);
+
+/*
+ * Serving loop (no counterpart in the reference): continuous batching of solve requests.
+ * Request i becomes eligible at arrival_s[i] seconds after the call starts (sorted). Up to
+ * `slots` requests run at once, each on its own HIP stream, so their sweep steps interleave
+ * on the device; a request is admitted, in arrival order, when a slot is idle and a
+ * contiguous region of its footprint (its solver workspace at the rank cap) is free in
+ * `workspace`, and keeps its cores and interfaces resident there until it finishes.
+ * start_s[i] / done_s[i] receive the admission and completion times (seconds since the
+ * call started); latency = done_s - arrival_s. Results equal those of ttamen_amen_solve.
+ */
+int ttamen_serve(ttamen_system* systems, const double* arrival_s, int count, int64_t nswp,
+                 double eps, int64_t max_full, int64_t kickrank, int64_t kick2,
+                 int64_t local_iterations, int64_t resets, int precision, int slots,
+                 void* workspace, size_t workspace_bytes, double* start_s, double* done_s,
+                 void* stream);
 
 /* Bytes of device workspace ttamen_amen_solve needs for these sizes (0 on bad input). */
 size_t ttamen_amen_solve_workspace_size(int d, const int64_t* N, const int64_t* rA,
@@ -173,9 +233,11 @@ int ttamen_gemm_f64(int64_t M, int64_t N, int64_t K, int64_t batch, double alpha
 /*
  * Measurement hooks (no counterpart in the reference). When enabled, every full-size
  * local-operator application of ttamen_amen_solve that is not speculative is bracketed
- * by HIP events on the solver's stream. ttamen_profile_read fills out4 with
- * {applications, total event time in ms, algorithmic flops, compulsory bytes}
- * accumulated since ttamen_profile_enable. Process-global; not for concurrent solves.
+ * by HIP events on the solver's stream; so are the FP32 applications inside the local GMRES
+ * of precision 1 (only the Arnoldi steps that ran are counted). ttamen_profile_read fills
+ * out4[0..7] with {FP64 applications, their event time in ms, algorithmic flops,
+ * compulsory bytes, then the same four for FP32} accumulated since ttamen_profile_enable.
+ * Process-global and lock-protected (batched solves serialise their timed launches).
  */
 int ttamen_profile_enable(int enable);
 int ttamen_profile_read(double* out4);

@@ -110,6 +110,45 @@ def test_laplace_potential_d10_against_reference():
     assert diff < 1e-8
 
 
+@pytest.mark.parametrize("name", ["potential_d4_n16", "laplace_d6_n16_eps1e-10"])
+def test_mixed_precision_against_reference_fixture(name):
+    """FP32 local GMRES inside FP64 refinement reaches the same solution as the FP64 reference."""
+    import torchtt_amd as tt
+    c = load_golden(name)
+    _say("amen fp32+refinement golden", name)
+    torch.manual_seed(c["seed"])
+    x = tt.solvers.amen_solve(_tt(c["A"], "cpu"), _tt(c["b"], "cpu"), nswp=c["nswp"], eps=c["eps"],
+                              rmax=c["rmax"], kickrank=c["kickrank"], precision="fp32")
+    st = dict(tt.solvers.last_stats)
+    xr = _tt(c["x"], "cpu")
+    diff = _rel_tt(x, xr)
+    res = tt.solvers.residual(_tt(c["A"], "cpu"), x, _tt(c["b"], "cpu"))
+    _say("  ranks %s (ref %s) rel diff vs FP64 reference %.3e residual %.3e refinements %d fp32 matvecs %d"
+         % (x.R, c["ranks"].tolist(), diff, res, st["refinements"], st["matvecs_f32"]))
+    assert st["matvecs_f32"] > 0
+    assert res <= 50 * c["eps"]
+    assert diff <= 1e-8
+
+
+def test_mixed_precision_d8_n48():
+    """Laplacian + potential d=8, n=48, rank cap 60: FP32 mode vs FP64 mode on the GPU."""
+    import torchtt_amd as tt
+    from torchtt_amd import workloads as W
+    A, b = W.laplace_potential(8, 48, seed=2)
+    A, b = tt.TT(A).cuda(), tt.TT(b).cuda()
+    _say("amen d8 n48 fp64 vs fp32")
+    torch.manual_seed(0)
+    x64 = tt.solvers.amen_solve(A, b, eps=1e-10, rmax=60)
+    torch.manual_seed(0)
+    x32 = tt.solvers.amen_solve(A, b, eps=1e-10, rmax=60, precision="fp32")
+    st = dict(tt.solvers.last_stats)
+    r64, r32 = tt.solvers.residual(A, x64, b), tt.solvers.residual(A, x32, b)
+    diff = _rel_tt(x32, x64)
+    _say("  residual fp64 %.3e fp32 %.3e, rel diff %.3e, stats %s" % (r64, r32, diff, st))
+    assert r32 < 1e-8 and r64 < 1e-8
+    assert diff < 1e-7
+
+
 def test_edge_cases():
     """d = 1, d = 2, no enrichment (kickrank = 0, which the reference rejects), warm start."""
     import torchtt_amd as tt

@@ -9,6 +9,7 @@ from ._extras import eye, ones, zeros, random, randn, kron, dot
 from . import errors
 from . import solvers
 from . import workloads
+from . import serving
 
 __all__ = ["TT", "eye", "ones", "zeros", "random", "randn", "kron", "dot", "errors", "solvers",
-           "workloads"]
+           "workloads", "serving"]

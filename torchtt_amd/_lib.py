@@ -27,10 +27,29 @@ class TTAmenStats(ctypes.Structure):
         ("max_res_last", ctypes.c_double),
         ("elapsed_s", ctypes.c_double),
         ("max_rank", ctypes.c_int64),
+        ("refinements", ctypes.c_int64),
+        ("matvecs_f32", ctypes.c_int64),
+        ("matvec_flops_f32", ctypes.c_double),
     ]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+class TTAmenSystem(ctypes.Structure):
+    _fields_ = [
+        ("d", ctypes.c_int),
+        ("N", c_i64p), ("rA", c_i64p), ("rb", c_i64p),
+        ("A_cores", ctypes.POINTER(c_dp)), ("b_cores", ctypes.POINTER(c_dp)),
+        ("x0_cores", ctypes.POINTER(c_dp)), ("rx0", c_i64p),
+        ("z0_cores", ctypes.POINTER(c_dp)),
+        ("seed", ctypes.c_uint64),
+        ("rmax", ctypes.c_int64),
+        ("x_out", c_dp), ("x_out_elems", c_size),
+        ("rx_out", c_i64p), ("offsets_out", c_i64p),
+        ("stats", ctypes.POINTER(TTAmenStats)),
+        ("status", ctypes.c_int),
+    ]
 
 
 # name -> (restype, argtypes)
@@ -42,6 +61,15 @@ _SIGNATURES = {
         ctypes.POINTER(c_dp), c_i64p, ctypes.POINTER(c_dp), ctypes.c_uint64, c_i64, ctypes.c_double,
         c_i64, c_i64, c_i64, c_i64, c_i64, c_i64, ctypes.c_int, ctypes.c_int, ctypes.c_int,
         c_dp, c_size, c_dp, c_size, c_i64p, c_i64p, ctypes.POINTER(TTAmenStats), c_dp]),
+    "ttamen_amen_solve_batch": (ctypes.c_int, [ctypes.POINTER(TTAmenSystem), ctypes.c_int, c_i64, ctypes.c_double,
+                                               c_i64, c_i64, c_i64, c_i64, c_i64, ctypes.c_int, ctypes.c_int,
+                                               c_dp, c_size, c_dp]),
+    "ttamen_amen_solve_batch_workspace_size": (c_size, [ctypes.POINTER(TTAmenSystem), ctypes.c_int, c_i64, c_i64,
+                                                        c_i64, c_i64, ctypes.c_int, ctypes.c_int]),
+    "ttamen_serve": (ctypes.c_int, [ctypes.POINTER(TTAmenSystem), ctypes.POINTER(ctypes.c_double), ctypes.c_int,
+                                    c_i64, ctypes.c_double, c_i64, c_i64, c_i64, c_i64, c_i64, ctypes.c_int,
+                                    ctypes.c_int, c_dp, c_size, ctypes.POINTER(ctypes.c_double),
+                                    ctypes.POINTER(ctypes.c_double), c_dp]),
     "ttamen_amen_solve_workspace_size": (c_size, [
         ctypes.c_int, c_i64p, c_i64p, c_i64p, c_i64p, c_i64, c_i64, c_i64, c_i64, c_i64, ctypes.c_int]),
     "ttamen_amen_solve_output_elems": (c_size, [ctypes.c_int, c_i64p, c_i64p, c_i64, c_i64, c_i64]),

@@ -22,6 +22,10 @@
 #include <cstdio>
 #include <cstring>
 #include <mutex>
+#include <atomic>
+#include <thread>
+#include <condition_variable>
+#include <map>
 
 namespace ttamen {
 
@@ -33,29 +37,56 @@ void set_last_error(const std::string& m) { g_last_error = m; }
 // Measurement hook: HIP events around every full-size, non-speculative local matvec
 // (the solver's dominant operation), on the stream it is launched on.
 struct MatvecProfile {
+  std::mutex mu;
   bool enabled = false;
-  std::vector<hipEvent_t> pool;
+  std::vector<hipEvent_t> free_events;  // recycled once their pair has been read
   std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
   double total_ms = 0.0, flops = 0.0, bytes = 0.0;
   int64_t count = 0;
-  size_t next = 0;
+  // FP32 applications inside GMRES: kept until their solver knows which steps ran
+  struct Tagged { const void* owner; hipEvent_t e0, e1; int cyc, step; double flops, bytes; };
+  std::vector<Tagged> tagged;
+  double total_ms32 = 0.0, flops32 = 0.0, bytes32 = 0.0;
+  int64_t count32 = 0;
   hipEvent_t get() {
-    if (next == pool.size()) {
-      hipEvent_t e;
-      TT_HIP(hipEventCreate(&e));
-      pool.push_back(e);
+    if (!free_events.empty()) {
+      hipEvent_t e = free_events.back();
+      free_events.pop_back();
+      return e;
     }
-    return pool[next++];
+    hipEvent_t e;
+    TT_HIP(hipEventCreate(&e));
+    return e;
+  }
+  float elapsed(hipEvent_t e0, hipEvent_t e1) {
+    float ms = 0.f;
+    TT_HIP(hipEventSynchronize(e1));
+    TT_HIP(hipEventElapsedTime(&ms, e0, e1));
+    free_events.push_back(e0);
+    free_events.push_back(e1);
+    return ms;
+  }
+  // keep the tagged pairs of `owner` whose step ran (step < steps[cyc]) and time them
+  void settle(const void* owner, const std::vector<int>& steps) {
+    std::vector<Tagged> rest;
+    for (auto& t : tagged) {
+      if (t.owner != owner) {
+        rest.push_back(t);
+        continue;
+      }
+      const float ms = elapsed(t.e0, t.e1);
+      if (t.cyc < (int)steps.size() && t.step < steps[t.cyc]) {
+        total_ms32 += ms;
+        flops32 += t.flops;
+        bytes32 += t.bytes;
+        count32 += 1;
+      }
+    }
+    tagged.swap(rest);
   }
   void flush() {
-    for (auto& pr : pending) {
-      float ms = 0.f;
-      TT_HIP(hipEventSynchronize(pr.second));
-      TT_HIP(hipEventElapsedTime(&ms, pr.first, pr.second));
-      total_ms += ms;
-    }
+    for (auto& pr : pending) total_ms += elapsed(pr.first, pr.second);
     pending.clear();
-    next = 0;
   }
 };
 static MatvecProfile g_prof;
@@ -238,7 +269,9 @@ class Solver {
   int NORMB(int k) const { return 1 + (d - 1) + k; }
   int NORMX(int k) const { return 1 + 2 * (d - 1) + k; }
   int TMP(int i) const { return 1 + 3 * (d - 1) + i; }
-  enum { T_NRHS = 0, T_DNORM, T_RESOLD, T_RESNEW, T_CUR, T_MAXRES, T_DX, T_SOLN, T_ONE, T_CAND };
+  enum { T_NRHS = 0, T_DNORM, T_RESOLD, T_RESNEW, T_CUR, T_MAXRES, T_DX, T_SOLN, T_ONE, T_R32, T_CAND };
+  static constexpr int kMaxRefine = 12;      // FP64 refinement steps per local solve (FP32 mode)
+  static constexpr double kInnerTolFloor = 1e-5;  // smallest relative tolerance asked of FP32 GMRES
   static constexpr int kNV = 32;  // truncation candidates per batch
 
   void allocate(const double* const* A, const double* const* b, const double* const* x0,
@@ -247,6 +280,10 @@ class Solver {
     b_.assign(b, b + d);
     Apm_.resize(d);
     for (int k = 0; k < d; ++k) Apm_[k] = ws_.alloc<double>(P.rA[k] * P.N[k] * P.N[k] * P.rA[k + 1]);
+    if (O.precision == 1) {
+      Apm32_.resize(d);
+      for (int k = 0; k < d; ++k) Apm32_[k] = ws_.alloc<float>(P.rA[k] * P.N[k] * P.N[k] * P.rA[k + 1]);
+    }
     x_.resize(d);
     for (int k = 0; k < d; ++k) x_[k] = ws_.alloc<double>(P.maxcore);
     spare_ = ws_.alloc<double>(P.maxcore);
@@ -276,6 +313,9 @@ class Solver {
     if (ws_.measuring()) return;
     TT_HIP(hipMemsetAsync(rs_.counter, 0, 64 * sizeof(unsigned), st_));
     for (int k = 0; k < d; ++k) permute_core_A(A_[k], Apm_[k], (int)P.rA[k], (int)P.N[k], (int)P.rA[k + 1], st_);
+    if (O.precision == 1)
+      for (int k = 0; k < d; ++k)
+        permute_core_A_f32(A_[k], Apm32_[k], (int)P.rA[k], (int)P.N[k], (int)P.rA[k + 1], st_);
     TT_LAUNCH_CHECK();
     // initial guess (reference amen_solve.h:161-167)
     rx_.assign(d + 1, 1);
@@ -338,7 +378,10 @@ class Solver {
       TT_HIP(hipMemcpyAsync(host_sync().host_dbl(), S_ + NORMX(0), sizeof(double) * (d - 1), hipMemcpyDeviceToHost, st_));
       TT_HIP(hipStreamSynchronize(st_));
     }
-    if (g_prof.enabled) g_prof.flush();
+    if (g_prof.enabled) {
+      std::lock_guard<std::mutex> lk(g_prof.mu);
+      g_prof.flush();
+    }
     double lg = 0.0;
     for (int i = 0; i < d - 1; ++i) lg += std::log(host_sync().host_dbl()[i]);
     norm_x_ = std::exp(lg / d);
@@ -370,6 +413,7 @@ class Solver {
   int d;
   std::vector<const double*> A_, b_;
   std::vector<double*> Apm_, x_, z_, Phi_, Phib_, Phiz_, Phizb_;
+  std::vector<float*> Apm32_;
   double* spare_ = nullptr;
   double* S_ = nullptr;
   int nscal_ = 0;
@@ -392,7 +436,9 @@ class Solver {
   void matvec(const LocalOp& op, const double* x, double* y, int nvec, const int* skip = nullptr,
               bool timed = false) {
     hipEvent_t e0 = nullptr, e1 = nullptr;
+    std::unique_lock<std::mutex> plk(g_prof.mu, std::defer_lock);
     if (timed && g_prof.enabled) {
+      plk.lock();
       e0 = g_prof.get();
       e1 = g_prof.get();
       TT_HIP(hipEventRecord(e0, st_));
@@ -582,6 +628,8 @@ class Solver {
       ws_.release(mk2);
       matvec(op, prev, e, 1);
       diffnorm(e, rhs, Nl, sc(TMP(T_NRHS)), sc(TMP(T_RESOLD)), rs_, st_);
+    } else if (O.precision == 1) {
+      refine_solve_f32(k, op, prev, rhs, sol, e, Nl, real_tol);
     } else {
       size_t mk2 = ws_.mark();
       double* drhs = ws_.alloc<double>(Nl);
@@ -604,9 +652,11 @@ class Solver {
       axpby(1.0, prev, 1.0, sol, Nl, st_);
       ws_.release(mk2);
     }
-    // res_new: e = B sol - rhs
-    matvec(op, sol, e, 1, nullptr, true);
-    axpby(-1.0, rhs, 1.0, e, Nl, st_);
+    // res_new: e = B sol - rhs (the refinement loop leaves it computed)
+    if (use_full || O.precision != 1) {
+      matvec(op, sol, e, 1, nullptr, true);
+      axpby(-1.0, rhs, 1.0, e, Nl, st_);
+    }
     norm2(e, Nl, sc(TMP(T_RESNEW)), rs_, st_);
     hipLaunchKernelGGL(k_sdiv, dim3(1), dim3(1), 0, st_, S_, TMP(T_RESNEW), TMP(T_RESNEW), TMP(T_NRHS));
     TT_LAUNCH_CHECK();
@@ -768,6 +818,87 @@ class Solver {
            h[TMP(T_RESOLD)], h[TMP(T_RESNEW)], h[TMP(T_DX)] / h[TMP(T_SOLN)]);
   }
 
+  // Local solve in mixed precision: FP32 GMRES on the FP32 local operator (MFMA f32)
+  // inside FP64 iterative refinement. Each step: r = rhs - B x in FP64, solve B d = r to a
+  // relative tolerance tau in FP32, x += d. Stops when ||r|| <= real_tol * ||rhs||, the
+  // reference's local GMRES target (amen_solve.h:340-343), or after kMaxRefine steps.
+  // On return sol = x and e = B x - rhs (FP64); res_old is set from the initial residual.
+  void refine_solve_f32(int k, const LocalOp& op, const double* prev, const double* rhs,
+                        double* sol, double* e, int64_t Nl, double real_tol) {
+    size_t mk = ws_.mark();
+    double* r = ws_.alloc<double>(Nl);
+    matvec(op, prev, e, 1, nullptr, true);
+    sub(rhs, e, r, Nl, st_);
+    norm2(r, Nl, sc(TMP(T_DNORM)), rs_, st_);
+    hipLaunchKernelGGL(k_sdiv, dim3(1), dim3(1), 0, st_, S_, TMP(T_RESOLD), TMP(T_DNORM), TMP(T_NRHS));
+    TT_LAUNCH_CHECK();
+    copy(prev, sol, Nl, st_);
+    const int64_t r1 = op.r1, r2 = op.r2, s = op.s, S = op.S;
+    float* pl = ws_.alloc<float>(r1 * s * r1);
+    float* pr = ws_.alloc<float>(r2 * S * r2);
+    cast_f64_f32(op.PhiL, pl, r1 * s * r1, st_);
+    cast_f64_f32(op.PhiR, pr, r2 * S * r2, st_);
+    LocalOpF32 op32;
+    op32.PhiL = pl; op32.Apm = Apm32_[k]; op32.PhiR = pr;
+    op32.l1 = op.l1; op32.s = op.s; op32.r1 = op.r1; op32.n = op.n; op32.S = op.S;
+    op32.l2 = op.l2; op32.r2 = op.r2;
+    float* r32 = ws_.alloc<float>(Nl);
+    float* d32 = ws_.alloc<float>(Nl);
+    GmresDev g = gmres_alloc(ws_, (int)O.local_iterations, Nl, sizeof(float));
+    GmresOp<float> gop;
+    gop.apply = [&](const float* xin, float* yout, const int* skip) {
+      local_matvec_f32(op32, xin, yout, 1, ws_, st_, skip);
+    };
+    const bool prof = g_prof.enabled;
+    if (prof) {
+      gop.apply_step = [&](const float* xin, float* yout, const int* skip, int cyc, int step) {
+        std::lock_guard<std::mutex> lk(g_prof.mu);
+        hipEvent_t e0 = g_prof.get(), e1 = g_prof.get();
+        TT_HIP(hipEventRecord(e0, st_));
+        local_matvec_f32(op32, xin, yout, 1, ws_, st_, skip);
+        TT_HIP(hipEventRecord(e1, st_));
+        const double n = op.n, s = op.s, S = op.S, r1 = op.r1, r2 = op.r2, l1 = op.l1, l2 = op.l2;
+        g_prof.tagged.push_back({(const void*)this, e0, e1, cyc, step,
+                                 2 * n * s * l1 * r1 * r2 + 2 * n * n * s * S * l1 * r2 + 2 * n * S * l1 * l2 * r2,
+                                 4.0 * (r1 * n * r2 + l1 * n * l2 + s * l1 * r1 + s * n * n * S + S * l2 * r2)});
+      };
+    }
+    gop.axpby = [&](double a, const float* xx, double b, float* yy, int64_t nn) { axpby_f32(a, xx, b, yy, nn, st_); };
+    gop.norm = [&](const float* xx, int64_t nn, double* slot) { norm2_f32(xx, nn, slot, rs_, st_); };
+    for (int it = 0; it < kMaxRefine; ++it) {
+      TT_HIP(hipMemcpyAsync(host_sync().host_dbl(), S_ + TMP(0), sizeof(double) * 4, hipMemcpyDeviceToHost, st_));
+      TT_HIP(hipStreamSynchronize(st_));
+      const double nrhs = host_sync().host_dbl()[T_NRHS], rn = host_sync().host_dbl()[T_DNORM];
+      if (!(rn > real_tol * nrhs)) break;
+      const double tau = std::min(0.5, std::max(kInnerTolFloor, 0.5 * real_tol * nrhs / rn));
+      cast_f64_f32(r, r32, Nl, st_);
+      norm2_f32(r32, Nl, sc(TMP(T_R32)), rs_, st_);
+      TT_HIP(hipMemsetAsync(d32, 0, sizeof(float) * Nl, st_));
+      GmresResult gr = gmres_run<float>(g, Nl, d32, r32, sc(TMP(T_R32)), sc(TMP(T_R32)), tau,
+                                        (int)O.resets, gop, rs_, host_sync(), st_);
+      if (prof) {
+        std::lock_guard<std::mutex> lk(g_prof.mu);
+        g_prof.settle((const void*)this, gr.cycle_steps);
+        g_prof.flush();
+      }
+      stats_.gmres_iters += gr.iters;
+      stats_.refinements += 1;
+      count_matvec32(op, gr.iters + (gr.cycles - 1));
+      add_f32_to_f64(d32, sol, Nl, st_);
+      matvec(op, sol, e, 1, nullptr, true);
+      sub(rhs, e, r, Nl, st_);
+      norm2(r, Nl, sc(TMP(T_DNORM)), rs_, st_);
+    }
+    axpby(-1.0, r, 0.0, e, Nl, st_);  // e = B x - rhs
+    ws_.release(mk);
+  }
+
+  void count_matvec32(const LocalOp& op, int nvec) {
+    const double n = op.n, s = op.s, S = op.S, r1 = op.r1, r2 = op.r2, l1 = op.l1, l2 = op.l2;
+    stats_.matvecs_f32 += nvec;
+    stats_.matvec_flops_f32 += nvec * (2 * n * s * l1 * r1 * r2 + 2 * n * n * s * S * l1 * r2 + 2 * n * S * l1 * l2 * r2);
+  }
+
   // Residual-driven rank choice (reference amen_solve.h:382-403): starting from the
   // full rank, decrease r while ||B(U_r S_r V_r^T) - rhs|| / ||rhs|| stays within
   // max(res_new, 2*real_tol); the result is the last accepted rank.
@@ -885,9 +1016,106 @@ static size_t solve_workspace_bound(const Plan& P, const Options& O) {
   doubles += 32.0 * Nl + 32.0 * R * s * n * 2 + 2 * R * s * 32;      // truncation batch
   doubles += 3 * R * n * cw + 2 * cw * cw + R * cw;    // enrichment QR
   doubles += 2 * R * n * rz * 4 + 8 * rz * n * rz;     // z update
+  if (O.precision == 1) doubles += Nl * (m + 6) * 0.5 + Nl;  // FP32 Krylov basis, r, d, FP64 residual
   size_t step = (size_t)(doubles * 8.0 * 1.25);
   const size_t splitk = 48ull << 20;                   // split-K partials
   return persistent + step + splitk + (16ull << 20);
+}
+
+static Options make_options(int64_t nswp, double eps, int64_t rmax, int64_t max_full,
+                            int64_t kickrank, int64_t kick2, int64_t local_iterations,
+                            int64_t resets, int verbose, int precision, uint64_t seed) {
+  TT_CHECK(precision == 0 || precision == 1, ERR_UNSUPPORTED,
+           "precision must be 0 (FP64) or 1 (FP32 + FP64 refinement)");
+  TT_CHECK(nswp >= 0 && eps >= 0 && kickrank >= 0 && kick2 >= 0 && resets >= 0 &&
+               local_iterations >= 1,
+           ERR_INVALID, "invalid solver parameters");
+  TT_CHECK(max_full <= (int64_t)lu_max_order() + 1, ERR_UNSUPPORTED,
+           "max_full above " + std::to_string(lu_max_order() + 1) + " is not supported");
+  Options O;
+  O.nswp = nswp; O.eps = eps; O.rmax = rmax; O.max_full = max_full; O.kickrank = kickrank;
+  O.kick2 = kick2; O.local_iterations = local_iterations; O.resets = resets;
+  O.verbose = verbose; O.precision = precision; O.seed = seed;
+  return O;
+}
+
+// One solve on one stream with one workspace (shared by the single and batched entries).
+static void solve_one(int d, const int64_t* N, const int64_t* rA, const int64_t* rb,
+                      const double* const* A_cores, const double* const* b_cores,
+                      const double* const* x0_cores, const int64_t* rx0,
+                      const double* const* z0_cores, const Options& O, void* workspace,
+                      size_t workspace_bytes, double* x_out, size_t x_out_elems, int64_t* rx_out,
+                      int64_t* offsets_out, ttamen_stats* stats, hipStream_t st) {
+  auto t0 = std::chrono::high_resolution_clock::now();
+  TT_CHECK(A_cores && b_cores && N && rA && rb && x_out && rx_out && offsets_out && workspace,
+           ERR_INVALID, "null argument");
+  Plan P = make_plan(d, N, rA, rb, x0_cores ? rx0 : nullptr, O.rmax, O.kickrank, O.kick2);
+  TT_CHECK(P.maxR <= kMaxSmall, ERR_UNSUPPORTED,
+           "rank bound " + std::to_string(P.maxR) + " exceeds " + std::to_string(kMaxSmall) +
+               " (lower rmax)");
+  Arena ws(workspace, workspace_bytes);
+  Solver s(P, O, ws, st);
+  s.allocate(A_cores, b_cores, x0_cores, rx0, z0_cores);
+  s.run();
+  s.write_out(x_out, x_out_elems, rx_out, offsets_out);
+  if (stats) {
+    *stats = s.stats_;
+    stats->elapsed_s =
+        std::chrono::duration<double>(std::chrono::high_resolution_clock::now() - t0).count();
+  }
+}
+
+static size_t batch_slice(size_t need) { return (need + 255) / 256 * 256; }
+
+// First-fit allocator of byte ranges of one workspace (the serving memory budget).
+class FirstFit {
+ public:
+  explicit FirstFit(size_t cap) { free_[0] = cap; }
+  bool alloc(size_t n, size_t* off) {
+    for (auto it = free_.begin(); it != free_.end(); ++it) {
+      if (it->second >= n) {
+        *off = it->first;
+        const size_t rest = it->second - n, start = it->first + n;
+        free_.erase(it);
+        if (rest) free_[start] = rest;
+        return true;
+      }
+    }
+    return false;
+  }
+  void release(size_t off, size_t n) {
+    auto it = free_.emplace(off, n).first;
+    auto nx = std::next(it);
+    if (nx != free_.end() && it->first + it->second == nx->first) {
+      it->second += nx->second;
+      free_.erase(nx);
+    }
+    if (it != free_.begin()) {
+      auto pv = std::prev(it);
+      if (pv->first + pv->second == it->first) {
+        pv->second += it->second;
+        free_.erase(it);
+      }
+    }
+  }
+
+ private:
+  std::map<size_t, size_t> free_;
+};
+
+// Streams for the batch workers: created once per device, default priority, reused.
+static std::vector<hipStream_t> stream_pool(int dev, int n) {
+  static std::mutex mu;
+  static std::vector<std::vector<hipStream_t>> pools;
+  std::lock_guard<std::mutex> lk(mu);
+  if ((int)pools.size() <= dev) pools.resize(dev + 1);
+  auto& p = pools[dev];
+  while ((int)p.size() < n) {
+    hipStream_t s;
+    TT_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    p.push_back(s);
+  }
+  return std::vector<hipStream_t>(p.begin(), p.begin() + n);
 }
 
 }  // namespace ttamen
@@ -951,37 +1179,178 @@ int ttamen_amen_solve(int d, const int64_t* N, const int64_t* rA, const int64_t*
                       size_t x_out_elems, int64_t* rx_out, int64_t* offsets_out,
                       ttamen_stats* stats, void* stream) {
   return guarded([&] {
-    auto t0 = std::chrono::high_resolution_clock::now();
-    TT_CHECK(A_cores && b_cores && N && rA && rb && x_out && rx_out && offsets_out, ERR_INVALID,
-             "null argument");
     TT_CHECK(preconditioner == 0, ERR_UNSUPPORTED, "local preconditioners are not supported");
-    TT_CHECK(precision == 0, ERR_UNSUPPORTED, "only precision 0 (FP64) is supported");
-    TT_CHECK(nswp >= 0 && eps >= 0 && kickrank >= 0 && kick2 >= 0 && resets >= 0 &&
-                 local_iterations >= 1,
-             ERR_INVALID, "invalid solver parameters");
+    Options O = make_options(nswp, eps, rmax, max_full, kickrank, kick2, local_iterations, resets,
+                             verbose, precision, seed);
     linalg_init();
     lu_init();
-    TT_CHECK(max_full <= (int64_t)lu_max_order() + 1, ERR_UNSUPPORTED,
-             "max_full above " + std::to_string(lu_max_order() + 1) + " is not supported");
-    Plan P = make_plan(d, N, rA, rb, x0_cores ? rx0 : nullptr, rmax, kickrank, kick2);
-    Options O;
-    O.nswp = nswp; O.eps = eps; O.rmax = rmax; O.max_full = max_full; O.kickrank = kickrank;
-    O.kick2 = kick2; O.local_iterations = local_iterations; O.resets = resets;
-    O.verbose = verbose; O.precision = precision; O.seed = seed;
-    TT_CHECK(P.maxR <= kMaxSmall, ERR_UNSUPPORTED,
-             "rank bound " + std::to_string(P.maxR) + " exceeds " + std::to_string(kMaxSmall) +
-                 " (lower rmax)");
-    Arena ws(workspace, workspace_bytes);
-    hipStream_t st = (hipStream_t)stream;
-    Solver s(P, O, ws, st);
-    s.allocate(A_cores, b_cores, x0_cores, rx0, z0_cores);
-    s.run();
-    s.write_out(x_out, x_out_elems, rx_out, offsets_out);
-    if (stats) {
-      *stats = s.stats_;
-      stats->elapsed_s =
-          std::chrono::duration<double>(std::chrono::high_resolution_clock::now() - t0).count();
+    solve_one(d, N, rA, rb, A_cores, b_cores, x0_cores, rx0, z0_cores, O, workspace,
+              workspace_bytes, x_out, x_out_elems, rx_out, offsets_out, stats, (hipStream_t)stream);
+  });
+}
+
+size_t ttamen_amen_solve_batch_workspace_size(const ttamen_system* systems, int count,
+                                              int64_t kickrank, int64_t kick2,
+                                              int64_t local_iterations, int64_t max_full,
+                                              int precision, int workers) {
+  size_t out = 0;
+  int rc = guarded([&] {
+    TT_CHECK(systems && count >= 0 && workers >= 1, ERR_INVALID, "bad batch arguments");
+    size_t mx = 0;
+    for (int i = 0; i < count; ++i) {
+      const ttamen_system& y = systems[i];
+      size_t w = ttamen_amen_solve_workspace_size(y.d, y.N, y.rA, y.rb, y.x0_cores ? y.rx0 : nullptr,
+                                                  y.rmax, kickrank, kick2, local_iterations,
+                                                  max_full, precision);
+      TT_CHECK(w > 0, ERR_INVALID, "system " + std::to_string(i) + ": " + g_last_error);
+      mx = std::max(mx, w);
     }
+    out = batch_slice(mx) * (size_t)std::min(workers, std::max(count, 1));
+  });
+  return rc == 0 ? out : 0;
+}
+
+int ttamen_amen_solve_batch(ttamen_system* systems, int count, int64_t nswp, double eps,
+                            int64_t max_full, int64_t kickrank, int64_t kick2,
+                            int64_t local_iterations, int64_t resets, int precision, int workers,
+                            void* workspace, size_t workspace_bytes, void* stream) {
+  return guarded([&] {
+    TT_CHECK(systems && count >= 0 && workers >= 1 && workspace, ERR_INVALID, "bad batch arguments");
+    if (count == 0) return;
+    workers = std::min(workers, count);
+    const size_t slice = workspace_bytes / (size_t)workers / 256 * 256;
+    linalg_init();
+    lu_init();
+    int dev = 0;
+    TT_HIP(hipGetDevice(&dev));
+    std::vector<hipStream_t> streams = stream_pool(dev, workers);
+    // Inputs were produced on the caller's stream: every worker stream waits for it.
+    hipEvent_t ready;
+    TT_HIP(hipEventCreateWithFlags(&ready, hipEventDisableTiming));
+    TT_HIP(hipEventRecord(ready, (hipStream_t)stream));
+    for (auto s : streams) TT_HIP(hipStreamWaitEvent(s, ready, 0));
+    std::atomic<int> next(0);
+    std::mutex err_mu;
+    std::string first_err;
+    auto work = [&](int w) {
+      (void)hipSetDevice(dev);
+      char* base = (char*)workspace + (size_t)w * slice;
+      for (;;) {
+        const int i = next.fetch_add(1);
+        if (i >= count) break;
+        ttamen_system& y = systems[i];
+        y.status = guarded([&] {
+          Options O = make_options(nswp, eps, y.rmax, max_full, kickrank, kick2, local_iterations,
+                                   resets, 0, precision, y.seed);
+          solve_one(y.d, y.N, y.rA, y.rb, y.A_cores, y.b_cores, y.x0_cores, y.rx0, y.z0_cores, O,
+                    base, slice, y.x_out, y.x_out_elems, y.rx_out, y.offsets_out, y.stats,
+                    streams[w]);
+        });
+        if (y.status != 0) {
+          std::lock_guard<std::mutex> lk(err_mu);
+          if (first_err.empty()) first_err = "system " + std::to_string(i) + ": " + g_last_error;
+        }
+      }
+    };
+    std::vector<std::thread> pool;
+    for (int w = 1; w < workers; ++w) pool.emplace_back(work, w);
+    work(0);
+    for (auto& t : pool) t.join();
+    (void)hipEventDestroy(ready);
+    TT_CHECK(first_err.empty(), ERR_NUMERIC, first_err);
+  });
+}
+
+int ttamen_serve(ttamen_system* systems, const double* arrival_s, int count, int64_t nswp,
+                 double eps, int64_t max_full, int64_t kickrank, int64_t kick2,
+                 int64_t local_iterations, int64_t resets, int precision, int slots,
+                 void* workspace, size_t workspace_bytes, double* start_s, double* done_s,
+                 void* stream) {
+  return guarded([&] {
+    TT_CHECK(systems && arrival_s && start_s && done_s && workspace && slots >= 1 && count >= 0,
+             ERR_INVALID, "bad serve arguments");
+    if (count == 0) return;
+    for (int i = 1; i < count; ++i)
+      TT_CHECK(arrival_s[i] >= arrival_s[i - 1], ERR_INVALID, "arrivals must be sorted");
+    linalg_init();
+    lu_init();
+    int dev = 0;
+    TT_HIP(hipGetDevice(&dev));
+    // footprint of every request = its solver workspace (rank-cap bound)
+    std::vector<size_t> need(count);
+    for (int i = 0; i < count; ++i) {
+      const ttamen_system& y = systems[i];
+      size_t w = ttamen_amen_solve_workspace_size(y.d, y.N, y.rA, y.rb, nullptr, y.rmax, kickrank,
+                                                  kick2, local_iterations, max_full, precision);
+      TT_CHECK(w > 0, ERR_INVALID, "request " + std::to_string(i) + ": " + g_last_error);
+      need[i] = batch_slice(w);
+      TT_CHECK(need[i] <= workspace_bytes, ERR_WORKSPACE,
+               "request " + std::to_string(i) + " does not fit the serving memory");
+    }
+    const int nslots = std::min(slots, count);
+    std::vector<hipStream_t> streams = stream_pool(dev, nslots);
+    hipEvent_t ready;
+    TT_HIP(hipEventCreateWithFlags(&ready, hipEventDisableTiming));
+    TT_HIP(hipEventRecord(ready, (hipStream_t)stream));
+    for (auto s : streams) TT_HIP(hipStreamWaitEvent(s, ready, 0));
+    TT_HIP(hipStreamSynchronize((hipStream_t)stream));
+
+    // Admission state: requests are admitted strictly in arrival order, when a slot is
+    // idle, the request has arrived and a contiguous region of its footprint is free.
+    std::mutex mu;
+    std::condition_variable cv;
+    int next = 0;
+    FirstFit mem(workspace_bytes);
+    std::string first_err;
+    const auto t0 = std::chrono::steady_clock::now();
+    auto now_s = [&] {
+      return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    };
+    auto work = [&](int w) {
+      (void)hipSetDevice(dev);
+      for (;;) {
+        int i;
+        size_t off;
+        {
+          std::unique_lock<std::mutex> lk(mu);
+          for (;;) {
+            if (next >= count) return;
+            const double wait = arrival_s[next] - now_s();
+            if (wait > 0) {
+              cv.wait_for(lk, std::chrono::duration<double>(wait));
+              continue;
+            }
+            if (mem.alloc(need[next], &off)) break;
+            cv.wait(lk);  // memory frees when a running request finishes
+          }
+          i = next++;
+          start_s[i] = now_s();
+        }
+        cv.notify_all();
+        ttamen_system& y = systems[i];
+        y.status = guarded([&] {
+          Options O = make_options(nswp, eps, y.rmax, max_full, kickrank, kick2, local_iterations,
+                                   resets, 0, precision, y.seed);
+          solve_one(y.d, y.N, y.rA, y.rb, y.A_cores, y.b_cores, y.x0_cores, y.rx0, y.z0_cores, O,
+                    (char*)workspace + off, need[i], y.x_out, y.x_out_elems, y.rx_out,
+                    y.offsets_out, y.stats, streams[w]);
+        });
+        {
+          std::lock_guard<std::mutex> lk(mu);
+          done_s[i] = now_s();
+          mem.release(off, need[i]);
+          if (y.status != 0 && first_err.empty())
+            first_err = "request " + std::to_string(i) + ": " + g_last_error;
+        }
+        cv.notify_all();
+      }
+    };
+    std::vector<std::thread> pool;
+    for (int w = 1; w < nslots; ++w) pool.emplace_back(work, w);
+    work(0);
+    for (auto& t : pool) t.join();
+    (void)hipEventDestroy(ready);
+    TT_CHECK(first_err.empty(), ERR_NUMERIC, first_err);
   });
 }
 
@@ -1130,20 +1499,28 @@ size_t ttamen_dense_solve_workspace_size(int64_t n) {
 
 int ttamen_profile_enable(int enable) {
   return guarded([&] {
+    std::lock_guard<std::mutex> lk(g_prof.mu);
     g_prof.flush();
+    g_prof.tagged.clear();
     g_prof.enabled = enable != 0;
     g_prof.total_ms = 0.0; g_prof.flops = 0.0; g_prof.bytes = 0.0; g_prof.count = 0;
+    g_prof.total_ms32 = 0.0; g_prof.flops32 = 0.0; g_prof.bytes32 = 0.0; g_prof.count32 = 0;
   });
 }
 
-int ttamen_profile_read(double* out4) {
+int ttamen_profile_read(double* out4) {  // 8 values
   return guarded([&] {
     TT_CHECK(out4, ERR_INVALID, "null argument");
+    std::lock_guard<std::mutex> lk(g_prof.mu);
     g_prof.flush();
     out4[0] = (double)g_prof.count;
     out4[1] = g_prof.total_ms;
     out4[2] = g_prof.flops;
     out4[3] = g_prof.bytes;
+    out4[4] = (double)g_prof.count32;
+    out4[5] = g_prof.total_ms32;
+    out4[6] = g_prof.flops32;
+    out4[7] = g_prof.bytes32;
   });
 }
 

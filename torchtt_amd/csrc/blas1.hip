@@ -130,6 +130,32 @@ void axpby(double a, const double* x, double b, double* y, int64_t n, hipStream_
   TT_LAUNCH_CHECK();
 }
 
+__global__ void k_axpby_f32(double a, const float* x, double b, float* y, int64_t n,
+                            const int* skip) {
+  if (skip && *skip) return;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    y[i] = (float)(a * (double)x[i] + b * (double)y[i]);
+}
+void axpby_f32(double a, const float* x, double b, float* y, int64_t n, hipStream_t st,
+               const int* skip) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_axpby_f32, dim3(grid_for(n)), dim3(256), 0, st, a, x, b, y, n, skip);
+  TT_LAUNCH_CHECK();
+}
+
+// y (FP64) += x (FP32)
+__global__ void k_add_f32_to_f64(const float* x, double* y, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    y[i] += (double)x[i];
+}
+void add_f32_to_f64(const float* x, double* y, int64_t n, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_add_f32_to_f64, dim3(grid_for(n)), dim3(256), 0, st, x, y, n);
+  TT_LAUNCH_CHECK();
+}
+
 __global__ void k_sub(const double* x, const double* y, double* z, int64_t n) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x)

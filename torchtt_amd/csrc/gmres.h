@@ -29,12 +29,16 @@ struct GmresDev {
 template <typename T>
 struct GmresOp {
   std::function<void(const T*, T*, const int*)> apply;          // y = Op x (skip flag)
+  // Optional: Arnoldi-step application, tagged (cycle, step) so a caller can attribute
+  // timing to the steps that actually ran (later steps of a converged cycle are skipped).
+  std::function<void(const T*, T*, const int*, int, int)> apply_step;
   std::function<void(double, const T*, double, T*, int64_t)> axpby;  // y = a x + b y
   std::function<void(const T*, int64_t, double*)> norm;         // *slot = ||x||
 };
 
 struct GmresResult {
   int flag = 0, iters = 0, cycles = 0, reorths = 0;
+  std::vector<int> cycle_steps;  // Arnoldi steps executed in each cycle
 };
 
 // Small pinned-host mailbox used to poll device flags without stalling the queue.

@@ -287,7 +287,8 @@ GmresResult gmres_run(GmresDev g, int64_t N, T* x, const T* rhs, const double* n
     for (k = 0; k < g.m && !stop; ++k) {
       T* qk = Q + (int64_t)k * N;
       T* qn = Q + (int64_t)(k + 1) * N;
-      op.apply(qk, w, done);
+      if (op.apply_step) op.apply_step(qk, w, done, cyc, k);
+      else op.apply(qk, w, done);
       double* Hcol = g.H + (int64_t)k * (g.m + 1);
       const double* Pw = P1 + (size_t)(k + 1) * kGmresMaxBlocks;  // row kc of P1: w.w
       // pass 1: project out the basis
@@ -313,10 +314,11 @@ GmresResult gmres_run(GmresDev g, int64_t N, T* x, const T* rhs, const double* n
     hipLaunchKernelGGL((k_gmres_xupd<T>), dim3(nb), dim3(kGB), 0, st, Q, N, x, g);
     TT_LAUNCH_CHECK();
     // Did this cycle converge?
-    TT_HIP(hipMemcpyAsync(hs.host_int(), g.istate + GM_FLAG, sizeof(int), hipMemcpyDeviceToHost,
+    TT_HIP(hipMemcpyAsync(hs.host_int(), g.istate + GM_FLAG, 2 * sizeof(int), hipMemcpyDeviceToHost,
                           st));
     TT_HIP(hipStreamSynchronize(st));
-    const int flag = *hs.host_int();
+    const int flag = hs.host_int()[0];
+    res.cycle_steps.push_back(hs.host_int()[1]);  // GM_KCOLS follows GM_FLAG
     res.cycles = cyc + 1;
     if (flag) break;
   }

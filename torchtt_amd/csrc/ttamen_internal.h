@@ -141,6 +141,10 @@ void scale_by(double* x, int64_t n, const double* slot, hipStream_t st);
 // y = a*x + b*y with host scalars
 void axpby(double a, const double* x, double b, double* y, int64_t n, hipStream_t st,
            const int* skip = nullptr);
+void axpby_f32(double a, const float* x, double b, float* y, int64_t n, hipStream_t st,
+               const int* skip = nullptr);
+// y (FP64) += x (FP32)
+void add_f32_to_f64(const float* x, double* y, int64_t n, hipStream_t st);
 // z = x - y
 void sub(const double* x, const double* y, double* z, int64_t n, hipStream_t st);
 void copy(const double* x, double* y, int64_t n, hipStream_t st);

@@ -37,10 +37,31 @@ def _solver_device(A):
     return torch.device("cuda", torch.cuda.current_device())
 
 
+def _check_system(A, b):
+    if not (isinstance(A, TT) and isinstance(b, TT)):
+        raise InvalidArguments('A and b must be TT instances.')
+    if not (A.is_ttm and not b.is_ttm):
+        raise IncompatibleTypes('A must be TT-matrix and b must be vector.')
+    if A.M != A.N:
+        raise ShapeMismatch('A is not quadratic.')
+    if A.N != b.N:
+        raise ShapeMismatch('Dimension mismatch.')
+
+
+def _draw_z(N, kickrank, kick2, device):
+    """Residual-proxy initial cores, drawn as the reference draws them (cpp/amen_solve.h:169-176)."""
+    d = len(N)
+    rzc = kickrank + kick2
+    if rzc <= 0:
+        return None
+    rz = [1] + [rzc] * (d - 1) + [1]
+    return [torch.randn((rz[i], N[i], rz[i + 1]), dtype=torch.float64, device=device) for i in range(d)]
+
+
 def amen_solve(A, b, nswp=22, x0=None, eps=1e-10, rmax=32768, max_full=500, kickrank=4, kick2=0,
                trunc_norm='res', local_solver=1, local_iterations=40, resets=2, verbose=False,
                preconditioner=None, use_cpp=True, band_diagonal=-1, use_single_precision=False,
-               precision=None, seed=None):
+               precision=None, seed=None, _z0=None):
     """Solve A x = b in the TT format with AMEn (Dolgov & Savostyanov, SIAM J. Sci. Comput. 36 (2014)).
 
     Args follow the reference. `trunc_norm`, `local_solver`, `use_cpp` and `band_diagonal`
@@ -54,14 +75,7 @@ def amen_solve(A, b, nswp=22, x0=None, eps=1e-10, rmax=32768, max_full=500, kick
         TT: the solution on the device and dtype of A.cores[0].
     """
     global last_stats
-    if not (isinstance(A, TT) and isinstance(b, TT)):
-        raise InvalidArguments('A and b must be TT instances.')
-    if not (A.is_ttm and not b.is_ttm):
-        raise IncompatibleTypes('A must be TT-matrix and b must be vector.')
-    if A.M != A.N:
-        raise ShapeMismatch('A is not quadratic.')
-    if A.N != b.N:
-        raise ShapeMismatch('Dimension mismatch.')
+    _check_system(A, b)
     if preconditioner is None:
         prec = 0
     elif preconditioner == 'c':
@@ -86,13 +100,9 @@ def amen_solve(A, b, nswp=22, x0=None, eps=1e-10, rmax=32768, max_full=500, kick
 
     # Residual-proxy initial cores: drawn like the reference draws them (torch.randn with
     # the options of A, cpp/amen_solve.h:174-176), so a seeded run matches it.
-    rzc = kickrank + kick2
-    z_cores = None
-    if rzc > 0:
-        rz = [1] + [rzc] * (d - 1) + [1]
-        z_cores = [torch.randn((rz[i], N[i], rz[i + 1]), dtype=torch.float64, device=out_device)
-                   for i in range(d)]
-        z_cores = [z.to(dev).contiguous() for z in z_cores]
+    z_cores = _z0 if _z0 is not None else _draw_z(N, kickrank, kick2, out_device)
+    if z_cores is not None:
+        z_cores = [z.to(device=dev, dtype=torch.float64).contiguous() for z in z_cores]
 
     with torch.cuda.device(dev):
         Ac = [c.to(device=dev, dtype=torch.float64).contiguous() for c in A.cores]
@@ -134,6 +144,110 @@ def amen_solve(A, b, nswp=22, x0=None, eps=1e-10, rmax=32768, max_full=500, kick
         last_stats["workspace_bytes"] = int(ws_bytes)
         del ws, xout
     return TT(cores)
+
+
+#: per-system statistics of the most recent batch (list of dicts)
+last_batch_stats = None
+
+
+def _build_systems(systems, dev, kickrank, kick2, z0, lib):
+    """ctypes array of ttamen_system for (A, b[, rmax]) items, with device copies and outputs."""
+    count = len(systems)
+    keep, outs = [], []
+    arr = (_lib.TTAmenSystem * count)()
+    stats = (_lib.TTAmenStats * count)()
+    for i, item in enumerate(systems):
+        A, b = item[0], item[1]
+        rmax = int(item[2]) if len(item) > 2 else 32768
+        _check_system(A, b)
+        N = list(b.N)
+        d = len(N)
+        Ac = [c.to(device=dev, dtype=torch.float64).contiguous() for c in A.cores]
+        bc = [c.to(device=dev, dtype=torch.float64).contiguous() for c in b.cores]
+        zc = z0[i] if z0 is not None else _draw_z(N, kickrank, kick2, A.cores[0].device)
+        if zc is not None:
+            zc = [z.to(device=dev, dtype=torch.float64).contiguous() for z in zc]
+        Na, rAa, rba = _lib.i64_array(N), _lib.i64_array(A.R), _lib.i64_array(b.R)
+        out_elems = lib.ttamen_amen_solve_output_elems(d, Na, None, rmax, int(kickrank), int(kick2))
+        xout = torch.empty(max(out_elems, 1), dtype=torch.float64, device=dev)
+        rx_out = (ctypes.c_int64 * (d + 1))()
+        off_out = (ctypes.c_int64 * d)()
+        pa, pb = _lib.ptr_array(Ac), _lib.ptr_array(bc)
+        pz = _lib.ptr_array(zc) if zc is not None else None
+        keep.extend([Ac, bc, zc, Na, rAa, rba, pa, pb, pz, xout, rx_out, off_out])
+        y = arr[i]
+        y.d = d
+        y.N, y.rA, y.rb = Na, rAa, rba
+        y.A_cores, y.b_cores = pa, pb
+        y.x0_cores, y.rx0 = None, None
+        y.z0_cores = pz
+        y.seed = 0
+        y.rmax = rmax
+        y.x_out = ctypes.c_void_p(xout.data_ptr())
+        y.x_out_elems = out_elems
+        y.rx_out, y.offsets_out = rx_out, off_out
+        y.stats = ctypes.pointer(stats[i])
+        y.status = 0
+        outs.append((A, N, xout, rx_out, off_out))
+    return arr, stats, outs, keep
+
+
+def _collect(outs):
+    result = []
+    for (A, N, xout, rx_out, off_out) in outs:
+        d = len(N)
+        cores = []
+        for k in range(d):
+            n = rx_out[k] * N[k] * rx_out[k + 1]
+            c = xout[off_out[k]:off_out[k] + n].reshape(rx_out[k], N[k], rx_out[k + 1]).clone()
+            cores.append(c.to(device=A.cores[0].device, dtype=A.cores[0].dtype))
+        result.append(TT(cores))
+    return result
+
+
+#: per-system statistics of the most recent batch (list of dicts)
+last_batch_stats = None
+
+
+def amen_solve_batch(systems, nswp=22, eps=1e-10, max_full=500, kickrank=4, kick2=0, local_iterations=40,
+                     resets=2, precision='fp64', workers=8, z0=None):
+    """Solve many independent TT systems concurrently (no counterpart in the reference).
+
+    systems: list of (A, b) or (A, b, rmax) with TT A, b. Each system runs the same solver as
+    `amen_solve` with the same options; `workers` host threads with one HIP stream each take
+    systems from a shared queue (libttamen: ttamen_amen_solve_batch). The residual-proxy
+    draws are made per system in list order (or taken from `z0`, a list of per-system core
+    lists), so system i's result equals `amen_solve(A_i, b_i, ..., _z0=z0[i])`.
+
+    Returns:
+        list[TT]: the solutions, on each A's device and dtype.
+    """
+    global last_batch_stats
+    if precision not in _PRECISIONS:
+        raise InvalidArguments("precision must be one of %s." % sorted(_PRECISIONS))
+    lib = _lib.load()
+    count = len(systems)
+    if count == 0:
+        return []
+    dev = _solver_device(systems[0][0])
+    with torch.cuda.device(dev):
+        arr, stats, outs, keep = _build_systems(systems, dev, kickrank, kick2, z0, lib)
+        workers = max(1, min(int(workers), count))
+        ws_bytes = lib.ttamen_amen_solve_batch_workspace_size(arr, count, int(kickrank), int(kick2),
+                                                              int(local_iterations), int(max_full),
+                                                              _PRECISIONS[precision], workers)
+        if ws_bytes == 0:
+            _lib.check("ttamen_amen_solve_batch_workspace_size", 1)
+        ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dev)
+        code = lib.ttamen_amen_solve_batch(arr, count, int(nswp), float(eps), int(max_full), int(kickrank),
+                                           int(kick2), int(local_iterations), int(resets),
+                                           _PRECISIONS[precision], workers, ctypes.c_void_p(ws.data_ptr()),
+                                           ws_bytes, ctypes.c_void_p(_lib.stream_handle(dev)))
+        _lib.check("ttamen_amen_solve_batch", code)
+        result = _collect(outs)
+        last_batch_stats = [stats[i].as_dict() for i in range(count)]
+        del ws, keep
+    return result
 
 
 def residual(A, x, b):
