@@ -4,9 +4,10 @@ Workloads (--workload):
   batch      (default) 256 independent anisotropic-diffusion systems: d ~ U{4..12},
              n ~ U{8..64}, coefficients log-uniform in [0.1, 10] per mode, operator TT rank 2-4,
              random rank-1..4 N(0,1) right-hand sides with ||b|| = 1, rank caps U{16..64},
-             eps 1e-8. A step is one pass over all 256 systems with the batched solver
-             (worker threads, one HIP stream each) in --precision (default fp32: FP32 local GMRES
-             on the matrix cores inside FP64 refinement). The plain FP64 solver, one system at a
+             eps 1e-8. A step is one pass over all 256 systems with the batched solver in
+             --precision (default fp32: FP32 local GMRES on the matrix cores inside FP64
+             refinement): the systems run as one cohort whose kernels are merged into grouped
+             launches at every host synchronisation point. The plain FP64 solver, one system at a
              time, is measured on a sample of the same systems for the comparison.
   potential  one system: d=10, n=64, Laplacian + potential (TT rank 4), random rank-4 rhs,
              rank cap 80, eps 1e-8; a step is one solve.
@@ -51,7 +52,8 @@ def parse():
     p.add_argument("--workload", default="batch", choices=["batch", "potential", "serve"])
     p.add_argument("--precision", default=None, choices=["fp64", "fp32"])
     p.add_argument("--systems", type=int, default=256)
-    p.add_argument("--workers", type=int, default=8)
+    p.add_argument("--workers", type=int, default=8, help="host threads running the systems' host code")
+    p.add_argument("--slots", type=int, default=64, help="serve: requests running at once")
     p.add_argument("--plain-sample", type=int, default=32)
     p.add_argument("--eps", type=float, default=1e-8)
     p.add_argument("--rate", type=float, default=200.0)
@@ -149,6 +151,7 @@ def run_batch(args, tt, W, lib, _lib):
     _lib.check("profile_read", lib.ttamen_profile_read(prof))
     lib.ttamen_profile_enable(0)
     stats = tt.solvers.last_batch_stats
+    cohort = dict(tt.solvers.last_cohort_stats)
     res = np.array([tt.solvers.residual(it[0], x, it[1]) for it, x in zip(items, sols)])
     ok = int(np.sum(res <= RES_BAR))
     lat_ms = np.array([s["elapsed_s"] for s in stats]) * 1e3
@@ -167,7 +170,7 @@ def run_batch(args, tt, W, lib, _lib):
     ok_sample = int(np.sum(res[:ns] <= RES_BAR))
 
     cfg = {"workload": "batched AMEn: %d anisotropic diffusion systems, d~U{4..12}, n~U{8..64}, op rank 2-4, "
-                       "rhs rank 1-4, rank caps U{16..64}, eps %g, %s, %d workers"
+                       "rhs rank 1-4, rank caps U{16..64}, eps %g, %s, one cohort, %d host threads"
                        % (len(items), args.eps, args.precision, args.workers),
            "systems": len(items), "precision": args.precision, "workers": args.workers, "eps": args.eps}
     out = base_record(args, value, total, cfg, dtype_of(args.precision))
@@ -186,6 +189,7 @@ def run_batch(args, tt, W, lib, _lib):
         "error_vs_plain_fp64_max": max(err) if err else None,
         "error_vs_plain_fp64_median": float(np.median(err)) if err else None,
         "roofline": roofline(prof, args.precision),
+        "cohort": cohort,
     })
     if not args.no_cpu_baseline:
         cpu_items = [(s["A"], s["b"], s["rmax"]) for s in systems]
@@ -245,7 +249,7 @@ def run_serve(args, tt, W, lib, _lib):
         tt.solvers.amen_solve_batch(items[:8], eps=args.eps, precision=args.precision, z0=z0[:8], workers=4)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    sols, info = tt.serving.serve(items, arrivals, slots=args.workers, eps=args.eps, precision=args.precision,
+    sols, info = tt.serving.serve(items, arrivals, slots=args.slots, eps=args.eps, precision=args.precision,
                                   z0=z0)
     torch.cuda.synchronize()
     total = time.perf_counter() - t0
@@ -254,8 +258,8 @@ def run_serve(args, tt, W, lib, _lib):
     lat = info["latency_s"] * 1e3
     cfg = {"workload": "served AMEn: Poisson arrivals at %g req/s for %g s (%d requests) of systems drawn as in "
                        "the batched workload, %d slots, %s, eps %g"
-                       % (args.rate, args.duration, len(items), args.workers, args.precision, args.eps),
-           "rate": args.rate, "duration_s": args.duration, "requests": len(items), "slots": args.workers}
+                       % (args.rate, args.duration, len(items), args.slots, args.precision, args.eps),
+           "rate": args.rate, "duration_s": args.duration, "requests": len(items), "slots": args.slots}
     out = base_record(args, ok / total, total, cfg, dtype_of(args.precision))
     out.update({"p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
                 "solved_le_1e-8": ok, "requests": len(items), "makespan_s": total,

@@ -91,12 +91,16 @@ int ttamen_amen_solve(int d, const int64_t* N, const int64_t* rA, const int64_t*
 
 /*
  * Batched solves of independent systems (no counterpart in the reference, which solves one
- * system per call): `workers` host threads, each with its own HIP stream and its own slice
- * of `workspace`, take systems from a shared queue and run the same solver as
- * ttamen_amen_solve; results do not depend on how systems are distributed. Shared
- * options apply to all systems; rmax and seed are per system. Each system's status is
- * written to its `status` field; the call returns the first failure's status.
- * The worker streams wait for the work already queued on `stream` before starting.
+ * system per call). The systems run as one cohort on `stream`: each runs the same solver as
+ * ttamen_amen_solve in its own region of `workspace`, and at every host synchronisation
+ * point the launches that all running systems have queued are merged into grouped launches
+ * (one grid covering the same kernel for every system), so each launch fills the device.
+ * Up to 256 systems run at once (as many as fit in the workspace, admitted in index order);
+ * `workers` host threads run the systems' host code. Results are bitwise those of
+ * ttamen_amen_solve with the same inputs. Shared options apply to all systems; rmax and
+ * seed are per system. Each system's status is written to its `status` field; the call
+ * returns the first failure's status. The first ttamen_cohort_staging_bytes() bytes of
+ * `workspace` hold the grouped launches' argument tables.
  */
 typedef struct ttamen_system {
   int d;
@@ -122,7 +126,8 @@ int ttamen_amen_solve_batch(ttamen_system* systems, int count, int64_t nswp, dou
                             int64_t max_full, int64_t kickrank, int64_t kick2,
                             int64_t local_iterations, int64_t resets, int precision, int workers,
                             void* workspace, size_t workspace_bytes, void* stream);
-/* Workspace for ttamen_amen_solve_batch: workers x the largest single-system need. */
+/* Workspace for ttamen_amen_solve_batch: staging plus room for the systems to run all at
+ * once (the 256 largest needs). A smaller workspace runs fewer systems at a time. */
 size_t ttamen_amen_solve_batch_workspace_size(const ttamen_system* systems, int count,
                                               int64_t kickrank, int64_t kick2,
                                               int64_t local_iterations, int64_t max_full,
@@ -131,18 +136,32 @@ size_t ttamen_amen_solve_batch_workspace_size(const ttamen_system* systems, int 
 /*
  * Serving loop (no counterpart in the reference): continuous batching of solve requests.
  * Request i becomes eligible at arrival_s[i] seconds after the call starts (sorted). Up to
- * `slots` requests run at once, each on its own HIP stream, so their sweep steps interleave
- * on the device; a request is admitted, in arrival order, when a slot is idle and a
- * contiguous region of its footprint (its solver workspace at the rank cap) is free in
- * `workspace`, and keeps its cores and interfaces resident there until it finishes.
- * start_s[i] / done_s[i] receive the admission and completion times (seconds since the
- * call started); latency = done_s - arrival_s. Results equal those of ttamen_amen_solve.
+ * `slots` (<= 256) requests run at once as one cohort on `stream` (see
+ * ttamen_amen_solve_batch), so the same sweep steps of different requests execute as one
+ * grouped launch; a request is admitted between two such rounds, in arrival order, when a
+ * slot is idle and a contiguous region of its footprint (its solver workspace at the rank
+ * cap) is free in `workspace` after the staging area, and keeps its cores and interfaces
+ * resident there until it finishes. start_s[i] / done_s[i] receive the admission and
+ * completion times (seconds since the call started); latency = done_s - arrival_s.
+ * Results equal those of ttamen_amen_solve.
  */
 int ttamen_serve(ttamen_system* systems, const double* arrival_s, int count, int64_t nswp,
                  double eps, int64_t max_full, int64_t kickrank, int64_t kick2,
                  int64_t local_iterations, int64_t resets, int precision, int slots,
                  void* workspace, size_t workspace_bytes, double* start_s, double* done_s,
                  void* stream);
+
+/* Bytes at the front of a batch / serving workspace that hold grouped-launch tables. */
+size_t ttamen_cohort_staging_bytes(void);
+/* Counters of the last batch / serving call (no counterpart in the reference): rounds,
+ * grouped launches, launch items (member launches merged into them), member resumptions,
+ * and seconds spent resuming members, building grouped launches, waiting for the device,
+ * in total. */
+int ttamen_cohort_stats(double* out8);
+/* Scheduling parameters of later batch / serving calls (process-wide; results do not
+ * depend on them): grouped launches per round (128), merge steps a system may wait before
+ * its next launch is served (32), GMRES iterations between convergence polls (8). */
+int ttamen_cohort_set_params(int window, int max_age, int gmres_poll);
 
 /* Bytes of device workspace ttamen_amen_solve needs for these sizes (0 on bad input). */
 size_t ttamen_amen_solve_workspace_size(int d, const int64_t* N, const int64_t* rA,

@@ -15,6 +15,7 @@
 //     and each B(u_j v_j^T) is a rank-1 local product (cost ~1/r of a matvec).
 #include "ttamen_internal.h"
 #include "gmres.h"
+#include "cohort.h"
 #include "../../include/ttamen.h"
 #include <algorithm>
 #include <chrono>
@@ -93,24 +94,35 @@ static MatvecProfile g_prof;
 
 // ---------------------------------------------------------------- small kernels
 // out[i][c] = in[i][c] * s[c]  for c < ncols (in row-major with ld ldi, out ld ldo)
-__global__ void k_scale_cols(const double* in, int64_t ldi, const double* s, double* out,
-                             int64_t ldo, int64_t rows, int ncols) {
-  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < rows * ncols;
-       t += (int64_t)gridDim.x * blockDim.x) {
-    int64_t i = t / ncols;
-    int c = (int)(t - i * ncols);
-    out[i * ldo + c] = in[i * ldi + c] * s[c];
+struct KScaleCols {
+  struct Args { const double* in; int64_t ldi; const double* s; double* out; int64_t ldo, rows; int ncols; };
+  static constexpr int kThreads = 256;
+  static __device__ void run(const Args& a, const Blk& bk) {
+    for (int64_t t = bk.x * (int64_t)blockDim.x + threadIdx.x; t < a.rows * a.ncols;
+         t += (int64_t)bk.gx * blockDim.x) {
+      int64_t i = t / a.ncols;
+      int c = (int)(t - i * a.ncols);
+      a.out[i * a.ldo + c] = a.in[i * a.ldi + c] * a.s[c];
+    }
   }
-}
+};
 
 // Candidate residuals of the truncation search. e (N) holds B*sol_{r0} - rhs for the
 // current top rank r0; candidate c removes component j_c = r0-1-c:
 //   e_c = e_{c-1} - s[j_c] * g_c,   res_c = ||e_c|| / nrhs.
 // e is left holding the last candidate's residual vector.
-__global__ __launch_bounds__(256) void k_trunc_prefix(double* e, const double* g, int64_t N,
-                                                      int nv, const double* sv, int j0,
-                                                      double* partial, unsigned* counter,
-                                                      double* out, const double* nrhs) {
+struct KTruncPrefix {
+  struct Args {
+    double* e; const double* g; int64_t N; int nv; const double* sv; int j0; double* partial;
+    unsigned* counter; double* out; const double* nrhs;
+  };
+  static constexpr int kThreads = 256;
+  static __device__ void run(const Args& a, const Blk& bk);
+};
+__device__ void KTruncPrefix::run(const Args& a, const Blk& bk) {
+  double* e = a.e; const double* g = a.g; const int64_t N = a.N; const int nv = a.nv;
+  const double* sv = a.sv; const int j0 = a.j0; double* partial = a.partial;
+  unsigned* counter = a.counter; double* out = a.out; const double* nrhs = a.nrhs;
   __shared__ double sh[4][32];
   __shared__ bool last;
   double acc[32];
@@ -119,7 +131,7 @@ __global__ __launch_bounds__(256) void k_trunc_prefix(double* e, const double* g
   double sc[32];
 #pragma unroll
   for (int c = 0; c < 32; ++c) sc[c] = (c < nv) ? sv[j0 - c] : 0.0;
-  for (int64_t i = blockIdx.x * 256 + threadIdx.x; i < N; i += (int64_t)gridDim.x * 256) {
+  for (int64_t i = bk.x * 256 + threadIdx.x; i < N; i += (int64_t)bk.gx * 256) {
     double v = e[i];
 #pragma unroll
     for (int c = 0; c < 32; ++c) {
@@ -140,14 +152,14 @@ __global__ __launch_bounds__(256) void k_trunc_prefix(double* e, const double* g
   }
   __syncthreads();
   if (threadIdx.x < 32) {
-    partial[blockIdx.x * 32 + threadIdx.x] =
+    partial[bk.x * 32 + threadIdx.x] =
         sh[0][threadIdx.x] + sh[1][threadIdx.x] + sh[2][threadIdx.x] + sh[3][threadIdx.x];
     __threadfence();
   }
   __syncthreads();
   if (threadIdx.x == 0) {
     unsigned prev = atomicAdd(counter, 1u);
-    last = (prev == gridDim.x - 1);
+    last = (prev == bk.gx - 1);
   }
   __syncthreads();
   if (!last) return;
@@ -155,15 +167,13 @@ __global__ __launch_bounds__(256) void k_trunc_prefix(double* e, const double* g
   if (threadIdx.x < nv) {
     volatile double* vp = partial;
     double s = 0.0;
-    for (unsigned b = 0; b < gridDim.x; ++b) s += vp[b * 32 + threadIdx.x];
+    for (unsigned b = 0; b < bk.gx; ++b) s += vp[b * 32 + threadIdx.x];
     out[threadIdx.x] = sqrt(s) / (*nrhs);
   }
   __syncthreads();
   if (threadIdx.x == 0) *counter = 0u;
 }
 
-// S[dst] = S[a] / S[b]
-__global__ void k_sdiv(double* S, int dst, int a, int b) { S[dst] = S[a] / S[b]; }
 
 // Normal random numbers: SplitMix64 counter stream + Box-Muller.
 __device__ __forceinline__ uint64_t splitmix64(uint64_t x) {
@@ -172,23 +182,23 @@ __device__ __forceinline__ uint64_t splitmix64(uint64_t x) {
   x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
   return x ^ (x >> 31);
 }
-__global__ void k_randn(double* x, int64_t n, uint64_t seed, uint64_t stream_id) {
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x) {
-    uint64_t a = splitmix64(seed ^ splitmix64(stream_id * 0x100000001B3ull + (uint64_t)i * 2));
-    uint64_t b = splitmix64(a + 0x632BE59BD9B4E019ull);
-    double u1 = ((a >> 11) + 1.0) * (1.0 / 9007199254740993.0);
-    double u2 = (b >> 11) * (1.0 / 9007199254740992.0);
-    x[i] = sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
+struct KRandn {
+  struct Args { double* x; int64_t n; uint64_t seed, stream_id; };
+  static constexpr int kThreads = 256;
+  static __device__ void run(const Args& p, const Blk& bk) {
+    for (int64_t i = bk.x * (int64_t)blockDim.x + threadIdx.x; i < p.n;
+         i += (int64_t)bk.gx * blockDim.x) {
+      uint64_t a = splitmix64(p.seed ^ splitmix64(p.stream_id * 0x100000001B3ull + (uint64_t)i * 2));
+      uint64_t b = splitmix64(a + 0x632BE59BD9B4E019ull);
+      double u1 = ((a >> 11) + 1.0) * (1.0 / 9007199254740993.0);
+      double u2 = (b >> 11) * (1.0 / 9007199254740992.0);
+      p.x[i] = sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
+    }
   }
-}
+};
 
-// One pinned mailbox per host thread, created on first use and kept for the process.
-static HostSync& host_sync() {
-  static thread_local HostSync hs;
-  hs.init();
-  return hs;
-}
+// Pinned mailbox of this solve: per cohort member, else per host thread.
+static HostSync& host_sync() { return member_host_sync(); }
 
 static int grid_of(int64_t n) {
   int64_t g = (n + 1023) / 1024;
@@ -311,7 +321,7 @@ class Solver {
 
   void run() {
     if (ws_.measuring()) return;
-    TT_HIP(hipMemsetAsync(rs_.counter, 0, 64 * sizeof(unsigned), st_));
+    zero_words(rs_.counter, 64, st_);
     for (int k = 0; k < d; ++k) permute_core_A(A_[k], Apm_[k], (int)P.rA[k], (int)P.N[k], (int)P.rA[k + 1], st_);
     if (O.precision == 1)
       for (int k = 0; k < d; ++k)
@@ -332,7 +342,7 @@ class Solver {
       int64_t n = rz_[k] * P.N[k] * rz_[k + 1];
       if (z0_) copy(z0_[k], z_[k], n, st_);
       else {
-        hipLaunchKernelGGL(k_randn, dim3(grid_of(n)), dim3(256), 0, st_, z_[k], n, O.seed, (uint64_t)k);
+        launch<KRandn>(dim3(grid_of(n)), dim3(256), 0, st_, {z_[k], n, O.seed, (uint64_t)k});
         TT_LAUNCH_CHECK();
       }
     }
@@ -357,8 +367,7 @@ class Solver {
       scalar_set(S_, TMP(T_MAXRES), 0.0, st_);
       for (int k = 0; k < d; ++k) lr_step(k, last);
       // max_res of this sweep
-      TT_HIP(hipMemcpyAsync(host_sync().host_dbl(), S_ + TMP(T_MAXRES), sizeof(double), hipMemcpyDeviceToHost, st_));
-      TT_HIP(hipStreamSynchronize(st_));
+      read_to_host(host_sync().host_dbl(), S_ + TMP(T_MAXRES), sizeof(double), st_);
       const double max_res = host_sync().host_dbl()[0];
       stats_.max_res_last = max_res;
       if (O.verbose) {
@@ -375,8 +384,7 @@ class Solver {
     }
     // rescale (reference :534-541)
     if (d > 1) {
-      TT_HIP(hipMemcpyAsync(host_sync().host_dbl(), S_ + NORMX(0), sizeof(double) * (d - 1), hipMemcpyDeviceToHost, st_));
-      TT_HIP(hipStreamSynchronize(st_));
+      read_to_host(host_sync().host_dbl(), S_ + NORMX(0), sizeof(double) * (d - 1), st_);
     }
     if (g_prof.enabled) {
       std::lock_guard<std::mutex> lk(g_prof.mu);
@@ -400,7 +408,7 @@ class Solver {
       off += n;
     }
     for (int k = 0; k <= d; ++k) rx_out[k] = rx_[k];
-    TT_HIP(hipStreamSynchronize(st_));
+    sync_point(st_);
   }
 
   ttamen_stats stats_{};
@@ -437,7 +445,7 @@ class Solver {
               bool timed = false) {
     hipEvent_t e0 = nullptr, e1 = nullptr;
     std::unique_lock<std::mutex> plk(g_prof.mu, std::defer_lock);
-    if (timed && g_prof.enabled) {
+    if (timed && g_prof.enabled && !current_recorder()) {
       plk.lock();
       e0 = g_prof.get();
       e1 = g_prof.get();
@@ -587,8 +595,8 @@ class Solver {
     size_t mk = ws_.mark();
     const int64_t nc = ld - c0;
     double* t = ws_.alloc<double>(rows * nc);
-    hipLaunchKernelGGL(k_randn, dim3(grid_of(rows * nc)), dim3(256), 0, st_, t, rows * nc,
-                       O.seed, (uint64_t)(1000003 * (++rand_calls_) + k));
+    launch<KRandn>(dim3(grid_of(rows * nc)), dim3(256), 0, st_,
+                   {t, rows * nc, O.seed, (uint64_t)(1000003 * (++rand_calls_) + k)});
     TT_LAUNCH_CHECK();
     copy2d(t, nc, 1, buf + c0, ld, rows, nc, st_);
     ws_.release(mk);
@@ -636,8 +644,7 @@ class Solver {
       matvec(op, prev, e, 1, nullptr, true);
       sub(rhs, e, drhs, Nl, st_);
       norm2(drhs, Nl, sc(TMP(T_DNORM)), rs_, st_);
-      hipLaunchKernelGGL(k_sdiv, dim3(1), dim3(1), 0, st_, S_, TMP(T_RESOLD), TMP(T_DNORM), TMP(T_NRHS));
-      TT_LAUNCH_CHECK();
+      scalar_div(S_, TMP(T_RESOLD), TMP(T_DNORM), TMP(T_NRHS), st_);
       fill(sol, Nl, 0.0, st_);
       GmresDev g = gmres_alloc(ws_, (int)O.local_iterations, Nl, sizeof(double));
       GmresOp<double> gop;
@@ -658,8 +665,7 @@ class Solver {
       axpby(-1.0, rhs, 1.0, e, Nl, st_);
     }
     norm2(e, Nl, sc(TMP(T_RESNEW)), rs_, st_);
-    hipLaunchKernelGGL(k_sdiv, dim3(1), dim3(1), 0, st_, S_, TMP(T_RESNEW), TMP(T_RESNEW), TMP(T_NRHS));
-    TT_LAUNCH_CHECK();
+    scalar_div(S_, TMP(T_RESNEW), TMP(T_RESNEW), TMP(T_NRHS), st_);
     // max_res = max(max_res, res_old)
     scalar_maxdiv(S_, TMP(T_MAXRES), TMP(T_RESOLD), TMP(T_ONE), st_);
     if (O.verbose) {
@@ -682,7 +688,7 @@ class Solver {
       r = std::max<int64_t>(r, 1);
       u = U; ldu = kc;
       vmat = ws_.alloc<double>(r2 * r);
-      hipLaunchKernelGGL(k_scale_cols, dim3(grid_of(r2 * r)), dim3(256), 0, st_, V, kc, sv, vmat, r, r2, (int)r);
+      launch<KScaleCols>(dim3(grid_of(r2 * r)), dim3(256), 0, st_, {V, kc, sv, vmat, r, r2, (int)r});
       TT_LAUNCH_CHECK();
     } else {
       const int64_t rows = r1 * n;
@@ -811,8 +817,7 @@ class Solver {
   }
 
   void sync_print_local(int k, int64_t Nl) {
-    TT_HIP(hipMemcpyAsync(host_sync().host_dbl(), S_, sizeof(double) * nscal_, hipMemcpyDeviceToHost, st_));
-    TT_HIP(hipStreamSynchronize(st_));
+    read_to_host(host_sync().host_dbl(), S_, sizeof(double) * nscal_, st_);
     const double* h = host_sync().host_dbl();
     printf("\tCore %d (local size %lld): res_old = %g, res_new = %g, dx = %g\n", k, (long long)Nl,
            h[TMP(T_RESOLD)], h[TMP(T_RESNEW)], h[TMP(T_DX)] / h[TMP(T_SOLN)]);
@@ -830,8 +835,7 @@ class Solver {
     matvec(op, prev, e, 1, nullptr, true);
     sub(rhs, e, r, Nl, st_);
     norm2(r, Nl, sc(TMP(T_DNORM)), rs_, st_);
-    hipLaunchKernelGGL(k_sdiv, dim3(1), dim3(1), 0, st_, S_, TMP(T_RESOLD), TMP(T_DNORM), TMP(T_NRHS));
-    TT_LAUNCH_CHECK();
+    scalar_div(S_, TMP(T_RESOLD), TMP(T_DNORM), TMP(T_NRHS), st_);
     copy(prev, sol, Nl, st_);
     const int64_t r1 = op.r1, r2 = op.r2, s = op.s, S = op.S;
     float* pl = ws_.alloc<float>(r1 * s * r1);
@@ -849,7 +853,7 @@ class Solver {
     gop.apply = [&](const float* xin, float* yout, const int* skip) {
       local_matvec_f32(op32, xin, yout, 1, ws_, st_, skip);
     };
-    const bool prof = g_prof.enabled;
+    const bool prof = g_prof.enabled && !current_recorder();
     if (prof) {
       gop.apply_step = [&](const float* xin, float* yout, const int* skip, int cyc, int step) {
         std::lock_guard<std::mutex> lk(g_prof.mu);
@@ -866,14 +870,13 @@ class Solver {
     gop.axpby = [&](double a, const float* xx, double b, float* yy, int64_t nn) { axpby_f32(a, xx, b, yy, nn, st_); };
     gop.norm = [&](const float* xx, int64_t nn, double* slot) { norm2_f32(xx, nn, slot, rs_, st_); };
     for (int it = 0; it < kMaxRefine; ++it) {
-      TT_HIP(hipMemcpyAsync(host_sync().host_dbl(), S_ + TMP(0), sizeof(double) * 4, hipMemcpyDeviceToHost, st_));
-      TT_HIP(hipStreamSynchronize(st_));
+      read_to_host(host_sync().host_dbl(), S_ + TMP(0), sizeof(double) * 4, st_);
       const double nrhs = host_sync().host_dbl()[T_NRHS], rn = host_sync().host_dbl()[T_DNORM];
       if (!(rn > real_tol * nrhs)) break;
       const double tau = std::min(0.5, std::max(kInnerTolFloor, 0.5 * real_tol * nrhs / rn));
       cast_f64_f32(r, r32, Nl, st_);
       norm2_f32(r32, Nl, sc(TMP(T_R32)), rs_, st_);
-      TT_HIP(hipMemsetAsync(d32, 0, sizeof(float) * Nl, st_));
+      zero_words(d32, Nl, st_);
       GmresResult gr = gmres_run<float>(g, Nl, d32, r32, sc(TMP(T_R32)), sc(TMP(T_R32)), tau,
                                         (int)O.resets, gop, rs_, host_sync(), st_);
       if (prof) {
@@ -917,8 +920,7 @@ class Solver {
     double* t2 = ws_.alloc<double>(kNV * r1 * n * S);
     double* pv = ws_.alloc<double>(l2 * S * kNV);
     // Read res_new (and res_old for verbose) once.
-    TT_HIP(hipMemcpyAsync(host_sync().host_dbl(), S_ + TMP(0), sizeof(double) * 8, hipMemcpyDeviceToHost, st_));
-    TT_HIP(hipStreamSynchronize(st_));
+    read_to_host(host_sync().host_dbl(), S_ + TMP(0), sizeof(double) * 8, st_);
     const double res_new = host_sync().host_dbl()[T_RESNEW];
     if (O.verbose) {
       const double* h = host_sync().host_dbl();
@@ -969,11 +971,10 @@ class Solver {
         gemm_f64(gd, ws_, st_);
       }
       stats_.matvec_flops += nv * 2.0 * (double)n * (s * r1 * r1 + n * s * S * r1 + S * r1 * l2) + 2.0 * nv * l2 * S * r2;
-      hipLaunchKernelGGL(k_trunc_prefix, dim3(std::min<int64_t>(1024, (Nl + 1023) / 1024)), dim3(256), 0, st_,
-                         e, g, Nl, nv, sv, j0, rs_.partial, rs_.counter, S_ + TMP(T_CAND), S_ + TMP(T_NRHS));
+      launch<KTruncPrefix>(dim3(std::min<int64_t>(1024, (Nl + 1023) / 1024)), dim3(256), 0, st_,
+                           {e, g, Nl, nv, sv, j0, rs_.partial, rs_.counter, S_ + TMP(T_CAND), S_ + TMP(T_NRHS)});
       TT_LAUNCH_CHECK();
-      TT_HIP(hipMemcpyAsync(host_sync().host_dbl(), S_ + TMP(T_CAND), sizeof(double) * nv, hipMemcpyDeviceToHost, st_));
-      TT_HIP(hipStreamSynchronize(st_));
+      read_to_host(host_sync().host_dbl(), S_ + TMP(T_CAND), sizeof(double) * nv, st_);
       const double* res = host_sync().host_dbl();
       for (int c = 0; c < nv; ++c) {
         const int64_t rc = rtop - 1 - c;
@@ -1103,20 +1104,13 @@ class FirstFit {
   std::map<size_t, size_t> free_;
 };
 
-// Streams for the batch workers: created once per device, default priority, reused.
-static std::vector<hipStream_t> stream_pool(int dev, int n) {
-  static std::mutex mu;
-  static std::vector<std::vector<hipStream_t>> pools;
-  std::lock_guard<std::mutex> lk(mu);
-  if ((int)pools.size() <= dev) pools.resize(dev + 1);
-  auto& p = pools[dev];
-  while ((int)p.size() < n) {
-    hipStream_t s;
-    TT_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
-    p.push_back(s);
-  }
-  return std::vector<hipStream_t>(p.begin(), p.begin() + n);
-}
+// Device staging for the argument tables of a cohort's grouped launches (taken from
+// the front of the batch / serving workspace).
+static constexpr size_t kStageBytes = 32ull << 20;
+
+static std::mutex g_err_mu;
+static std::mutex g_cohort_mu;
+static Cohort::Counters g_cohort_last;
 
 }  // namespace ttamen
 
@@ -1137,9 +1131,101 @@ static int guarded(F&& f) {
   }
 }
 
+extern "C" size_t ttamen_amen_solve_workspace_size(int d, const int64_t* N, const int64_t* rA,
+                                                   const int64_t* rb, const int64_t* rx0, int64_t rmax,
+                                                   int64_t kickrank, int64_t kick2,
+                                                   int64_t local_iterations, int64_t max_full,
+                                                   int precision);
+
+// Shared by the batched and the serving entry: run `count` systems as one cohort on
+// `stream`, admitting them in index order as `ready(i)` allows and memory frees up.
+template <class Ready, class Started, class Done>
+static void run_cohort(ttamen_system* systems, int count, int64_t nswp, double eps,
+                       int64_t max_full, int64_t kickrank, int64_t kick2, int64_t local_iterations,
+                       int64_t resets, int precision, int workers, int slots, void* workspace,
+                       size_t workspace_bytes, hipStream_t stream, Ready ready, Started started,
+                       Done done) {
+  TT_CHECK(workspace_bytes > kStageBytes, ERR_WORKSPACE, "workspace smaller than the staging area");
+  linalg_init();
+  lu_init();
+  std::vector<size_t> need(count);
+  for (int i = 0; i < count; ++i) {
+    const ttamen_system& y = systems[i];
+    size_t w = ttamen_amen_solve_workspace_size(y.d, y.N, y.rA, y.rb, y.x0_cores ? y.rx0 : nullptr,
+                                                y.rmax, kickrank, kick2, local_iterations, max_full,
+                                                precision);
+    TT_CHECK(w > 0, ERR_INVALID, "system " + std::to_string(i) + ": " + g_last_error);
+    need[i] = batch_slice(w);
+    TT_CHECK(need[i] <= workspace_bytes - kStageBytes, ERR_WORKSPACE,
+             "system " + std::to_string(i) + " does not fit the workspace");
+  }
+  char* stage = (char*)workspace;
+  char* base = stage + kStageBytes;
+  FirstFit mem(workspace_bytes - kStageBytes);
+  slots = std::max(1, std::min(slots, kMaxGroup));
+  std::string first_err;
+  int next = 0;
+  Cohort co(stream, stage, kStageBytes, workers);
+  auto admit = [&](Cohort& c) -> bool {
+    while (next < count && c.members() < slots && ready(next, c.members())) {
+      size_t off;
+      if (!mem.alloc(need[next], &off)) break;
+      const int i = next++;
+      started(i);
+      ttamen_system* y = &systems[i];
+      char* slice = base + off;
+      const size_t bytes = need[i];
+      c.add(
+          [=, &first_err] {
+            y->status = guarded([&] {
+              Options O = make_options(nswp, eps, y->rmax, max_full, kickrank, kick2,
+                                       local_iterations, resets, 0, precision, y->seed);
+              solve_one(y->d, y->N, y->rA, y->rb, y->A_cores, y->b_cores, y->x0_cores, y->rx0,
+                        y->z0_cores, O, slice, bytes, y->x_out, y->x_out_elems, y->rx_out,
+                        y->offsets_out, y->stats, stream);
+            });
+            if (y->status != 0) {
+              std::lock_guard<std::mutex> lk(g_err_mu);
+              if (first_err.empty()) first_err = "system " + std::to_string(i) + ": " + g_last_error;
+            }
+          },
+          [=, &mem] {
+            mem.release(off, bytes);
+            done(i);
+          });
+    }
+    return next < count;
+  };
+  co.run(admit);
+  {
+    std::lock_guard<std::mutex> lk(g_cohort_mu);
+    g_cohort_last = co.counters();
+  }
+  TT_CHECK(first_err.empty(), ERR_NUMERIC, first_err);
+}
+
 extern "C" {
 
 const char* ttamen_version(void) { return "ttamen 0.1.0 (gfx950)"; }
+size_t ttamen_cohort_staging_bytes(void) { return kStageBytes; }
+
+int ttamen_cohort_set_params(int window, int max_age, int gmres_poll) {
+  if (window < 1 || max_age < 1 || gmres_poll < 1) return ERR_INVALID;
+  CohortParams& p = cohort_params();
+  p.window = window;
+  p.max_age = max_age;
+  p.gmres_poll = gmres_poll;
+  return 0;
+}
+
+int ttamen_cohort_stats(double* out8) {
+  if (!out8) return ERR_INVALID;
+  std::lock_guard<std::mutex> lk(g_cohort_mu);
+  const Cohort::Counters& c = g_cohort_last;
+  const double v[8] = {c.rounds, c.launches, c.items, c.resumes, c.t_resume, c.t_build, c.t_gpu, c.t_total};
+  for (int i = 0; i < 8; ++i) out8[i] = v[i];
+  return 0;
+}
 const char* ttamen_last_error(void) { return g_last_error.c_str(); }
 
 size_t ttamen_amen_solve_workspace_size(int d, const int64_t* N, const int64_t* rA,
@@ -1196,16 +1282,19 @@ size_t ttamen_amen_solve_batch_workspace_size(const ttamen_system* systems, int 
   size_t out = 0;
   int rc = guarded([&] {
     TT_CHECK(systems && count >= 0 && workers >= 1, ERR_INVALID, "bad batch arguments");
-    size_t mx = 0;
+    std::vector<size_t> need;
     for (int i = 0; i < count; ++i) {
       const ttamen_system& y = systems[i];
       size_t w = ttamen_amen_solve_workspace_size(y.d, y.N, y.rA, y.rb, y.x0_cores ? y.rx0 : nullptr,
                                                   y.rmax, kickrank, kick2, local_iterations,
                                                   max_full, precision);
       TT_CHECK(w > 0, ERR_INVALID, "system " + std::to_string(i) + ": " + g_last_error);
-      mx = std::max(mx, w);
+      need.push_back(batch_slice(w));
     }
-    out = batch_slice(mx) * (size_t)std::min(workers, std::max(count, 1));
+    // room for every system at once (at most kMaxGroup of them run together)
+    std::sort(need.begin(), need.end(), std::greater<size_t>());
+    out = kStageBytes;
+    for (int i = 0; i < std::min<int>(count, kMaxGroup); ++i) out += need[i];
   });
   return rc == 0 ? out : 0;
 }
@@ -1217,47 +1306,9 @@ int ttamen_amen_solve_batch(ttamen_system* systems, int count, int64_t nswp, dou
   return guarded([&] {
     TT_CHECK(systems && count >= 0 && workers >= 1 && workspace, ERR_INVALID, "bad batch arguments");
     if (count == 0) return;
-    workers = std::min(workers, count);
-    const size_t slice = workspace_bytes / (size_t)workers / 256 * 256;
-    linalg_init();
-    lu_init();
-    int dev = 0;
-    TT_HIP(hipGetDevice(&dev));
-    std::vector<hipStream_t> streams = stream_pool(dev, workers);
-    // Inputs were produced on the caller's stream: every worker stream waits for it.
-    hipEvent_t ready;
-    TT_HIP(hipEventCreateWithFlags(&ready, hipEventDisableTiming));
-    TT_HIP(hipEventRecord(ready, (hipStream_t)stream));
-    for (auto s : streams) TT_HIP(hipStreamWaitEvent(s, ready, 0));
-    std::atomic<int> next(0);
-    std::mutex err_mu;
-    std::string first_err;
-    auto work = [&](int w) {
-      (void)hipSetDevice(dev);
-      char* base = (char*)workspace + (size_t)w * slice;
-      for (;;) {
-        const int i = next.fetch_add(1);
-        if (i >= count) break;
-        ttamen_system& y = systems[i];
-        y.status = guarded([&] {
-          Options O = make_options(nswp, eps, y.rmax, max_full, kickrank, kick2, local_iterations,
-                                   resets, 0, precision, y.seed);
-          solve_one(y.d, y.N, y.rA, y.rb, y.A_cores, y.b_cores, y.x0_cores, y.rx0, y.z0_cores, O,
-                    base, slice, y.x_out, y.x_out_elems, y.rx_out, y.offsets_out, y.stats,
-                    streams[w]);
-        });
-        if (y.status != 0) {
-          std::lock_guard<std::mutex> lk(err_mu);
-          if (first_err.empty()) first_err = "system " + std::to_string(i) + ": " + g_last_error;
-        }
-      }
-    };
-    std::vector<std::thread> pool;
-    for (int w = 1; w < workers; ++w) pool.emplace_back(work, w);
-    work(0);
-    for (auto& t : pool) t.join();
-    (void)hipEventDestroy(ready);
-    TT_CHECK(first_err.empty(), ERR_NUMERIC, first_err);
+    run_cohort(systems, count, nswp, eps, max_full, kickrank, kick2, local_iterations, resets,
+               precision, workers, kMaxGroup, workspace, workspace_bytes, (hipStream_t)stream,
+               [](int, int) { return true; }, [](int) {}, [](int) {});
   });
 }
 
@@ -1272,85 +1323,24 @@ int ttamen_serve(ttamen_system* systems, const double* arrival_s, int count, int
     if (count == 0) return;
     for (int i = 1; i < count; ++i)
       TT_CHECK(arrival_s[i] >= arrival_s[i - 1], ERR_INVALID, "arrivals must be sorted");
-    linalg_init();
-    lu_init();
-    int dev = 0;
-    TT_HIP(hipGetDevice(&dev));
-    // footprint of every request = its solver workspace (rank-cap bound)
-    std::vector<size_t> need(count);
-    for (int i = 0; i < count; ++i) {
-      const ttamen_system& y = systems[i];
-      size_t w = ttamen_amen_solve_workspace_size(y.d, y.N, y.rA, y.rb, nullptr, y.rmax, kickrank,
-                                                  kick2, local_iterations, max_full, precision);
-      TT_CHECK(w > 0, ERR_INVALID, "request " + std::to_string(i) + ": " + g_last_error);
-      need[i] = batch_slice(w);
-      TT_CHECK(need[i] <= workspace_bytes, ERR_WORKSPACE,
-               "request " + std::to_string(i) + " does not fit the serving memory");
-    }
-    const int nslots = std::min(slots, count);
-    std::vector<hipStream_t> streams = stream_pool(dev, nslots);
-    hipEvent_t ready;
-    TT_HIP(hipEventCreateWithFlags(&ready, hipEventDisableTiming));
-    TT_HIP(hipEventRecord(ready, (hipStream_t)stream));
-    for (auto s : streams) TT_HIP(hipStreamWaitEvent(s, ready, 0));
-    TT_HIP(hipStreamSynchronize((hipStream_t)stream));
-
-    // Admission state: requests are admitted strictly in arrival order, when a slot is
-    // idle, the request has arrived and a contiguous region of its footprint is free.
-    std::mutex mu;
-    std::condition_variable cv;
-    int next = 0;
-    FirstFit mem(workspace_bytes);
-    std::string first_err;
     const auto t0 = std::chrono::steady_clock::now();
     auto now_s = [&] {
       return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     };
-    auto work = [&](int w) {
-      (void)hipSetDevice(dev);
-      for (;;) {
-        int i;
-        size_t off;
-        {
-          std::unique_lock<std::mutex> lk(mu);
-          for (;;) {
-            if (next >= count) return;
-            const double wait = arrival_s[next] - now_s();
-            if (wait > 0) {
-              cv.wait_for(lk, std::chrono::duration<double>(wait));
-              continue;
-            }
-            if (mem.alloc(need[next], &off)) break;
-            cv.wait(lk);  // memory frees when a running request finishes
-          }
-          i = next++;
-          start_s[i] = now_s();
-        }
-        cv.notify_all();
-        ttamen_system& y = systems[i];
-        y.status = guarded([&] {
-          Options O = make_options(nswp, eps, y.rmax, max_full, kickrank, kick2, local_iterations,
-                                   resets, 0, precision, y.seed);
-          solve_one(y.d, y.N, y.rA, y.rb, y.A_cores, y.b_cores, y.x0_cores, y.rx0, y.z0_cores, O,
-                    (char*)workspace + off, need[i], y.x_out, y.x_out_elems, y.rx_out,
-                    y.offsets_out, y.stats, streams[w]);
-        });
-        {
-          std::lock_guard<std::mutex> lk(mu);
-          done_s[i] = now_s();
-          mem.release(off, need[i]);
-          if (y.status != 0 && first_err.empty())
-            first_err = "request " + std::to_string(i) + ": " + g_last_error;
-        }
-        cv.notify_all();
+    // Requests are admitted in arrival order, once arrived, when a slot and a contiguous
+    // region of their footprint are free. With nothing running, wait for the next arrival.
+    auto ready = [&](int i, int running) {
+      double wait = arrival_s[i] - now_s();
+      if (wait > 0 && running == 0) {
+        std::this_thread::sleep_for(std::chrono::duration<double>(wait));
+        wait = 0;
       }
+      return wait <= 0;
     };
-    std::vector<std::thread> pool;
-    for (int w = 1; w < nslots; ++w) pool.emplace_back(work, w);
-    work(0);
-    for (auto& t : pool) t.join();
-    (void)hipEventDestroy(ready);
-    TT_CHECK(first_err.empty(), ERR_NUMERIC, first_err);
+    const int workers = std::max(1, std::min(8, slots));
+    run_cohort(systems, count, nswp, eps, max_full, kickrank, kick2, local_iterations, resets,
+               precision, workers, slots, workspace, workspace_bytes, (hipStream_t)stream, ready,
+               [&](int i) { start_s[i] = now_s(); }, [&](int i) { done_s[i] = now_s(); });
   });
 }
 

@@ -56,6 +56,7 @@ struct GemmParams {
   int scale_mode;
   const int* skip;
   T* partial;  // split-K scratch: [split][batch][M][N]
+  int tile;    // 0: 32x32, 1: 32x64, 2: 64x32, 3: 64x64 (BM x BN)
 };
 
 template <typename T>
@@ -66,23 +67,48 @@ __device__ __forceinline__ double eff_alpha(const GemmParams<T>& p) {
   return s;
 }
 
+// One tile shape; the LDS for the double-buffered A and B tiles is passed in.
 template <typename T, int BM, int BN, int BK>
-__global__ __launch_bounds__(256) void k_gemm(GemmParams<T> p) {
-  if (p.skip && *p.skip) return;
+struct GemmTile {
+  static constexpr int PA = 1, PB = 1;
+  static constexpr size_t kLds = sizeof(T) * 2 * BK * ((BM + PA) + (BN + PB));
+  static __device__ void body(const GemmParams<T>& p, const Blk& bk, char* smem);
+};
+
+// All tile shapes are one launch kind (the tile is chosen per launch by p.tile), so that
+// a cohort merges the GEMMs of systems whose shapes pick different tiles.
+template <typename T>
+struct KGemm {
+  typedef GemmParams<T> Args;
+  static constexpr int kThreads = 256;
+  static __device__ void run(const Args& pp, const Blk& bk) {
+    __shared__ __attribute__((aligned(16))) char smem[GemmTile<T, 64, 64, 16>::kLds];
+    const GemmParams<T> p = pp;
+    if (p.skip && *p.skip) return;
+    switch (p.tile) {
+      case 0: GemmTile<T, 32, 32, 16>::body(p, bk, smem); break;
+      case 1: GemmTile<T, 32, 64, 16>::body(p, bk, smem); break;
+      case 2: GemmTile<T, 64, 32, 16>::body(p, bk, smem); break;
+      default: GemmTile<T, 64, 64, 16>::body(p, bk, smem); break;
+    }
+  }
+};
+
+template <typename T, int BM, int BN, int BK>
+__device__ void GemmTile<T, BM, BN, BK>::body(const GemmParams<T>& p, const Blk& bk, char* smem) {
   constexpr int MI = BM / 32, NI = BN / 32;  // 16x16 blocks per wave
   constexpr int AE = BM * BK / 256, BE = BK * BN / 256;
-  constexpr int PA = 1, PB = 1;
-  __shared__ T As[2][BK][BM + PA];
-  __shared__ T Bs[2][BK][BN + PB];
+  T (*As)[BK][BM + PA] = reinterpret_cast<T (*)[BK][BM + PA]>(smem);
+  T (*Bs)[BK][BN + PB] = reinterpret_cast<T (*)[BK][BN + PB]>(smem + sizeof(T) * 2 * BK * (BM + PA));
   typedef typename AccT<T>::type acc_t;
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave >> 1, wn = wave & 1;
   const int tiles_n = (p.N + BN - 1) / BN;
-  const int tm = blockIdx.x / tiles_n, tn = blockIdx.x - tm * tiles_n;
-  const int bidx = blockIdx.y;
+  const int tm = bk.x / tiles_n, tn = bk.x - tm * tiles_n;
+  const int bidx = bk.y;
   const int b1 = bidx / p.B2, b2 = bidx - b1 * p.B2;
-  const int64_t k0 = (int64_t)blockIdx.z * p.kchunk;
+  const int64_t k0 = (int64_t)bk.z * p.kchunk;
   const int64_t k1 = min((int64_t)p.K, k0 + p.kchunk);
   const T* A = p.a + b1 * p.ab1 + b2 * p.ab2;
   const T* B = p.b + b1 * p.bb1 + b2 * p.bb2;
@@ -172,7 +198,7 @@ __global__ __launch_bounds__(256) void k_gemm(GemmParams<T> p) {
   }
 
   if (p.partial) {
-    T* P = p.partial + ((int64_t)blockIdx.z * p.B + bidx) * (int64_t)p.M * p.N;
+    T* P = p.partial + ((int64_t)bk.z * p.B + bidx) * (int64_t)p.M * p.N;
 #pragma unroll
     for (int mi = 0; mi < MI; ++mi)
 #pragma unroll
@@ -206,13 +232,22 @@ __global__ __launch_bounds__(256) void k_gemm(GemmParams<T> p) {
 
 // Sum split-K partials in split order and apply alpha / beta.
 template <typename T>
-__global__ void k_gemm_reduce(GemmParams<T> p, int splits) {
+struct KGemmReduce {
+  struct Args { GemmParams<T> p; int splits; };
+  static constexpr int kThreads = 256;
+  static __device__ void run(const Args& a, const Blk& bk);
+};
+
+template <typename T>
+__device__ void KGemmReduce<T>::run(const Args& a, const Blk& bk) {
+  const GemmParams<T> p = a.p;
+  const int splits = a.splits;
   if (p.skip && *p.skip) return;
   const int64_t MN = (int64_t)p.M * p.N;
   const int64_t total = MN * p.B;
   const double al = eff_alpha(p);
-  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
-       t += (int64_t)gridDim.x * blockDim.x) {
+  for (int64_t t = bk.x * (int64_t)blockDim.x + threadIdx.x; t < total;
+       t += (int64_t)bk.gx * blockDim.x) {
     int64_t bidx = t / MN, r = t - bidx * MN;
     int i = (int)(r / p.N), j = (int)(r - (int64_t)i * p.N);
     double s = 0.0;
@@ -227,10 +262,10 @@ __global__ void k_gemm_reduce(GemmParams<T> p, int splits) {
 
 template <typename T, int BM, int BN>
 static void launch_tile(GemmParams<T>& p, int splits, hipStream_t st) {
-  constexpr int BK = 16;
   int tiles = ((p.M + BM - 1) / BM) * ((p.N + BN - 1) / BN);
   dim3 grid(tiles, p.B, splits);
-  hipLaunchKernelGGL((k_gemm<T, BM, BN, BK>), grid, dim3(256), 0, st, p);
+  p.tile = (BM == 32 ? 0 : 2) + (BN == 32 ? 0 : 1);
+  launch<KGemm<T>>(grid, dim3(256), 0, st, p);
   TT_LAUNCH_CHECK();
 }
 
@@ -279,7 +314,7 @@ static void gemm_impl(const GemmDesc& g, Arena& ws, hipStream_t st) {
     if (splits > 1) {
       int64_t total = (int64_t)g.M * g.N * p.B;
       int blocks = (int)std::min<int64_t>((total + 255) / 256, 4096);
-      hipLaunchKernelGGL((k_gemm_reduce<T>), dim3(blocks), dim3(256), 0, st, p, splits);
+      launch<KGemmReduce<T>>(dim3(blocks), dim3(256), 0, st, {p, splits});
       TT_LAUNCH_CHECK();
     }
   }

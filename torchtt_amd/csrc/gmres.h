@@ -53,13 +53,14 @@ class HostSync {
   }
   void post(const int* dflag, int idx, hipStream_t st) {
     int s = idx % NS;
-    TT_HIP(hipMemcpyAsync(host_ + s, dflag, sizeof(int), hipMemcpyDeviceToHost, st));
-    TT_HIP(hipEventRecord(ev_[s], st));
+    post_to_host(host_ + s, dflag, sizeof(int), st);
+    if (!current_recorder()) TT_HIP(hipEventRecord(ev_[s], st));
   }
-  bool wait(int idx) {
+  bool wait(int idx, hipStream_t st) {
     if (idx < 0) return false;
     int s = idx % NS;
-    TT_HIP(hipEventSynchronize(ev_[s]));
+    if (current_recorder()) sync_point(st);
+    else TT_HIP(hipEventSynchronize(ev_[s]));
     return host_[s] != 0;
   }
   int* host_int() { return host_ + NS; }

@@ -18,6 +18,7 @@
 // inner solve); dot products and the Hessenberg are accumulated in double.
 #include "ttamen_internal.h"
 #include "gmres.h"
+#include "cohort.h"
 #include <algorithm>
 #include <cmath>
 
@@ -72,43 +73,61 @@ __device__ bool gm_need_reorth(const double* Pw, const double* Pn1, int nb) {
 // P[i][b] = Q_i . w over block b's chunk (i < kc), P[kc][b] = w . w. Each wave streams
 // whole rows over the chunk (coalesced); the chunk of w stays in L1.
 template <typename T>
-__global__ __launch_bounds__(kGB) void k_cgs_dots(const T* Q, int64_t N, int kc, const T* w,
-                                                  double* P, int with_ww, const int* skip,
-                                                  const double* Pw, const double* Pn1) {
-  if (gm_skip(skip)) return;
-  if (Pw && !gm_need_reorth(Pw, Pn1, gridDim.x)) return;
+struct KCgsDots {
+  struct Args {
+    const T* Q; int64_t N; int kc; const T* w; double* P; int with_ww; const int* skip;
+    const double* Pw; const double* Pn1;
+  };
+  static constexpr int kThreads = kGB;
+  static __device__ void run(const Args& a, const Blk& bk);
+};
+template <typename T>
+__device__ void KCgsDots<T>::run(const Args& a, const Blk& bk) {
+  const T* Q = a.Q; const int64_t N = a.N; const int kc = a.kc; const T* w = a.w;
+  double* P = a.P;
+  if (gm_skip(a.skip)) return;
+  if (a.Pw && !gm_need_reorth(a.Pw, a.Pn1, bk.gx)) return;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int64_t chunk = (N + gridDim.x - 1) / gridDim.x;
-  const int64_t lo = blockIdx.x * chunk, hi = min(N, lo + chunk);
+  const int64_t chunk = (N + bk.gx - 1) / bk.gx;
+  const int64_t lo = bk.x * chunk, hi = min(N, lo + chunk);
+  const int with_ww = a.with_ww;
   const int rows = kc + (with_ww ? 1 : 0);
   for (int i = wv; i < rows; i += kGB / 64) {
     const T* row = (i < kc) ? Q + (int64_t)i * N : w;
     double acc = 0.0;
     for (int64_t j = lo + lane; j < hi; j += 64) acc += (double)row[j] * (double)w[j];
     acc = gm_wave_sum(acc);
-    if (lane == 0) P[(int64_t)i * kGmresMaxBlocks + blockIdx.x] = acc;
+    if (lane == 0) P[(int64_t)i * kGmresMaxBlocks + bk.x] = acc;
   }
 }
 
 // h = sum_b P[:, b]; w -= sum_i h_i Q_i over the block's chunk; Pn[b] = ||w chunk||^2.
 // Block 0 writes (pass 1) or adds (pass 2) h into the Hessenberg column.
 template <typename T>
-__global__ __launch_bounds__(kGB) void k_cgs_update(const T* Q, int64_t N, int kc, T* w,
-                                                    const double* P, double* Hcol, int accumulate,
-                                                    double* Pn, const int* skip, const double* Pw,
-                                                    const double* Pn1, int* reorth_count) {
-  if (gm_skip(skip)) return;
-  if (Pw && !gm_need_reorth(Pw, Pn1, gridDim.x)) return;
+struct KCgsUpdate {
+  struct Args {
+    const T* Q; int64_t N; int kc; T* w; const double* P; double* Hcol; int accumulate;
+    double* Pn; const int* skip; const double* Pw; const double* Pn1; int* reorth_count;
+  };
+  static constexpr int kThreads = kGB;
+  static __device__ void run(const Args& a, const Blk& bk);
+};
+template <typename T>
+__device__ void KCgsUpdate<T>::run(const Args& a, const Blk& bk) {
+  const T* Q = a.Q; const int64_t N = a.N; const int kc = a.kc; T* w = a.w;
+  double* Hcol = a.Hcol; const int accumulate = a.accumulate; int* reorth_count = a.reorth_count;
+  if (gm_skip(a.skip)) return;
+  if (a.Pw && !gm_need_reorth(a.Pw, a.Pn1, bk.gx)) return;
   __shared__ double hs[kGmresMaxCols + 1];
   __shared__ double sh[kGB / 64];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  gm_reduce_rows(P, kc, gridDim.x, hs);
-  if (blockIdx.x == 0) {
+  gm_reduce_rows(a.P, kc, bk.gx, hs);
+  if (bk.x == 0) {
     for (int i = threadIdx.x; i < kc; i += kGB) Hcol[i] = accumulate ? Hcol[i] + hs[i] : hs[i];
     if (reorth_count && threadIdx.x == 0) *reorth_count += 1;
   }
-  const int64_t chunk = (N + gridDim.x - 1) / gridDim.x;
-  const int64_t lo = blockIdx.x * chunk, hi = min(N, lo + chunk);
+  const int64_t chunk = (N + bk.gx - 1) / bk.gx;
+  const int64_t lo = bk.x * chunk, hi = min(N, lo + chunk);
   double acc = 0.0;
   for (int64_t j = lo + threadIdx.x; j < hi; j += kGB) {
     double v = (double)w[j];
@@ -122,26 +141,33 @@ __global__ __launch_bounds__(kGB) void k_cgs_update(const T* Q, int64_t N, int k
   if (threadIdx.x == 0) {
     double s = 0.0;
     for (int i = 0; i < kGB / 64; ++i) s += sh[i];
-    Pn[blockIdx.x] = s;
+    a.Pn[bk.x] = s;
   }
 }
 
 // q_{k+1} = w / ||w||; block 0 / thread 0 finishes the Hessenberg column with Givens
 // rotations (reference gmres.h:20-31, 106-129) and tests convergence.
 template <typename T>
-__global__ __launch_bounds__(kGB) void k_gmres_close(const T* w, T* qn, int64_t N,
-                                                     const double* Pw, const double* Pn1,
-                                                     const double* Pn2, GmresDev g, int k,
-                                                     const int* done_in) {
-  if (*done_in) return;
+struct KGmresClose {
+  struct Args {
+    const T* w; T* qn; int64_t N; const double* Pw; const double* Pn1; const double* Pn2;
+    GmresDev g; int k; const int* done_in;
+  };
+  static constexpr int kThreads = kGB;
+  static __device__ void run(const Args& a, const Blk& bk);
+};
+template <typename T>
+__device__ void KGmresClose<T>::run(const Args& a, const Blk& bk) {
+  const T* w = a.w; T* qn = a.qn; const int64_t N = a.N; const GmresDev g = a.g; const int k = a.k;
+  if (*a.done_in) return;
   __shared__ double nn[2];
-  const bool reorth = gm_need_reorth(Pw, Pn1, gridDim.x);
-  gm_reduce_rows(reorth ? Pn2 : Pn1, 1, gridDim.x, nn);
+  const bool reorth = gm_need_reorth(a.Pw, a.Pn1, bk.gx);
+  gm_reduce_rows(reorth ? a.Pn2 : a.Pn1, 1, bk.gx, nn);
   const double hn = sqrt(nn[0]);
   const double inv = (hn > 0.0) ? 1.0 / hn : 0.0;
-  for (int64_t j = blockIdx.x * (int64_t)kGB + threadIdx.x; j < N; j += (int64_t)gridDim.x * kGB)
+  for (int64_t j = bk.x * (int64_t)kGB + threadIdx.x; j < N; j += (int64_t)bk.gx * kGB)
     qn[j] = (T)((double)w[j] * inv);
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
+  if (bk.x == 0 && threadIdx.x == 0) {
     const int ld = g.m + 1;
     double* h = g.H + (int64_t)k * ld;
     h[k + 1] = hn;
@@ -175,12 +201,19 @@ __global__ __launch_bounds__(kGB) void k_gmres_close(const T* w, T* qn, int64_t 
 // Start of a cycle: r already holds rhs - Op(x0) and dstate[GM_RNORM] = ||r||.
 // q_0 = r / ||r||, beta = (||r||, 0, ...), done = (||r|| <= 0).
 template <typename T>
-__global__ __launch_bounds__(kGB) void k_gmres_begin(const T* r, T* q0, int64_t N, GmresDev g) {
+struct KGmresBegin {
+  struct Args { const T* r; T* q0; int64_t N; GmresDev g; };
+  static constexpr int kThreads = kGB;
+  static __device__ void run(const Args& a, const Blk& bk);
+};
+template <typename T>
+__device__ void KGmresBegin<T>::run(const Args& a, const Blk& bk) {
+  const T* r = a.r; T* q0 = a.q0; const int64_t N = a.N; const GmresDev g = a.g;
   const double rn = g.dstate[GM_RNORM];
   const double inv = (rn > 0.0) ? 1.0 / rn : 0.0;
-  for (int64_t j = blockIdx.x * (int64_t)kGB + threadIdx.x; j < N; j += (int64_t)gridDim.x * kGB)
+  for (int64_t j = bk.x * (int64_t)kGB + threadIdx.x; j < N; j += (int64_t)bk.gx * kGB)
     q0[j] = (T)((double)r[j] * inv);
-  if (blockIdx.x == 0) {
+  if (bk.x == 0) {
     for (int i = threadIdx.x; i <= g.m; i += kGB) g.beta[i] = 0.0;
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -194,8 +227,15 @@ __global__ __launch_bounds__(kGB) void k_gmres_begin(const T* r, T* q0, int64_t 
 }
 
 // thr = real_tol * norm_rhs / ||drhs||,  bnorm = ||drhs||  (reference amen_solve.h:340-343)
-__global__ void k_gmres_setup(GmresDev g, const double* norm_rhs, const double* dnorm,
-                              double real_tol) {
+struct KGmresSetup {
+  struct Args { GmresDev g; const double* norm_rhs; const double* dnorm; double real_tol; };
+  static constexpr int kThreads = 64;
+  static __device__ void run(const Args& a, const Blk&);
+};
+__device__ void KGmresSetup::run(const Args& a, const Blk&) {
+  if (threadIdx.x != 0) return;
+  const GmresDev g = a.g; const double* norm_rhs = a.norm_rhs; const double* dnorm = a.dnorm;
+  const double real_tol = a.real_tol;
   g.dstate[GM_BNORM] = *dnorm;
   g.dstate[GM_THR] = real_tol * (*norm_rhs) / (*dnorm);
   g.dstate[GM_RNORM] = *dnorm;
@@ -207,7 +247,14 @@ __global__ void k_gmres_setup(GmresDev g, const double* norm_rhs, const double* 
 }
 
 // y = H(0:kc,0:kc) \ beta(0:kc) (upper triangular), one thread.
-__global__ void k_gmres_y(GmresDev g) {
+struct KGmresY {
+  struct Args { GmresDev g; };
+  static constexpr int kThreads = 64;
+  static __device__ void run(const Args& a, const Blk&);
+};
+__device__ void KGmresY::run(const Args& a, const Blk&) {
+  if (threadIdx.x != 0) return;
+  const GmresDev g = a.g;
   const int kc = g.istate[GM_KCOLS];
   const int ld = g.m + 1;
   for (int i = kc - 1; i >= 0; --i) {
@@ -220,19 +267,32 @@ __global__ void k_gmres_y(GmresDev g) {
 
 // x += sum_{i<kc} y_i Q_i
 template <typename T>
-__global__ __launch_bounds__(kGB) void k_gmres_xupd(const T* Q, int64_t N, T* x, GmresDev g) {
+struct KGmresXupd {
+  struct Args { const T* Q; int64_t N; T* x; GmresDev g; };
+  static constexpr int kThreads = kGB;
+  static __device__ void run(const Args& a, const Blk& bk);
+};
+template <typename T>
+__device__ void KGmresXupd<T>::run(const Args& a, const Blk& bk) {
+  const T* Q = a.Q; const int64_t N = a.N; T* x = a.x; const GmresDev g = a.g;
   const int kc = g.istate[GM_KCOLS];
   __shared__ double y[kGmresMaxCols];
   for (int i = threadIdx.x; i < kc; i += kGB) y[i] = g.y[i];
   __syncthreads();
-  for (int64_t j = blockIdx.x * (int64_t)kGB + threadIdx.x; j < N; j += (int64_t)gridDim.x * kGB) {
+  for (int64_t j = bk.x * (int64_t)kGB + threadIdx.x; j < N; j += (int64_t)bk.gx * kGB) {
     double v = (double)x[j];
     for (int i = 0; i < kc; ++i) v += y[i] * (double)Q[(int64_t)i * N + j];
     x[j] = (T)v;
   }
 }
 
-__global__ void k_gmres_rearm(GmresDev g) { g.istate[GM_DONE] = 0; }
+struct KGmresRearm {
+  struct Args { GmresDev g; };
+  static constexpr int kThreads = 64;
+  static __device__ void run(const Args& a, const Blk&) {
+    if (threadIdx.x == 0) a.g.istate[GM_DONE] = 0;
+  }
+};
 
 // ---------------------------------------------------------------- host driver
 GmresDev gmres_alloc(Arena& ws, int m, int64_t N, int esize) {
@@ -267,20 +327,22 @@ GmresResult gmres_run(GmresDev g, int64_t N, T* x, const T* rhs, const double* n
   double* Pn2 = Pn1 + kGmresMaxBlocks;
   GmresResult res;
 
-  hipLaunchKernelGGL(k_gmres_setup, dim3(1), dim3(1), 0, st, g, norm_rhs, dnorm, real_tol);
+  launch<KGmresSetup>(dim3(1), dim3(1), 0, st, {g, norm_rhs, dnorm, real_tol});
   TT_LAUNCH_CHECK();
+  const bool cohort = current_recorder() != nullptr;
+  const int poll = std::max(1, cohort_params().gmres_poll);
   for (int cyc = 0; cyc < resets; ++cyc) {
     // r = rhs - Op(x); first cycle starts from x = 0 so r = rhs.
     const T* r = rhs;
     if (cyc > 0) {
-      hipLaunchKernelGGL(k_gmres_rearm, dim3(1), dim3(1), 0, st, g);
+      launch<KGmresRearm>(dim3(1), dim3(1), 0, st, {g});
       TT_LAUNCH_CHECK();
       op.apply(x, w, nullptr);
       op.axpby(1.0, rhs, -1.0, w, N);  // w = rhs - Op(x)
       op.norm(w, N, g.dstate + GM_RNORM);
       r = w;
     }
-    hipLaunchKernelGGL((k_gmres_begin<T>), dim3(nb), dim3(kGB), 0, st, r, Q, N, g);
+    launch<KGmresBegin<T>>(dim3(nb), dim3(kGB), 0, st, {r, Q, N, g});
     TT_LAUNCH_CHECK();
     int k = 0;
     bool stop = false;
@@ -292,38 +354,39 @@ GmresResult gmres_run(GmresDev g, int64_t N, T* x, const T* rhs, const double* n
       double* Hcol = g.H + (int64_t)k * (g.m + 1);
       const double* Pw = P1 + (size_t)(k + 1) * kGmresMaxBlocks;  // row kc of P1: w.w
       // pass 1: project out the basis
-      hipLaunchKernelGGL((k_cgs_dots<T>), dim3(nb), dim3(kGB), 0, st, Q, N, k + 1, w, P1, 1, done,
-                         (const double*)nullptr, (const double*)nullptr);
-      hipLaunchKernelGGL((k_cgs_update<T>), dim3(nb), dim3(kGB), 0, st, Q, N, k + 1, w, P1, Hcol, 0,
-                         Pn1, done, (const double*)nullptr, (const double*)nullptr, (int*)nullptr);
+      launch<KCgsDots<T>>(dim3(nb), dim3(kGB), 0, st,
+                          {Q, N, k + 1, w, P1, 1, done, nullptr, nullptr});
+      launch<KCgsUpdate<T>>(dim3(nb), dim3(kGB), 0, st,
+                            {Q, N, k + 1, w, P1, Hcol, 0, Pn1, done, nullptr, nullptr, nullptr});
       // pass 2 (re-orthogonalisation): each kernel re-derives the decision on the device
-      hipLaunchKernelGGL((k_cgs_dots<T>), dim3(nb), dim3(kGB), 0, st, Q, N, k + 1, w, P2, 0, done,
-                         Pw, (const double*)Pn1);
-      hipLaunchKernelGGL((k_cgs_update<T>), dim3(nb), dim3(kGB), 0, st, Q, N, k + 1, w, P2, Hcol, 1,
-                         Pn2, done, Pw, (const double*)Pn1, g.istate + GM_REORTHS);
-      hipLaunchKernelGGL((k_gmres_close<T>), dim3(nb), dim3(kGB), 0, st, w, qn, N, Pw,
-                         (const double*)Pn1, (const double*)Pn2, g, k, done);
+      launch<KCgsDots<T>>(dim3(nb), dim3(kGB), 0, st, {Q, N, k + 1, w, P2, 0, done, Pw, Pn1});
+      launch<KCgsUpdate<T>>(dim3(nb), dim3(kGB), 0, st,
+                            {Q, N, k + 1, w, P2, Hcol, 1, Pn2, done, Pw, Pn1, g.istate + GM_REORTHS});
+      launch<KGmresClose<T>>(dim3(nb), dim3(kGB), 0, st, {w, qn, N, Pw, Pn1, Pn2, g, k, done});
       TT_LAUNCH_CHECK();
-      // Read the flag of the previous iteration while this one runs.
       hs.post(done, k, st);
-      if (k >= 1 && hs.wait(k - 1)) stop = true;
+      if (cohort) {
+        // Inside a cohort a host read suspends the member until its queue has run: poll
+        // every gmres_poll iterations (iterations enqueued after convergence are no-ops).
+        if ((k + 1) % poll == 0 && hs.wait(k, st)) stop = true;
+      } else if (k >= 1 && hs.wait(k - 1, st)) {
+        // Read the flag of the previous iteration while this one runs.
+        stop = true;
+      }
     }
     // Flag of the last enqueued iteration.
-    (void)hs.wait(k - 1);
-    hipLaunchKernelGGL(k_gmres_y, dim3(1), dim3(1), 0, st, g);
-    hipLaunchKernelGGL((k_gmres_xupd<T>), dim3(nb), dim3(kGB), 0, st, Q, N, x, g);
+    if (!cohort) (void)hs.wait(k - 1, st);
+    launch<KGmresY>(dim3(1), dim3(1), 0, st, {g});
+    launch<KGmresXupd<T>>(dim3(nb), dim3(kGB), 0, st, {Q, N, x, g});
     TT_LAUNCH_CHECK();
     // Did this cycle converge?
-    TT_HIP(hipMemcpyAsync(hs.host_int(), g.istate + GM_FLAG, 2 * sizeof(int), hipMemcpyDeviceToHost,
-                          st));
-    TT_HIP(hipStreamSynchronize(st));
+    read_to_host(hs.host_int(), g.istate + GM_FLAG, 2 * sizeof(int), st);
     const int flag = hs.host_int()[0];
     res.cycle_steps.push_back(hs.host_int()[1]);  // GM_KCOLS follows GM_FLAG
     res.cycles = cyc + 1;
     if (flag) break;
   }
-  TT_HIP(hipMemcpyAsync(hs.host_int(), g.istate, sizeof(int) * 8, hipMemcpyDeviceToHost, st));
-  TT_HIP(hipStreamSynchronize(st));
+  read_to_host(hs.host_int(), g.istate, sizeof(int) * 8, st);
   res.flag = hs.host_int()[GM_FLAG];
   res.iters = hs.host_int()[GM_ITERS];
   res.reorths = hs.host_int()[GM_REORTHS];

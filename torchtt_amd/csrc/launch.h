@@ -1,0 +1,123 @@
+// Kernel launch plumbing: every kernel of the solver is a struct K with
+//   struct Args { ... };                      plain-old-data launch arguments
+//   static constexpr int kThreads;            block size upper bound
+//   static __device__ void run(const Args&, const Blk&);
+// and is launched through launch<K>(grid, block, lds, stream, args).
+//
+// Outside a cohort the launch is an ordinary kernel launch of k_one<K> on the
+// given stream. Inside a cohort (a host thread that is running one system of a
+// batched solve) the launch is recorded instead; the cohort later runs the
+// recorded launches of many systems together as one k_grp<K> launch whose grid
+// is the concatenation of their grids. Blk carries the block coordinates of the
+// launch the block belongs to, so a kernel body computes exactly the same thing
+// in either form.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstddef>
+#include <cstdint>
+#include <typeinfo>
+
+namespace ttamen {
+
+struct Blk {
+  unsigned x, y, z;     // block index within its own launch
+  unsigned gx, gy, gz;  // grid of its own launch
+};
+
+constexpr int kMaxGroup = 256;  // launches merged into one grouped launch, at most
+
+// Items of a grouped launch live in device memory, `stride` bytes apart:
+//   unsigned grid[4] (gx, gy, gz, unused), then the Args of that launch.
+struct GroupHdr {
+  const char* items;
+  unsigned stride;
+  int n;
+  unsigned start[kMaxGroup + 1];  // first block of item i; start[n] = total blocks
+};
+
+template <class K>
+__global__ __launch_bounds__(K::kThreads) void k_one(typename K::Args a) {
+  K::run(a, Blk{blockIdx.x, blockIdx.y, blockIdx.z, gridDim.x, gridDim.y, gridDim.z});
+}
+
+template <class K>
+__global__ __launch_bounds__(K::kThreads) void k_grp(GroupHdr h) {
+  const unsigned b = blockIdx.x;
+  int lo = 0, hi = h.n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (h.start[mid] <= b) lo = mid;
+    else hi = mid - 1;
+  }
+  const char* it = h.items + (size_t)lo * h.stride;
+  const unsigned* g = (const unsigned*)it;
+  const unsigned gx = g[0], gy = g[1], gz = g[2];
+  unsigned l = b - h.start[lo];
+  Blk bk;
+  bk.gx = gx; bk.gy = gy; bk.gz = gz;
+  bk.x = l % gx;
+  l /= gx;
+  bk.y = l % gy;
+  bk.z = l / gy;
+  K::run(*(const typename K::Args*)(it + 16), bk);
+}
+
+// One grouped launch as prepared by the cohort: items already copied to the device.
+struct GroupLaunch {
+  GroupHdr hdr;
+  unsigned total_blocks;
+  dim3 block;
+  size_t lds;
+  hipStream_t stream;
+};
+typedef void (*GroupFn)(const GroupLaunch&);
+
+template <class K>
+void group_exec(const GroupLaunch& gl) {
+  hipLaunchKernelGGL((k_grp<K>), dim3(gl.total_blocks), gl.block, gl.lds, gl.stream, gl.hdr);
+}
+
+// Receives launches made while a cohort member runs on this thread.
+class Recorder {
+ public:
+  virtual void record(GroupFn fn, dim3 grid, dim3 block, size_t lds, const void* args,
+                      size_t size) = 0;
+  // Run everything recorded so far and wait for it (a host synchronisation point).
+  virtual void sync() = 0;
+  virtual ~Recorder() = default;
+};
+
+// The recorder of the cohort member running on this host thread, or null.
+Recorder*& current_recorder();
+
+// Names of launch kinds, for the cohort's optional launch histogram.
+void register_kind_name(GroupFn fn, const char* name);
+template <class K>
+inline void register_kind() {
+  static const bool once = (register_kind_name(&group_exec<K>, typeid(K).name()), true);
+  (void)once;
+}
+
+template <class K>
+inline void launch(dim3 grid, dim3 block, size_t lds, hipStream_t st, const typename K::Args& a) {
+  if (grid.x == 0 || grid.y == 0 || grid.z == 0) return;
+  if (Recorder* r = current_recorder()) {
+    register_kind<K>();
+    r->record(&group_exec<K>, grid, block, lds, &a, sizeof(a));
+    return;
+  }
+  hipLaunchKernelGGL((k_one<K>), grid, block, lds, st, a);
+}
+
+// Allow `bytes` of dynamic LDS for both launch forms of K.
+template <class K>
+inline hipError_t allow_lds(int bytes) {
+  hipError_t e1 = hipFuncSetAttribute((const void*)k_one<K>,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+  hipError_t e2 = hipFuncSetAttribute((const void*)k_grp<K>,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+  return e1 != hipSuccess ? e1 : e2;
+}
+
+}  // namespace ttamen

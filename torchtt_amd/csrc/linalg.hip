@@ -35,10 +35,19 @@ constexpr int kQrPasses = 4;
 //     row j is zero; dead[j] records it and *ndead counts them.
 constexpr double kNearIdentity = 1e-6;
 
-__global__ __launch_bounds__(1024) void k_cholesky(const double* G, int n, double shift_c,
-                                                   double dead_rel, int last_pass, double* Rout,
-                                                   int* dead, int* ndead, const int* skip_in,
-                                                   int* final_out) {
+struct KCholesky {
+  struct Args {
+    const double* G; int n; double shift_c; double dead_rel; int last_pass; double* Rout;
+    int* dead; int* ndead; const int* skip_in; int* final_out;
+  };
+  static constexpr int kThreads = 1024;
+  static __device__ void run(const Args& a, const Blk&);
+};
+__device__ void KCholesky::run(const Args& a, const Blk&) {
+  const double* G = a.G; const int n = a.n; const double shift_c = a.shift_c;
+  const double dead_rel = a.dead_rel; const int last_pass = a.last_pass; double* Rout = a.Rout;
+  int* dead = a.dead; int* ndead = a.ndead; const int* skip_in = a.skip_in;
+  int* final_out = a.final_out;
   extern __shared__ double W[];
   __shared__ double s_piv, s_maxd, s_shift, s_red[16];
   __shared__ int s_dead, s_nd, s_final;
@@ -119,9 +128,16 @@ __device__ __forceinline__ int packed_idx(int i, int j, int n) {
   return i * n - (i * (i - 1)) / 2 + (j - i);
 }
 
-__global__ __launch_bounds__(64) void k_trsm_rows(const double* A, int64_t sr, int64_t sc,
-                                                   int64_t m, int n, const double* R, double* X,
-                                                   const int* skip) {
+struct KTrsmRows {
+  struct Args {
+    const double* A; int64_t sr, sc, m; int n; const double* R; double* X; const int* skip;
+  };
+  static constexpr int kThreads = 64;
+  static __device__ void run(const Args& a, const Blk& bk);
+};
+__device__ void KTrsmRows::run(const Args& args, const Blk& bk) {
+  const double* A = args.A; const int64_t sr = args.sr, sc = args.sc, m = args.m;
+  const int n = args.n; const double* R = args.R; double* X = args.X; const int* skip = args.skip;
   if (skip && *skip) return;
   extern __shared__ double P[];
   for (int t = threadIdx.x; t < n * n; t += blockDim.x) {
@@ -129,7 +145,7 @@ __global__ __launch_bounds__(64) void k_trsm_rows(const double* A, int64_t sr, i
     if (i <= j) P[packed_idx(i, j, n)] = R[t];
   }
   __syncthreads();
-  const int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t row = (int64_t)bk.x * blockDim.x + threadIdx.x;
   if (row >= m) return;
   const double* a = A + row * sr;
   double* x = X + row * n;
@@ -176,9 +192,14 @@ __device__ double block_sum_1024(double v, double* sh) {
   return r;
 }
 
-__global__ __launch_bounds__(1024) void k_complete_dead(double* Q, int64_t m, int n, int* dead,
-                                                        const int* ndead) {
-  if (*ndead == 0) return;
+struct KCompleteDead {
+  struct Args { double* Q; int64_t m; int n; int* dead; const int* ndead; };
+  static constexpr int kThreads = 1024;
+  static __device__ void run(const Args& a, const Blk&);
+};
+__device__ void KCompleteDead::run(const Args& a, const Blk&) {
+  double* Q = a.Q; const int64_t m = a.m; const int n = a.n; int* dead = a.dead;
+  if (*a.ndead == 0) return;
   __shared__ double sh[16];
   __shared__ int s_ok;
   const int tid = threadIdx.x, nt = blockDim.x;
@@ -257,17 +278,17 @@ int qr(const double* A, int64_t sr, int64_t sc, int64_t m, int64_t n, double* Q,
     if (!ws.measuring()) {
       const double shift_c = (pass < 2) ? 11.0 * ((double)m * n + (double)n * (n + 1)) : 0.0;
       const double dead_rel = (pass < 2) ? 0.0 : 1e-26;
-      hipLaunchKernelGGL(k_cholesky, dim3(1), dim3(1024), chol_lds, st, G, (int)n, shift_c,
-                         dead_rel, (int)(pass == kQrPasses - 1), Rk, dead, ndead, skip,
-                         flags + pass + 1);
+      launch<KCholesky>(dim3(1), dim3(1024), chol_lds, st,
+                        {G, (int)n, shift_c, dead_rel, (int)(pass == kQrPasses - 1), Rk, dead, ndead,
+                         skip, flags + pass + 1});
       TT_LAUNCH_CHECK();
-      hipLaunchKernelGGL(k_trsm_rows, dim3(trsm_blocks), dim3(64), trsm_lds, st, src, ssr, ssc,
-                         m, (int)n, Rk, Q, skip);
+      launch<KTrsmRows>(dim3(trsm_blocks), dim3(64), trsm_lds, st,
+                        {src, ssr, ssc, m, (int)n, Rk, Q, skip});
       TT_LAUNCH_CHECK();
     }
   }
   if (!ws.measuring()) {
-    hipLaunchKernelGGL(k_complete_dead, dim3(1), dim3(1024), 0, st, Q, m, (int)n, dead, ndead);
+    launch<KCompleteDead>(dim3(1), dim3(1024), 0, st, {Q, m, (int)n, dead, ndead});
     TT_LAUNCH_CHECK();
   }
   {
@@ -299,9 +320,18 @@ __device__ __forceinline__ void rr_pair(int t, int i, int K, int& p, int& q) {
 // Outputs: s (k, descending), Ur (k x k row-major; column r = r-th left singular
 // vector), rank_of[j] (sorted position of original column j), log of rotations
 // (c, s) per (sweep, round, pair), *nsweeps.
-__global__ __launch_bounds__(1024) void k_jacobi(const double* M, int64_t sr, int64_t sc, int k,
-                                                 double* s_out, double* Ur, int* rank_of,
-                                                 double2* log, int* nsweeps) {
+struct KJacobi {
+  struct Args {
+    const double* M; int64_t sr, sc; int k; double* s_out; double* Ur; int* rank_of;
+    double2* log; int* nsweeps;
+  };
+  static constexpr int kThreads = 1024;
+  static __device__ void run(const Args& a, const Blk&);
+};
+__device__ void KJacobi::run(const Args& a, const Blk&) {
+  const double* M = a.M; const int64_t sr = a.sr, sc = a.sc; const int k = a.k;
+  double* s_out = a.s_out; double* Ur = a.Ur; int* rank_of = a.rank_of; double2* log = a.log;
+  int* nsweeps = a.nsweeps;
   extern __shared__ double W[];  // column-major k x k, then sig[k]
   double* sig = W + (size_t)k * k;
   __shared__ int s_rot;
@@ -437,10 +467,16 @@ __global__ __launch_bounds__(1024) void k_jacobi(const double* M, int64_t sr, in
 }
 
 // Row rho of V: apply the logged rotations to e_rho; write V[rho][rank_of[j]].
-__global__ __launch_bounds__(64) void k_jacobi_v(int k, const double2* log, const int* nsweeps,
-                                                 const int* rank_of, double* V, int64_t ldv) {
+struct KJacobiV {
+  struct Args { int k; const double2* log; const int* nsweeps; const int* rank_of; double* V; int64_t ldv; };
+  static constexpr int kThreads = 64;
+  static __device__ void run(const Args& a, const Blk& bk);
+};
+__device__ void KJacobiV::run(const Args& a, const Blk& bk) {
+  const int k = a.k; const double2* log = a.log; const int* nsweeps = a.nsweeps;
+  const int* rank_of = a.rank_of; double* V = a.V; const int64_t ldv = a.ldv;
   extern __shared__ double v[];
-  const int rho = blockIdx.x, lane = threadIdx.x;
+  const int rho = bk.x, lane = threadIdx.x;
   const int K = (k % 2) ? k + 1 : k;
   const int npairs = K / 2;
   for (int j = lane; j < K; j += 64) v[j] = (j == rho) ? 1.0 : 0.0;
@@ -477,11 +513,10 @@ static void jacobi_svd_small(const double* M, int64_t sr, int64_t sc, int k, dou
   double2* log = ws.alloc<double2>((size_t)kJacobiMaxSweeps * std::max(1, K - 1) * (K / 2 + 1));
   if (!ws.measuring()) {
     size_t lds = ((size_t)k * k + k) * sizeof(double);
-    hipLaunchKernelGGL(k_jacobi, dim3(1), dim3(1024), lds, st, M, sr, sc, k, s, Ur, rank_of, log,
-                       nsw);
+    launch<KJacobi>(dim3(1), dim3(1024), lds, st, {M, sr, sc, k, s, Ur, rank_of, log, nsw});
     TT_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_jacobi_v, dim3(k), dim3(64), (size_t)K * sizeof(double), st, k, log, nsw,
-                       rank_of, V, ldv);
+    launch<KJacobiV>(dim3(k), dim3(64), (size_t)K * sizeof(double), st,
+                     {k, log, nsw, rank_of, V, ldv});
     TT_LAUNCH_CHECK();
   }
   ws.release(mk);
@@ -527,6 +562,11 @@ static void raise_lds_limit(const void* fn, int dev_max) {
     (void)hipGetLastError();  // keep the error out of later checks; launches report it
   }
 }
+template <class K>
+static void raise_lds(int dev_max) {
+  raise_lds_limit((const void*)k_one<K>, dev_max);
+  raise_lds_limit((const void*)k_grp<K>, dev_max);
+}
 
 void linalg_init() {
   static int done_dev = -1;
@@ -535,9 +575,9 @@ void linalg_init() {
   if (done_dev == dev) return;
   int dev_max = 0;
   TT_HIP(hipDeviceGetAttribute(&dev_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
-  raise_lds_limit((const void*)k_cholesky, dev_max);
-  raise_lds_limit((const void*)k_jacobi, dev_max);
-  raise_lds_limit((const void*)k_trsm_rows, dev_max);
+  raise_lds<KCholesky>(dev_max);
+  raise_lds<KJacobi>(dev_max);
+  raise_lds<KTrsmRows>(dev_max);
   done_dev = dev;
 }
 

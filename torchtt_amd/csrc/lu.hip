@@ -19,8 +19,13 @@ namespace ttamen {
 
 constexpr int kLuNb = 32;
 
-__global__ __launch_bounds__(1024) void k_lu_panel(double* Aug, int n, int ld, int j0, int jb,
-                                                   int* piv) {
+struct KLuPanel {
+  struct Args { double* Aug; int n, ld, j0, jb; int* piv; };
+  static constexpr int kThreads = 1024;
+  static __device__ void run(const Args& a, const Blk&);
+};
+__device__ void KLuPanel::run(const Args& a, const Blk&) {
+  double* Aug = a.Aug; const int n = a.n, ld = a.ld, j0 = a.j0, jb = a.jb; int* piv = a.piv;
   extern __shared__ double P[];  // column-major: P[c * rows + i]
   __shared__ double shv[16];
   __shared__ int shi[16];
@@ -84,15 +89,21 @@ __global__ __launch_bounds__(1024) void k_lu_panel(double* Aug, int n, int ld, i
 }
 
 // Trailing columns c in [j0+jb, n]: apply the panel's row swaps, then U12 = L11^-1 A12.
-__global__ __launch_bounds__(256) void k_lu_swap_trsm(double* Aug, int n, int ld, int j0, int jb,
-                                                      const int* piv) {
+struct KLuSwapTrsm {
+  struct Args { double* Aug; int n, ld, j0, jb; const int* piv; };
+  static constexpr int kThreads = 256;
+  static __device__ void run(const Args& a, const Blk& bk);
+};
+__device__ void KLuSwapTrsm::run(const Args& args, const Blk& bk) {
+  double* Aug = args.Aug; const int n = args.n, ld = args.ld, j0 = args.j0, jb = args.jb;
+  const int* piv = args.piv;
   __shared__ double L[kLuNb][kLuNb];
   for (int t = threadIdx.x; t < jb * jb; t += blockDim.x) {
     int i = t / jb, c = t - i * jb;
     L[i][c] = Aug[(int64_t)(j0 + i) * ld + j0 + c];
   }
   __syncthreads();
-  const int c = j0 + jb + blockIdx.x * blockDim.x + threadIdx.x;
+  const int c = j0 + jb + bk.x * blockDim.x + threadIdx.x;
   if (c > n) return;
   double a[kLuNb];
 #pragma unroll
@@ -124,8 +135,13 @@ __global__ __launch_bounds__(256) void k_lu_swap_trsm(double* Aug, int n, int ld
 }
 
 // Back substitution for rows [j0, j0+jb): x_blk = U_blk^-1 y_blk, then y[0:j0] -= U[0:j0, blk] x_blk.
-__global__ __launch_bounds__(256) void k_lu_backsolve(double* Aug, int n, int ld, int j0, int jb,
-                                                      double* x) {
+struct KLuBacksolve {
+  struct Args { double* Aug; int n, ld, j0, jb; double* x; };
+  static constexpr int kThreads = 256;
+  static __device__ void run(const Args& a, const Blk& bk);
+};
+__device__ void KLuBacksolve::run(const Args& a, const Blk& bk) {
+  double* Aug = a.Aug; const int n = a.n, ld = a.ld, j0 = a.j0, jb = a.jb; double* x = a.x;
   __shared__ double U[kLuNb][kLuNb];
   __shared__ double xb[kLuNb];
   for (int t = threadIdx.x; t < jb * jb; t += blockDim.x) {
@@ -142,7 +158,7 @@ __global__ __launch_bounds__(256) void k_lu_backsolve(double* Aug, int n, int ld
   }
   __syncthreads();
   for (int i = threadIdx.x; i < jb; i += blockDim.x) x[j0 + i] = xb[i];
-  for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < j0; r += gridDim.x * blockDim.x) {
+  for (int r = bk.x * blockDim.x + threadIdx.x; r < j0; r += bk.gx * blockDim.x) {
     double v = Aug[(int64_t)r * ld + n];
     for (int t = 0; t < jb; ++t) v -= Aug[(int64_t)r * ld + j0 + t] * xb[t];
     Aug[(int64_t)r * ld + n] = v;
@@ -164,12 +180,11 @@ void lu_solve_aug(double* Aug, int n, double* x, Arena& ws, hipStream_t st) {
     const int jb = std::min(nb, n - j0);
     if (!ws.measuring()) {
       size_t lds = (size_t)(n - j0) * jb * sizeof(double);
-      hipLaunchKernelGGL(k_lu_panel, dim3(1), dim3(1024), lds, st, Aug, n, ld, j0, jb, piv);
+      launch<KLuPanel>(dim3(1), dim3(1024), lds, st, {Aug, n, ld, j0, jb, piv});
       TT_LAUNCH_CHECK();
       const int ncols = n + 1 - (j0 + jb);
       if (ncols > 0) {
-        hipLaunchKernelGGL(k_lu_swap_trsm, dim3((ncols + 255) / 256), dim3(256), 0, st, Aug, n, ld,
-                           j0, jb, piv);
+        launch<KLuSwapTrsm>(dim3((ncols + 255) / 256), dim3(256), 0, st, {Aug, n, ld, j0, jb, piv});
         TT_LAUNCH_CHECK();
       }
     }
@@ -189,7 +204,7 @@ void lu_solve_aug(double* Aug, int n, double* x, Arena& ws, hipStream_t st) {
     for (int j0 = last; j0 >= 0; j0 -= nb) {
       const int jb = std::min(nb, n - j0);
       const int blocks = std::max(1, std::min(64, (j0 + 255) / 256));
-      hipLaunchKernelGGL(k_lu_backsolve, dim3(blocks), dim3(256), 0, st, Aug, n, ld, j0, jb, x);
+      launch<KLuBacksolve>(dim3(blocks), dim3(256), 0, st, {Aug, n, ld, j0, jb, x});
       TT_LAUNCH_CHECK();
     }
   }
@@ -203,11 +218,13 @@ void lu_init() {
   if (done_dev == dev) return;
   int dev_max = 0;
   TT_HIP(hipDeviceGetAttribute(&dev_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
-  hipFuncAttributes fa;
-  TT_HIP(hipFuncGetAttributes(&fa, (const void*)k_lu_panel));
-  if (hipFuncSetAttribute((const void*)k_lu_panel, hipFuncAttributeMaxDynamicSharedMemorySize,
-                          dev_max - (int)fa.sharedSizeBytes) != hipSuccess)
-    (void)hipGetLastError();
+  for (const void* fn : {(const void*)k_one<KLuPanel>, (const void*)k_grp<KLuPanel>}) {
+    hipFuncAttributes fa;
+    TT_HIP(hipFuncGetAttributes(&fa, fn));
+    if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            dev_max - (int)fa.sharedSizeBytes) != hipSuccess)
+      (void)hipGetLastError();
+  }
   done_dev = dev;
 }
 

@@ -7,6 +7,7 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include "launch.h"
 #include <cstdint>
 #include <cstddef>
 #include <string>
@@ -149,6 +150,7 @@ void add_f32_to_f64(const float* x, double* y, int64_t n, hipStream_t st);
 void sub(const double* x, const double* y, double* z, int64_t n, hipStream_t st);
 void copy(const double* x, double* y, int64_t n, hipStream_t st);
 void fill(double* x, int64_t n, double v, hipStream_t st);
+void zero_words(void* x, int64_t nwords, hipStream_t st);  // nwords 32-bit words set to 0
 void eye(double* x, int64_t m, int64_t n, hipStream_t st);  // row-major m x n identity
 // Strided 2-D copy: dst(i,j) = src(i*sr + j*sc), dst row-major with leading dim ldd
 void copy2d(const double* src, int64_t sr, int64_t sc, double* dst, int64_t ldd, int64_t m,
@@ -171,6 +173,17 @@ void nrmsc_update(double* S, int i_nrmsc, int i_a, int i_x, int i_b, int mode, h
 // S[dst] = max(S[dst], S[src] / S[den])
 void scalar_maxdiv(double* S, int dst, int src, int den, hipStream_t st);
 void scalar_set(double* S, int dst, double v, hipStream_t st);
+// S[dst] = S[a] / S[b]
+void scalar_div(double* S, int dst, int a, int b, hipStream_t st);
+
+// ---------------------------------------------------------------- host transfers
+// Queue a copy of `bytes` (a multiple of 4) of device memory into pinned host memory.
+void post_to_host(void* host, const void* dev, size_t bytes, hipStream_t st);
+// Wait until everything queued so far has run (inside a cohort: until the cohort has
+// run this system's recorded work).
+void sync_point(hipStream_t st);
+// post_to_host + sync_point.
+void read_to_host(void* host, const void* dev, size_t bytes, hipStream_t st);
 
 // ---------------------------------------------------------------- dense linear algebra
 constexpr int kMaxSmall = 140;   // largest square block kept in LDS (FP64)

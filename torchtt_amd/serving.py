@@ -3,9 +3,9 @@
 `serve(requests, arrivals, ...)` hands a whole request stream to libttamen's native loop
 (include/ttamen.h: ttamen_serve): requests are admitted in arrival order whenever a slot is
 idle and a contiguous region of their footprint (the solver workspace at their rank cap) is
-free in the serving memory; each admitted request runs on its own HIP stream with its cores
-and interfaces resident in that region, so sweep steps of different requests interleave on
-the GPU. Results are the same as solving each request alone.
+free in the serving memory. The running requests form one cohort: the same sweep steps of
+different requests run as one grouped launch, with each request's cores and interfaces
+resident in its region. Results are the same as solving each request alone.
 """
 import ctypes
 
@@ -28,12 +28,14 @@ def poisson_arrivals(rate, duration, seed=0):
         out.append(t)
 
 
-def serve(requests, arrivals, slots=8, memory_bytes=None, nswp=22, eps=1e-10, max_full=500, kickrank=4,
+def serve(requests, arrivals, slots=64, memory_bytes=None, nswp=22, eps=1e-10, max_full=500, kickrank=4,
           kick2=0, local_iterations=40, resets=2, precision='fp64', z0=None):
     """Serve `requests` ((A, b[, rmax]) items) arriving at `arrivals` (seconds, sorted).
 
+    slots: requests running at once (at most 256).
     memory_bytes: serving memory (HBM) budget shared by the running requests; default
-    slots x the largest request footprint.
+    slots x the largest request footprint. The launch-table staging area
+    (ttamen_cohort_staging_bytes) is allocated on top of it.
 
     Returns:
         (solutions, info) where info holds per-request arrival/start/done times (s),
@@ -51,18 +53,24 @@ def serve(requests, arrivals, slots=8, memory_bytes=None, nswp=22, eps=1e-10, ma
     with torch.cuda.device(dev):
         arr, stats, outs, keep = _build_systems(requests, dev, kickrank, kick2, z0, lib)
         if memory_bytes is None:
-            memory_bytes = lib.ttamen_amen_solve_batch_workspace_size(arr, count, int(kickrank), int(kick2),
-                                                                      int(local_iterations), int(max_full),
-                                                                      _PRECISIONS[precision], int(slots))
-            if memory_bytes == 0:
-                _lib.check("ttamen_amen_solve_batch_workspace_size", 1)
-        ws = torch.empty(int(memory_bytes), dtype=torch.uint8, device=dev)
+            need = 0
+            for i in range(count):
+                y = arr[i]
+                w = lib.ttamen_amen_solve_workspace_size(y.d, y.N, y.rA, y.rb, None, y.rmax, int(kickrank),
+                                                         int(kick2), int(local_iterations), int(max_full),
+                                                         _PRECISIONS[precision])
+                if w == 0:
+                    _lib.check("ttamen_amen_solve_workspace_size", 1)
+                need = max(need, w)
+            memory_bytes = min(int(slots), count) * (need + 256)
+        ws_bytes = int(memory_bytes) + int(lib.ttamen_cohort_staging_bytes())
+        ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dev)
         arr_t = (ctypes.c_double * count)(*[float(a) for a in arrivals])
         start = (ctypes.c_double * count)()
         done = (ctypes.c_double * count)()
         code = lib.ttamen_serve(arr, arr_t, count, int(nswp), float(eps), int(max_full), int(kickrank), int(kick2),
                                 int(local_iterations), int(resets), _PRECISIONS[precision], int(slots),
-                                ctypes.c_void_p(ws.data_ptr()), int(memory_bytes), start, done,
+                                ctypes.c_void_p(ws.data_ptr()), ws_bytes, start, done,
                                 ctypes.c_void_p(_lib.stream_handle(dev)))
         _lib.check("ttamen_serve", code)
         sols = _collect(outs)

@@ -214,10 +214,12 @@ def amen_solve_batch(systems, nswp=22, eps=1e-10, max_full=500, kickrank=4, kick
     """Solve many independent TT systems concurrently (no counterpart in the reference).
 
     systems: list of (A, b) or (A, b, rmax) with TT A, b. Each system runs the same solver as
-    `amen_solve` with the same options; `workers` host threads with one HIP stream each take
-    systems from a shared queue (libttamen: ttamen_amen_solve_batch). The residual-proxy
-    draws are made per system in list order (or taken from `z0`, a list of per-system core
-    lists), so system i's result equals `amen_solve(A_i, b_i, ..., _z0=z0[i])`.
+    `amen_solve` with the same options; the systems run together as one cohort
+    (libttamen: ttamen_amen_solve_batch): at every host synchronisation point the kernels
+    that all running systems have queued are merged into grouped launches. `workers` host
+    threads run the systems' host code. The residual-proxy draws are made per system in list
+    order (or taken from `z0`, a list of per-system core lists), so system i's result equals
+    `amen_solve(A_i, b_i, ..., _z0=z0[i])` bit for bit.
 
     Returns:
         list[TT]: the solutions, on each A's device and dtype.
@@ -246,8 +248,26 @@ def amen_solve_batch(systems, nswp=22, eps=1e-10, max_full=500, kickrank=4, kick
         _lib.check("ttamen_amen_solve_batch", code)
         result = _collect(outs)
         last_batch_stats = [stats[i].as_dict() for i in range(count)]
+        cohort_stats()
         del ws, keep
     return result
+
+
+#: counters of the most recent batched / served call (see cohort_stats)
+last_cohort_stats = None
+
+
+def cohort_stats():
+    """Counters of the last batched or served call: rounds, grouped launches, merged launch
+    items, member resumptions and the seconds spent resuming members, building launches,
+    waiting for the device and in total (libttamen: ttamen_cohort_stats)."""
+    global last_cohort_stats
+    lib = _lib.load()
+    out = (ctypes.c_double * 8)()
+    _lib.check("ttamen_cohort_stats", lib.ttamen_cohort_stats(out))
+    keys = ("rounds", "launches", "items", "resumes", "t_resume", "t_build", "t_gpu", "t_total")
+    last_cohort_stats = dict(zip(keys, list(out)))
+    return last_cohort_stats
 
 
 def residual(A, x, b):
