@@ -95,7 +95,8 @@ int ttamen_amen_solve(int d, const int64_t* N, const int64_t* rA, const int64_t*
  * ttamen_amen_solve in its own region of `workspace`, and at every host synchronisation
  * point the launches that all running systems have queued are merged into grouped launches
  * (one grid covering the same kernel for every system), so each launch fills the device.
- * Up to 256 systems run at once (as many as fit in the workspace, admitted in index order);
+ * As many systems as fit in the workspace run at once (admitted in index order), spread
+ * over a few concurrent cohorts ("lanes") of up to 256 systems, each on its own stream;
  * `workers` host threads run the systems' host code. Results are bitwise those of
  * ttamen_amen_solve with the same inputs. Shared options apply to all systems; rmax and
  * seed are per system. Each system's status is written to its `status` field; the call
@@ -160,8 +161,11 @@ size_t ttamen_cohort_staging_bytes(void);
 int ttamen_cohort_stats(double* out8);
 /* Scheduling parameters of later batch / serving calls (process-wide; results do not
  * depend on them): grouped launches per round (128), merge steps a system may wait before
- * its next launch is served (32), GMRES iterations between convergence polls (8). */
-int ttamen_cohort_set_params(int window, int max_age, int gmres_poll);
+ * its next launch is served (64), GMRES iterations between convergence polls (8), and the
+ * number of lanes (3, at most 8): independent cohorts on their own streams that run
+ * concurrently, the systems being spread over them as they are admitted; a round also ends
+ * early once idle_pct % (15; 100: never) of the members' weight waits to be resumed. */
+int ttamen_cohort_set_params(int window, int max_age, int gmres_poll, int lanes, int idle_pct);
 
 /* Bytes of device workspace ttamen_amen_solve needs for these sizes (0 on bad input). */
 size_t ttamen_amen_solve_workspace_size(int d, const int64_t* N, const int64_t* rA,

@@ -58,7 +58,7 @@ _SIGNATURES = {
     "ttamen_last_error": (ctypes.c_char_p, []),
     "ttamen_cohort_staging_bytes": (c_size, []),
     "ttamen_cohort_stats": (ctypes.c_int, [ctypes.POINTER(ctypes.c_double)]),
-    "ttamen_cohort_set_params": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+    "ttamen_cohort_set_params": (ctypes.c_int, [ctypes.c_int] * 5),
     "ttamen_amen_solve": (ctypes.c_int, [
         ctypes.c_int, c_i64p, c_i64p, c_i64p, ctypes.POINTER(c_dp), ctypes.POINTER(c_dp),
         ctypes.POINTER(c_dp), c_i64p, ctypes.POINTER(c_dp), ctypes.c_uint64, c_i64, ctypes.c_double,

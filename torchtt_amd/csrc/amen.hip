@@ -16,6 +16,7 @@
 #include "ttamen_internal.h"
 #include "gmres.h"
 #include "cohort.h"
+#include "kernels_blas1.h"
 #include "../../include/ttamen.h"
 #include <algorithm>
 #include <chrono>
@@ -93,20 +94,6 @@ struct MatvecProfile {
 static MatvecProfile g_prof;
 
 // ---------------------------------------------------------------- small kernels
-// out[i][c] = in[i][c] * s[c]  for c < ncols (in row-major with ld ldi, out ld ldo)
-struct KScaleCols {
-  struct Args { const double* in; int64_t ldi; const double* s; double* out; int64_t ldo, rows; int ncols; };
-  static constexpr int kThreads = 256;
-  static __device__ void run(const Args& a, const Blk& bk) {
-    for (int64_t t = bk.x * (int64_t)blockDim.x + threadIdx.x; t < a.rows * a.ncols;
-         t += (int64_t)bk.gx * blockDim.x) {
-      int64_t i = t / a.ncols;
-      int c = (int)(t - i * a.ncols);
-      a.out[i * a.ldo + c] = a.in[i * a.ldi + c] * a.s[c];
-    }
-  }
-};
-
 // Candidate residuals of the truncation search. e (N) holds B*sol_{r0} - rhs for the
 // current top rank r0; candidate c removes component j_c = r0-1-c:
 //   e_c = e_{c-1} - s[j_c] * g_c,   res_c = ||e_c|| / nrhs.
@@ -174,28 +161,6 @@ __device__ void KTruncPrefix::run(const Args& a, const Blk& bk) {
   if (threadIdx.x == 0) *counter = 0u;
 }
 
-
-// Normal random numbers: SplitMix64 counter stream + Box-Muller.
-__device__ __forceinline__ uint64_t splitmix64(uint64_t x) {
-  x += 0x9E3779B97F4A7C15ull;
-  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
-  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
-  return x ^ (x >> 31);
-}
-struct KRandn {
-  struct Args { double* x; int64_t n; uint64_t seed, stream_id; };
-  static constexpr int kThreads = 256;
-  static __device__ void run(const Args& p, const Blk& bk) {
-    for (int64_t i = bk.x * (int64_t)blockDim.x + threadIdx.x; i < p.n;
-         i += (int64_t)bk.gx * blockDim.x) {
-      uint64_t a = splitmix64(p.seed ^ splitmix64(p.stream_id * 0x100000001B3ull + (uint64_t)i * 2));
-      uint64_t b = splitmix64(a + 0x632BE59BD9B4E019ull);
-      double u1 = ((a >> 11) + 1.0) * (1.0 / 9007199254740993.0);
-      double u2 = (b >> 11) * (1.0 / 9007199254740992.0);
-      p.x[i] = sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
-    }
-  }
-};
 
 // Pinned mailbox of this solve: per cohort member, else per host thread.
 static HostSync& host_sync() { return member_host_sync(); }
@@ -1104,9 +1069,26 @@ class FirstFit {
   std::map<size_t, size_t> free_;
 };
 
-// Device staging for the argument tables of a cohort's grouped launches (taken from
-// the front of the batch / serving workspace).
-static constexpr size_t kStageBytes = 32ull << 20;
+// Device staging for the argument tables of the cohorts' grouped launches (taken from
+// the front of the batch / serving workspace): one region per lane.
+static constexpr int kMaxLanes = 8;
+static constexpr size_t kStagePerLane = 16ull << 20;
+static constexpr size_t kStageBytes = kStagePerLane * kMaxLanes;
+
+// Streams of the cohort lanes: created once per device, default priority, reused.
+static std::vector<hipStream_t> lane_streams(int dev, int n) {
+  static std::mutex mu;
+  static std::vector<std::vector<hipStream_t>> pools;
+  std::lock_guard<std::mutex> lk(mu);
+  if ((int)pools.size() <= dev) pools.resize(dev + 1);
+  auto& p = pools[dev];
+  while ((int)p.size() < n) {
+    hipStream_t s;
+    TT_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    p.push_back(s);
+  }
+  return std::vector<hipStream_t>(p.begin(), p.begin() + n);
+}
 
 static std::mutex g_err_mu;
 static std::mutex g_cohort_mu;
@@ -1149,8 +1131,13 @@ static void run_cohort(ttamen_system* systems, int count, int64_t nswp, double e
   linalg_init();
   lu_init();
   std::vector<size_t> need(count);
+  std::vector<double> cost(count);
   for (int i = 0; i < count; ++i) {
     const ttamen_system& y = systems[i];
+    // work estimate: sum of the local problem sizes at the rank bounds
+    Plan P = make_plan(y.d, y.N, y.rA, y.rb, y.x0_cores ? y.rx0 : nullptr, y.rmax, kickrank, kick2);
+    cost[i] = 0;
+    for (int k = 0; k < y.d; ++k) cost[i] += (double)P.Rb[k] * P.N[k] * P.Rb[k + 1];
     size_t w = ttamen_amen_solve_workspace_size(y.d, y.N, y.rA, y.rb, y.x0_cores ? y.rx0 : nullptr,
                                                 y.rmax, kickrank, kick2, local_iterations, max_full,
                                                 precision);
@@ -1159,18 +1146,40 @@ static void run_cohort(ttamen_system* systems, int count, int64_t nswp, double e
     TT_CHECK(need[i] <= workspace_bytes - kStageBytes, ERR_WORKSPACE,
              "system " + std::to_string(i) + " does not fit the workspace");
   }
+  // member weights: square of the cost relative to the median, within [1/16, 16]
+  std::vector<double> sorted_cost(cost);
+  std::sort(sorted_cost.begin(), sorted_cost.end());
+  const double med = std::max(1.0, sorted_cost[count / 2]);
   char* stage = (char*)workspace;
   char* base = stage + kStageBytes;
   FirstFit mem(workspace_bytes - kStageBytes);
-  slots = std::max(1, std::min(slots, kMaxGroup));
+  slots = std::max(1, std::min(slots, kMaxLanes * kMaxGroup));
+  const int lanes = std::max(1, std::min({cohort_params().lanes, kMaxLanes, count}));
+  const int lane_slots =
+      std::max(1, std::min(kMaxGroup, (std::min(slots, count) + lanes - 1) / lanes));
+  const int lane_workers = std::max(1, (workers + lanes - 1) / lanes);
+  int dev = 0;
+  TT_HIP(hipGetDevice(&dev));
+  // One stream per lane; all start after the work already queued on the caller's stream.
+  std::vector<hipStream_t> streams = lane_streams(dev, lanes);
+  hipEvent_t ready_ev;
+  TT_HIP(hipEventCreateWithFlags(&ready_ev, hipEventDisableTiming));
+  TT_HIP(hipEventRecord(ready_ev, stream));
+  for (auto s : streams) TT_HIP(hipStreamWaitEvent(s, ready_ev, 0));
+  std::mutex adm_mu;  // admission state: next, running, mem
   std::string first_err;
-  int next = 0;
-  Cohort co(stream, stage, kStageBytes, workers);
+  int next = 0, running = 0;
+  std::vector<std::unique_ptr<Cohort>> co;
+  for (int l = 0; l < lanes; ++l)
+    co.emplace_back(new Cohort(streams[l], stage + (size_t)l * kStagePerLane, kStagePerLane,
+                               lane_workers));
   auto admit = [&](Cohort& c) -> bool {
-    while (next < count && c.members() < slots && ready(next, c.members())) {
+    std::lock_guard<std::mutex> lk(adm_mu);
+    while (next < count && c.members() < lane_slots && running < slots && ready(next, running)) {
       size_t off;
       if (!mem.alloc(need[next], &off)) break;
       const int i = next++;
+      ++running;
       started(i);
       ttamen_system* y = &systems[i];
       char* slice = base + off;
@@ -1185,22 +1194,50 @@ static void run_cohort(ttamen_system* systems, int count, int64_t nswp, double e
                         y->offsets_out, y->stats, stream);
             });
             if (y->status != 0) {
-              std::lock_guard<std::mutex> lk(g_err_mu);
+              std::lock_guard<std::mutex> lk2(g_err_mu);
               if (first_err.empty()) first_err = "system " + std::to_string(i) + ": " + g_last_error;
             }
           },
-          [=, &mem] {
+          [=, &mem, &adm_mu, &running] {
+            std::lock_guard<std::mutex> lk2(adm_mu);
             mem.release(off, bytes);
+            --running;
             done(i);
-          });
+          },
+          std::min(16.0, std::max(1.0 / 16, (cost[i] / med) * (cost[i] / med))));
     }
     return next < count;
   };
-  co.run(admit);
+  std::string lane_err;
+  std::mutex lane_err_mu;
+  auto run_lane = [&](int l) {
+    try {
+      (void)hipSetDevice(dev);
+      co[l]->run(admit);
+    } catch (const std::exception& e) {
+      std::lock_guard<std::mutex> lk(lane_err_mu);
+      if (lane_err.empty()) lane_err = e.what();
+    }
+  };
+  std::vector<std::thread> threads;
+  for (int l = 1; l < lanes; ++l) threads.emplace_back(run_lane, l);
+  run_lane(0);
+  for (auto& t : threads) t.join();
+  for (auto s : streams) TT_HIP(hipStreamSynchronize(s));
+  (void)hipEventDestroy(ready_ev);
   {
     std::lock_guard<std::mutex> lk(g_cohort_mu);
-    g_cohort_last = co.counters();
+    Cohort::Counters sum;
+    for (auto& c : co) {
+      const Cohort::Counters& k = c->counters();
+      sum.rounds += k.rounds; sum.launches += k.launches; sum.items += k.items;
+      sum.resumes += k.resumes; sum.t_resume += k.t_resume; sum.t_build += k.t_build;
+      sum.t_gpu += k.t_gpu; sum.t_total = std::max(sum.t_total, k.t_total);
+      sum.head_kinds += k.head_kinds;
+    }
+    g_cohort_last = sum;
   }
+  TT_CHECK(lane_err.empty(), ERR_HIP, lane_err);
   TT_CHECK(first_err.empty(), ERR_NUMERIC, first_err);
 }
 
@@ -1209,12 +1246,16 @@ extern "C" {
 const char* ttamen_version(void) { return "ttamen 0.1.0 (gfx950)"; }
 size_t ttamen_cohort_staging_bytes(void) { return kStageBytes; }
 
-int ttamen_cohort_set_params(int window, int max_age, int gmres_poll) {
-  if (window < 1 || max_age < 1 || gmres_poll < 1) return ERR_INVALID;
+int ttamen_cohort_set_params(int window, int max_age, int gmres_poll, int lanes, int idle_pct) {
+  if (window < 1 || max_age < 1 || gmres_poll < 1 || lanes < 1 || lanes > kMaxLanes ||
+      idle_pct < 1 || idle_pct > 100)
+    return ERR_INVALID;
   CohortParams& p = cohort_params();
   p.window = window;
   p.max_age = max_age;
   p.gmres_poll = gmres_poll;
+  p.lanes = lanes;
+  p.idle_pct = idle_pct;
   return 0;
 }
 
@@ -1291,7 +1332,7 @@ size_t ttamen_amen_solve_batch_workspace_size(const ttamen_system* systems, int 
       TT_CHECK(w > 0, ERR_INVALID, "system " + std::to_string(i) + ": " + g_last_error);
       need.push_back(batch_slice(w));
     }
-    // room for every system at once (at most kMaxGroup of them run together)
+    // room for every system at once (at most 256 of them; more run as others finish)
     std::sort(need.begin(), need.end(), std::greater<size_t>());
     out = kStageBytes;
     for (int i = 0; i < std::min<int>(count, kMaxGroup); ++i) out += need[i];
@@ -1307,7 +1348,8 @@ int ttamen_amen_solve_batch(ttamen_system* systems, int count, int64_t nswp, dou
     TT_CHECK(systems && count >= 0 && workers >= 1 && workspace, ERR_INVALID, "bad batch arguments");
     if (count == 0) return;
     run_cohort(systems, count, nswp, eps, max_full, kickrank, kick2, local_iterations, resets,
-               precision, workers, kMaxGroup, workspace, workspace_bytes, (hipStream_t)stream,
+               precision, workers, kMaxLanes * kMaxGroup, workspace, workspace_bytes,
+               (hipStream_t)stream,
                [](int, int) { return true; }, [](int) {}, [](int) {});
   });
 }

@@ -26,11 +26,14 @@ namespace ttamen {
 class HostSync;
 
 // Scheduling parameters (ttamen_cohort_set_params): grouped launches per round, merge
-// steps a member may wait before it is served, GMRES iterations between polls.
+// steps a member may wait before it is served, GMRES iterations between polls, lanes,
+// and the share of the members' weight that, once idle, ends a round early.
 struct CohortParams {
   int window = 128;
-  int max_age = 32;
+  int max_age = 64;
   int gmres_poll = 8;
+  int lanes = 3;       // independent cohorts, each on its own stream, run concurrently
+  int idle_pct = 15;   // a round also ends once this % of the members' weight waits to resume
 };
 CohortParams& cohort_params();
 
@@ -44,7 +47,9 @@ class Cohort {
 
   // Add a member; it starts at the next round. `on_done` runs on the calling thread of
   // run() once the member's last recorded work has completed on the device.
-  void add(std::function<void()> body, std::function<void()> on_done);
+  // `weight` (> 0) ranks members when launch kinds compete: the kind whose waiting
+  // members weigh most is launched first (heavier systems are served first).
+  void add(std::function<void()> body, std::function<void()> on_done, double weight = 1.0);
   int members() const { return (int)members_.size(); }
 
   // Rounds until no member is left and `admit` (called before every round, may add

@@ -62,7 +62,7 @@ struct Cohort::Member final : Recorder {
   struct Op {
     GroupFn fn;
     dim3 grid, block;
-    uint32_t lds, off, size;
+    uint32_t lds, off, size, kind;
   };
   std::vector<Op> ops;
   std::vector<char> args;
@@ -74,14 +74,16 @@ struct Cohort::Member final : Recorder {
   int worker = 0;
   size_t head = 0;  // first op not yet issued
   int age = 0;      // merge steps waited with a non-empty queue
+  double weight = 1.0;
   HostSync* hs = nullptr;
   bool idle() const { return head == ops.size(); }
 
-  void record(GroupFn fn, dim3 grid, dim3 block, size_t lds, const void* a, size_t size) override {
+  void record(GroupFn fn, dim3 grid, dim3 block, size_t lds, const void* a, size_t size,
+              unsigned kind) override {
     const size_t off = (args.size() + 15) & ~size_t(15);
     args.resize(off + size);
     std::memcpy(args.data() + off, a, size);
-    ops.push_back({fn, grid, block, (uint32_t)lds, (uint32_t)off, (uint32_t)size});
+    ops.push_back({fn, grid, block, (uint32_t)lds, (uint32_t)off, (uint32_t)size, kind});
   }
   // Yield to the worker; the cohort runs the recorded work before resuming us.
   void sync() override {
@@ -154,6 +156,7 @@ Cohort::Cohort(hipStream_t st, void* dstage, size_t stage_bytes, int workers)
   TT_CHECK(dstage && stage_bytes >= 65536, ERR_INVALID, "cohort: staging buffer too small");
   TT_HIP(hipHostMalloc((void**)&hstage_, stage_cap_, hipHostMallocDefault));
   trace_ = std::getenv("TTAMEN_COHORT_TRACE") != nullptr;
+  register_kind_name(&mega_exec, "merged (mega) kind");
   sync_->lists.resize(workers_);
 }
 
@@ -164,11 +167,12 @@ Cohort::~Cohort() {
   if (hstage_) (void)hipHostFree(hstage_);
 }
 
-void Cohort::add(std::function<void()> body, std::function<void()> on_done) {
+void Cohort::add(std::function<void()> body, std::function<void()> on_done, double weight) {
   TT_CHECK((int)members_.size() < kMaxGroup, ERR_INVALID, "cohort: too many members");
   std::unique_ptr<Member> m(new Member());
   m->body = std::move(body);
   m->on_done = std::move(on_done);
+  m->weight = weight > 0 ? weight : 1.0;
   m->hs = hs_acquire();
   // balance: the worker with the fewest members
   std::vector<int> load(workers_, 0);
@@ -219,13 +223,21 @@ void Cohort::dispatch(int max_steps) {
   struct Key {
     GroupFn fn;
     unsigned bx, by, bz;
-    int count;
+    double score;
   };
   std::vector<Key> keys;
   std::vector<int> sel;
+  double total_w = 0.0, busy_w = 0.0;
+  for (auto& m : members_) {
+    total_w += m->weight;
+    if (!m->idle()) busy_w += m->weight;
+  }
+  const double idle_stop = total_w * cohort_params().idle_pct / 100.0;
   for (int step = 0; step < max_steps; ++step) {
-    // Launch kinds at the heads of the queues: take the most common one, unless a member
-    // has waited max_age steps (then its kind, so that no member starves).
+    // end the round early once enough of the cohort waits to be resumed
+    if (step > 0 && total_w - busy_w >= idle_stop && idle_stop < total_w) break;
+    // Launch kinds at the heads of the queues: take the one whose waiting members weigh
+    // most, unless a member has waited max_age steps (then its kind, so none starves).
     keys.clear();
     int oldest = -1;
     for (size_t i = 0; i < act.size(); ++i) {
@@ -236,17 +248,17 @@ void Cohort::dispatch(int max_steps) {
       bool found = false;
       for (auto& k : keys)
         if (k.fn == op.fn && k.bx == op.block.x && k.by == op.block.y && k.bz == op.block.z) {
-          ++k.count;
+          k.score += m->weight;
           found = true;
           break;
         }
-      if (!found) keys.push_back({op.fn, op.block.x, op.block.y, op.block.z, 1});
+      if (!found) keys.push_back({op.fn, op.block.x, op.block.y, op.block.z, m->weight});
     }
     if (keys.empty()) break;
     cnt_.head_kinds += (double)keys.size();
     Key best = keys[0];
     for (const auto& k : keys)
-      if (k.count > best.count) best = k;
+      if (k.score > best.score) best = k;
     if (act[oldest]->age >= cohort_params().max_age) {
       const Member::Op& op = act[oldest]->ops[act[oldest]->head];
       best = Key{op.fn, op.block.x, op.block.y, op.block.z, 0};
@@ -262,7 +274,8 @@ void Cohort::dispatch(int max_steps) {
       else
         m->age += 1;
     }
-    const uint32_t size = act[sel[0]]->ops[act[sel[0]]->head].size;
+    uint32_t size = 0;
+    for (int i : sel) size = std::max(size, act[i]->ops[act[i]->head].size);
     const size_t stride = 16 + ((size + 15) & ~size_t(15));
     const size_t bytes = stride * sel.size();
     TT_CHECK(bytes <= stage_cap_, ERR_WORKSPACE, "cohort: staging buffer too small");
@@ -287,11 +300,12 @@ void Cohort::dispatch(int max_steps) {
       g[0] = op.grid.x;
       g[1] = op.grid.y;
       g[2] = op.grid.z;
-      g[3] = 0;
+      g[3] = op.kind;
       std::memcpy(hp + j * stride + 16, m->args.data() + op.off, op.size);
       gl.lds = std::max<size_t>(gl.lds, op.lds);
       m->head += 1;
       m->age = 0;
+      if (m->idle()) busy_w -= m->weight;
     }
     TT_CHECK(total < (1ull << 31), ERR_INVALID, "cohort: grouped grid too large");
     gl.hdr.start[sel.size()] = (unsigned)total;
@@ -329,6 +343,7 @@ void Cohort::run(const std::function<bool(Cohort&)>& admit) {
     if (more) more = admit(*this);
     if (members_.empty()) {
       if (!more) break;
+      std::this_thread::sleep_for(std::chrono::microseconds(50));
       continue;
     }
     // Resume the members whose recorded work has all run.

@@ -1,0 +1,289 @@
+// Kernels of the device-resident GMRES (bodies in the launch.h form), shared by the
+// GMRES driver and the cohort's merged launch kind.
+#pragma once
+#include "ttamen_internal.h"
+#include <type_traits>
+#include "gmres.h"
+#include <algorithm>
+
+namespace ttamen {
+
+constexpr int kGB = 256;  // threads per block (kGB >= kGmresMaxBlocks)
+static_assert(kGB >= kGmresMaxBlocks, "one thread per partial in the block reductions");
+
+inline int gm_blocks(int64_t N) {
+  int64_t b = (N + kGB * 4 - 1) / (kGB * 4);
+  return (int)std::max<int64_t>(1, std::min<int64_t>(b, kGmresMaxBlocks));
+}
+
+__device__ __forceinline__ double gm_wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+__device__ __forceinline__ bool gm_skip(const int* a) { return a && *a; }
+
+// out[i] = sum_b P[i * kGmresMaxBlocks + b] over b < nb, for i < rows; every thread of
+// the block gets the results in LDS. Fixed reduction tree: deterministic.
+__device__ __forceinline__ void gm_reduce_rows(const double* P, int rows, int nb, double* out) {
+  __shared__ double red[kGB / 64][kGmresMaxCols + 1];
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  for (int i = 0; i < rows; ++i) {
+    double v = (t < nb) ? P[(int64_t)i * kGmresMaxBlocks + t] : 0.0;
+    v = gm_wave_sum(v);
+    if (lane == 0) red[wv][i] = v;
+  }
+  __syncthreads();
+  for (int i = t; i < rows; i += kGB) {
+    double s = 0.0;
+    for (int w = 0; w < kGB / 64; ++w) s += red[w][i];
+    out[i] = s;
+  }
+  __syncthreads();
+}
+
+// Whether the second (re-orthogonalisation) pass runs: the first pass cancelled more
+// than 1 - 1/sqrt(2) of ||w|| (Daniel, Gragg, Kaufman and Stewart, Math. Comp. 30 (1976)).
+// Pw: row kc of the first dot partials (w.w); Pn1: ||w'||^2 partials after pass 1.
+__device__ __forceinline__ bool gm_need_reorth(const double* Pw, const double* Pn1, int nb) {
+  __shared__ double r2[2];
+  gm_reduce_rows(Pw, 1, nb, r2);
+  __shared__ double r1[2];
+  gm_reduce_rows(Pn1, 1, nb, r1);
+  return !(sqrt(r1[0]) > 0.7071067811865476 * sqrt(r2[0]));
+}
+
+// P[i][b] = Q_i . w over block b's chunk (i < kc), P[kc][b] = w . w. Each wave streams
+// whole rows over the chunk (coalesced); the chunk of w stays in L1.
+template <typename T>
+struct KCgsDots {
+  struct Args {
+    const T* Q; int64_t N; int kc; const T* w; double* P; int with_ww; const int* skip;
+    const double* Pw; const double* Pn1;
+  };
+  static constexpr int kThreads = kGB;
+  static constexpr int kMega = std::is_same<T, double>::value ? 4 : 5;
+  static __device__ __forceinline__ void run(const Args& a, const Blk& bk);
+};
+template <typename T>
+__device__ __forceinline__ void KCgsDots<T>::run(const Args& a, const Blk& bk) {
+  const T* Q = a.Q; const int64_t N = a.N; const int kc = a.kc; const T* w = a.w;
+  double* P = a.P;
+  if (gm_skip(a.skip)) return;
+  if (a.Pw && !gm_need_reorth(a.Pw, a.Pn1, bk.gx)) return;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int64_t chunk = (N + bk.gx - 1) / bk.gx;
+  const int64_t lo = bk.x * chunk, hi = min(N, lo + chunk);
+  const int with_ww = a.with_ww;
+  const int rows = kc + (with_ww ? 1 : 0);
+  for (int i = wv; i < rows; i += kGB / 64) {
+    const T* row = (i < kc) ? Q + (int64_t)i * N : w;
+    double acc = 0.0;
+    for (int64_t j = lo + lane; j < hi; j += 64) acc += (double)row[j] * (double)w[j];
+    acc = gm_wave_sum(acc);
+    if (lane == 0) P[(int64_t)i * kGmresMaxBlocks + bk.x] = acc;
+  }
+}
+
+// h = sum_b P[:, b]; w -= sum_i h_i Q_i over the block's chunk; Pn[b] = ||w chunk||^2.
+// Block 0 writes (pass 1) or adds (pass 2) h into the Hessenberg column.
+template <typename T>
+struct KCgsUpdate {
+  struct Args {
+    const T* Q; int64_t N; int kc; T* w; const double* P; double* Hcol; int accumulate;
+    double* Pn; const int* skip; const double* Pw; const double* Pn1; int* reorth_count;
+  };
+  static constexpr int kThreads = kGB;
+  static constexpr int kMega = std::is_same<T, double>::value ? 6 : 7;
+  static __device__ __forceinline__ void run(const Args& a, const Blk& bk);
+};
+template <typename T>
+__device__ __forceinline__ void KCgsUpdate<T>::run(const Args& a, const Blk& bk) {
+  const T* Q = a.Q; const int64_t N = a.N; const int kc = a.kc; T* w = a.w;
+  double* Hcol = a.Hcol; const int accumulate = a.accumulate; int* reorth_count = a.reorth_count;
+  if (gm_skip(a.skip)) return;
+  if (a.Pw && !gm_need_reorth(a.Pw, a.Pn1, bk.gx)) return;
+  __shared__ double hs[kGmresMaxCols + 1];
+  __shared__ double sh[kGB / 64];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  gm_reduce_rows(a.P, kc, bk.gx, hs);
+  if (bk.x == 0) {
+    for (int i = threadIdx.x; i < kc; i += kGB) Hcol[i] = accumulate ? Hcol[i] + hs[i] : hs[i];
+    if (reorth_count && threadIdx.x == 0) *reorth_count += 1;
+  }
+  const int64_t chunk = (N + bk.gx - 1) / bk.gx;
+  const int64_t lo = bk.x * chunk, hi = min(N, lo + chunk);
+  double acc = 0.0;
+  for (int64_t j = lo + threadIdx.x; j < hi; j += kGB) {
+    double v = (double)w[j];
+    for (int i = 0; i < kc; ++i) v -= hs[i] * (double)Q[(int64_t)i * N + j];
+    w[j] = (T)v;
+    acc += v * v;
+  }
+  acc = gm_wave_sum(acc);
+  if (lane == 0) sh[wv] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s = 0.0;
+    for (int i = 0; i < kGB / 64; ++i) s += sh[i];
+    a.Pn[bk.x] = s;
+  }
+}
+
+// q_{k+1} = w / ||w||; block 0 / thread 0 finishes the Hessenberg column with Givens
+// rotations (reference gmres.h:20-31, 106-129) and tests convergence.
+template <typename T>
+struct KGmresClose {
+  struct Args {
+    const T* w; T* qn; int64_t N; const double* Pw; const double* Pn1; const double* Pn2;
+    GmresDev g; int k; const int* done_in;
+  };
+  static constexpr int kThreads = kGB;
+  static constexpr int kMega = std::is_same<T, double>::value ? 8 : 9;
+  static __device__ __forceinline__ void run(const Args& a, const Blk& bk);
+};
+template <typename T>
+__device__ __forceinline__ void KGmresClose<T>::run(const Args& a, const Blk& bk) {
+  const T* w = a.w; T* qn = a.qn; const int64_t N = a.N; const GmresDev g = a.g; const int k = a.k;
+  if (*a.done_in) return;
+  __shared__ double nn[2];
+  const bool reorth = gm_need_reorth(a.Pw, a.Pn1, bk.gx);
+  gm_reduce_rows(reorth ? a.Pn2 : a.Pn1, 1, bk.gx, nn);
+  const double hn = sqrt(nn[0]);
+  const double inv = (hn > 0.0) ? 1.0 / hn : 0.0;
+  for (int64_t j = bk.x * (int64_t)kGB + threadIdx.x; j < N; j += (int64_t)bk.gx * kGB)
+    qn[j] = (T)((double)w[j] * inv);
+  if (bk.x == 0 && threadIdx.x == 0) {
+    const int ld = g.m + 1;
+    double* h = g.H + (int64_t)k * ld;
+    h[k + 1] = hn;
+    for (int i = 0; i < k; ++i) {
+      double t = g.cs[i] * h[i] + g.sn[i] * h[i + 1];
+      h[i + 1] = -g.sn[i] * h[i] + g.cs[i] * h[i + 1];
+      h[i] = t;
+    }
+    double den = sqrt(h[k] * h[k] + h[k + 1] * h[k + 1]);
+    double c = 1.0, sn = 0.0;
+    if (den > 0.0) { c = h[k] / den; sn = h[k + 1] / den; }
+    h[k] = c * h[k] + sn * h[k + 1];
+    h[k + 1] = 0.0;
+    g.cs[k] = c;
+    g.sn[k] = sn;
+    g.beta[k + 1] = -sn * g.beta[k];
+    g.beta[k] = c * g.beta[k];
+    const double err = fabs(g.beta[k + 1]) / g.dstate[GM_BNORM];
+    g.istate[GM_KCOLS] = k + 1;
+    g.istate[GM_ITERS] += 1;
+    g.dstate[GM_ERR] = err;
+    if (err <= g.dstate[GM_THR]) {
+      g.istate[GM_FLAG] = 1;
+      g.istate[GM_DONE] = 1;
+    } else if (k + 1 >= g.m) {
+      g.istate[GM_DONE] = 1;  // cycle exhausted (flag stays 0)
+    }
+  }
+}
+
+// Start of a cycle: r already holds rhs - Op(x0) and dstate[GM_RNORM] = ||r||.
+// q_0 = r / ||r||, beta = (||r||, 0, ...), done = (||r|| <= 0).
+template <typename T>
+struct KGmresBegin {
+  struct Args { const T* r; T* q0; int64_t N; GmresDev g; };
+  static constexpr int kThreads = kGB;
+  static constexpr int kMega = std::is_same<T, double>::value ? 10 : 11;
+  static __device__ __forceinline__ void run(const Args& a, const Blk& bk);
+};
+template <typename T>
+__device__ __forceinline__ void KGmresBegin<T>::run(const Args& a, const Blk& bk) {
+  const T* r = a.r; T* q0 = a.q0; const int64_t N = a.N; const GmresDev g = a.g;
+  const double rn = g.dstate[GM_RNORM];
+  const double inv = (rn > 0.0) ? 1.0 / rn : 0.0;
+  for (int64_t j = bk.x * (int64_t)kGB + threadIdx.x; j < N; j += (int64_t)bk.gx * kGB)
+    q0[j] = (T)((double)r[j] * inv);
+  if (bk.x == 0) {
+    for (int i = threadIdx.x; i <= g.m; i += kGB) g.beta[i] = 0.0;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      g.beta[0] = rn;
+      g.istate[GM_KCOLS] = 0;
+      const int zero = !(rn > 0.0);
+      g.istate[GM_DONE] = zero;
+      if (zero) g.istate[GM_FLAG] = 1;
+    }
+  }
+}
+
+// thr = real_tol * norm_rhs / ||drhs||,  bnorm = ||drhs||  (reference amen_solve.h:340-343)
+struct KGmresSetup {
+  struct Args { GmresDev g; const double* norm_rhs; const double* dnorm; double real_tol; };
+  static constexpr int kThreads = 64;
+  static constexpr int kMega = 14;
+  static __device__ __forceinline__ void run(const Args& a, const Blk&);
+};
+__device__ __forceinline__ void KGmresSetup::run(const Args& a, const Blk&) {
+  if (threadIdx.x != 0) return;
+  const GmresDev g = a.g; const double* norm_rhs = a.norm_rhs; const double* dnorm = a.dnorm;
+  const double real_tol = a.real_tol;
+  g.dstate[GM_BNORM] = *dnorm;
+  g.dstate[GM_THR] = real_tol * (*norm_rhs) / (*dnorm);
+  g.dstate[GM_RNORM] = *dnorm;
+  g.istate[GM_FLAG] = 0;
+  g.istate[GM_ITERS] = 0;
+  g.istate[GM_DONE] = 0;
+  g.istate[GM_KCOLS] = 0;
+  g.istate[GM_REORTHS] = 0;
+}
+
+// y = H(0:kc,0:kc) \ beta(0:kc) (upper triangular), one thread.
+struct KGmresY {
+  struct Args { GmresDev g; };
+  static constexpr int kThreads = 64;
+  static constexpr int kMega = 15;
+  static __device__ __forceinline__ void run(const Args& a, const Blk&);
+};
+__device__ __forceinline__ void KGmresY::run(const Args& a, const Blk&) {
+  if (threadIdx.x != 0) return;
+  const GmresDev g = a.g;
+  const int kc = g.istate[GM_KCOLS];
+  const int ld = g.m + 1;
+  for (int i = kc - 1; i >= 0; --i) {
+    double s = g.beta[i];
+    for (int j = i + 1; j < kc; ++j) s -= g.H[(int64_t)j * ld + i] * g.y[j];
+    const double d = g.H[(int64_t)i * ld + i];
+    g.y[i] = (d != 0.0) ? s / d : 0.0;
+  }
+}
+
+// x += sum_{i<kc} y_i Q_i
+template <typename T>
+struct KGmresXupd {
+  struct Args { const T* Q; int64_t N; T* x; GmresDev g; };
+  static constexpr int kThreads = kGB;
+  static constexpr int kMega = std::is_same<T, double>::value ? 12 : 13;
+  static __device__ __forceinline__ void run(const Args& a, const Blk& bk);
+};
+template <typename T>
+__device__ __forceinline__ void KGmresXupd<T>::run(const Args& a, const Blk& bk) {
+  const T* Q = a.Q; const int64_t N = a.N; T* x = a.x; const GmresDev g = a.g;
+  const int kc = g.istate[GM_KCOLS];
+  __shared__ double y[kGmresMaxCols];
+  for (int i = threadIdx.x; i < kc; i += kGB) y[i] = g.y[i];
+  __syncthreads();
+  for (int64_t j = bk.x * (int64_t)kGB + threadIdx.x; j < N; j += (int64_t)bk.gx * kGB) {
+    double v = (double)x[j];
+    for (int i = 0; i < kc; ++i) v += y[i] * (double)Q[(int64_t)i * N + j];
+    x[j] = (T)v;
+  }
+}
+
+struct KGmresRearm {
+  struct Args { GmresDev g; };
+  static constexpr int kThreads = 64;
+  static constexpr int kMega = 16;
+  static __device__ __forceinline__ void run(const Args& a, const Blk&) {
+    if (threadIdx.x == 0) a.g.istate[GM_DONE] = 0;
+  }
+};
+
+}  // namespace ttamen

@@ -16,9 +16,28 @@
 #include <hip/hip_runtime.h>
 #include <cstddef>
 #include <cstdint>
+#include <type_traits>
 #include <typeinfo>
 
 namespace ttamen {
+
+// Address space of read-only device memory (scalar loads on AMDGPU).
+#define TT_CONST __attribute__((address_space(4)))
+
+// Copy a plain struct out of read-only device memory with scalar loads.
+template <class A>
+__device__ __forceinline__ A load_const(const char* p) {
+  static_assert(sizeof(A) % 4 == 0, "launch arguments are made of 4-byte words");
+  union U {
+    A a;
+    unsigned w[sizeof(A) / 4];
+    __device__ U() {}
+  } u;
+  const TT_CONST unsigned* q = (const TT_CONST unsigned*)p;
+#pragma unroll
+  for (int i = 0; i < (int)(sizeof(A) / 4); ++i) u.w[i] = q[i];
+  return u.a;
+}
 
 struct Blk {
   unsigned x, y, z;     // block index within its own launch
@@ -28,7 +47,7 @@ struct Blk {
 constexpr int kMaxGroup = 256;  // launches merged into one grouped launch, at most
 
 // Items of a grouped launch live in device memory, `stride` bytes apart:
-//   unsigned grid[4] (gx, gy, gz, unused), then the Args of that launch.
+//   unsigned grid[4] (gx, gy, gz, kind), then the Args of that launch.
 struct GroupHdr {
   const char* items;
   unsigned stride;
@@ -51,16 +70,18 @@ __global__ __launch_bounds__(K::kThreads) void k_grp(GroupHdr h) {
     else hi = mid - 1;
   }
   const char* it = h.items + (size_t)lo * h.stride;
-  const unsigned* g = (const unsigned*)it;
-  const unsigned gx = g[0], gy = g[1], gz = g[2];
+  const uint4 g = load_const<uint4>(it);
   unsigned l = b - h.start[lo];
   Blk bk;
-  bk.gx = gx; bk.gy = gy; bk.gz = gz;
-  bk.x = l % gx;
-  l /= gx;
-  bk.y = l % gy;
-  bk.z = l / gy;
-  K::run(*(const typename K::Args*)(it + 16), bk);
+  bk.gx = g.x; bk.gy = g.y; bk.gz = g.z;
+  bk.x = l % g.x;
+  l /= g.x;
+  bk.y = l % g.y;
+  bk.z = l / g.y;
+  // The item table is read-only for the kernel's lifetime: load it through the constant
+  // address space so the arguments land in scalar registers, as kernel arguments do.
+  const typename K::Args a = load_const<typename K::Args>(it + 16);
+  K::run(a, bk);
 }
 
 // One grouped launch as prepared by the cohort: items already copied to the device.
@@ -81,8 +102,9 @@ void group_exec(const GroupLaunch& gl) {
 // Receives launches made while a cohort member runs on this thread.
 class Recorder {
  public:
+  // kind: the kernel index within a merged launch kind (mega_exec), else 0.
   virtual void record(GroupFn fn, dim3 grid, dim3 block, size_t lds, const void* args,
-                      size_t size) = 0;
+                      size_t size, unsigned kind) = 0;
   // Run everything recorded so far and wait for it (a host synchronisation point).
   virtual void sync() = 0;
   virtual ~Recorder() = default;
@@ -99,12 +121,29 @@ inline void register_kind() {
   (void)once;
 }
 
+// Kernels that declare `static constexpr int kMega` (their index in mega.hip) are, inside
+// a cohort, recorded as items of one merged launch kind: a 256-thread kernel that runs
+// each item's own body, so that the cohort can serve systems that are at different
+// steps of the sweep with one launch. They use no dynamic LDS and work for any block
+// size up to 256 threads.
+constexpr int kMegaThreads = 256;
+void mega_exec(const GroupLaunch& gl);
+
+template <class K, class = void>
+struct has_mega : std::false_type {};
+template <class K>
+struct has_mega<K, std::void_t<decltype(K::kMega)>> : std::true_type {};
+
 template <class K>
 inline void launch(dim3 grid, dim3 block, size_t lds, hipStream_t st, const typename K::Args& a) {
   if (grid.x == 0 || grid.y == 0 || grid.z == 0) return;
   if (Recorder* r = current_recorder()) {
-    register_kind<K>();
-    r->record(&group_exec<K>, grid, block, lds, &a, sizeof(a));
+    if constexpr (has_mega<K>::value) {
+      r->record(&mega_exec, grid, dim3(kMegaThreads), 0, &a, sizeof(a), (unsigned)K::kMega);
+    } else {
+      register_kind<K>();
+      r->record(&group_exec<K>, grid, block, lds, &a, sizeof(a), 0);
+    }
     return;
   }
   hipLaunchKernelGGL((k_one<K>), grid, block, lds, st, a);
