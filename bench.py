@@ -54,6 +54,7 @@ def parse():
     p.add_argument("--systems", type=int, default=256)
     p.add_argument("--workers", type=int, default=8, help="host threads running the systems' host code")
     p.add_argument("--slots", type=int, default=256, help="serve: requests running at once")
+    p.add_argument("--lanes", type=int, default=3, help="concurrent cohorts (streams) of the batched solver")
     p.add_argument("--plain-sample", type=int, default=32)
     p.add_argument("--eps", type=float, default=1e-8)
     p.add_argument("--rate", type=float, default=200.0)
@@ -304,6 +305,7 @@ def main():
     from torchtt_amd import workloads as W
     from torchtt_amd import _lib
     lib = _lib.load()
+    _lib.check("cohort_set_params", lib.ttamen_cohort_set_params(128, 64, 8, args.lanes, 15))
     fn = {"batch": run_batch, "potential": run_potential, "serve": run_serve}[args.workload]
     print(json.dumps(fn(args, tt, W, lib, _lib)), flush=True)
 
