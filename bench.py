@@ -11,6 +11,8 @@ Workloads (--workload):
              time, is measured on a sample of the same systems for the comparison.
   potential  one system: d=10, n=64, Laplacian + potential (TT rank 4), random rank-4 rhs,
              rank cap 80, eps 1e-8; a step is one solve.
+  precision  one system: d=12, n=64, rank cap 128, solved in FP64 and in mixed precision; the
+             error of the mixed solution vs the FP64 one is reported.
   serve      Poisson arrivals at --rate requests/s of systems drawn as in `batch`, for
              --duration seconds, served by the continuous-batching loop; reports p50/p99
              latency (completion - arrival) and solves/s.
@@ -49,7 +51,7 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=1)
     p.add_argument("--warmup", type=int, default=1)
-    p.add_argument("--workload", default="batch", choices=["batch", "potential", "serve"])
+    p.add_argument("--workload", default="batch", choices=["batch", "potential", "precision", "serve"])
     p.add_argument("--precision", default=None, choices=["fp64", "fp32"])
     p.add_argument("--systems", type=int, default=256)
     p.add_argument("--workers", type=int, default=8, help="host threads running the systems' host code")
@@ -266,6 +268,48 @@ def run_potential(args, tt, W, lib, _lib):
     return out
 
 
+def run_precision(args, tt, W, lib, _lib):
+    """d=12, n=64, Laplacian + potential (TT rank 4), random rank-4 rhs, rank cap 128, eps 1e-8:
+    one FP64 solve and one mixed-precision solve (FP32 local GMRES on the FP32 local operator
+    inside FP64 refinement) from the same residual-proxy draws; error of the mixed solution vs
+    the FP64 one, residual of each."""
+    dev = torch.device("cuda", 0)
+    A, b = W.laplace_potential(12, 64, seed=0)
+    Ag = tt.TT([c.to(dev) for c in A])
+    bg = tt.TT([c.to(dev) for c in b])
+    torch.manual_seed(0)
+    z = tt.solvers._draw_z(bg.N, 4, 0, "cpu")
+    rows = {}
+    for prec in ("fp64", "fp32"):
+        for _ in range(args.warmup):
+            tt.solvers.amen_solve(Ag, bg, eps=args.eps, rmax=128, precision=prec, _z0=z)
+        torch.cuda.synchronize()
+        _lib.check("profile_enable", lib.ttamen_profile_enable(1))
+        t0 = time.perf_counter()
+        x = tt.solvers.amen_solve(Ag, bg, eps=args.eps, rmax=128, precision=prec, _z0=z)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        prof = (ctypes.c_double * 8)()
+        _lib.check("profile_read", lib.ttamen_profile_read(prof))
+        lib.ttamen_profile_enable(0)
+        st = dict(tt.solvers.last_stats)
+        rows[prec] = {"x": x, "time_s": dt, "residual": tt.solvers.residual(Ag, x, bg), "sweeps": st["sweeps"],
+                      "gmres_iters": st["gmres_iters"], "max_rank": st["max_rank"],
+                      "refinements": st.get("refinements", 0), "roofline": roofline(prof, prec)}
+    x64, x32 = rows["fp64"].pop("x"), rows["fp32"].pop("x")
+    err = float((x32 - x64).norm() / x64.norm())
+    ok = sum(1 for r in rows.values() if r["residual"] <= RES_BAR)
+    total = rows["fp64"]["time_s"] + rows["fp32"]["time_s"]
+    cfg = {"workload": "AMEn solve, Laplacian + potential, d=12, n=64, TT rank 4, random rank-4 rhs, rank cap "
+                       "128, eps %g: FP64 and FP32-inner + FP64-refinement" % args.eps,
+           "d": 12, "n": 64, "rmax": 128, "eps": args.eps}
+    out = base_record(args, 1.0 / rows["fp32"]["time_s"] if rows["fp32"]["residual"] <= RES_BAR else 0.0,
+                      total, cfg, dtype_of("fp32"))
+    out.update({"fp64": rows["fp64"], "fp32": rows["fp32"], "error_fp32_vs_fp64": err, "solved_le_1e-8": ok,
+                "roofline": rows["fp32"]["roofline"]})
+    return out
+
+
 def run_serve(args, tt, W, lib, _lib):
     dev = torch.device("cuda", 0)
     arrivals = tt.serving.poisson_arrivals(args.rate, args.duration, seed=1)
@@ -306,7 +350,8 @@ def main():
     from torchtt_amd import _lib
     lib = _lib.load()
     _lib.check("cohort_set_params", lib.ttamen_cohort_set_params(128, 64, 8, args.lanes, 15))
-    fn = {"batch": run_batch, "potential": run_potential, "serve": run_serve}[args.workload]
+    fn = {"batch": run_batch, "potential": run_potential, "precision": run_precision,
+          "serve": run_serve}[args.workload]
     print(json.dumps(fn(args, tt, W, lib, _lib)), flush=True)
 
 

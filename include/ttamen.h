@@ -152,6 +152,19 @@ int ttamen_serve(ttamen_system* systems, const double* arrival_s, int count, int
                  void* workspace, size_t workspace_bytes, double* start_s, double* done_s,
                  void* stream);
 
+/*
+ * TT rounding (replaces the reference's native round_this, cpp/compression.h:3-41, bound as
+ * torchttcpp.round_this at cpp/cpp_ext.cpp:10, which rounds in place): left-to-right QR, then
+ * right-to-left truncated SVD with relative accuracy eps / sqrt(d-1) per bond and rank cap
+ * rmax (rank rule of cpp/ortho.h:16-44). Input cores (device, FP64) are not modified; the
+ * rounded cores are written to `out` at offsets_out[k] with ranks r_out (d+1). N[k] is the
+ * mode size of core k (m*n for an operator core). Ranks up to 140. Synchronous at the end.
+ */
+size_t ttamen_tt_round_workspace_size(int d, const int64_t* N, const int64_t* R);
+int ttamen_tt_round(int d, const int64_t* N, const int64_t* R, const double* const* cores,
+                    double eps, int64_t rmax, void* workspace, size_t workspace_bytes, double* out,
+                    size_t out_elems, int64_t* r_out, int64_t* offsets_out, void* stream);
+
 /* Bytes at the front of a batch / serving workspace that hold grouped-launch tables. */
 size_t ttamen_cohort_staging_bytes(void);
 /* Counters of the last batch / serving call (no counterpart in the reference): rounds,

@@ -257,10 +257,17 @@ class TT:
         return nrm ** 2 if squared else nrm
 
     def round(self, eps=1e-12, rmax=sys.maxsize):
-        """TT rounding: left-to-right QR, then right-to-left truncated SVD (rank_chop)."""
+        """TT rounding: left-to-right QR, then right-to-left truncated SVD (rank_chop).
+
+        FP64 cores on a ROCm GPU go through libttamen (solvers.round_tt, the reference's native
+        round_this); other inputs use torch's QR/SVD with the same rule."""
         d = len(self.cores)
         if d <= 1:
             return self.clone()
+        if self.cores[0].is_cuda and self.cores[0].dtype == torch.float64 and isinstance(rmax, int) \
+                and max(self.R) <= 140:
+            from .solvers import round_tt
+            return round_tt(self, eps, rmax)
         cores = [c.clone() for c in self.cores]
         for k in range(d - 1):
             shp = cores[k].shape

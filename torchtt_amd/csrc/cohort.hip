@@ -156,7 +156,8 @@ Cohort::Cohort(hipStream_t st, void* dstage, size_t stage_bytes, int workers)
   TT_CHECK(dstage && stage_bytes >= 65536, ERR_INVALID, "cohort: staging buffer too small");
   TT_HIP(hipHostMalloc((void**)&hstage_, stage_cap_, hipHostMallocDefault));
   trace_ = std::getenv("TTAMEN_COHORT_TRACE") != nullptr;
-  register_kind_name(&mega_exec, "merged (mega) kind");
+  register_kind_name(&mega_exec, "merged kind: GMRES / streaming / scalar kernels");
+  register_kind_name(&mega_gemm_exec, "merged kind: GEMMs");
   sync_->lists.resize(workers_);
 }
 

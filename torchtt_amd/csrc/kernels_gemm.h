@@ -72,6 +72,7 @@ struct KGemm {
   typedef GemmParams<T> Args;
   static constexpr int kThreads = 256;
   static constexpr int kMega = std::is_same<T, double>::value ? 0 : 1;
+  static constexpr int kMegaGroup = 1;
   static __device__ __forceinline__ void run(const Args& pp, const Blk& bk) {
     static_assert(GemmTile<T, 64, 64, 16>::kLds <= 2 * 16 * (65 + 65) * sizeof(double), "gemm LDS");
     char* smem = gemm_lds();
@@ -228,6 +229,7 @@ struct KGemmReduce {
   struct Args { GemmParams<T> p; int splits; };
   static constexpr int kThreads = 256;
   static constexpr int kMega = std::is_same<T, double>::value ? 2 : 3;
+  static constexpr int kMegaGroup = 1;
   static __device__ __forceinline__ void run(const Args& a, const Blk& bk);
 };
 

@@ -126,20 +126,29 @@ inline void register_kind() {
 // each item's own body, so that the cohort can serve systems that are at different
 // steps of the sweep with one launch. They use no dynamic LDS and work for any block
 // size up to 256 threads.
+// Kernels that also declare `kMegaGroup = 1` (the GEMMs) form a second merged kind, so
+// that the light kernels of the first run at the occupancy their own registers allow.
 constexpr int kMegaThreads = 256;
 void mega_exec(const GroupLaunch& gl);
+void mega_gemm_exec(const GroupLaunch& gl);
 
 template <class K, class = void>
 struct has_mega : std::false_type {};
 template <class K>
 struct has_mega<K, std::void_t<decltype(K::kMega)>> : std::true_type {};
+template <class K, class = void>
+struct mega_group : std::integral_constant<int, 0> {};
+template <class K>
+struct mega_group<K, std::void_t<decltype(K::kMegaGroup)>>
+    : std::integral_constant<int, K::kMegaGroup> {};
 
 template <class K>
 inline void launch(dim3 grid, dim3 block, size_t lds, hipStream_t st, const typename K::Args& a) {
   if (grid.x == 0 || grid.y == 0 || grid.z == 0) return;
   if (Recorder* r = current_recorder()) {
     if constexpr (has_mega<K>::value) {
-      r->record(&mega_exec, grid, dim3(kMegaThreads), 0, &a, sizeof(a), (unsigned)K::kMega);
+      r->record(mega_group<K>::value ? &mega_gemm_exec : &mega_exec, grid, dim3(kMegaThreads), 0,
+                &a, sizeof(a), (unsigned)K::kMega);
     } else {
       register_kind<K>();
       r->record(&group_exec<K>, grid, block, lds, &a, sizeof(a), 0);

@@ -37,7 +37,8 @@ typedef KStoreHost M35; typedef KScaleCols M36; typedef KRandn M37;
     }                                                                               \
     break;
 
-__global__ __launch_bounds__(kMegaThreads) void k_mega(GroupHdr h) {
+// Locate the item of block blockIdx.x; returns its kernel index, arguments and block.
+__device__ __forceinline__ unsigned mega_item(const GroupHdr& h, const char** args, Blk* bk) {
   const unsigned b = blockIdx.x;
   int lo = 0, hi = h.n - 1;
   while (lo < hi) {
@@ -47,17 +48,22 @@ __global__ __launch_bounds__(kMegaThreads) void k_mega(GroupHdr h) {
   }
   const char* it = h.items + (size_t)lo * h.stride;
   const uint4 g = load_const<uint4>(it);
-  const unsigned kind = g.w;
   unsigned l = b - h.start[lo];
-  Blk bk;
-  bk.gx = g.x; bk.gy = g.y; bk.gz = g.z;
-  bk.x = l % g.x;
+  bk->gx = g.x; bk->gy = g.y; bk->gz = g.z;
+  bk->x = l % g.x;
   l /= g.x;
-  bk.y = l % g.y;
-  bk.z = l / g.y;
-  const char* args = it + 16;
+  bk->y = l % g.y;
+  bk->z = l / g.y;
+  *args = it + 16;
+  return g.w;
+}
+
+__global__ __launch_bounds__(kMegaThreads) void k_mega(GroupHdr h) {
+  const char* args;
+  Blk bk;
+  const unsigned kind = mega_item(h, &args, &bk);
   switch (kind) {
-    TT_MEGA_CASE(0) TT_MEGA_CASE(1) TT_MEGA_CASE(2) TT_MEGA_CASE(3) TT_MEGA_CASE(4)
+    TT_MEGA_CASE(4)
     TT_MEGA_CASE(5) TT_MEGA_CASE(6) TT_MEGA_CASE(7) TT_MEGA_CASE(8) TT_MEGA_CASE(9)
     TT_MEGA_CASE(10) TT_MEGA_CASE(11) TT_MEGA_CASE(12) TT_MEGA_CASE(13) TT_MEGA_CASE(14)
     TT_MEGA_CASE(15) TT_MEGA_CASE(16) TT_MEGA_CASE(17) TT_MEGA_CASE(18) TT_MEGA_CASE(19)
@@ -70,8 +76,23 @@ __global__ __launch_bounds__(kMegaThreads) void k_mega(GroupHdr h) {
   }
 }
 
+// The GEMMs (both precisions, every tile shape) and their split-K reductions.
+__global__ __launch_bounds__(kMegaThreads) void k_mega_gemm(GroupHdr h) {
+  const char* args;
+  Blk bk;
+  const unsigned kind = mega_item(h, &args, &bk);
+  switch (kind) {
+    TT_MEGA_CASE(0) TT_MEGA_CASE(1) TT_MEGA_CASE(2) TT_MEGA_CASE(3)
+    default:
+      break;
+  }
+}
+
 void mega_exec(const GroupLaunch& gl) {
   hipLaunchKernelGGL(k_mega, dim3(gl.total_blocks), dim3(kMegaThreads), 0, gl.stream, gl.hdr);
+}
+void mega_gemm_exec(const GroupLaunch& gl) {
+  hipLaunchKernelGGL(k_mega_gemm, dim3(gl.total_blocks), dim3(kMegaThreads), 0, gl.stream, gl.hdr);
 }
 
 }  // namespace ttamen

@@ -7,6 +7,9 @@ cpp/amen_solve.h:122-549). There is no CPU fallback: without the library or a RO
 the call raises.
 """
 import ctypes
+import sys
+
+import numpy as np
 import torch
 
 from ._tt import TT
@@ -251,6 +254,43 @@ def amen_solve_batch(systems, nswp=22, eps=1e-10, max_full=500, kickrank=4, kick
         cohort_stats()
         del ws, keep
     return result
+
+
+def round_tt(x, eps=1e-12, rmax=sys.maxsize):
+    """TT rounding on the GPU (libttamen: ttamen_tt_round), replacing the reference's native
+    `round_this` (cpp/compression.h:3-41): left-to-right QR, then right-to-left truncated SVD
+    with accuracy eps/sqrt(d-1) per bond (rank_chop, cpp/ortho.h:16-44) and rank cap rmax.
+    FP64 cores on one device; ranks up to 140. Returns a new TT."""
+    if not isinstance(x, TT):
+        raise InvalidArguments("Input must be a TT.")
+    lib = _lib.load()
+    dev = x.cores[0].device
+    if dev.type != "cuda":
+        raise RuntimeError("round_tt runs on a ROCm GPU; the input cores are on %s" % dev)
+    d = len(x.cores)
+    shapes = [tuple(c.shape[1:-1]) for c in x.cores]
+    N = [int(np.prod(sh)) for sh in shapes]
+    R = list(x.R)
+    cores = [c.to(torch.float64).contiguous() for c in x.cores]
+    Na, Ra = _lib.i64_array(N), _lib.i64_array(R)
+    with torch.cuda.device(dev):
+        ws_bytes = lib.ttamen_tt_round_workspace_size(d, Na, Ra)
+        if ws_bytes == 0:
+            _lib.check("ttamen_tt_round_workspace_size", 1)
+        ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dev)
+        total = sum(R[k] * N[k] * R[k + 1] for k in range(d))
+        out = torch.empty(total, dtype=torch.float64, device=dev)
+        r_out = (ctypes.c_int64 * (d + 1))()
+        off = (ctypes.c_int64 * d)()
+        rmax = int(min(rmax, 2 ** 62))
+        code = lib.ttamen_tt_round(d, Na, Ra, _lib.ptr_array(cores), float(eps), rmax,
+                                   ctypes.c_void_p(ws.data_ptr()), ws_bytes, ctypes.c_void_p(out.data_ptr()),
+                                   total, r_out, off, ctypes.c_void_p(_lib.stream_handle(dev)))
+        _lib.check("ttamen_tt_round", code)
+        res = [out[off[k]:off[k] + r_out[k] * N[k] * r_out[k + 1]].reshape((r_out[k],) + shapes[k] + (r_out[k + 1],))
+               .clone().to(x.cores[0].dtype) for k in range(d)]
+        del ws
+    return TT(res)
 
 
 #: counters of the most recent batched / served call (see cohort_stats)
