@@ -68,7 +68,10 @@ typedef struct ttamen_stats {
  *                    rz = (1, kickrank+kick2, ..., 1), normal random (device), or NULL
  *                    to draw them on the device from `seed`
  *  nswp, eps, rmax, max_full, kickrank, kick2, local_iterations, resets, verbose,
- *  preconditioner    as in the reference (preconditioner: 0 none; 1, 2 unsupported)
+ *  preconditioner    as in the reference: 0 none, 1 'c' (per-(l,L) n x n block inverses),
+ *                    2 'r' (per-l (n r2) x (n r2) block inverses, order <= 90), right
+ *                    preconditioning of the iterative local solve (FP64 only); the workspace
+ *                    must then also hold ttamen_amen_solve_prec_workspace_size(...) bytes
  *  precision         0 = FP64 everywhere (plain); 1 = local GMRES in FP32 on the FP32
  *                    local operator (MFMA f32), inside FP64 iterative refinement of every
  *                    local solve (FP64 residual rhs - B x each step); interfaces, QR/SVD,
@@ -179,6 +182,19 @@ int ttamen_cohort_stats(double* out8);
  * concurrently, the systems being spread over them as they are admitted; a round also ends
  * early once idle_pct % (15; 100: never) of the members' weight waits to be resumed. */
 int ttamen_cohort_set_params(int window, int max_age, int gmres_poll, int lanes, int idle_pct);
+
+/* Extra workspace bytes for a local preconditioner (1 'c', 2 'r'; 0 for none). */
+size_t ttamen_amen_solve_prec_workspace_size(int d, const int64_t* N, const int64_t* rA,
+                                             const int64_t* rb, const int64_t* rx0, int64_t rmax,
+                                             int64_t kickrank, int64_t kick2, int preconditioner);
+/*
+ * One local preconditioner application y = P x (reference AMENsolveMV::setter/apply_prec,
+ * cpp/matvecs.h:153-194): mode 1 'c', 2 'r'; PhiL r1 x s x r1, A s x n x n x S, PhiR
+ * r2 x S x r2, x and y r1 x n x r2 (device, FP64). Synchronous.
+ */
+int ttamen_local_prec(int mode, const double* PhiL, const double* A, const double* PhiR,
+                      const double* x, double* y, int64_t r1, int64_t s, int64_t n, int64_t S,
+                      int64_t r2, void* workspace, size_t workspace_bytes, void* stream);
 
 /* Bytes of device workspace ttamen_amen_solve needs for these sizes (0 on bad input). */
 size_t ttamen_amen_solve_workspace_size(int d, const int64_t* N, const int64_t* rA,

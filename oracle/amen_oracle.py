@@ -34,6 +34,31 @@ def local_product(phi_right, phi_left, core_a, core):
     return np.tensordot(t2, phi_right, axes=([0, 3], [2, 1]))    # l m L
 
 
+def prec_blocks(mode, phi_left, core_a, phi_right):
+    """Local preconditioner blocks (AMENsolveMV::setter, matvecs.h:161-174).
+
+    mode 'c' (C_PREC): J[l, L] = inv_{m,n}( sum_{s,S} PhiL[l,s,l] A[s,m,n,S] PhiR[L,S,L] ), shape r1 x r2 x n x n.
+    mode 'r' (R_PREC): J[l] = inv_{(m,L),(n,R)}( sum_{s,S} PhiL[l,s,l] A[s,m,n,S] PhiR[L,S,R] ),
+    shape r1 x n x r2 x n x r2.
+    """
+    jl = np.tensordot(np.diagonal(phi_left, 0, 0, 2), core_a, axes=([0], [0]))   # d m n S
+    if mode == 'c':
+        jr = np.diagonal(phi_right, 0, 0, 2)                                       # S D
+        j = np.tensordot(jl, jr, axes=([3], [0])).transpose(0, 3, 1, 2)            # d D m n
+        return np.linalg.inv(j)
+    jt = np.tensordot(jl, phi_right, axes=([3], [1])).transpose(0, 1, 3, 2, 4)      # d m L n R
+    sh = jt.shape
+    j = jt.reshape(sh[0], sh[1] * sh[2], sh[3] * sh[4])
+    return np.linalg.inv(j).reshape(sh)
+
+
+def prec_apply(mode, j, x):
+    """y = P x (AMENsolveMV::apply_prec, matvecs.h:177-193); x is r1 x n x r2."""
+    if mode == 'c':
+        return np.einsum('rRmn,rnR->rmR', j, x)
+    return np.einsum('rmLnR,rnR->rmL', j, x)
+
+
 def phi_bck_A(phi_now, core_left, core_a, core_right):
     """Phi_prev[l,s,r] = sum xL[l,M,L] Phi[L,S,R] A[s,M,N,S] xR[r,N,R]  (amen_solve.h:38-47)."""
     p = np.tensordot(core_left, phi_now, axes=([2], [0]))        # l M S R
@@ -189,8 +214,10 @@ def round_tt(cores, epsilon, rmax):
 
 # --------------------------------------------------------------------------- AMEn
 def amen_solve(A, b, x0=None, z0=None, nswp=22, eps=1e-10, rmax=32768, max_full=500,
-               kickrank=4, kick2=0, local_iterations=40, resets=2, rng=None, trace=None):
-    """AMEn solve following cpp/amen_solve.h:122-549 (FP64, no preconditioner).
+               kickrank=4, kick2=0, local_iterations=40, resets=2, rng=None, trace=None,
+               preconditioner=None):
+    """AMEn solve following cpp/amen_solve.h:122-549 (FP64; preconditioner None, 'c' or 'r'
+    right-preconditions the iterative local solve as in amen_solve.h:346-356).
 
     A: operator cores rA x n x n x rA'; b: rhs cores; x0: optional initial cores;
     z0: initial residual-proxy cores rz x n x rz' (normal random; the reference draws
@@ -306,7 +333,15 @@ def amen_solve(A, b, x0=None, z0=None, nswp=22, eps=1e-10, rmax=32768, max_full=
                 drhs = rhs - op(previous_solution)
                 eps_local /= np.linalg.norm(drhs)
                 ps = 0.0 * previous_solution
-                solution_now, flag, nit = gmres(op, drhs, ps, local_iterations, eps_local, resets)
+                if preconditioner is None:
+                    solution_now, flag, nit = gmres(op, drhs, ps, local_iterations, eps_local, resets)
+                else:                                             # matvecs.h:198-212, amen_solve.h:353-356
+                    J = prec_blocks(preconditioner, pl, ak, pr)
+
+                    def pop(v, J=J, shape_now=shape_now):
+                        return op(prec_apply(preconditioner, J, v.reshape(shape_now)).reshape(-1))
+                    solution_now, flag, nit = gmres(pop, drhs, ps, local_iterations, eps_local, resets)
+                    solution_now = prec_apply(preconditioner, J, solution_now.reshape(shape_now)).reshape(-1)
                 gm_its.append(nit)
                 solution_now = solution_now + previous_solution
                 res_old = np.linalg.norm(op(previous_solution) - rhs) / norm_rhs

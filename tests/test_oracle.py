@@ -169,3 +169,33 @@ def test_tt_residual_matches_dense():
     Amat = Af.reshape([v for n in N for v in (n, n)]).transpose(0, 2, 4, 1, 3, 5).reshape(36, 36)
     want = np.linalg.norm(Amat @ _full(x).reshape(-1) - _full(b).reshape(-1)) / np.linalg.norm(_full(b))
     assert np.isclose(O.tt_residual(A, x, b), want, rtol=1e-10)
+
+
+def _diag_phi(rng, r, s):
+    """Interface with Phi[l, s, r] = 0 for l != r (makes the local operator block diagonal)."""
+    phi = np.zeros((r, s, r))
+    for l in range(r):
+        phi[l, :, l] = rng.standard_normal(s) + 2.0
+    return phi
+
+
+@pytest.mark.parametrize("mode", ["c", "r"])
+def test_preconditioner_inverts_block_diagonal_operator(mode):
+    """With interfaces diagonal in (l, r) (and in (L, R) for 'c') the local operator is block
+    diagonal with exactly the blocks the preconditioner inverts, so P(B x) = x."""
+    rng = np.random.default_rng(7)
+    r1, s, n, S, r2 = 3, 2, 5, 2, 4
+    A = rng.standard_normal((s, n, n, S))
+    for si in range(s):
+        for Si in range(S):
+            A[si, :, :, Si] += 4.0 * np.eye(n)
+    PL = _diag_phi(rng, r1, s)
+    PR = _diag_phi(rng, r2, S) if mode == "c" else rng.standard_normal((r2, S, r2)) + 3.0 * np.eye(r2)[:, None, :]
+    x = rng.standard_normal((r1, n, r2))
+    Bx = O.local_product(PR, PL, A, x)
+    J = O.prec_blocks(mode, PL, A, PR)
+    y = O.prec_apply(mode, J, Bx)
+    assert np.linalg.norm(y - x) <= 1e-10 * np.linalg.norm(x)
+    # a wrong orientation of the blocks (transposed J) must fail this check
+    Jt = np.swapaxes(J, -1, -2) if mode == "c" else J.reshape(r1, n * r2, n * r2).transpose(0, 2, 1).reshape(J.shape)
+    assert np.linalg.norm(O.prec_apply(mode, Jt, Bx) - x) > 1e-3 * np.linalg.norm(x)

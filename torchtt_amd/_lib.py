@@ -57,6 +57,10 @@ _SIGNATURES = {
     "ttamen_version": (ctypes.c_char_p, []),
     "ttamen_last_error": (ctypes.c_char_p, []),
     "ttamen_cohort_staging_bytes": (c_size, []),
+    "ttamen_amen_solve_prec_workspace_size": (c_size, [ctypes.c_int, c_i64p, c_i64p, c_i64p, c_i64p, c_i64,
+                                                       c_i64, c_i64, ctypes.c_int]),
+    "ttamen_local_prec": (ctypes.c_int, [ctypes.c_int, c_dp, c_dp, c_dp, c_dp, c_dp, c_i64, c_i64, c_i64,
+                                         c_i64, c_i64, c_dp, c_size, c_dp]),
     "ttamen_tt_round_workspace_size": (c_size, [ctypes.c_int, c_i64p, c_i64p]),
     "ttamen_tt_round": (ctypes.c_int, [ctypes.c_int, c_i64p, c_i64p, ctypes.POINTER(c_dp), ctypes.c_double,
                                        c_i64, c_dp, c_size, c_dp, c_size, c_i64p, c_i64p, c_dp]),

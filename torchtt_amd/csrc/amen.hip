@@ -32,6 +32,14 @@
 namespace ttamen {
 
 void linalg_init();
+// local preconditioners (prec.hip)
+int prec_max_order();
+size_t prec_elems(int mode, int r1, int n, int r2);
+void prec_build(int mode, const double* PhiL, const double* A, const double* PhiR, double* J,
+                int r1, int s, int n, int S, int r2, hipStream_t st);
+void prec_apply(int mode, const double* J, const double* x, double* y, int r1, int n, int r2,
+                Arena& ws, hipStream_t st, const int* skip);
+void prec_init();
 
 static thread_local std::string g_last_error;
 void set_last_error(const std::string& m) { g_last_error = m; }
@@ -613,7 +621,21 @@ class Solver {
       fill(sol, Nl, 0.0, st_);
       GmresDev g = gmres_alloc(ws_, (int)O.local_iterations, Nl, sizeof(double));
       GmresOp<double> gop;
-      gop.apply = [&](const double* xin, double* yout, const int* skip) { matvec(op, xin, yout, 1, skip); };
+      // Right preconditioning (reference amen_solve.h:346-356): GMRES on B P, solution P y.
+      double* J = nullptr;
+      double* ptmp = nullptr;
+      if (O.prec) {
+        J = ws_.alloc<double>(prec_elems(O.prec, (int)r1, (int)n, (int)r2));
+        ptmp = ws_.alloc<double>(Nl);
+        prec_build(O.prec, Phi_[k], A_[k], Phi_[k + 1], J, (int)r1, (int)s, (int)n, (int)S,
+                   (int)r2, st_);
+        gop.apply = [&](const double* xin, double* yout, const int* skip) {
+          prec_apply(O.prec, J, xin, ptmp, (int)r1, (int)n, (int)r2, ws_, st_, skip);
+          matvec(op, ptmp, yout, 1, skip);
+        };
+      } else {
+        gop.apply = [&](const double* xin, double* yout, const int* skip) { matvec(op, xin, yout, 1, skip); };
+      }
       gop.axpby = [&](double a, const double* xx, double b, double* yy, int64_t nn) { axpby(a, xx, b, yy, nn, st_); };
       gop.norm = [&](const double* xx, int64_t nn, double* slot) { norm2(xx, nn, slot, rs_, st_); };
       GmresResult gr = gmres_run<double>(g, Nl, sol, drhs, sc(TMP(T_NRHS)), sc(TMP(T_DNORM)),
@@ -621,6 +643,10 @@ class Solver {
       stats_.gmres_iters += gr.iters;
       count_matvec(op, gr.iters + (gr.cycles - 1));
       if (O.verbose) printf("\t\tGMRES flag %d after %d iterations (local size %lld)\n", gr.flag, gr.iters, (long long)Nl);
+      if (O.prec) {
+        prec_apply(O.prec, J, sol, ptmp, (int)r1, (int)n, (int)r2, ws_, st_, nullptr);
+        copy(ptmp, sol, Nl, st_);
+      }
       axpby(1.0, prev, 1.0, sol, Nl, st_);
       ws_.release(mk2);
     }
@@ -990,9 +1016,14 @@ static size_t solve_workspace_bound(const Plan& P, const Options& O) {
 
 static Options make_options(int64_t nswp, double eps, int64_t rmax, int64_t max_full,
                             int64_t kickrank, int64_t kick2, int64_t local_iterations,
-                            int64_t resets, int verbose, int precision, uint64_t seed) {
+                            int64_t resets, int verbose, int precision, uint64_t seed,
+                            int preconditioner = 0) {
   TT_CHECK(precision == 0 || precision == 1, ERR_UNSUPPORTED,
            "precision must be 0 (FP64) or 1 (FP32 + FP64 refinement)");
+  TT_CHECK(preconditioner >= 0 && preconditioner <= 2, ERR_INVALID,
+           "preconditioner must be 0 (none), 1 ('c') or 2 ('r')");
+  TT_CHECK(preconditioner == 0 || precision == 0, ERR_UNSUPPORTED,
+           "local preconditioners are available with FP64 local solves");
   TT_CHECK(nswp >= 0 && eps >= 0 && kickrank >= 0 && kick2 >= 0 && resets >= 0 &&
                local_iterations >= 1,
            ERR_INVALID, "invalid solver parameters");
@@ -1001,7 +1032,7 @@ static Options make_options(int64_t nswp, double eps, int64_t rmax, int64_t max_
   Options O;
   O.nswp = nswp; O.eps = eps; O.rmax = rmax; O.max_full = max_full; O.kickrank = kickrank;
   O.kick2 = kick2; O.local_iterations = local_iterations; O.resets = resets;
-  O.verbose = verbose; O.precision = precision; O.seed = seed;
+  O.verbose = verbose; O.precision = precision; O.seed = seed; O.prec = preconditioner;
   return O;
 }
 
@@ -1285,6 +1316,41 @@ size_t ttamen_amen_solve_workspace_size(int d, const int64_t* N, const int64_t* 
   return rc == 0 ? out : 0;
 }
 
+size_t ttamen_amen_solve_prec_workspace_size(int d, const int64_t* N, const int64_t* rA,
+                                             const int64_t* rb, const int64_t* rx0, int64_t rmax,
+                                             int64_t kickrank, int64_t kick2, int preconditioner) {
+  size_t out = 0;
+  int rc = guarded([&] {
+    TT_CHECK(preconditioner >= 0 && preconditioner <= 2, ERR_INVALID, "bad preconditioner");
+    if (preconditioner == 0) return;
+    Plan P = make_plan(d, N, rA, rb, rx0, rmax, kickrank, kick2);
+    size_t mx = 0;
+    for (int k = 0; k < d; ++k) {
+      const int64_t r1 = P.Rb[k], n = P.N[k], r2 = P.Rb[k + 1];
+      if (preconditioner == 2 && n * r2 > prec_max_order()) continue;  // raises if reached
+      mx = std::max(mx, prec_elems(preconditioner, (int)r1, (int)n, (int)r2) + (size_t)(r1 * n * r2));
+    }
+    out = mx * sizeof(double) + 4096;
+  });
+  return rc == 0 ? out : 0;
+}
+
+int ttamen_local_prec(int mode, const double* PhiL, const double* A, const double* PhiR,
+                      const double* x, double* y, int64_t r1, int64_t s, int64_t n, int64_t S,
+                      int64_t r2, void* workspace, size_t workspace_bytes, void* stream) {
+  return guarded([&] {
+    TT_CHECK(PhiL && A && PhiR && x && y && workspace, ERR_INVALID, "null argument");
+    TT_CHECK(r1 > 0 && s > 0 && n > 0 && S > 0 && r2 > 0, ERR_INVALID, "sizes must be positive");
+    prec_init();
+    hipStream_t st = (hipStream_t)stream;
+    Arena ws(workspace, workspace_bytes);
+    double* J = ws.alloc<double>(prec_elems(mode, (int)r1, (int)n, (int)r2));
+    prec_build(mode, PhiL, A, PhiR, J, (int)r1, (int)s, (int)n, (int)S, (int)r2, st);
+    prec_apply(mode, J, x, y, (int)r1, (int)n, (int)r2, ws, st, nullptr);
+    TT_HIP(hipStreamSynchronize(st));
+  });
+}
+
 size_t ttamen_amen_solve_output_elems(int d, const int64_t* N, const int64_t* rx0, int64_t rmax,
                                       int64_t kickrank, int64_t kick2) {
   size_t out = 0;
@@ -1306,11 +1372,11 @@ int ttamen_amen_solve(int d, const int64_t* N, const int64_t* rA, const int64_t*
                       size_t x_out_elems, int64_t* rx_out, int64_t* offsets_out,
                       ttamen_stats* stats, void* stream) {
   return guarded([&] {
-    TT_CHECK(preconditioner == 0, ERR_UNSUPPORTED, "local preconditioners are not supported");
     Options O = make_options(nswp, eps, rmax, max_full, kickrank, kick2, local_iterations, resets,
-                             verbose, precision, seed);
+                             verbose, precision, seed, preconditioner);
     linalg_init();
     lu_init();
+    if (preconditioner) prec_init();
     solve_one(d, N, rA, rb, A_cores, b_cores, x0_cores, rx0, z0_cores, O, workspace,
               workspace_bytes, x_out, x_out_elems, rx_out, offsets_out, stats, (hipStream_t)stream);
   });

@@ -87,12 +87,12 @@ def amen_solve(A, b, nswp=22, x0=None, eps=1e-10, rmax=32768, max_full=500, kick
         prec = 2
     else:
         raise InvalidArguments("Invalid preconditioner.")
-    if prec != 0:
-        raise NotImplementedError("Local preconditioners ('c', 'r') are not available in the GPU solver yet.")
     if precision is None:
         precision = 'fp32' if use_single_precision else 'fp64'
     if precision not in _PRECISIONS:
         raise InvalidArguments("precision must be one of %s." % sorted(_PRECISIONS))
+    if prec != 0 and precision != 'fp64':
+        raise InvalidArguments("Local preconditioners are available with FP64 local solves.")
 
     lib = _lib.load()
     out_device, out_dtype = A.cores[0].device, A.cores[0].dtype
@@ -122,6 +122,12 @@ def amen_solve(A, b, nswp=22, x0=None, eps=1e-10, rmax=32768, max_full=500, kick
                                                         _PRECISIONS[precision])
         if ws_bytes == 0:
             _lib.check("ttamen_amen_solve_workspace_size", 1)
+        if prec:
+            extra = lib.ttamen_amen_solve_prec_workspace_size(d, Na, rAa, rba, rx0a, int(rmax), int(kickrank),
+                                                              int(kick2), prec)
+            if extra == 0:
+                _lib.check("ttamen_amen_solve_prec_workspace_size", 1)
+            ws_bytes += extra
         out_elems = lib.ttamen_amen_solve_output_elems(d, Na, rx0a, int(rmax), int(kickrank), int(kick2))
         ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dev)
         xout = torch.empty(max(out_elems, 1), dtype=torch.float64, device=dev)
