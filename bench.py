@@ -51,7 +51,7 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=1)
     p.add_argument("--warmup", type=int, default=1)
-    p.add_argument("--workload", default="batch", choices=["batch", "potential", "precision", "serve"])
+    p.add_argument("--workload", default="batch", choices=["batch", "potential", "precision", "serve", "solo"])
     p.add_argument("--precision", default=None, choices=["fp64", "fp32"])
     p.add_argument("--systems", type=int, default=256)
     p.add_argument("--workers", type=int, default=8, help="host threads running the systems' host code")
@@ -108,6 +108,20 @@ def cpu_baseline(items, eps, args, budget_s=25.0):
             "sample": "first %d system(s) of the workload solved one after another (%.1f s)" % (done, dt)}
 
 
+def pmc_traffic(precision):
+    """HBM bytes per local-matvec application from the committed PMC summary (FETCH_SIZE +
+    WRITE_SIZE, scaled by a calibration copy), for the FP32 matvec it was measured on."""
+    path = os.path.join(ROOT, "profiles", "matvec_fp32_largest_batch_system_pmc.md")
+    if precision != "fp32" or not os.path.exists(path):
+        return None, None
+    txt = open(path).read()
+    i = txt.find("<!-- pmc-json")
+    if i < 0:
+        return None, None
+    blob = txt[i + len("<!-- pmc-json"):txt.index("-->", i)]
+    return json.loads(blob)["traffic_bytes_per_application"], os.path.relpath(path, ROOT)
+
+
 def roofline(prof, precision):
     n64, ms64, fl64, by64, n32, ms32, fl32, by32 = list(prof)
     if precision == "fp32" and n32 > 0:
@@ -118,8 +132,10 @@ def roofline(prof, precision):
         kern = "local matvec FP64 (3 MFMA GEMM launches per application)"
     avg_s = (ms / 1e3) / max(n, 1)
     ach = (fl / max(n, 1)) / avg_s / 1e12 if n else 0.0
+    traffic, src = pmc_traffic(precision)
     return {"kernel": kern, "bound": "mfma", "achieved": ach, "peak": peak, "unit": "TFLOP/s",
-            "frac": ach / peak, "traffic": None, "applications": int(n), "avg_us": avg_s * 1e6,
+            "frac": ach / peak, "traffic": traffic, "traffic_source": src,
+            "applications": int(n), "avg_us": avg_s * 1e6,
             "flops_per_application": fl / max(n, 1), "compulsory_bytes_per_application": by / max(n, 1)}
 
 
@@ -310,6 +326,31 @@ def run_precision(args, tt, W, lib, _lib):
     return out
 
 
+def run_solo(args, tt, W, lib, _lib):
+    """The single solve the batch roofline is measured on (the largest batch system), preceded
+    by a 64 MiB device copy that calibrates the HBM counters of a PMC run of this command."""
+    dev = torch.device("cuda", 0)
+    src = torch.ones(8 << 20, dtype=torch.float64, device=dev)
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    systems = W.batch_systems(args.systems, seed=0)
+    big = int(np.argmax([system_cost(sy) for sy in systems]))
+    sy = systems[big]
+    item = (tt.TT([c.to(dev) for c in sy["A"]]), tt.TT([c.to(dev) for c in sy["b"]]), sy["rmax"])
+    torch.manual_seed(0)
+    # the same residual-proxy draws as in the batched run (drawn system by system)
+    z0 = [tt.solvers._draw_z([c.shape[1] for c in s["b"]], 4, 0, "cpu") for s in systems[:big + 1]][big]
+    t0 = time.perf_counter()
+    roof = solo_roofline(tt, lib, _lib, item, z0, args.precision, args.eps)
+    total = time.perf_counter() - t0
+    cfg = {"workload": "single solve of the largest batch system (index %d: d=%d, n=%d, rank cap %d), %s"
+                       % (big, sy["d"], sy["n"], sy["rmax"], args.precision), "calibration_copy_bytes": 64 << 20}
+    out = base_record(args, 1.0 / total, total, cfg, dtype_of(args.precision))
+    out["roofline"] = roof
+    return out
+
+
 def run_serve(args, tt, W, lib, _lib):
     dev = torch.device("cuda", 0)
     arrivals = tt.serving.poisson_arrivals(args.rate, args.duration, seed=1)
@@ -351,7 +392,7 @@ def main():
     lib = _lib.load()
     _lib.check("cohort_set_params", lib.ttamen_cohort_set_params(128, 64, 8, args.lanes, 15))
     fn = {"batch": run_batch, "potential": run_potential, "precision": run_precision,
-          "serve": run_serve}[args.workload]
+          "serve": run_serve, "solo": run_solo}[args.workload]
     print(json.dumps(fn(args, tt, W, lib, _lib)), flush=True)
 
 
