@@ -55,7 +55,6 @@ GmresResult gmres_run(GmresDev g, int64_t N, T* x, const T* rhs, const double* n
   double* P1 = g.partial;                                        // (kc + 1) rows
   double* P2 = P1 + (size_t)kGmresMaxBlocks * (kGmresMaxCols + 1);  // kc rows
   double* Pn1 = P2 + (size_t)kGmresMaxBlocks * kGmresMaxCols;
-  double* Pn2 = Pn1 + kGmresMaxBlocks;
   GmresResult res;
 
   launch<KGmresSetup>(dim3(1), dim3(1), 0, st, {g, norm_rhs, dnorm, real_tol});
@@ -85,15 +84,10 @@ GmresResult gmres_run(GmresDev g, int64_t N, T* x, const T* rhs, const double* n
       double* Hcol = g.H + (int64_t)k * (g.m + 1);
       const double* Pw = P1 + (size_t)(k + 1) * kGmresMaxBlocks;  // row kc of P1: w.w
       // pass 1: project out the basis
-      launch<KCgsDots<T>>(dim3(nb), dim3(kGB), 0, st,
-                          {Q, N, k + 1, w, P1, 1, done, nullptr, nullptr});
-      launch<KCgsUpdate<T>>(dim3(nb), dim3(kGB), 0, st,
-                            {Q, N, k + 1, w, P1, Hcol, 0, Pn1, done, nullptr, nullptr, nullptr});
-      // pass 2 (re-orthogonalisation): each kernel re-derives the decision on the device
-      launch<KCgsDots<T>>(dim3(nb), dim3(kGB), 0, st, {Q, N, k + 1, w, P2, 0, done, Pw, Pn1});
-      launch<KCgsUpdate<T>>(dim3(nb), dim3(kGB), 0, st,
-                            {Q, N, k + 1, w, P2, Hcol, 1, Pn2, done, Pw, Pn1, g.istate + GM_REORTHS});
-      launch<KGmresClose<T>>(dim3(nb), dim3(kGB), 0, st, {w, qn, N, Pw, Pn1, Pn2, g, k, done});
+      launch<KCgsDots<T>>(dim3(nb), dim3(kGB), 0, st, {Q, N, k + 1, w, P1, done});
+      launch<KCgsUpdate<T>>(dim3(nb), dim3(kGB), 0, st, {Q, N, k + 1, w, P1, Hcol, Pn1, P2, done});
+      launch<KGmresClose<T>>(dim3(nb), dim3(kGB), 0, st,
+                             {Q, N, k + 1, w, qn, Pw, Pn1, P2, Hcol, g, k, done, g.istate + GM_REORTHS});
       TT_LAUNCH_CHECK();
       hs.post(done, k, st);
       if (cohort) {

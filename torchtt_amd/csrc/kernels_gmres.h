@@ -54,136 +54,156 @@ __device__ __forceinline__ bool gm_need_reorth(const double* Pw, const double* P
   return !(sqrt(r1[0]) > 0.7071067811865476 * sqrt(r2[0]));
 }
 
-// P[i][b] = Q_i . w over block b's chunk (i < kc), P[kc][b] = w . w. Each wave streams
-// whole rows over the chunk (coalesced); the chunk of w stays in L1.
+// Per-block partial dot products of the rows Q_0..Q_{rows-1} (and optionally of w itself)
+// with the block's chunk [lo, hi) of w: P[i * kGmresMaxBlocks + blk]. Each wave streams whole
+// rows over the chunk (coalesced); the chunk of w stays in L1.
 template <typename T>
-struct KCgsDots {
-  struct Args {
-    const T* Q; int64_t N; int kc; const T* w; double* P; int with_ww; const int* skip;
-    const double* Pw; const double* Pn1;
-  };
-  static constexpr int kThreads = kGB;
-  static constexpr int kMega = std::is_same<T, double>::value ? 4 : 5;
-  static __device__ __forceinline__ void run(const Args& a, const Blk& bk);
-};
-template <typename T>
-__device__ __forceinline__ void KCgsDots<T>::run(const Args& a, const Blk& bk) {
-  const T* Q = a.Q; const int64_t N = a.N; const int kc = a.kc; const T* w = a.w;
-  double* P = a.P;
-  if (gm_skip(a.skip)) return;
-  if (a.Pw && !gm_need_reorth(a.Pw, a.Pn1, bk.gx)) return;
+__device__ __forceinline__ void gm_chunk_dots(const T* Q, int64_t N, int rows, const T* w,
+                                              bool with_ww, int64_t lo, int64_t hi, double* P,
+                                              unsigned blk) {
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int64_t chunk = (N + bk.gx - 1) / bk.gx;
-  const int64_t lo = bk.x * chunk, hi = min(N, lo + chunk);
-  const int with_ww = a.with_ww;
-  const int rows = kc + (with_ww ? 1 : 0);
-  for (int i = wv; i < rows; i += kGB / 64) {
-    const T* row = (i < kc) ? Q + (int64_t)i * N : w;
+  const int nr = rows + (with_ww ? 1 : 0);
+  for (int i = wv; i < nr; i += kGB / 64) {
+    const T* row = (i < rows) ? Q + (int64_t)i * N : w;
     double acc = 0.0;
     for (int64_t j = lo + lane; j < hi; j += 64) acc += (double)row[j] * (double)w[j];
     acc = gm_wave_sum(acc);
-    if (lane == 0) P[(int64_t)i * kGmresMaxBlocks + bk.x] = acc;
+    if (lane == 0) P[(int64_t)i * kGmresMaxBlocks + blk] = acc;
   }
 }
 
-// h = sum_b P[:, b]; w -= sum_i h_i Q_i over the block's chunk; Pn[b] = ||w chunk||^2.
-// Block 0 writes (pass 1) or adds (pass 2) h into the Hessenberg column.
+// w[lo:hi] -= sum_i h_i Q_i[lo:hi]; returns this thread's share of ||w[lo:hi]||^2.
 template <typename T>
-struct KCgsUpdate {
-  struct Args {
-    const T* Q; int64_t N; int kc; T* w; const double* P; double* Hcol; int accumulate;
-    double* Pn; const int* skip; const double* Pw; const double* Pn1; int* reorth_count;
-  };
-  static constexpr int kThreads = kGB;
-  static constexpr int kMega = std::is_same<T, double>::value ? 6 : 7;
-  static __device__ __forceinline__ void run(const Args& a, const Blk& bk);
-};
-template <typename T>
-__device__ __forceinline__ void KCgsUpdate<T>::run(const Args& a, const Blk& bk) {
-  const T* Q = a.Q; const int64_t N = a.N; const int kc = a.kc; T* w = a.w;
-  double* Hcol = a.Hcol; const int accumulate = a.accumulate; int* reorth_count = a.reorth_count;
-  if (gm_skip(a.skip)) return;
-  if (a.Pw && !gm_need_reorth(a.Pw, a.Pn1, bk.gx)) return;
-  __shared__ double hs[kGmresMaxCols + 1];
-  __shared__ double sh[kGB / 64];
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  gm_reduce_rows(a.P, kc, bk.gx, hs);
-  if (bk.x == 0) {
-    for (int i = threadIdx.x; i < kc; i += kGB) Hcol[i] = accumulate ? Hcol[i] + hs[i] : hs[i];
-    if (reorth_count && threadIdx.x == 0) *reorth_count += 1;
-  }
-  const int64_t chunk = (N + bk.gx - 1) / bk.gx;
-  const int64_t lo = bk.x * chunk, hi = min(N, lo + chunk);
+__device__ __forceinline__ double gm_chunk_update(const T* Q, int64_t N, int kc, T* w,
+                                                  const double* h, int64_t lo, int64_t hi) {
   double acc = 0.0;
   for (int64_t j = lo + threadIdx.x; j < hi; j += kGB) {
     double v = (double)w[j];
-    for (int i = 0; i < kc; ++i) v -= hs[i] * (double)Q[(int64_t)i * N + j];
+    for (int i = 0; i < kc; ++i) v -= h[i] * (double)Q[(int64_t)i * N + j];
     w[j] = (T)v;
     acc += v * v;
   }
-  acc = gm_wave_sum(acc);
-  if (lane == 0) sh[wv] = acc;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double s = 0.0;
-    for (int i = 0; i < kGB / 64; ++i) s += sh[i];
-    a.Pn[bk.x] = s;
-  }
+  return acc;
 }
 
-// q_{k+1} = w / ||w||; block 0 / thread 0 finishes the Hessenberg column with Givens
-// rotations (reference gmres.h:20-31, 106-129) and tests convergence.
+// Arnoldi step k, orthogonalisation in three kernels (classical Gram-Schmidt, selective second
+// pass with the DGKS criterion 1/sqrt(2); Daniel, Gragg, Kaufman, Stewart, Math. Comp. 30
+// (1976)), all deterministic (fixed-order partials reduced by every consumer block):
+//   KCgsDots     P1 = [Q^T w ; w.w] partials
+//   KCgsUpdate   h = sum P1, w' = w - Q h, Pn1 = ||w'||^2 partials and P2 = Q^T w' partials
+//                (the second pass's dots, from the chunk the block just updated)
+//   KGmresClose  if ||w'|| < ||w|| / sqrt(2): h2 = sum P2, w'' = w' - Q h2, H(:,k) += h2 and
+//                ||w''||^2 = ||w'||^2 - ||h2||^2; q_{k+1} = w'' / ||w''||; Givens on H(:,k)
+template <typename T>
+struct KCgsDots {
+  struct Args { const T* Q; int64_t N; int kc; const T* w; double* P; const int* skip; };
+  static constexpr int kThreads = kGB;
+  static constexpr int kMega = std::is_same<T, double>::value ? 4 : 5;
+  static __device__ __forceinline__ void run(const Args& a, const Blk& bk) {
+    if (gm_skip(a.skip)) return;
+    const int64_t chunk = (a.N + bk.gx - 1) / bk.gx;
+    const int64_t lo = bk.x * chunk, hi = min(a.N, lo + chunk);
+    gm_chunk_dots(a.Q, a.N, a.kc, a.w, true, lo, hi, a.P, bk.x);
+  }
+};
+
+template <typename T>
+struct KCgsUpdate {
+  struct Args {
+    const T* Q; int64_t N; int kc; T* w; const double* P1; double* Hcol; double* Pn1; double* P2;
+    const int* skip;
+  };
+  static constexpr int kThreads = kGB;
+  static constexpr int kMega = std::is_same<T, double>::value ? 6 : 7;
+  static __device__ __forceinline__ void run(const Args& a, const Blk& bk) {
+    if (gm_skip(a.skip)) return;
+    __shared__ double hs[kGmresMaxCols + 1];
+    __shared__ double sh[kGB / 64];
+    const int kc = a.kc;
+    gm_reduce_rows(a.P1, kc, bk.gx, hs);
+    if (bk.x == 0)
+      for (int i = threadIdx.x; i < kc; i += kGB) a.Hcol[i] = hs[i];
+    const int64_t chunk = (a.N + bk.gx - 1) / bk.gx;
+    const int64_t lo = bk.x * chunk, hi = min(a.N, lo + chunk);
+    double acc = gm_chunk_update(a.Q, a.N, kc, a.w, hs, lo, hi);
+    acc = gm_wave_sum(acc);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = acc;
+    __syncthreads();  // also orders this block's writes of w before the dots below
+    if (threadIdx.x == 0) {
+      double t = 0.0;
+      for (int i = 0; i < kGB / 64; ++i) t += sh[i];
+      a.Pn1[bk.x] = t;
+    }
+    gm_chunk_dots(a.Q, a.N, kc, (const T*)a.w, false, lo, hi, a.P2, bk.x);
+  }
+};
+
 template <typename T>
 struct KGmresClose {
   struct Args {
-    const T* w; T* qn; int64_t N; const double* Pw; const double* Pn1; const double* Pn2;
-    GmresDev g; int k; const int* done_in;
+    const T* Q; int64_t N; int kc; T* w; T* qn; const double* Pw; const double* Pn1;
+    const double* P2; double* Hcol; GmresDev g; int k; const int* done_in; int* reorth_count;
   };
   static constexpr int kThreads = kGB;
   static constexpr int kMega = std::is_same<T, double>::value ? 8 : 9;
-  static __device__ __forceinline__ void run(const Args& a, const Blk& bk);
-};
-template <typename T>
-__device__ __forceinline__ void KGmresClose<T>::run(const Args& a, const Blk& bk) {
-  const T* w = a.w; T* qn = a.qn; const int64_t N = a.N; const GmresDev g = a.g; const int k = a.k;
-  if (*a.done_in) return;
-  __shared__ double nn[2];
-  const bool reorth = gm_need_reorth(a.Pw, a.Pn1, bk.gx);
-  gm_reduce_rows(reorth ? a.Pn2 : a.Pn1, 1, bk.gx, nn);
-  const double hn = sqrt(nn[0]);
-  const double inv = (hn > 0.0) ? 1.0 / hn : 0.0;
-  for (int64_t j = bk.x * (int64_t)kGB + threadIdx.x; j < N; j += (int64_t)bk.gx * kGB)
-    qn[j] = (T)((double)w[j] * inv);
-  if (bk.x == 0 && threadIdx.x == 0) {
-    const int ld = g.m + 1;
-    double* h = g.H + (int64_t)k * ld;
-    h[k + 1] = hn;
-    for (int i = 0; i < k; ++i) {
-      double t = g.cs[i] * h[i] + g.sn[i] * h[i + 1];
-      h[i + 1] = -g.sn[i] * h[i] + g.cs[i] * h[i + 1];
-      h[i] = t;
+  static __device__ __forceinline__ void run(const Args& a, const Blk& bk) {
+    if (*a.done_in) return;
+    __shared__ double h2[kGmresMaxCols + 1];
+    __shared__ double nn[2];
+    const int kc = a.kc;
+    const bool reorth = gm_need_reorth(a.Pw, a.Pn1, bk.gx);
+    gm_reduce_rows(a.Pn1, 1, bk.gx, nn);
+    double hn2 = nn[0];
+    const int64_t chunk = (a.N + bk.gx - 1) / bk.gx;
+    const int64_t lo = bk.x * chunk, hi = min(a.N, lo + chunk);
+    if (reorth) {
+      gm_reduce_rows(a.P2, kc, bk.gx, h2);
+      double s2 = 0.0;
+      for (int i = 0; i < kc; ++i) s2 += h2[i] * h2[i];
+      hn2 -= s2;
+      (void)gm_chunk_update(a.Q, a.N, kc, a.w, h2, lo, hi);
+      if (bk.x == 0) {
+        for (int i = threadIdx.x; i < kc; i += kGB) a.Hcol[i] += h2[i];
+        if (threadIdx.x == 0 && a.reorth_count) *a.reorth_count += 1;
+      }
+      __syncthreads();
     }
-    double den = sqrt(h[k] * h[k] + h[k + 1] * h[k + 1]);
-    double c = 1.0, sn = 0.0;
-    if (den > 0.0) { c = h[k] / den; sn = h[k + 1] / den; }
-    h[k] = c * h[k] + sn * h[k + 1];
-    h[k + 1] = 0.0;
-    g.cs[k] = c;
-    g.sn[k] = sn;
-    g.beta[k + 1] = -sn * g.beta[k];
-    g.beta[k] = c * g.beta[k];
-    const double err = fabs(g.beta[k + 1]) / g.dstate[GM_BNORM];
-    g.istate[GM_KCOLS] = k + 1;
-    g.istate[GM_ITERS] += 1;
-    g.dstate[GM_ERR] = err;
-    if (err <= g.dstate[GM_THR]) {
-      g.istate[GM_FLAG] = 1;
-      g.istate[GM_DONE] = 1;
-    } else if (k + 1 >= g.m) {
-      g.istate[GM_DONE] = 1;  // cycle exhausted (flag stays 0)
+    const double hn = sqrt(hn2 > 0.0 ? hn2 : 0.0);
+    const double inv = (hn > 0.0) ? 1.0 / hn : 0.0;
+    for (int64_t j = lo + threadIdx.x; j < hi; j += kGB) a.qn[j] = (T)((double)a.w[j] * inv);
+    if (bk.x == 0 && threadIdx.x == 0) {
+      const GmresDev g = a.g;
+      const int k = a.k;
+      const int ld = g.m + 1;
+      double* h = g.H + (int64_t)k * ld;
+      h[k + 1] = hn;
+      for (int i = 0; i < k; ++i) {
+        double t = g.cs[i] * h[i] + g.sn[i] * h[i + 1];
+        h[i + 1] = -g.sn[i] * h[i] + g.cs[i] * h[i + 1];
+        h[i] = t;
+      }
+      double den = sqrt(h[k] * h[k] + h[k + 1] * h[k + 1]);
+      double c = 1.0, sn = 0.0;
+      if (den > 0.0) { c = h[k] / den; sn = h[k + 1] / den; }
+      h[k] = c * h[k] + sn * h[k + 1];
+      h[k + 1] = 0.0;
+      g.cs[k] = c;
+      g.sn[k] = sn;
+      g.beta[k + 1] = -sn * g.beta[k];
+      g.beta[k] = c * g.beta[k];
+      const double err = fabs(g.beta[k + 1]) / g.dstate[GM_BNORM];
+      g.istate[GM_KCOLS] = k + 1;
+      g.istate[GM_ITERS] += 1;
+      g.dstate[GM_ERR] = err;
+      if (err <= g.dstate[GM_THR]) {
+        g.istate[GM_FLAG] = 1;
+        g.istate[GM_DONE] = 1;
+      } else if (k + 1 >= g.m) {
+        g.istate[GM_DONE] = 1;  // cycle exhausted (flag stays 0)
+      }
     }
   }
-}
+};
 
 // Start of a cycle: r already holds rhs - Op(x0) and dstate[GM_RNORM] = ||r||.
 // q_0 = r / ||r||, beta = (||r||, 0, ...), done = (||r|| <= 0).
