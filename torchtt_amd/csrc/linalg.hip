@@ -22,7 +22,7 @@ namespace ttamen {
 
 constexpr double kUnitRoundoff = 1.1102230246251565e-16;  // 2^-53
 constexpr int kJacobiMaxSweeps = 48;
-constexpr int kQrPasses = 4;
+constexpr int kQrPasses = 3;
 
 // ---------------------------------------------------------------- Cholesky
 // G (n x n, row-major, symmetric) + shift*I = R^T R, R upper triangular written
@@ -261,7 +261,7 @@ int qr(const double* A, int64_t sr, int64_t sc, int64_t m, int64_t n, double* Q,
   const int trsm_blocks = (int)((m + 63) / 64);
 
   // Passes: two shifted Cholesky-QR passes (each shrinks the condition number by about
-  // sqrt(11 (m n + n (n+1)) u)), then plain passes; a pass whose input Gram matrix is
+  // sqrt(11 (m n + n (n+1)) u)), then one plain pass; a pass whose input Gram matrix is
   // within 1e-6 of the identity is unshifted and the last one. Later passes see the flag
   // and return at once.
   for (int pass = 0; pass < kQrPasses; ++pass) {
