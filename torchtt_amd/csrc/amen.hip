@@ -252,7 +252,7 @@ class Solver {
   int NORMB(int k) const { return 1 + (d - 1) + k; }
   int NORMX(int k) const { return 1 + 2 * (d - 1) + k; }
   int TMP(int i) const { return 1 + 3 * (d - 1) + i; }
-  enum { T_NRHS = 0, T_DNORM, T_RESOLD, T_RESNEW, T_CUR, T_MAXRES, T_DX, T_SOLN, T_ONE, T_R32, T_CAND };
+  enum { T_NRHS = 0, T_DNORM, T_RESOLD, T_RESNEW, T_CUR, T_MAXRES, T_DX, T_SOLN, T_ONE, T_R32, T_TAUN, T_CAND };
   static constexpr int kMaxRefine = 12;      // FP64 refinement steps per local solve (FP32 mode)
   static constexpr double kInnerTolFloor = 1e-5;  // smallest relative tolerance asked of FP32 GMRES
   static constexpr int kNV = 32;  // truncation candidates per batch
@@ -861,14 +861,22 @@ class Solver {
     gop.axpby = [&](double a, const float* xx, double b, float* yy, int64_t nn) { axpby_f32(a, xx, b, yy, nn, st_); };
     gop.norm = [&](const float* xx, int64_t nn, double* slot) { norm2_f32(xx, nn, slot, rs_, st_); };
     for (int it = 0; it < kMaxRefine; ++it) {
-      read_to_host(host_sync().host_dbl(), S_ + TMP(0), sizeof(double) * 4, st_);
-      const double nrhs = host_sync().host_dbl()[T_NRHS], rn = host_sync().host_dbl()[T_DNORM];
-      if (!(rn > real_tol * nrhs)) break;
-      const double tau = std::min(0.5, std::max(kInnerTolFloor, 0.5 * real_tol * nrhs / rn));
+      // The first step always runs (its tolerance is computed on the device); later steps
+      // run while the FP64 residual is above the target.
+      if (it > 0) {
+        read_to_host(host_sync().host_dbl(), S_ + TMP(0), sizeof(double) * 4, st_);
+        const double nrhs = host_sync().host_dbl()[T_NRHS], rn = host_sync().host_dbl()[T_DNORM];
+        if (!(rn > real_tol * nrhs)) break;
+      }
       cast_f64_f32(r, r32, Nl, st_);
       norm2_f32(r32, Nl, sc(TMP(T_R32)), rs_, st_);
+      // tau = min(0.5, max(floor, 0.5 real_tol ||rhs|| / ||r||)); GMRES stops at tau ||r32||
+      launch<KTauScale>(dim3(1), dim3(1), 0, st_,
+                        {S_, TMP(T_TAUN), TMP(T_NRHS), TMP(T_DNORM), TMP(T_R32), real_tol,
+                         kInnerTolFloor});
+      TT_LAUNCH_CHECK();
       zero_words(d32, Nl, st_);
-      GmresResult gr = gmres_run<float>(g, Nl, d32, r32, sc(TMP(T_R32)), sc(TMP(T_R32)), tau,
+      GmresResult gr = gmres_run<float>(g, Nl, d32, r32, sc(TMP(T_TAUN)), sc(TMP(T_R32)), 1.0,
                                         (int)O.resets, gop, rs_, host_sync(), st_);
       if (prof) {
         std::lock_guard<std::mutex> lk(g_prof.mu);
@@ -910,15 +918,15 @@ class Solver {
     double* t1 = ws_.alloc<double>(kNV * r1 * s * n);
     double* t2 = ws_.alloc<double>(kNV * r1 * n * S);
     double* pv = ws_.alloc<double>(l2 * S * kNV);
-    // Read res_new (and res_old for verbose) once.
-    read_to_host(host_sync().host_dbl(), S_ + TMP(0), sizeof(double) * 8, st_);
-    const double res_new = host_sync().host_dbl()[T_RESNEW];
     if (O.verbose) {
+      read_to_host(host_sync().host_dbl(), S_ + TMP(0), sizeof(double) * 8, st_);
       const double* h = host_sync().host_dbl();
       printf("\t\tres_old = %g, res_new = %g, dx = %g\n", h[T_RESOLD], h[T_RESNEW],
              (h[T_SOLN] > 0) ? h[T_DX] / h[T_SOLN] : 0.0);
     }
-    const double thr = std::max(res_new, real_tol * damp);
+    // res_new (for the threshold) is read together with the first batch of candidates
+    double thr = 0.0;
+    bool have_thr = false;
     int64_t rtop = kc;  // residual in e currently corresponds to rank rtop
     int64_t result = -1;
     while (rtop > 1 && result < 0) {
@@ -965,8 +973,12 @@ class Solver {
       launch<KTruncPrefix>(dim3(std::min<int64_t>(1024, (Nl + 1023) / 1024)), dim3(256), 0, st_,
                            {e, g, Nl, nv, sv, j0, rs_.partial, rs_.counter, S_ + TMP(T_CAND), S_ + TMP(T_NRHS)});
       TT_LAUNCH_CHECK();
-      read_to_host(host_sync().host_dbl(), S_ + TMP(T_CAND), sizeof(double) * nv, st_);
-      const double* res = host_sync().host_dbl();
+      read_to_host(host_sync().host_dbl(), S_ + TMP(0), sizeof(double) * (T_CAND + nv), st_);
+      if (!have_thr) {
+        thr = std::max(host_sync().host_dbl()[T_RESNEW], real_tol * damp);
+        have_thr = true;
+      }
+      const double* res = host_sync().host_dbl() + T_CAND;
       for (int c = 0; c < nv; ++c) {
         const int64_t rc = rtop - 1 - c;
         if (res[c] > thr) {

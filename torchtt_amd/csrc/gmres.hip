@@ -56,6 +56,8 @@ GmresResult gmres_run(GmresDev g, int64_t N, T* x, const T* rhs, const double* n
   double* P2 = P1 + (size_t)kGmresMaxBlocks * (kGmresMaxCols + 1);  // kc rows
   double* Pn1 = P2 + (size_t)kGmresMaxBlocks * kGmresMaxCols;
   GmresResult res;
+  int state[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  bool have_state = false;
 
   launch<KGmresSetup>(dim3(1), dim3(1), 0, st, {g, norm_rhs, dnorm, real_tol});
   TT_LAUNCH_CHECK();
@@ -104,17 +106,21 @@ GmresResult gmres_run(GmresDev g, int64_t N, T* x, const T* rhs, const double* n
     launch<KGmresY>(dim3(1), dim3(1), 0, st, {g});
     launch<KGmresXupd<T>>(dim3(nb), dim3(kGB), 0, st, {Q, N, x, g});
     TT_LAUNCH_CHECK();
-    // Did this cycle converge?
-    read_to_host(hs.host_int(), g.istate + GM_FLAG, 2 * sizeof(int), st);
-    const int flag = hs.host_int()[0];
-    res.cycle_steps.push_back(hs.host_int()[1]);  // GM_KCOLS follows GM_FLAG
+    // Did this cycle converge? (the whole state in one read: flag, columns, iterations, ...)
+    read_to_host(hs.host_int(), g.istate, sizeof(int) * 8, st);
+    for (int i = 0; i < 8; ++i) state[i] = hs.host_int()[i];
+    have_state = true;
+    res.cycle_steps.push_back(state[GM_KCOLS]);
     res.cycles = cyc + 1;
-    if (flag) break;
+    if (state[GM_FLAG]) break;
   }
-  read_to_host(hs.host_int(), g.istate, sizeof(int) * 8, st);
-  res.flag = hs.host_int()[GM_FLAG];
-  res.iters = hs.host_int()[GM_ITERS];
-  res.reorths = hs.host_int()[GM_REORTHS];
+  if (!have_state) {
+    read_to_host(hs.host_int(), g.istate, sizeof(int) * 8, st);
+    for (int i = 0; i < 8; ++i) state[i] = hs.host_int()[i];
+  }
+  res.flag = state[GM_FLAG];
+  res.iters = state[GM_ITERS];
+  res.reorths = state[GM_REORTHS];
   return res;
 }
 

@@ -273,6 +273,19 @@ struct KSdiv {
   }
 };
 
+// Inner tolerance of a mixed-precision refinement step, times ||r32||:
+//   S[dst] = min(0.5, max(floor, 0.5 * real_tol * S[nrhs] / S[rn])) * S[r32]
+struct KTauScale {
+  struct Args { double* S; int dst, nrhs, rn, r32; double real_tol, floor; };
+  static constexpr int kThreads = 64;
+  static constexpr int kMega = 38;
+  static __device__ __forceinline__ void run(const Args& p, const Blk&) {
+    if (threadIdx.x != 0) return;
+    const double t = fmin(0.5, fmax(p.floor, 0.5 * p.real_tol * p.S[p.nrhs] / p.S[p.rn]));
+    p.S[p.dst] = t * p.S[p.r32];
+  }
+};
+
 // ---------------------------------------------------------------- host transfers
 // Device -> pinned host copy done by a kernel (used inside a cohort, where the
 // copies of many systems become one launch).

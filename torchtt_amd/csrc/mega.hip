@@ -24,7 +24,7 @@ typedef KScaleSlot M19; typedef KAxpby M20; typedef KAxpbyF32 M21; typedef KAddF
 typedef KSub M23; typedef KCopy M24; typedef KFill M25; typedef KZero32 M26; typedef KEye M27;
 typedef KCopy2d M28; typedef KCast<double, float> M29; typedef KCast<float, double> M30;
 typedef KNrmsc M31; typedef KMaxdiv M32; typedef KSset M33; typedef KSdiv M34;
-typedef KStoreHost M35; typedef KScaleCols M36; typedef KRandn M37;
+typedef KStoreHost M35; typedef KScaleCols M36; typedef KRandn M37; typedef KTauScale M38;
 }  // namespace
 
 #define TT_MEGA_CASE(N)                                                             \
@@ -70,7 +70,7 @@ __global__ __launch_bounds__(kMegaThreads) void k_mega(GroupHdr h) {
     TT_MEGA_CASE(20) TT_MEGA_CASE(21) TT_MEGA_CASE(22) TT_MEGA_CASE(23) TT_MEGA_CASE(24)
     TT_MEGA_CASE(25) TT_MEGA_CASE(26) TT_MEGA_CASE(27) TT_MEGA_CASE(28) TT_MEGA_CASE(29)
     TT_MEGA_CASE(30) TT_MEGA_CASE(31) TT_MEGA_CASE(32) TT_MEGA_CASE(33) TT_MEGA_CASE(34)
-    TT_MEGA_CASE(35) TT_MEGA_CASE(36) TT_MEGA_CASE(37)
+    TT_MEGA_CASE(35) TT_MEGA_CASE(36) TT_MEGA_CASE(37) TT_MEGA_CASE(38)
     default:
       break;
   }
