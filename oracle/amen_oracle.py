@@ -222,7 +222,8 @@ def amen_solve(A, b, x0=None, z0=None, nswp=22, eps=1e-10, rmax=32768, max_full=
     A: operator cores rA x n x n x rA'; b: rhs cores; x0: optional initial cores;
     z0: initial residual-proxy cores rz x n x rz' (normal random; the reference draws
     them with torch.randn, so a test passes the same draws). `trace`, if a dict, receives
-    per-sweep ranks, max_res and GMRES iteration counts.
+    per-sweep ranks, max_res and GMRES iteration counts, and per core (sweep, core, res_old,
+    res_new, rank).
     """
     rng = np.random.default_rng(0) if rng is None else rng
     d = len(A)
@@ -365,6 +366,8 @@ def amen_solve(A, b, x0=None, z0=None, nswp=22, eps=1e-10, rmax=32768, max_full=
                 u, v = _qr(solution_now)
                 r = u.shape[1]
                 s = np.ones(r)
+            if trace is not None:
+                trace.setdefault("cores", []).append((swp, k, res_old, res_new, int(r)))
             u = u[:, :r]                                          # :413-416
             v = (np.diag(s[:r]) @ v[:r, :]).T
             if not last:                                          # :418-442

@@ -12,7 +12,7 @@ import torch
 
 import amen_oracle as O
 import laplace_exact as LE
-from conftest import GOLDEN_CASES, gpu_available, load_golden, load_ref
+from conftest import GOLDEN_CASES, GOLDEN_SWEEPS, gpu_available, load_golden, load_ref
 
 pytestmark = [pytest.mark.gpu,
               pytest.mark.skipif(not gpu_available(), reason="needs a ROCm GPU")]
@@ -61,6 +61,7 @@ def test_amen_matches_reference_fixture(name):
     _say("  rel diff vs reference %.3e, residual %.3e, stats %s" % (diff, res, tt.solvers.last_stats))
     assert diff <= 1e-8
     assert res <= 50 * c["eps"]
+    assert tt.solvers.last_stats["sweeps"] == GOLDEN_SWEEPS[name], tt.solvers.last_stats
 
 
 def test_laplace_closed_form():

@@ -9,6 +9,7 @@ import torch
 
 import amen_oracle as O
 from conftest import gpu_available
+from qr_cases import qr_case
 
 pytestmark = [pytest.mark.gpu,
               pytest.mark.skipif(not gpu_available(), reason="needs a ROCm GPU")]
@@ -170,26 +171,20 @@ def test_phi_rhs_updates(dims):
 
 # ----------------------------------------------------------------- QR / SVD
 def _qr_case(kind, m, n, seed):
-    rng = np.random.default_rng(seed)
-    A = rng.standard_normal((m, n))
-    if kind == "zero_cols":
-        A[:, 1] = 0.0
-        A[:, n - 1] = 0.0
-    elif kind == "dependent":
-        A[:, n - 1] = A[:, 0] * 2.0 - A[:, 1]
-    elif kind == "graded":
-        A = A * np.logspace(0, -14, n)[None, :]
-    elif kind == "tiny_tail":
-        # an orthonormal block plus a tiny residual block (the enrichment QR shape)
-        q, _ = np.linalg.qr(rng.standard_normal((m, n - 4)))
-        A = np.concatenate([q, 1e-11 * rng.standard_normal((m, 4))], 1)
-    return A
+    return qr_case(kind, m, n, seed)
 
 
 @pytest.mark.parametrize("kind,m,n", [("rand", 200, 30), ("rand", 5376, 84), ("rand", 30, 30), ("rand", 10, 25),
                                       ("zero_cols", 100, 12), ("dependent", 80, 10), ("graded", 300, 40),
                                       ("tiny_tail", 1000, 40), ("rand", 1, 1), ("rand", 256, 4),
-                                      ("rand", 8448, 136)])
+                                      ("rand", 8448, 136),
+                                      # single-workgroup path (up to 4 columns)
+                                      ("zero_cols", 300, 4), ("dependent", 64, 3), ("graded", 256, 4),
+                                      ("rand", 4352, 4), ("rand", 7, 2), ("zero_cols", 40, 2),
+                                      # numerically rank-deficient and near-square inputs
+                                      ("rank_deficient", 16, 12), ("rank_deficient", 120, 110),
+                                      ("rank_deficient", 3000, 60), ("rank_deficient", 64, 4),
+                                      ("square_zero_col", 148, 137), ("square_zero_col", 5, 4)])
 def test_qr(kind, m, n):
     L, lib = _lib()
     _say("qr", kind, m, n)
@@ -207,7 +202,9 @@ def test_qr(kind, m, n):
 
 @pytest.mark.parametrize("kind,m,n", [("rand", 200, 30), ("rand", 5376, 84), ("rand", 4, 256), ("rand", 30, 30),
                                       ("graded", 300, 40), ("zero_cols", 60, 9), ("rand", 1, 7), ("rand", 9, 1),
-                                      ("rand", 1000, 136)])
+                                      ("rand", 1000, 136), ("rand", 4, 256), ("rand", 256, 4), ("zero_cols", 128, 4),
+                                      ("rank_deficient", 16, 12), ("rank_deficient", 12, 16),
+                                      ("rank_deficient", 192, 14), ("square_zero_col", 148, 137)])
 def test_svd(kind, m, n):
     L, lib = _lib()
     _say("svd", kind, m, n)

@@ -8,7 +8,7 @@ import pytest
 
 import amen_oracle as O
 import laplace_exact as LE
-from conftest import GOLDEN_CASES, load_golden
+from conftest import GOLDEN_CASES, GOLDEN_SWEEPS, load_golden
 
 
 def _full(cores):
@@ -19,10 +19,12 @@ def _full(cores):
 @pytest.mark.parametrize("name", GOLDEN_CASES)
 def test_oracle_matches_reference_fixture(name):
     c = load_golden(name)
+    tr = {}
     x = O.amen_solve(c["A"], c["b"], x0=c["x0"], z0=c["z"], nswp=c["nswp"], eps=c["eps"], rmax=c["rmax"],
-                     kickrank=c["kickrank"], kick2=c["kick2"])
+                     kickrank=c["kickrank"], kick2=c["kick2"], trace=tr)
     ranks = [1] + [core.shape[2] for core in x]
     assert ranks == c["ranks"].tolist(), "trajectory (final ranks) differs from the reference"
+    assert tr["sweeps"] == GOLDEN_SWEEPS[name], tr["max_res"]
     fx, fr = _full(x), _full(c["x"])
     rel = np.linalg.norm(fx - fr) / np.linalg.norm(fr)
     assert rel <= 1e-8, rel

@@ -677,6 +677,7 @@ class Solver {
       r = truncation_rank(op, U, kc, sv, V, e, Nl, rows, r2, real_tol, damp, use_full);
       r = std::min<int64_t>(r, std::min<int64_t>(kc, O.rmax));
       r = std::max<int64_t>(r, 1);
+      if (O.verbose) sync_print_local(k, Nl, r);
       u = U; ldu = kc;
       vmat = ws_.alloc<double>(r2 * r);
       launch<KScaleCols>(dim3(grid_of(r2 * r)), dim3(256), 0, st_, {V, kc, sv, vmat, r, r2, (int)r});
@@ -690,7 +691,7 @@ class Solver {
       u = Q; ldu = kq; r = kq;
       vmat = ws_.alloc<double>(r2 * r);
       copy2d(R, 1, r2, vmat, r, r2, r, st_);  // vmat = R^T
-      if (O.verbose) sync_print_local(k, Nl);
+      if (O.verbose) sync_print_local(k, Nl, r);
     }
     // truncated solution tmp = u vmat^T ((r1 n) x r2)
     double* tmp = ws_.alloc<double>(Nl);
@@ -807,11 +808,11 @@ class Solver {
     ws_.release(mk);
   }
 
-  void sync_print_local(int k, int64_t Nl) {
+  void sync_print_local(int k, int64_t Nl, int64_t r) {
     read_to_host(host_sync().host_dbl(), S_, sizeof(double) * nscal_, st_);
     const double* h = host_sync().host_dbl();
-    printf("\tCore %d (local size %lld): res_old = %g, res_new = %g, dx = %g\n", k, (long long)Nl,
-           h[TMP(T_RESOLD)], h[TMP(T_RESNEW)], h[TMP(T_DX)] / h[TMP(T_SOLN)]);
+    printf("\tCore %d (local size %lld): res_old = %g, res_new = %g, dx = %g, rank %lld\n", k,
+           (long long)Nl, h[TMP(T_RESOLD)], h[TMP(T_RESNEW)], h[TMP(T_DX)] / h[TMP(T_SOLN)], (long long)r);
   }
 
   // Local solve in mixed precision: FP32 GMRES on the FP32 local operator (MFMA f32)

@@ -3,7 +3,7 @@
 // at::linalg_solve calls of the reference (cpp/amen_solve.h:231, 242, 254, 326,
 // 383, 407, 430, 437, 454; cpp/ortho.h:46-73).
 //
-// QR: shifted Cholesky-QR in three passes (Fukaya, Kannan, Nakatsukasa, Yamamoto,
+// QR: shifted Cholesky-QR in up to four passes (Fukaya, Kannan, Nakatsukasa, Yamamoto,
 //   Yanagisawa, "Shifted Cholesky QR for computing the QR factorization of
 //   ill-conditioned matrices", SIAM J. Sci. Comput. 42 (2020)). Each pass is a Gram
 //   GEMM on the matrix cores, a one-workgroup Cholesky in LDS and a row-parallel
@@ -22,7 +22,7 @@ namespace ttamen {
 
 constexpr double kUnitRoundoff = 1.1102230246251565e-16;  // 2^-53
 constexpr int kJacobiMaxSweeps = 48;
-constexpr int kQrPasses = 3;
+constexpr int kQrPasses = 4;
 
 // ---------------------------------------------------------------- Cholesky
 // G (n x n, row-major, symmetric) + shift*I = R^T R, R upper triangular written
@@ -178,8 +178,11 @@ __device__ void KTrsmRows::run(const Args& args, const Blk& bk) {
 }
 
 // ---------------------------------------------------------------- completion of dead columns
-// Replace the columns j with dead[j] != 0 of Q (m x n row-major, other columns
-// orthonormal) by unit vectors orthogonalised twice against all live columns.
+// Replace the columns j with dead[j] != 0 of Q (m x n row-major, other columns orthonormal)
+// by unit vectors orthogonalised twice against all live columns. The unit vector is e_p for
+// the row p with the largest 1 - sum_{c live} Q[p][c]^2 (the squared norm of e_p after the
+// projection): over all rows those sum to m - (live columns) >= 1, so the chosen e_p keeps at
+// least a 1/m share of its squared norm, whatever the shape (ties: smallest row).
 __device__ double block_sum_1024(double v, double* sh) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
@@ -192,6 +195,25 @@ __device__ double block_sum_1024(double v, double* sh) {
   return r;
 }
 
+// (max v, then smallest index) over the block; every thread gets the winning index.
+__device__ int block_argmax(double v, int idx, double* shv, int* shi) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const double ov = __shfl_xor(v, o, 64);
+    const int oi = __shfl_xor(idx, o, 64);
+    if (ov > v || (ov == v && oi < idx)) { v = ov; idx = oi; }
+  }
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  __syncthreads();
+  if (lane == 0) { shv[w] = v; shi[w] = idx; }
+  __syncthreads();
+  double bv = shv[0];
+  int bi = shi[0];
+  for (int i = 1; i < nw; ++i)
+    if (shv[i] > bv || (shv[i] == bv && shi[i] < bi)) { bv = shv[i]; bi = shi[i]; }
+  return bi;
+}
+
 struct KCompleteDead {
   struct Args { double* Q; int64_t m; int n; int* dead; const int* ndead; };
   static constexpr int kThreads = 1024;
@@ -201,40 +223,189 @@ __device__ void KCompleteDead::run(const Args& a, const Blk&) {
   double* Q = a.Q; const int64_t m = a.m; const int n = a.n; int* dead = a.dead;
   if (*a.ndead == 0) return;
   __shared__ double sh[16];
-  __shared__ int s_ok;
+  __shared__ int shi[16];
   const int tid = threadIdx.x, nt = blockDim.x;
-  int probe = 0;
   for (int j = 0; j < n; ++j) {
     if (!dead[j]) continue;
-    for (int tries = 0; tries < (int)min<int64_t>(m, 4 * n + 8); ++tries) {
-      const int64_t p = (probe++ * 7919LL + j * 104729LL) % m;
-      for (int64_t i = tid; i < m; i += nt) Q[i * n + j] = (i == p) ? 1.0 : 0.0;
-      __syncthreads();
-      for (int pass = 0; pass < 2; ++pass) {
-        for (int c = 0; c < n; ++c) {
-          if (c == j || dead[c]) continue;
-          double part = 0.0;
-          for (int64_t i = tid; i < m; i += nt) part += Q[i * n + c] * Q[i * n + j];
-          double dot = block_sum_1024(part, sh);
-          for (int64_t i = tid; i < m; i += nt) Q[i * n + j] -= dot * Q[i * n + c];
-          __syncthreads();
+    double best = -1.0;
+    int bidx = 0x7fffffff;
+    for (int64_t i = tid; i < m; i += nt) {
+      double w = 1.0;
+      for (int c = 0; c < n; ++c)
+        if (c != j && !dead[c]) w -= Q[i * n + c] * Q[i * n + c];
+      if (w > best) { best = w; bidx = (int)i; }
+    }
+    const int64_t p = block_argmax(best, bidx, sh, shi);
+    for (int64_t i = tid; i < m; i += nt) Q[i * n + j] = (i == p) ? 1.0 : 0.0;
+    __syncthreads();
+    for (int pass = 0; pass < 2; ++pass) {
+      for (int c = 0; c < n; ++c) {
+        if (c == j || dead[c]) continue;
+        double part = 0.0;
+        for (int64_t i = tid; i < m; i += nt) part += Q[i * n + c] * Q[i * n + j];
+        double dot = block_sum_1024(part, sh);
+        for (int64_t i = tid; i < m; i += nt) Q[i * n + j] -= dot * Q[i * n + c];
+        __syncthreads();
+      }
+    }
+    double part = 0.0;
+    for (int64_t i = tid; i < m; i += nt) part += Q[i * n + j] * Q[i * n + j];
+    const double nrm = sqrt(block_sum_1024(part, sh));
+    const double inv = (nrm > 0.0) ? 1.0 / nrm : 0.0;
+    for (int64_t i = tid; i < m; i += nt) Q[i * n + j] *= inv;
+    __syncthreads();
+    if (tid == 0) dead[j] = 0;
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------- QR of a few columns
+// The same shifted Cholesky-QR (passes, shifts, near-identity exit, dead columns and their
+// completion, R = Q^T A) for n <= kQrSmallCols columns in one workgroup and one launch: the
+// Gram matrix is accumulated in registers over each thread's rows and reduced in LDS, the
+// n x n Cholesky runs in one thread, each thread solves its rows against R.
+constexpr int kQrSmallCols = 4;
+
+__device__ __forceinline__ double block_sum_256(double v, double* sh) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) sh[w] = v;
+  __syncthreads();
+  double r = 0.0;
+  for (int i = 0; i < (int)(blockDim.x >> 6); ++i) r += sh[i];
+  return r;
+}
+
+struct KQrSmall {
+  struct Args {
+    const double* A; int64_t sr, sc, m; int n; double* Q; double* R; int* dead; int* ndead;
+  };
+  static constexpr int kThreads = 256;
+  static __device__ void run(const Args& a, const Blk&);
+};
+
+__device__ void KQrSmall::run(const Args& a, const Blk&) {
+  constexpr int NC = kQrSmallCols;
+  __shared__ double G[NC][NC];
+  __shared__ double Rk[NC][NC];
+  __shared__ double sh[16];
+  __shared__ int s_last;
+  const int tid = threadIdx.x, nt = blockDim.x;
+  const int n = a.n;
+  const int64_t m = a.m;
+  for (int pass = 0; pass < kQrPasses; ++pass) {
+    const double* src = (pass == 0) ? a.A : a.Q;
+    const int64_t ssr = (pass == 0) ? a.sr : n, ssc = (pass == 0) ? a.sc : 1;
+    // Gram = src^T src
+    double g[NC][NC];
+#pragma unroll
+    for (int i = 0; i < NC; ++i)
+#pragma unroll
+      for (int j = 0; j < NC; ++j) g[i][j] = 0.0;
+    for (int64_t r = tid; r < m; r += nt) {
+      double x[NC];
+#pragma unroll
+      for (int i = 0; i < NC; ++i) x[i] = (i < n) ? src[r * ssr + i * ssc] : 0.0;
+#pragma unroll
+      for (int i = 0; i < NC; ++i)
+#pragma unroll
+        for (int j = 0; j <= i; ++j) g[i][j] += x[i] * x[j];
+    }
+#pragma unroll
+    for (int i = 0; i < NC; ++i)
+#pragma unroll
+      for (int j = 0; j <= i; ++j) {
+        if (i < n) {
+          const double v = block_sum_256(g[i][j], sh);
+          if (tid == 0) { G[i][j] = v; G[j][i] = v; }
         }
       }
-      double part = 0.0;
-      for (int64_t i = tid; i < m; i += nt) part += Q[i * n + j] * Q[i * n + j];
-      double nrm = sqrt(block_sum_1024(part, sh));
-      if (tid == 0) s_ok = (nrm > 0.5);
-      __syncthreads();
-      if (s_ok) {
-        for (int64_t i = tid; i < m; i += nt) Q[i * n + j] /= nrm;
-        __syncthreads();
-        if (tid == 0) dead[j] = 0;
-        __syncthreads();
-        break;
+    __syncthreads();
+    if (tid == 0) {
+      // the Cholesky of KCholesky::run, for one thread
+      double delta = 0.0, tr = 0.0, md = 0.0;
+      for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j) delta = fmax(delta, fabs(G[i][j] - (i == j ? 1.0 : 0.0)));
+      for (int j = 0; j < n; ++j) { tr += G[j][j]; md = fmax(md, G[j][j]); }
+      const int near_id = delta < kNearIdentity;
+      const double shift_c = (pass < 2) ? 11.0 * ((double)m * n + (double)n * (n + 1)) : 0.0;
+      const double dead_rel = (pass < 2) ? 0.0 : 1e-26;
+      const double shv = (shift_c > 0.0 && !near_id) ? shift_c * kUnitRoundoff * tr : 0.0;
+      const double thr = dead_rel * (md + shv);
+      for (int j = 0; j < n; ++j) G[j][j] += shv;
+      int nd = 0;
+      for (int j = 0; j < n; ++j) {
+        const double d = G[j][j];
+        const int isdead = !(d > thr) || !(d > 0.0);
+        const double piv = isdead ? 1.0 : sqrt(d);
+        G[j][j] = piv;
+        a.dead[j] = isdead;
+        nd += isdead;
+        for (int c = j + 1; c < n; ++c) G[j][c] = isdead ? 0.0 : G[j][c] / piv;
+        if (!isdead)
+          for (int r = j + 1; r < n; ++r)
+            for (int c = r; c < n; ++c) G[r][c] -= G[j][r] * G[j][c];
       }
-      __syncthreads();
+      for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j) Rk[i][j] = (i <= j) ? G[i][j] : 0.0;
+      *a.ndead = nd;
+      s_last = near_id || pass == kQrPasses - 1;
     }
+    __syncthreads();
+    // Q rows = src rows R^{-1}
+    for (int64_t r = tid; r < m; r += nt) {
+      double x[NC];
+#pragma unroll
+      for (int i = 0; i < NC; ++i) x[i] = (i < n) ? src[r * ssr + i * ssc] : 0.0;
+#pragma unroll
+      for (int j = 0; j < NC; ++j) {
+        if (j < n) {
+          double v = x[j];
+#pragma unroll
+          for (int i = 0; i < NC; ++i)
+            if (i < j) v -= x[i] * Rk[i][j];
+          x[j] = v / Rk[j][j];
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < NC; ++j)
+        if (j < n) a.Q[r * n + j] = x[j];
+    }
+    __syncthreads();
+    if (s_last) break;
   }
+  // orthonormal completion of dead columns (as KCompleteDead, with this block)
+  if (*a.ndead > 0) KCompleteDead::run({a.Q, m, n, a.dead, a.ndead}, Blk{0, 0, 0, 1, 1, 1});
+  __syncthreads();
+  // R = Q^T A
+  double rr[NC][NC];
+#pragma unroll
+  for (int i = 0; i < NC; ++i)
+#pragma unroll
+    for (int j = 0; j < NC; ++j) rr[i][j] = 0.0;
+  for (int64_t r = tid; r < m; r += nt) {
+    double q[NC], x[NC];
+#pragma unroll
+    for (int i = 0; i < NC; ++i) {
+      q[i] = (i < n) ? a.Q[r * n + i] : 0.0;
+      x[i] = (i < n) ? a.A[r * a.sr + i * a.sc] : 0.0;
+    }
+#pragma unroll
+    for (int i = 0; i < NC; ++i)
+#pragma unroll
+      for (int j = 0; j < NC; ++j) rr[i][j] += q[i] * x[j];
+  }
+#pragma unroll
+  for (int i = 0; i < NC; ++i)
+#pragma unroll
+    for (int j = 0; j < NC; ++j) {
+      if (i < n && j < n) {
+        const double v = block_sum_256(rr[i][j], sh);
+        if (tid == 0) a.R[i * n + j] = v;
+      }
+    }
 }
 
 // ---------------------------------------------------------------- QR driver
@@ -247,6 +418,16 @@ int qr(const double* A, int64_t sr, int64_t sc, int64_t m, int64_t n, double* Q,
       copy2d(A, sr, sc, R, n, m, n, st);
     }
     return (int)m;
+  }
+  if (n <= kQrSmallCols) {
+    size_t mk = ws.mark();
+    int* dead = ws.alloc<int>(n + 1);
+    if (!ws.measuring()) {
+      launch<KQrSmall>(dim3(1), dim3(256), 0, st, {A, sr, sc, m, (int)n, Q, R, dead, dead + n});
+      TT_LAUNCH_CHECK();
+    }
+    ws.release(mk);
+    return (int)n;
   }
   TT_CHECK(n <= kMaxSmall, ERR_UNSUPPORTED,
            "qr: more than " + std::to_string(kMaxSmall) + " columns is not supported");
@@ -261,9 +442,12 @@ int qr(const double* A, int64_t sr, int64_t sc, int64_t m, int64_t n, double* Q,
   const int trsm_blocks = (int)((m + 63) / 64);
 
   // Passes: two shifted Cholesky-QR passes (each shrinks the condition number by about
-  // sqrt(11 (m n + n (n+1)) u)), then one plain pass; a pass whose input Gram matrix is
+  // sqrt(11 (m n + n (n+1)) u)), then plain passes; a pass whose input Gram matrix is
   // within 1e-6 of the identity is unshifted and the last one. Later passes see the flag
-  // and return at once.
+  // and return at once. Numerically rank-deficient inputs (singular values from 1 down
+  // to below u, as AMEn's local solutions have) can leave the third pass's input with a
+  // condition number near 1e10, which one plain pass does not orthonormalise (Q^T Q off
+  // by 1e-2, A - Q R by 1e-9 of A); the fourth pass does.
   for (int pass = 0; pass < kQrPasses; ++pass) {
     const double* src = (pass == 0) ? A : Q;
     int64_t ssr = (pass == 0) ? sr : n, ssc = (pass == 0) ? sc : 1;
