@@ -11,8 +11,9 @@ Workloads (--workload):
              time, is measured on a sample of the same systems for the comparison.
   potential  one system: d=10, n=64, Laplacian + potential (TT rank 4), random rank-4 rhs,
              rank cap 80, eps 1e-8; a step is one solve.
-  precision  one system: d=12, n=64, rank cap 128, solved in FP64 and in mixed precision; the
-             error of the mixed solution vs the FP64 one is reported.
+  precision  one system: d=12, n=64, rank cap 128, solved in FP64 and in the two mixed
+             precisions (FP32 products; BF16 products with FP32 accumulation); the error of each
+             mixed solution vs the FP64 one is reported.
   serve      Poisson arrivals at --rate requests/s of systems drawn as in `batch`, for
              --duration seconds, served by the continuous-batching loop; reports p50/p99
              latency (completion - arrival) and solves/s.
@@ -42,7 +43,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "solves/sec at rel. residual ≤1e-8, mixed vs FP64 plain; p50/p99 solve latency"
-PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3}
+PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "bf16": 2500.0}
 RES_BAR = 1e-8
 
 
@@ -52,7 +53,7 @@ def parse():
     p.add_argument("--steps", type=int, default=1)
     p.add_argument("--warmup", type=int, default=1)
     p.add_argument("--workload", default="batch", choices=["batch", "potential", "precision", "serve", "solo"])
-    p.add_argument("--precision", default=None, choices=["fp64", "fp32"])
+    p.add_argument("--precision", default=None, choices=["fp64", "fp32", "bf16"])
     p.add_argument("--systems", type=int, default=256)
     p.add_argument("--workers", type=int, default=8, help="host threads running the systems' host code")
     p.add_argument("--slots", type=int, default=256, help="serve: requests running at once")
@@ -124,7 +125,10 @@ def pmc_traffic(precision):
 
 def roofline(prof, precision):
     n64, ms64, fl64, by64, n32, ms32, fl32, by32 = list(prof)
-    if precision == "fp32" and n32 > 0:
+    if precision == "bf16" and n32 > 0:
+        n, ms, fl, by, peak = n32, ms32, fl32, by32, PEAK_TFLOPS["bf16"]
+        kern = "local matvec BF16 products, FP32 accumulation (3 MFMA GEMM launches per application), Arnoldi steps that ran"
+    elif precision == "fp32" and n32 > 0:
         n, ms, fl, by, peak = n32, ms32, fl32, by32, PEAK_TFLOPS["fp32"]
         kern = "local matvec FP32 (3 MFMA GEMM launches per application), Arnoldi steps that ran"
     else:
@@ -146,7 +150,7 @@ def base_record(args, value, total, config, dtype):
 
 
 def dtype_of(p):
-    return "f64" if p == "fp64" else "f32-inner+f64-refinement"
+    return {"fp64": "f64", "fp32": "f32-inner+f64-refinement", "bf16": "bf16-products+f32-acc+f64-refinement"}[p]
 
 
 def system_cost(sy):
@@ -286,9 +290,9 @@ def run_potential(args, tt, W, lib, _lib):
 
 def run_precision(args, tt, W, lib, _lib):
     """d=12, n=64, Laplacian + potential (TT rank 4), random rank-4 rhs, rank cap 128, eps 1e-8:
-    one FP64 solve and one mixed-precision solve (FP32 local GMRES on the FP32 local operator
-    inside FP64 refinement) from the same residual-proxy draws; error of the mixed solution vs
-    the FP64 one, residual of each."""
+    one FP64 solve and two mixed-precision solves (FP32 local GMRES on the local operator with
+    FP32 products, and with BF16 products and FP32 accumulation, inside FP64 refinement) from the
+    same residual-proxy draws; error of each mixed solution vs the FP64 one, residual of each."""
     dev = torch.device("cuda", 0)
     A, b = W.laplace_potential(12, 64, seed=0)
     Ag = tt.TT([c.to(dev) for c in A])
@@ -296,7 +300,7 @@ def run_precision(args, tt, W, lib, _lib):
     torch.manual_seed(0)
     z = tt.solvers._draw_z(bg.N, 4, 0, "cpu")
     rows = {}
-    for prec in ("fp64", "fp32"):
+    for prec in ("fp64", "fp32", "bf16"):
         for _ in range(args.warmup):
             tt.solvers.amen_solve(Ag, bg, eps=args.eps, rmax=128, precision=prec, _z0=z)
         torch.cuda.synchronize()
@@ -312,16 +316,18 @@ def run_precision(args, tt, W, lib, _lib):
         rows[prec] = {"x": x, "time_s": dt, "residual": tt.solvers.residual(Ag, x, bg), "sweeps": st["sweeps"],
                       "gmres_iters": st["gmres_iters"], "max_rank": st["max_rank"],
                       "refinements": st.get("refinements", 0), "roofline": roofline(prof, prec)}
-    x64, x32 = rows["fp64"].pop("x"), rows["fp32"].pop("x")
+    x64, x32, xbf = rows["fp64"].pop("x"), rows["fp32"].pop("x"), rows["bf16"].pop("x")
     err = float((x32 - x64).norm() / x64.norm())
+    err_bf = float((xbf - x64).norm() / x64.norm())
     ok = sum(1 for r in rows.values() if r["residual"] <= RES_BAR)
-    total = rows["fp64"]["time_s"] + rows["fp32"]["time_s"]
+    total = rows["fp64"]["time_s"] + rows["fp32"]["time_s"] + rows["bf16"]["time_s"]
     cfg = {"workload": "AMEn solve, Laplacian + potential, d=12, n=64, TT rank 4, random rank-4 rhs, rank cap "
-                       "128, eps %g: FP64 and FP32-inner + FP64-refinement" % args.eps,
+                       "128, eps %g: FP64, FP32-inner and BF16-products-inner, both + FP64-refinement" % args.eps,
            "d": 12, "n": 64, "rmax": 128, "eps": args.eps}
     out = base_record(args, 1.0 / rows["fp32"]["time_s"] if rows["fp32"]["residual"] <= RES_BAR else 0.0,
                       total, cfg, dtype_of("fp32"))
-    out.update({"fp64": rows["fp64"], "fp32": rows["fp32"], "error_fp32_vs_fp64": err, "solved_le_1e-8": ok,
+    out.update({"fp64": rows["fp64"], "fp32": rows["fp32"], "bf16": rows["bf16"], "error_fp32_vs_fp64": err,
+                "error_bf16_vs_fp64": err_bf, "solved_le_1e-8": ok,
                 "roofline": rows["fp32"]["roofline"]})
     return out
 

@@ -50,8 +50,8 @@ typedef struct ttamen_stats {
   double max_res_last;       /* max local residual of the last sweep */
   double elapsed_s;          /* wall time inside the call */
   int64_t max_rank;          /* largest TT rank reached */
-  int64_t refinements;       /* FP64 refinement steps (precision 1) */
-  int64_t matvecs_f32;       /* FP32 local-operator applications (precision 1) */
+  int64_t refinements;       /* FP64 refinement steps (precision 1 or 2) */
+  int64_t matvecs_f32;       /* FP32 / BF16 local-operator applications (precision 1 or 2) */
   double matvec_flops_f32;   /* their algorithmic flops */
 } ttamen_stats;
 
@@ -75,7 +75,9 @@ typedef struct ttamen_stats {
  *  precision         0 = FP64 everywhere (plain); 1 = local GMRES in FP32 on the FP32
  *                    local operator (MFMA f32), inside FP64 iterative refinement of every
  *                    local solve (FP64 residual rhs - B x each step); interfaces, QR/SVD,
- *                    truncation and the direct solver stay FP64
+ *                    truncation and the direct solver stay FP64; 2 = as 1 with BF16
+ *                    products (operands rounded to BF16 on entry to the matrix cores,
+ *                    FP32 accumulation, FP32 Krylov vectors)
  *  workspace         device scratch of ttamen_amen_solve_workspace_size(...) bytes
  *  x_out             device buffer receiving the solution cores back to back
  *  x_out_elems       its capacity in doubles (ttamen_amen_solve_output_elems(...))
@@ -222,6 +224,14 @@ int ttamen_local_matvec_f32(const float* PhiL, const float* A, const float* PhiR
                             float* y, int64_t l1, int64_t s, int64_t r1, int64_t n, int64_t S,
                             int64_t l2, int64_t r2, int64_t nvec, void* workspace,
                             size_t workspace_bytes, void* stream);
+/* Same with BF16 products: FP32 inputs/outputs, every operand rounded to BF16 (round to nearest
+ * even) as it enters the matrix cores, FP32 accumulation. The operator of the BF16 local solves
+ * (precision 2 of ttamen_amen_solve); no counterpart in the reference, whose only reduced
+ * precision is FP32 (torchtt/solvers.py:469-496). */
+int ttamen_local_matvec_bf16(const float* PhiL, const float* A, const float* PhiR, const float* x,
+                             float* y, int64_t l1, int64_t s, int64_t r1, int64_t n, int64_t S,
+                             int64_t l2, int64_t r2, int64_t nvec, void* workspace,
+                             size_t workspace_bytes, void* stream);
 size_t ttamen_local_matvec_workspace_size(int64_t l1, int64_t s, int64_t r1, int64_t n,
                                           int64_t S, int64_t l2, int64_t r2, int64_t nvec);
 

@@ -131,6 +131,26 @@ def test_mixed_precision_against_reference_fixture(name):
     assert diff <= 1e-8
 
 
+@pytest.mark.parametrize("name", ["potential_d4_n16", "laplace_d6_n16_eps1e-10"])
+def test_bf16_against_reference_fixture(name):
+    """BF16 products (FP32 accumulation) inside FP64 refinement reach the FP64 reference's solution:
+    the refinement residual is FP64, so only the number of refinement steps depends on BF16."""
+    import torchtt_amd as tt
+    c = load_golden(name)
+    _say("amen bf16+refinement golden", name)
+    torch.manual_seed(c["seed"])
+    x = tt.solvers.amen_solve(_tt(c["A"], "cpu"), _tt(c["b"], "cpu"), nswp=c["nswp"], eps=c["eps"],
+                              rmax=c["rmax"], kickrank=c["kickrank"], precision="bf16")
+    st = dict(tt.solvers.last_stats)
+    diff = _rel_tt(x, _tt(c["x"], "cpu"))
+    res = tt.solvers.residual(_tt(c["A"], "cpu"), x, _tt(c["b"], "cpu"))
+    _say("  ranks %s (ref %s) rel diff vs FP64 reference %.3e residual %.3e refinements %d bf16 matvecs %d"
+         % (x.R, c["ranks"].tolist(), diff, res, st["refinements"], st["matvecs_f32"]))
+    assert st["matvecs_f32"] > 0 and st["refinements"] > 0
+    assert res <= 50 * c["eps"]
+    assert diff <= 1e-7
+
+
 def test_mixed_precision_d8_n48():
     """Laplacian + potential d=8, n=48, rank cap 60: FP32 mode vs FP64 mode on the GPU."""
     import torchtt_amd as tt

@@ -121,6 +121,44 @@ def test_local_matvec_f32(shape):
         assert _rel(y[v].cpu().double().numpy(), want) < 2e-5, v
 
 
+def _bf16(a):
+    """Round float32 values to BF16 (round to nearest even), returned as float32."""
+    u = np.ascontiguousarray(a, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    u = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16) << 16
+    return u.astype(np.uint32).view(np.float32)
+
+
+@pytest.mark.parametrize("shape", SHAPES[:3])
+def test_local_matvec_bf16(shape):
+    """BF16 products, FP32 accumulation: against a stage-by-stage emulation (operands of each of
+    the three contractions rounded to BF16, results stored in FP32) and against FP64."""
+    L, lib = _lib()
+    l1, s, r1, n, S, l2, r2, nv = shape
+    _say("local_matvec bf16", shape)
+    pl, a, pr, x = _rand_local(shape, torch.float32, seed=3)
+    y = torch.empty(nv, l1, n, l2, dtype=torch.float32, device=DEV)
+    ws = _ws(lib.ttamen_local_matvec_workspace_size(l1, s, r1, n, S, l2, r2, nv) * 2)
+    t = [v.to(DEV) for v in (pl, a, pr, x)]
+    code = lib.ttamen_local_matvec_bf16(_p(t[0]), _p(t[1]), _p(t[2]), _p(t[3]), _p(y), l1, s, r1, n, S, l2, r2,
+                                        nv, _p(ws), ws.numel(), _stream())
+    L.check("local_matvec_bf16", code)
+    torch.cuda.synchronize()
+    PL, A4, PR = pl.numpy(), a.numpy(), pr.numpy()
+    for v in range(nv):
+        X = x[v].numpy()
+        t1 = np.einsum("lsr,rnR->lsnR", _bf16(PL).astype(np.float64), _bf16(X).astype(np.float64)).astype(np.float32)
+        t2 = np.einsum("smnS,lsnR->lmSR", _bf16(A4).astype(np.float64), _bf16(t1).astype(np.float64)).astype(np.float32)
+        emu = np.einsum("lmSR,LSR->lmL", _bf16(t2).astype(np.float64), _bf16(PR).astype(np.float64))
+        exact = O.local_product(PR.astype(np.float64), PL.astype(np.float64), A4.astype(np.float64),
+                                X.astype(np.float64))
+        got = y[v].cpu().double().numpy()
+        # FP32 intermediates that differ in the last bit (another accumulation order) can round
+        # to neighbouring BF16 values in the next stage: a few 1e-5 of relative difference; a
+        # wrong fragment layout or rounding would give O(1)
+        assert _rel(got, emu) < 5e-4, v
+        assert _rel(got, exact) < 3e-2, v
+
+
 # ----------------------------------------------------------------- interfaces
 @pytest.mark.parametrize("dims", [(6, 2, 5, 16, 3, 7, 4), (84, 4, 84, 64, 4, 84, 84), (1, 1, 1, 9, 2, 3, 2)])
 def test_phi_A_updates(dims):

@@ -85,6 +85,7 @@ _SIGNATURES = {
     "ttamen_amen_solve_output_elems": (c_size, [ctypes.c_int, c_i64p, c_i64p, c_i64, c_i64, c_i64]),
     "ttamen_local_matvec": (ctypes.c_int, [c_dp, c_dp, c_dp, c_dp, c_dp] + [c_i64] * 8 + [c_dp, c_size, c_dp]),
     "ttamen_local_matvec_f32": (ctypes.c_int, [c_dp, c_dp, c_dp, c_dp, c_dp] + [c_i64] * 8 + [c_dp, c_size, c_dp]),
+    "ttamen_local_matvec_bf16": (ctypes.c_int, [c_dp, c_dp, c_dp, c_dp, c_dp] + [c_i64] * 8 + [c_dp, c_size, c_dp]),
     "ttamen_local_matvec_workspace_size": (c_size, [c_i64] * 8),
     "ttamen_phi_fwd_A": (ctypes.c_int, [c_dp] * 5 + [c_i64] * 7 + [c_dp, c_size, c_dp]),
     "ttamen_phi_bck_A": (ctypes.c_int, [c_dp] * 5 + [c_i64] * 7 + [c_dp, c_size, c_dp]),

@@ -255,6 +255,10 @@ class Solver {
   enum { T_NRHS = 0, T_DNORM, T_RESOLD, T_RESNEW, T_CUR, T_MAXRES, T_DX, T_SOLN, T_ONE, T_R32, T_TAUN, T_CAND };
   static constexpr int kMaxRefine = 12;      // FP64 refinement steps per local solve (FP32 mode)
   static constexpr double kInnerTolFloor = 1e-5;  // smallest relative tolerance asked of FP32 GMRES
+  // BF16 products carry 8 significant bits: each refinement step gains about two digits at
+  // best, so the inner tolerance stops at 1e-3 and more steps are allowed.
+  static constexpr int kMaxRefineBf16 = 40;
+  static constexpr double kInnerTolFloorBf16 = 1e-3;
   static constexpr int kNV = 32;  // truncation candidates per batch
 
   void allocate(const double* const* A, const double* const* b, const double* const* x0,
@@ -263,7 +267,7 @@ class Solver {
     b_.assign(b, b + d);
     Apm_.resize(d);
     for (int k = 0; k < d; ++k) Apm_[k] = ws_.alloc<double>(P.rA[k] * P.N[k] * P.N[k] * P.rA[k + 1]);
-    if (O.precision == 1) {
+    if (O.precision >= 1) {
       Apm32_.resize(d);
       for (int k = 0; k < d; ++k) Apm32_[k] = ws_.alloc<float>(P.rA[k] * P.N[k] * P.N[k] * P.rA[k + 1]);
     }
@@ -296,7 +300,7 @@ class Solver {
     if (ws_.measuring()) return;
     zero_words(rs_.counter, 64, st_);
     for (int k = 0; k < d; ++k) permute_core_A(A_[k], Apm_[k], (int)P.rA[k], (int)P.N[k], (int)P.rA[k + 1], st_);
-    if (O.precision == 1)
+    if (O.precision >= 1)
       for (int k = 0; k < d; ++k)
         permute_core_A_f32(A_[k], Apm32_[k], (int)P.rA[k], (int)P.N[k], (int)P.rA[k + 1], st_);
     TT_LAUNCH_CHECK();
@@ -609,7 +613,7 @@ class Solver {
       ws_.release(mk2);
       matvec(op, prev, e, 1);
       diffnorm(e, rhs, Nl, sc(TMP(T_NRHS)), sc(TMP(T_RESOLD)), rs_, st_);
-    } else if (O.precision == 1) {
+    } else if (O.precision >= 1) {
       refine_solve_f32(k, op, prev, rhs, sol, e, Nl, real_tol);
     } else {
       size_t mk2 = ws_.mark();
@@ -651,7 +655,7 @@ class Solver {
       ws_.release(mk2);
     }
     // res_new: e = B sol - rhs (the refinement loop leaves it computed)
-    if (use_full || O.precision != 1) {
+    if (use_full || O.precision == 0) {
       matvec(op, sol, e, 1, nullptr, true);
       axpby(-1.0, rhs, 1.0, e, Nl, st_);
     }
@@ -815,10 +819,11 @@ class Solver {
            (long long)Nl, h[TMP(T_RESOLD)], h[TMP(T_RESNEW)], h[TMP(T_DX)] / h[TMP(T_SOLN)], (long long)r);
   }
 
-  // Local solve in mixed precision: FP32 GMRES on the FP32 local operator (MFMA f32)
-  // inside FP64 iterative refinement. Each step: r = rhs - B x in FP64, solve B d = r to a
-  // relative tolerance tau in FP32, x += d. Stops when ||r|| <= real_tol * ||rhs||, the
-  // reference's local GMRES target (amen_solve.h:340-343), or after kMaxRefine steps.
+  // Local solve in mixed precision: FP32 GMRES on the FP32 local operator (MFMA f32, or MFMA
+  // bf16 with FP32 accumulation in BF16 mode) inside FP64 iterative refinement. Each step:
+  // r = rhs - B x in FP64, solve B d = r to a relative tolerance tau in FP32, x += d. Stops
+  // when ||r|| <= real_tol * ||rhs||, the reference's local GMRES target
+  // (amen_solve.h:340-343), or after kMaxRefine (kMaxRefineBf16) steps.
   // On return sol = x and e = B x - rhs (FP64); res_old is set from the initial residual.
   void refine_solve_f32(int k, const LocalOp& op, const double* prev, const double* rhs,
                         double* sol, double* e, int64_t Nl, double real_tol) {
@@ -838,6 +843,9 @@ class Solver {
     op32.PhiL = pl; op32.Apm = Apm32_[k]; op32.PhiR = pr;
     op32.l1 = op.l1; op32.s = op.s; op32.r1 = op.r1; op32.n = op.n; op32.S = op.S;
     op32.l2 = op.l2; op32.r2 = op.r2;
+    op32.bf16 = (O.precision == 2);
+    const int max_refine = op32.bf16 ? kMaxRefineBf16 : kMaxRefine;
+    const double tol_floor = op32.bf16 ? kInnerTolFloorBf16 : kInnerTolFloor;
     float* r32 = ws_.alloc<float>(Nl);
     float* d32 = ws_.alloc<float>(Nl);
     GmresDev g = gmres_alloc(ws_, (int)O.local_iterations, Nl, sizeof(float));
@@ -861,7 +869,7 @@ class Solver {
     }
     gop.axpby = [&](double a, const float* xx, double b, float* yy, int64_t nn) { axpby_f32(a, xx, b, yy, nn, st_); };
     gop.norm = [&](const float* xx, int64_t nn, double* slot) { norm2_f32(xx, nn, slot, rs_, st_); };
-    for (int it = 0; it < kMaxRefine; ++it) {
+    for (int it = 0; it < max_refine; ++it) {
       // The first step always runs (its tolerance is computed on the device); later steps
       // run while the FP64 residual is above the target.
       if (it > 0) {
@@ -874,7 +882,7 @@ class Solver {
       // tau = min(0.5, max(floor, 0.5 real_tol ||rhs|| / ||r||)); GMRES stops at tau ||r32||
       launch<KTauScale>(dim3(1), dim3(1), 0, st_,
                         {S_, TMP(T_TAUN), TMP(T_NRHS), TMP(T_DNORM), TMP(T_R32), real_tol,
-                         kInnerTolFloor});
+                         tol_floor});
       TT_LAUNCH_CHECK();
       zero_words(d32, Nl, st_);
       GmresResult gr = gmres_run<float>(g, Nl, d32, r32, sc(TMP(T_TAUN)), sc(TMP(T_R32)), 1.0,
@@ -1021,7 +1029,7 @@ static size_t solve_workspace_bound(const Plan& P, const Options& O) {
   doubles += 32.0 * Nl + 32.0 * R * s * n * 2 + 2 * R * s * 32;      // truncation batch
   doubles += 3 * R * n * cw + 2 * cw * cw + R * cw;    // enrichment QR
   doubles += 2 * R * n * rz * 4 + 8 * rz * n * rz;     // z update
-  if (O.precision == 1) doubles += Nl * (m + 6) * 0.5 + Nl;  // FP32 Krylov basis, r, d, FP64 residual
+  if (O.precision >= 1) doubles += Nl * (m + 6) * 0.5 + Nl;  // FP32 Krylov basis, r, d, FP64 residual
   size_t step = (size_t)(doubles * 8.0 * 1.25);
   const size_t splitk = 48ull << 20;                   // split-K partials
   return persistent + step + splitk + (16ull << 20);
@@ -1031,8 +1039,9 @@ static Options make_options(int64_t nswp, double eps, int64_t rmax, int64_t max_
                             int64_t kickrank, int64_t kick2, int64_t local_iterations,
                             int64_t resets, int verbose, int precision, uint64_t seed,
                             int preconditioner = 0) {
-  TT_CHECK(precision == 0 || precision == 1, ERR_UNSUPPORTED,
-           "precision must be 0 (FP64) or 1 (FP32 + FP64 refinement)");
+  TT_CHECK(precision >= 0 && precision <= 2, ERR_UNSUPPORTED,
+           "precision must be 0 (FP64), 1 (FP32 + FP64 refinement) or 2 (BF16 products, FP32 "
+           "accumulation + FP64 refinement)");
   TT_CHECK(preconditioner >= 0 && preconditioner <= 2, ERR_INVALID,
            "preconditioner must be 0 (none), 1 ('c') or 2 ('r')");
   TT_CHECK(preconditioner == 0 || precision == 0, ERR_UNSUPPORTED,
@@ -1493,18 +1502,17 @@ int ttamen_local_matvec(const double* PhiL, const double* A, const double* PhiR,
   });
 }
 
-int ttamen_local_matvec_f32(const float* PhiL, const float* A, const float* PhiR, const float* x,
-                            float* y, int64_t l1, int64_t s, int64_t r1, int64_t n, int64_t S,
-                            int64_t l2, int64_t r2, int64_t nvec, void* workspace,
-                            size_t workspace_bytes, void* stream) {
+static int local_matvec_f32_abi(const float* PhiL, const float* A, const float* PhiR, const float* x,
+                                float* y, int64_t l1, int64_t s, int64_t r1, int64_t n, int64_t S,
+                                int64_t l2, int64_t r2, int64_t nvec, void* workspace,
+                                size_t workspace_bytes, void* stream, bool bf16) {
   return guarded([&] {
     TT_CHECK(PhiL && A && PhiR && x && y && workspace, ERR_INVALID, "null argument");
     Arena ws(workspace, workspace_bytes);
     hipStream_t st = (hipStream_t)stream;
-    // permute in double then cast is not needed: A is float here; permute via a float path
+    // A (s, n, n, S) in float: widened to double and permuted into the float operand layout
     float* Apm = ws.alloc<float>(s * n * n * S);
     {
-      // A float -> double temp -> permuted float
       double* Ad = ws.alloc<double>(s * n * n * S);
       cast_f32_f64(A, Ad, s * n * n * S, st);
       permute_core_A_f32(Ad, Apm, (int)s, (int)n, (int)S, st);
@@ -1513,8 +1521,25 @@ int ttamen_local_matvec_f32(const float* PhiL, const float* A, const float* PhiR
     op.PhiL = PhiL; op.Apm = Apm; op.PhiR = PhiR;
     op.l1 = (int)l1; op.s = (int)s; op.r1 = (int)r1; op.n = (int)n; op.S = (int)S;
     op.l2 = (int)l2; op.r2 = (int)r2;
+    op.bf16 = bf16;
     local_matvec_f32(op, x, y, (int)nvec, ws, st);
   });
+}
+
+int ttamen_local_matvec_f32(const float* PhiL, const float* A, const float* PhiR, const float* x,
+                            float* y, int64_t l1, int64_t s, int64_t r1, int64_t n, int64_t S,
+                            int64_t l2, int64_t r2, int64_t nvec, void* workspace,
+                            size_t workspace_bytes, void* stream) {
+  return local_matvec_f32_abi(PhiL, A, PhiR, x, y, l1, s, r1, n, S, l2, r2, nvec, workspace,
+                              workspace_bytes, stream, false);
+}
+
+int ttamen_local_matvec_bf16(const float* PhiL, const float* A, const float* PhiR, const float* x,
+                             float* y, int64_t l1, int64_t s, int64_t r1, int64_t n, int64_t S,
+                             int64_t l2, int64_t r2, int64_t nvec, void* workspace,
+                             size_t workspace_bytes, void* stream) {
+  return local_matvec_f32_abi(PhiL, A, PhiR, x, y, l1, s, r1, n, S, l2, r2, nvec, workspace,
+                              workspace_bytes, stream, true);
 }
 
 size_t ttamen_phi_workspace_size(int64_t l, int64_t s, int64_t r, int64_t n, int64_t S, int64_t L,

@@ -67,7 +67,7 @@ void local_matvec(const LocalOp& op, const double* x, double* y, int nvec, Arena
 void local_matvec_f32(const LocalOpF32& op, const float* x, float* y, int nvec, Arena& ws,
                       hipStream_t st, const int* skip) {
   matvec_impl<float>(op.PhiL, op.Apm, op.PhiR, op.l1, op.s, op.r1, op.n, op.S, op.l2, op.r2, x,
-                     y, nvec, ws, st, skip, gemm_f32);
+                     y, nvec, ws, st, skip, op.bf16 ? gemm_bf16 : gemm_f32);
 }
 
 size_t local_matvec_ws(const LocalOp& op, int nvec) {

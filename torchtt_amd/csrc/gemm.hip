@@ -24,16 +24,16 @@
 namespace ttamen {
 
 template <typename T, int BM, int BN>
-static void launch_tile(GemmParams<T>& p, int splits, hipStream_t st) {
+static void launch_tile(GemmParams<T>& p, int splits, hipStream_t st, bool bf16) {
   int tiles = ((p.M + BM - 1) / BM) * ((p.N + BN - 1) / BN);
   dim3 grid(tiles, p.B, splits);
-  p.tile = (BM == 32 ? 0 : 2) + (BN == 32 ? 0 : 1);
+  p.tile = (BM == 32 ? 0 : 2) + (BN == 32 ? 0 : 1) + (bf16 ? 4 : 0);
   launch<KGemm<T>>(grid, dim3(256), 0, st, p);
   TT_LAUNCH_CHECK();
 }
 
 template <typename T>
-static void gemm_impl(const GemmDesc& g, Arena& ws, hipStream_t st) {
+static void gemm_impl(const GemmDesc& g, Arena& ws, hipStream_t st, bool bf16 = false) {
   TT_CHECK(g.M >= 0 && g.N >= 0 && g.K >= 0 && g.B1 >= 1 && g.B2 >= 1, ERR_INVALID,
            "gemm: bad sizes");
   if (g.M == 0 || g.N == 0) return;
@@ -70,10 +70,10 @@ static void gemm_impl(const GemmDesc& g, Arena& ws, hipStream_t st) {
       // C = beta * C: run the kernel with an empty K range.
       splits = 1;
     }
-    if (BM == 32 && BN == 32) launch_tile<T, 32, 32>(p, splits, st);
-    else if (BM == 32) launch_tile<T, 32, 64>(p, splits, st);
-    else if (BN == 32) launch_tile<T, 64, 32>(p, splits, st);
-    else launch_tile<T, 64, 64>(p, splits, st);
+    if (BM == 32 && BN == 32) launch_tile<T, 32, 32>(p, splits, st, bf16);
+    else if (BM == 32) launch_tile<T, 32, 64>(p, splits, st, bf16);
+    else if (BN == 32) launch_tile<T, 64, 32>(p, splits, st, bf16);
+    else launch_tile<T, 64, 64>(p, splits, st, bf16);
     if (splits > 1) {
       int64_t total = (int64_t)g.M * g.N * p.B;
       int blocks = (int)std::min<int64_t>((total + 255) / 256, 4096);
@@ -86,5 +86,7 @@ static void gemm_impl(const GemmDesc& g, Arena& ws, hipStream_t st) {
 
 void gemm_f64(const GemmDesc& g, Arena& ws, hipStream_t st) { gemm_impl<double>(g, ws, st); }
 void gemm_f32(const GemmDesc& g, Arena& ws, hipStream_t st) { gemm_impl<float>(g, ws, st); }
+// FP32 operands and result, BF16 products (operands rounded to BF16), FP32 accumulation.
+void gemm_bf16(const GemmDesc& g, Arena& ws, hipStream_t st) { gemm_impl<float>(g, ws, st, true); }
 
 }  // namespace ttamen

@@ -39,7 +39,7 @@ struct GemmParams {
   int scale_mode;
   const int* skip;
   T* partial;  // split-K scratch: [split][batch][M][N]
-  int tile;    // 0: 32x32, 1: 32x64, 2: 64x32, 3: 64x64 (BM x BN)
+  int tile;    // 0: 32x32, 1: 32x64, 2: 64x32, 3: 64x64 (BM x BN); +4: BF16 MFMA (FP32 operands)
 };
 
 template <typename T>
@@ -67,6 +67,28 @@ __device__ __forceinline__ char* gemm_lds() {
   return smem;
 }
 
+// FP32 operands, BF16 arithmetic: each element is rounded to BF16 (round to nearest even)
+// when it is staged into LDS, the products run on v_mfma_f32_16x16x16bf16_1k and accumulate
+// in FP32. Fragment layout (16x16xK16 block): lane l holds A[i = l%16][k = 4*(l/16) + j] and
+// B[k = 4*(l/16) + j][n = l%16], j = 0..3; result register v holds D[4*(l/16) + v][l%16].
+// The LDS tiles are stored k-contiguous (A as [m][k], B as [n][k]) so that a lane's four
+// k values are one 8-byte read.
+typedef short s4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ unsigned short bf16_rne(float f) {
+  unsigned u = __float_as_uint(f);
+  if ((u & 0x7f800000u) == 0x7f800000u) return (unsigned short)((u >> 16) | ((u & 0xffffu) ? 0x40u : 0u));
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (unsigned short)(u >> 16);
+}
+
+template <int BM, int BN>
+struct GemmTileBf16 {
+  static constexpr int BK = 16, PK = 4;  // row of BK + PK bf16 = 40 bytes (8-byte aligned reads)
+  static constexpr size_t kLds = sizeof(unsigned short) * 2 * (BM + BN) * (BK + PK);
+  static __device__ __forceinline__ void body(const GemmParams<float>& p, const Blk& bk, char* smem);
+};
+
 template <typename T>
 struct KGemm {
   typedef GemmParams<T> Args;
@@ -75,9 +97,21 @@ struct KGemm {
   static constexpr int kMegaGroup = 1;
   static __device__ __forceinline__ void run(const Args& pp, const Blk& bk) {
     static_assert(GemmTile<T, 64, 64, 16>::kLds <= 2 * 16 * (65 + 65) * sizeof(double), "gemm LDS");
+    static_assert(GemmTileBf16<64, 64>::kLds <= 2 * 16 * (65 + 65) * sizeof(double), "gemm LDS");
     char* smem = gemm_lds();
     const GemmParams<T> p = pp;
     if (p.skip && *p.skip) return;
+    if constexpr (std::is_same<T, float>::value) {
+      if (p.tile >= 4) {
+        switch (p.tile) {
+          case 4: GemmTileBf16<32, 32>::body(p, bk, smem); break;
+          case 5: GemmTileBf16<32, 64>::body(p, bk, smem); break;
+          case 6: GemmTileBf16<64, 32>::body(p, bk, smem); break;
+          default: GemmTileBf16<64, 64>::body(p, bk, smem); break;
+        }
+        return;
+      }
+    }
     switch (p.tile) {
       case 0: GemmTile<T, 32, 32, 16>::body(p, bk, smem); break;
       case 1: GemmTile<T, 32, 64, 16>::body(p, bk, smem); break;
@@ -219,6 +253,143 @@ __device__ __forceinline__ void GemmTile<T, BM, BN, BK>::body(const GemmParams<T
           double val = al * (double)acc[mi][ni][v];
           if (p.beta != 0.0) val += p.beta * (double)(*cp);
           *cp = (T)val;
+        }
+      }
+}
+
+template <int BM, int BN>
+__device__ __forceinline__ void GemmTileBf16<BM, BN>::body(const GemmParams<float>& p, const Blk& bk,
+                                                          char* smem) {
+  constexpr int MI = BM / 32, NI = BN / 32;
+  constexpr int AE = BM * BK / 256, BE = BK * BN / 256;
+  typedef unsigned short bf;
+  bf (*As)[BM][BK + PK] = reinterpret_cast<bf (*)[BM][BK + PK]>(smem);
+  bf (*Bs)[BN][BK + PK] = reinterpret_cast<bf (*)[BN][BK + PK]>(smem + sizeof(bf) * 2 * BM * (BK + PK));
+
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+  const int tiles_n = (p.N + BN - 1) / BN;
+  const int tm = bk.x / tiles_n, tn = bk.x - tm * tiles_n;
+  const int bidx = bk.y;
+  const int b1 = bidx / p.B2, b2 = bidx - b1 * p.B2;
+  const int64_t k0 = (int64_t)bk.z * p.kchunk;
+  const int64_t k1 = min((int64_t)p.K, k0 + p.kchunk);
+  const float* A = p.a + b1 * p.ab1 + b2 * p.ab2;
+  const float* B = p.b + b1 * p.bb1 + b2 * p.bb2;
+  const int m0 = tm * BM, n0 = tn * BN;
+  const bool a_kfast = (p.ac == 1);
+  const bool b_nfast = (p.bc == 1);
+
+  float ra[AE], rb[BE];
+  auto loadA = [&](int64_t kt) {
+#pragma unroll
+    for (int e = 0; e < AE; ++e) {
+      int idx = tid + 256 * e, i, kk;
+      if (a_kfast) { i = idx / BK; kk = idx - i * BK; }
+      else { kk = idx / BM; i = idx - kk * BM; }
+      int64_t gi = m0 + i, gk = kt + kk;
+      ra[e] = (gi < p.M && gk < k1) ? A[gi * p.ar + gk * p.ac] : 0.0f;
+    }
+  };
+  auto storeA = [&](int buf) {
+#pragma unroll
+    for (int e = 0; e < AE; ++e) {
+      int idx = tid + 256 * e, i, kk;
+      if (a_kfast) { i = idx / BK; kk = idx - i * BK; }
+      else { kk = idx / BM; i = idx - kk * BM; }
+      As[buf][i][kk] = bf16_rne(ra[e]);
+    }
+  };
+  auto loadB = [&](int64_t kt) {
+#pragma unroll
+    for (int e = 0; e < BE; ++e) {
+      int idx = tid + 256 * e, j, kk;
+      if (b_nfast) { kk = idx / BN; j = idx - kk * BN; }
+      else { j = idx / BK; kk = idx - j * BK; }
+      int64_t gj = n0 + j, gk = kt + kk;
+      rb[e] = (gj < p.N && gk < k1) ? B[gk * p.br + gj * p.bc] : 0.0f;
+    }
+  };
+  auto storeB = [&](int buf) {
+#pragma unroll
+    for (int e = 0; e < BE; ++e) {
+      int idx = tid + 256 * e, j, kk;
+      if (b_nfast) { kk = idx / BN; j = idx - kk * BN; }
+      else { j = idx / BK; kk = idx - j * BK; }
+      Bs[buf][j][kk] = bf16_rne(rb[e]);
+    }
+  };
+
+  f4 acc[MI][NI];
+#pragma unroll
+  for (int mi = 0; mi < MI; ++mi)
+#pragma unroll
+    for (int ni = 0; ni < NI; ++ni) acc[mi][ni] = f4{0, 0, 0, 0};
+
+  if (k0 < k1) {
+    loadA(k0);
+    loadB(k0);
+    storeA(0);
+    storeB(0);
+  }
+  __syncthreads();
+  int cur = 0;
+  const int lr = lane & 15, lk = lane >> 4;
+  for (int64_t kt = k0; kt < k1; kt += BK) {
+    const bool more = kt + BK < k1;
+    if (more) {
+      loadA(kt + BK);
+      loadB(kt + BK);
+    }
+    s4 af[MI], bfr[NI];
+#pragma unroll
+    for (int mi = 0; mi < MI; ++mi)
+      af[mi] = *reinterpret_cast<const s4*>(&As[cur][wm * (BM / 2) + mi * 16 + lr][4 * lk]);
+#pragma unroll
+    for (int ni = 0; ni < NI; ++ni)
+      bfr[ni] = *reinterpret_cast<const s4*>(&Bs[cur][wn * (BN / 2) + ni * 16 + lr][4 * lk]);
+#pragma unroll
+    for (int mi = 0; mi < MI; ++mi)
+#pragma unroll
+      for (int ni = 0; ni < NI; ++ni)
+        acc[mi][ni] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(af[mi], bfr[ni], acc[mi][ni], 0, 0, 0);
+    if (more) {
+      storeA(cur ^ 1);
+      storeB(cur ^ 1);
+    }
+    __syncthreads();
+    cur ^= 1;
+  }
+
+  if (p.partial) {
+    float* P = p.partial + ((int64_t)bk.z * p.B + bidx) * (int64_t)p.M * p.N;
+#pragma unroll
+    for (int mi = 0; mi < MI; ++mi)
+#pragma unroll
+      for (int ni = 0; ni < NI; ++ni)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          int i = m0 + wm * (BM / 2) + mi * 16 + 4 * lk + v;
+          int j = n0 + wn * (BN / 2) + ni * 16 + lr;
+          if (i < p.M && j < p.N) P[(int64_t)i * p.N + j] = acc[mi][ni][v];
+        }
+    return;
+  }
+  const double al = eff_alpha(p);
+  float* C = p.c + b1 * p.cb1 + b2 * p.cb2;
+#pragma unroll
+  for (int mi = 0; mi < MI; ++mi)
+#pragma unroll
+    for (int ni = 0; ni < NI; ++ni)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        int i = m0 + wm * (BM / 2) + mi * 16 + 4 * lk + v;
+        int j = n0 + wn * (BN / 2) + ni * 16 + lr;
+        if (i < p.M && j < p.N) {
+          float* cp = C + (int64_t)i * p.cr + (int64_t)j * p.cc;
+          double val = al * (double)acc[mi][ni][v];
+          if (p.beta != 0.0) val += p.beta * (double)(*cp);
+          *cp = (float)val;
         }
       }
 }

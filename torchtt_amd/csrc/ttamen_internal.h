@@ -112,6 +112,7 @@ struct GemmDesc {
 void gemm_f64(const GemmDesc& g, Arena& ws, hipStream_t st);
 // FP32 inputs and outputs, FP32 accumulation on MFMA.
 void gemm_f32(const GemmDesc& g, Arena& ws, hipStream_t st);
+void gemm_bf16(const GemmDesc& g, Arena& ws, hipStream_t st);
 
 // Row-major helpers for building descriptors.
 inline MatRef rm(const void* p, int64_t ld) { MatRef m; m.p = p; m.r = ld; m.c = 1; return m; }
@@ -224,6 +225,7 @@ struct LocalOpF32 {
   const float* Apm = nullptr;
   const float* PhiR = nullptr;
   int l1 = 1, s = 1, r1 = 1, n = 1, S = 1, l2 = 1, r2 = 1;
+  bool bf16 = false;  // BF16 products (operands rounded to BF16), FP32 accumulation
 };
 void local_matvec_f32(const LocalOpF32& op, const float* x, float* y, int nvec, Arena& ws,
                       hipStream_t st, const int* skip = nullptr);
