@@ -49,19 +49,29 @@ __device__ void KCholesky::run(const Args& a, const Blk&) {
   int* dead = a.dead; int* ndead = a.ndead; const int* skip_in = a.skip_in;
   int* final_out = a.final_out;
   extern __shared__ double W[];
-  __shared__ double s_piv, s_maxd, s_shift, s_red[16];
-  __shared__ int s_dead, s_nd, s_final;
+  __shared__ double s_maxd, s_shift, s_red[16];
+  __shared__ double pivs[kMaxSmall];
+  __shared__ int s_nd, s_final;
   const int tid = threadIdx.x, nt = blockDim.x;
   if (skip_in && *skip_in) {
     if (tid == 0) *final_out = 1;
     return;
   }
   double dl = 0.0;
-  for (int t = tid; t < n * n; t += nt) {
-    double g = G[t];
-    W[t] = g;
-    int a = t / n, c = t - a * n;
-    dl = fmax(dl, fabs(g - (a == c ? 1.0 : 0.0)));
+  const int nn = n * n;
+  for (int t0 = tid; t0 < nn; t0 += 8 * nt) {  // 8 independent loads in flight per thread
+    double v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v[u] = (t0 + u * nt < nn) ? G[t0 + u * nt] : 0.0;
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int t = t0 + u * nt;
+      if (t < nn) {
+        W[t] = v[u];
+        const int a = t / n, c = t - a * n;
+        dl = fmax(dl, fabs(v[u] - (a == c ? 1.0 : 0.0)));
+      }
+    }
   }
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) dl = fmax(dl, __shfl_xor(dl, o, 64));
@@ -86,19 +96,18 @@ __device__ void KCholesky::run(const Args& a, const Blk&) {
   for (int j = tid; j < n; j += nt) W[j * n + j] += s_shift;
   __syncthreads();
   const double thr = dead_rel * s_maxd;
+  // Two barriers per column: every thread derives the pivot from the diagonal itself (the
+  // pivots are kept apart, so the diagonal is never rewritten), then row j is scaled, then the
+  // trailing submatrix is updated.
   for (int j = 0; j < n; ++j) {
+    const double d = W[j * n + j];
+    const int isdead = !(d > thr) || !(d > 0.0);
+    const double piv = isdead ? 1.0 : sqrt(d);
     if (tid == 0) {
-      double d = W[j * n + j];
-      int isdead = !(d > thr) || !(d > 0.0);
-      s_dead = isdead;
-      s_piv = isdead ? 1.0 : sqrt(d);
-      W[j * n + j] = s_piv;
+      pivs[j] = piv;
       dead[j] = isdead;
       if (isdead) s_nd += 1;
     }
-    __syncthreads();
-    const double piv = s_piv;
-    const int isdead = s_dead;
     for (int c = j + 1 + tid; c < n; c += nt) W[j * n + c] = isdead ? 0.0 : W[j * n + c] / piv;
     __syncthreads();
     if (!isdead) {
@@ -112,7 +121,7 @@ __device__ void KCholesky::run(const Args& a, const Blk&) {
   }
   for (int t = tid; t < n * n; t += nt) {
     int a = t / n, c = t % n;
-    Rout[t] = (a <= c) ? W[t] : 0.0;
+    Rout[t] = (a < c) ? W[t] : (a == c ? pivs[a] : 0.0);
   }
   if (tid == 0) {
     *ndead = s_nd;
@@ -122,17 +131,25 @@ __device__ void KCholesky::run(const Args& a, const Blk&) {
 
 // ---------------------------------------------------------------- triangular solve
 // X R = A for X (m x n row-major), R upper triangular (n x n row-major). One thread per
-// row; R is staged in LDS in packed upper-triangular form and read by broadcast.
+// row; R is staged in LDS in packed upper-triangular form and read by broadcast, and each
+// thread keeps the part of its row solved so far in LDS (column-major over the block's 64
+// rows, so the reads are conflict-free) instead of re-reading it from X.
 // In-place (X == A with unit column stride and row stride n) is allowed.
 __device__ __forceinline__ int packed_idx(int i, int j, int n) {
   return i * n - (i * (i - 1)) / 2 + (j - i);
+}
+
+constexpr int kTrsmRowsPerBlock = 64;
+
+inline size_t trsm_lds_bytes(int64_t n) {
+  return ((size_t)n * (n + 1) / 2 + (size_t)kTrsmRowsPerBlock * n) * sizeof(double);
 }
 
 struct KTrsmRows {
   struct Args {
     const double* A; int64_t sr, sc, m; int n; const double* R; double* X; const int* skip;
   };
-  static constexpr int kThreads = 64;
+  static constexpr int kThreads = kTrsmRowsPerBlock;
   static __device__ void run(const Args& a, const Blk& bk);
 };
 __device__ void KTrsmRows::run(const Args& args, const Blk& bk) {
@@ -140,12 +157,29 @@ __device__ void KTrsmRows::run(const Args& args, const Blk& bk) {
   const int n = args.n; const double* R = args.R; double* X = args.X; const int* skip = args.skip;
   if (skip && *skip) return;
   extern __shared__ double P[];
-  for (int t = threadIdx.x; t < n * n; t += blockDim.x) {
-    int i = t / n, j = t % n;
-    if (i <= j) P[packed_idx(i, j, n)] = R[t];
+  double* Xs = P + (size_t)n * (n + 1) / 2;  // Xs[i * 64 + t]: solved x[i] of thread t's row
+  // Stage R with 8 independent loads in flight per thread (one wave: a plain strided loop
+  // would wait one memory round trip per element it stages).
+  const int nn = n * n;
+  for (int t0 = threadIdx.x; t0 < nn; t0 += 8 * blockDim.x) {
+    double v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int t = t0 + u * (int)blockDim.x;
+      v[u] = (t < nn) ? R[t] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int t = t0 + u * (int)blockDim.x;
+      if (t < nn) {
+        const int i = t / n, j = t - i * n;
+        if (i <= j) P[packed_idx(i, j, n)] = v[u];
+      }
+    }
   }
   __syncthreads();
-  const int64_t row = (int64_t)bk.x * blockDim.x + threadIdx.x;
+  const int t = threadIdx.x;
+  const int64_t row = (int64_t)bk.x * kTrsmRowsPerBlock + t;
   if (row >= m) return;
   const double* a = A + row * sr;
   double* x = X + row * n;
@@ -155,7 +189,7 @@ __device__ void KTrsmRows::run(const Args& args, const Blk& bk) {
 #pragma unroll
     for (int c = 0; c < 16; ++c) acc[c] = (c < jb) ? a[(int64_t)(J + c) * sc] : 0.0;
     for (int i = 0; i < J; ++i) {
-      const double xi = x[i];
+      const double xi = Xs[i * kTrsmRowsPerBlock + t];
       const int base = packed_idx(i, J, n);
 #pragma unroll
       for (int c = 0; c < 16; ++c)
@@ -173,7 +207,10 @@ __device__ void KTrsmRows::run(const Args& args, const Blk& bk) {
     }
 #pragma unroll
     for (int c = 0; c < 16; ++c)
-      if (c < jb) x[J + c] = acc[c];
+      if (c < jb) {
+        x[J + c] = acc[c];
+        Xs[(J + c) * kTrsmRowsPerBlock + t] = acc[c];
+      }
   }
 }
 
@@ -266,16 +303,25 @@ __device__ void KCompleteDead::run(const Args& a, const Blk&) {
 // n x n Cholesky runs in one thread, each thread solves its rows against R.
 constexpr int kQrSmallCols = 4;
 
-__device__ __forceinline__ double block_sum_256(double v, double* sh) {
+// out[k] = sum over the block of v[k], k < K, with one pair of barriers for all K sums (wave
+// sums by shuffles, then one row of partials per wave in LDS); sh holds (blockDim / 64) * K.
+template <int K>
+__device__ __forceinline__ void block_sum_vec(const double (&v)[K], double* sh, double* out) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
 #pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  for (int k = 0; k < K; ++k) {
+    double x = v[k];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+    if (lane == 0) sh[w * K + k] = x;
+  }
   __syncthreads();
-  if (lane == 0) sh[w] = v;
+  for (int k = threadIdx.x; k < K; k += blockDim.x) {
+    double r = 0.0;
+    for (int i = 0; i < nw; ++i) r += sh[i * K + k];
+    out[k] = r;
+  }
   __syncthreads();
-  double r = 0.0;
-  for (int i = 0; i < (int)(blockDim.x >> 6); ++i) r += sh[i];
-  return r;
 }
 
 struct KQrSmall {
@@ -288,9 +334,11 @@ struct KQrSmall {
 
 __device__ void KQrSmall::run(const Args& a, const Blk&) {
   constexpr int NC = kQrSmallCols;
+  constexpr int NG = NC * (NC + 1) / 2;
   __shared__ double G[NC][NC];
   __shared__ double Rk[NC][NC];
-  __shared__ double sh[16];
+  __shared__ double sh[4 * NC * NC];
+  __shared__ double red[NC * NC];
   __shared__ int s_last;
   const int tid = threadIdx.x, nt = blockDim.x;
   const int n = a.n;
@@ -298,12 +346,10 @@ __device__ void KQrSmall::run(const Args& a, const Blk&) {
   for (int pass = 0; pass < kQrPasses; ++pass) {
     const double* src = (pass == 0) ? a.A : a.Q;
     const int64_t ssr = (pass == 0) ? a.sr : n, ssc = (pass == 0) ? a.sc : 1;
-    // Gram = src^T src
-    double g[NC][NC];
+    // Gram = src^T src (lower triangle, packed)
+    double g[NG];
 #pragma unroll
-    for (int i = 0; i < NC; ++i)
-#pragma unroll
-      for (int j = 0; j < NC; ++j) g[i][j] = 0.0;
+    for (int q = 0; q < NG; ++q) g[q] = 0.0;
     for (int64_t r = tid; r < m; r += nt) {
       double x[NC];
 #pragma unroll
@@ -311,19 +357,15 @@ __device__ void KQrSmall::run(const Args& a, const Blk&) {
 #pragma unroll
       for (int i = 0; i < NC; ++i)
 #pragma unroll
-        for (int j = 0; j <= i; ++j) g[i][j] += x[i] * x[j];
+        for (int j = 0; j <= i; ++j) g[i * (i + 1) / 2 + j] += x[i] * x[j];
     }
-#pragma unroll
-    for (int i = 0; i < NC; ++i)
-#pragma unroll
-      for (int j = 0; j <= i; ++j) {
-        if (i < n) {
-          const double v = block_sum_256(g[i][j], sh);
-          if (tid == 0) { G[i][j] = v; G[j][i] = v; }
-        }
-      }
-    __syncthreads();
+    block_sum_vec<NG>(g, sh, red);
     if (tid == 0) {
+      for (int i = 0; i < NC; ++i)
+        for (int j = 0; j <= i; ++j) {
+          G[i][j] = red[i * (i + 1) / 2 + j];
+          G[j][i] = G[i][j];
+        }
       // the Cholesky of KCholesky::run, for one thread
       double delta = 0.0, tr = 0.0, md = 0.0;
       for (int i = 0; i < n; ++i)
@@ -380,11 +422,9 @@ __device__ void KQrSmall::run(const Args& a, const Blk&) {
   if (*a.ndead > 0) KCompleteDead::run({a.Q, m, n, a.dead, a.ndead}, Blk{0, 0, 0, 1, 1, 1});
   __syncthreads();
   // R = Q^T A
-  double rr[NC][NC];
+  double rr[NC * NC];
 #pragma unroll
-  for (int i = 0; i < NC; ++i)
-#pragma unroll
-    for (int j = 0; j < NC; ++j) rr[i][j] = 0.0;
+  for (int q = 0; q < NC * NC; ++q) rr[q] = 0.0;
   for (int64_t r = tid; r < m; r += nt) {
     double q[NC], x[NC];
 #pragma unroll
@@ -395,17 +435,13 @@ __device__ void KQrSmall::run(const Args& a, const Blk&) {
 #pragma unroll
     for (int i = 0; i < NC; ++i)
 #pragma unroll
-      for (int j = 0; j < NC; ++j) rr[i][j] += q[i] * x[j];
+      for (int j = 0; j < NC; ++j) rr[i * NC + j] += q[i] * x[j];
   }
-#pragma unroll
-  for (int i = 0; i < NC; ++i)
-#pragma unroll
-    for (int j = 0; j < NC; ++j) {
-      if (i < n && j < n) {
-        const double v = block_sum_256(rr[i][j], sh);
-        if (tid == 0) a.R[i * n + j] = v;
-      }
-    }
+  block_sum_vec<NC * NC>(rr, sh, red);
+  for (int t = tid; t < n * n; t += nt) {
+    const int i = t / n, j = t - i * n;
+    a.R[t] = red[i * NC + j];
+  }
 }
 
 // ---------------------------------------------------------------- QR driver
@@ -438,8 +474,8 @@ int qr(const double* A, int64_t sr, int64_t sc, int64_t m, int64_t n, double* Q,
   int* ndead = dead + n;
   int* flags = ws.alloc<int>(kQrPasses + 1);  // flags[p]: a pass before p was the last one
   const size_t chol_lds = (size_t)n * n * sizeof(double);
-  const size_t trsm_lds = (size_t)n * (n + 1) / 2 * sizeof(double);
-  const int trsm_blocks = (int)((m + 63) / 64);
+  const size_t trsm_lds = trsm_lds_bytes(n);
+  const int trsm_blocks = (int)((m + kTrsmRowsPerBlock - 1) / kTrsmRowsPerBlock);
 
   // Passes: two shifted Cholesky-QR passes (each shrinks the condition number by about
   // sqrt(11 (m n + n (n+1)) u)), then plain passes; a pass whose input Gram matrix is
@@ -466,7 +502,7 @@ int qr(const double* A, int64_t sr, int64_t sc, int64_t m, int64_t n, double* Q,
                         {G, (int)n, shift_c, dead_rel, (int)(pass == kQrPasses - 1), Rk, dead, ndead,
                          skip, flags + pass + 1});
       TT_LAUNCH_CHECK();
-      launch<KTrsmRows>(dim3(trsm_blocks), dim3(64), trsm_lds, st,
+      launch<KTrsmRows>(dim3(trsm_blocks), dim3(kTrsmRowsPerBlock), trsm_lds, st,
                         {src, ssr, ssc, m, (int)n, Rk, Q, skip});
       TT_LAUNCH_CHECK();
     }
