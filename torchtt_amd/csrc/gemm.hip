@@ -46,25 +46,43 @@ static void gemm_impl(const GemmDesc& g, Arena& ws, hipStream_t st, bool bf16 = 
   p.skip = g.skip; p.partial = nullptr;
   TT_CHECK(p.B <= 65535, ERR_INVALID, "gemm: batch too large");
 
-  const int BM = (g.M <= 32) ? 32 : 64;
-  const int BN = (g.N <= 32) ? 32 : 64;
-  int tiles = ((g.M + BM - 1) / BM) * ((g.N + BN - 1) / BN) * p.B;
+  // 64x64 tiles unless the grid of 64x64 tiles would be under 4 workgroups per CU (1024):
+  // the AMEn shapes (batched local-operator stages, Gram matrices, thin products) then run
+  // with one wave per SIMD and expose the MFMA and load latencies, and 32x32 tiles (4x the
+  // workgroups) measured 1.1-2x faster on them. Inside a cohort the GEMMs of many systems
+  // share each launch and already fill the device, so the larger tiles stay.
+  const int64_t tiles64 = (int64_t)((g.M + 63) / 64) * ((g.N + 63) / 64) * p.B;
+  const bool small_grid = tiles64 < 1024 && current_recorder() == nullptr;
+  const int BM = (g.M <= 32 || small_grid) ? 32 : 64;
+  const int BN = (g.N <= 32 || small_grid) ? 32 : 64;
   // Split K when the output grid is too small to fill the 256 CUs and K is long.
-  int splits = 1;
-  if (g.K == 0) {
-    p.kchunk = 1;
-  } else {
+  auto split_plan = [&](int bm, int bn, int64_t* kchunk_out) {
+    const int tiles = ((g.M + bm - 1) / bm) * ((g.N + bn - 1) / bn) * p.B;
+    int splits = 1;
+    if (g.K == 0) {
+      *kchunk_out = 1;
+      return 1;
+    }
     if (tiles < 256 && g.K >= 512) {
       splits = std::min<int64_t>((512 + tiles - 1) / tiles, g.K / 256);
       splits = std::max(1, std::min(splits, 64));
     }
     int64_t kchunk = (g.K + splits - 1) / splits;
     kchunk = (kchunk + 15) / 16 * 16;
-    splits = (int)((g.K + kchunk - 1) / kchunk);
-    p.kchunk = kchunk;
-  }
+    *kchunk_out = kchunk;
+    return (int)((g.K + kchunk - 1) / kchunk);
+  };
+  int splits = split_plan(BM, BN, &p.kchunk);
   size_t mk = ws.mark();
-  if (splits > 1) p.partial = ws.alloc<T>((size_t)splits * p.B * g.M * g.N);
+  size_t partial_elems = splits > 1 ? (size_t)splits * p.B * g.M * g.N : 0;
+  if (ws.measuring()) {
+    // A workspace is sized without a cohort but may be used inside one: reserve the split-K
+    // partials of both tile rules.
+    int64_t kc;
+    const int s64 = split_plan(g.M <= 32 ? 32 : 64, g.N <= 32 ? 32 : 64, &kc);
+    if (s64 > 1) partial_elems = std::max(partial_elems, (size_t)s64 * p.B * g.M * g.N);
+  }
+  if (partial_elems) p.partial = ws.alloc<T>(partial_elems);
   if (!ws.measuring()) {
     if (g.K == 0) {
       // C = beta * C: run the kernel with an empty K range.
