@@ -165,6 +165,37 @@ int ttamen_serve(ttamen_system* systems, const double* arrival_s, int count, int
  * rounded cores are written to `out` at offsets_out[k] with ranks r_out (d+1). N[k] is the
  * mode size of core k (m*n for an operator core). Ranks up to 140. Synchronous at the end.
  */
+/*
+ * Elementwise division x = b ./ a of two TT tensors by AMEn. Replaces the reference's Python
+ * solver torchtt._division.amen_divide (torchtt/_division.py:76-398; no native binding exists
+ * in the reference), reached from TT.__truediv__ / __rtruediv__ (torchtt/_tt_base.py:898-970)
+ * and torchtt.elementwise_divide (torchtt/_extras.py:588-612).
+ *  a_cores: ra[k] x N[k] x ra[k+1] (TT tensor a), b_cores: rb[k] x N[k] x rb[k+1]; FP64 device
+ *  x0_cores/rx0: initial guess (the reference's default is all-ones cores of ranks 1,2,..,2,1,
+ *               _division.py:86-88, which the caller passes); z0_cores as in ttamen_amen_solve
+ *  rmax: d+1 rank caps, one per interface (_division.py:94-95)
+ *  local_iterations, resets: each GMRES restart runs up to local_iterations + 1 steps, as the
+ *               reference's call does (_division.py:253)
+ *  preconditioner: 0 none, 1 'c'
+ * The operator diag(a) is formed in the workspace as TT-matrix cores a[s,m,S] delta_mn; the
+ * sweep is ttamen_amen_solve's with the reference Python solver's truncation rule (rank >= 2
+ * when every candidate passes, _division.py:283-296). Status codes as ttamen_amen_solve.
+ */
+int ttamen_amen_divide(int d, const int64_t* N, const int64_t* ra, const int64_t* rb,
+                       const double* const* a_cores, const double* const* b_cores,
+                       const double* const* x0_cores, const int64_t* rx0,
+                       const double* const* z0_cores, int64_t nswp, double eps, const int64_t* rmax,
+                       int64_t max_full, int64_t kickrank, int64_t kick2, int64_t local_iterations,
+                       int64_t resets, int verbose, int preconditioner, void* workspace,
+                       size_t workspace_bytes, double* x_out, size_t x_out_elems, int64_t* rx_out,
+                       int64_t* offsets_out, ttamen_stats* stats, void* stream);
+size_t ttamen_amen_divide_workspace_size(int d, const int64_t* N, const int64_t* ra,
+                                         const int64_t* rb, const int64_t* rx0, const int64_t* rmax,
+                                         int64_t kickrank, int64_t kick2, int64_t local_iterations,
+                                         int64_t max_full, int preconditioner);
+size_t ttamen_amen_divide_output_elems(int d, const int64_t* N, const int64_t* rx0,
+                                       const int64_t* rmax, int64_t kickrank, int64_t kick2);
+
 size_t ttamen_tt_round_workspace_size(int d, const int64_t* N, const int64_t* R);
 int ttamen_tt_round(int d, const int64_t* N, const int64_t* R, const double* const* cores,
                     double eps, int64_t rmax, void* workspace, size_t workspace_bytes, double* out,

@@ -215,7 +215,7 @@ def round_tt(cores, epsilon, rmax):
 # --------------------------------------------------------------------------- AMEn
 def amen_solve(A, b, x0=None, z0=None, nswp=22, eps=1e-10, rmax=32768, max_full=500,
                kickrank=4, kick2=0, local_iterations=40, resets=2, rng=None, trace=None,
-               preconditioner=None):
+               preconditioner=None, python_rules=False, rank_caps=None):
     """AMEn solve following cpp/amen_solve.h:122-549 (FP64; preconditioner None, 'c' or 'r'
     right-preconditions the iterative local solve as in amen_solve.h:346-356).
 
@@ -223,7 +223,8 @@ def amen_solve(A, b, x0=None, z0=None, nswp=22, eps=1e-10, rmax=32768, max_full=
     z0: initial residual-proxy cores rz x n x rz' (normal random; the reference draws
     them with torch.randn, so a test passes the same draws). `trace`, if a dict, receives
     per-sweep ranks, max_res and GMRES iteration counts, and per core (sweep, core, res_old,
-    res_new, rank).
+    res_new, rank). python_rules: the truncation loop and zero-norm guard of the reference's
+    Python solvers (see amen_divide); rank_caps: d+1 per-interface rank caps instead of rmax.
     """
     rng = np.random.default_rng(0) if rng is None else rng
     d = len(A)
@@ -250,6 +251,9 @@ def amen_solve(A, b, x0=None, z0=None, nswp=22, eps=1e-10, rmax=32768, max_full=
     normx = np.ones(max(d - 1, 0))
     nrmsc = 1.0
     damp = 2.0
+    # a zero interface norm divides by 0 in the C++ solver (amen_solve.h:275, 280) and by 1 in the
+    # Python solvers (_division.py:166-173)
+    zero_norm = 1.0 if python_rules else 0.0
     last = False
     if trace is not None:
         trace.setdefault("ranks", [])
@@ -291,11 +295,11 @@ def amen_solve(A, b, x0=None, z0=None, nswp=22, eps=1e-10, rmax=32768, max_full=
             Phis[k] = phi_bck_A(Phis[k + 1], x[k], A[k], x[k])   # :270-285
             Phis_b[k] = phi_bck_rhs(Phis_b[k + 1], b[k], x[k])
             nrm = np.linalg.norm(Phis[k])
-            nrm = nrm if nrm > 0 else 0.0
+            nrm = nrm if nrm > 0 else zero_norm
             normA[k - 1] = nrm
             Phis[k] = Phis[k] / nrm
             nrm = np.linalg.norm(Phis_b[k])
-            nrm = nrm if nrm > 0 else 0.0
+            nrm = nrm if nrm > 0 else zero_norm
             normb[k - 1] = nrm
             Phis_b[k] = Phis_b[k] / nrm
             nrmsc = nrmsc * normb[k - 1] / (normA[k - 1] * normx[k - 1])
@@ -353,15 +357,26 @@ def amen_solve(A, b, x0=None, z0=None, nswp=22, eps=1e-10, rmax=32768, max_full=
             if k < d - 1:                                         # :382-404
                 u, s, vt = np.linalg.svd(solution_now, full_matrices=False)
                 v = vt
-                r = u.shape[1]
-                while r > 0:
-                    sol = (u[:, :r] * s[:r]) @ v[:r, :]
-                    res = np.linalg.norm(apply(sol.reshape(-1)) - rhs) / norm_rhs
-                    if res > max(res_new, real_tol * damp):
-                        break
-                    r -= 1
-                r += 1
-                r = r if (r < u.shape[1] and r < rmax) else min(u.shape[1], rmax)
+                cap = rmax if rank_caps is None else rank_caps[k + 1]
+                if python_rules:                                  # _division.py:283-296
+                    r = 0
+                    for r in range(u.shape[1] - 1, 0, -1):
+                        sol = (u[:, :r] * s[:r]) @ v[:r, :]
+                        res = np.linalg.norm(apply(sol.reshape(-1)) - rhs) / norm_rhs
+                        if res > max(real_tol * damp, res_new):
+                            break
+                    r += 1
+                    r = min(r, s.shape[0], cap)
+                else:
+                    r = u.shape[1]
+                    while r > 0:
+                        sol = (u[:, :r] * s[:r]) @ v[:r, :]
+                        res = np.linalg.norm(apply(sol.reshape(-1)) - rhs) / norm_rhs
+                        if res > max(res_new, real_tol * damp):
+                            break
+                        r -= 1
+                    r += 1
+                    r = r if (r < u.shape[1] and r < cap) else min(u.shape[1], cap)
             else:                                                 # :406-410
                 u, v = _qr(solution_now)
                 r = u.shape[1]
@@ -437,6 +452,37 @@ def amen_solve(A, b, x0=None, z0=None, nswp=22, eps=1e-10, rmax=32768, max_full=
         trace["sweeps"] = swp + 1
     norm_x = np.exp(np.sum(np.log(normx)) / d) if d > 1 else 1.0   # :534-541
     return [c * norm_x for c in x]
+
+
+def diag_operator(a):
+    """TT-matrix cores of diag(a) for a TT tensor a: D[s,m,n,S] = a[s,m,S] if m == n else 0 (the
+    operator of the reference's LinearOp, _division.py:30-75, 'lsr,smS,LSR,rmR->lmL')."""
+    out = []
+    for c in a:
+        r0, n, r1 = c.shape
+        D = np.zeros((r0, n, n, r1))
+        idx = np.arange(n)
+        D[:, idx, idx, :] = c
+        out.append(D)
+    return out
+
+
+def amen_divide(a, b, x0=None, z0=None, nswp=22, eps=1e-10, rmax=100, max_full=500, kickrank=4,
+                kick2=0, local_iterations=40, resets=2, rng=None, trace=None, preconditioner=None):
+    """x = b ./ a by AMEn, following torchtt/_division.py:76-398 (the reference's Python solver):
+    the sweep of amen_solve on the operator diag(a), started from all-ones cores of ranks
+    1, 2, ..., 2, 1 (:86-88) with per-interface rank caps (:94-95), GMRES restarts of
+    local_iterations + 1 steps (:253), the Python truncation loop (:283-296) and zero-norm
+    guard 1 (:166-173)."""
+    d = len(a)
+    N = [c.shape[1] for c in a]
+    caps = [1] + (d - 1) * [int(rmax)] + [1] if np.isscalar(rmax) else [int(v) for v in rmax]
+    if x0 is None:
+        rx = [1] + (d - 1) * [2] + [1]
+        x0 = [np.ones((rx[k], N[k], rx[k + 1])) for k in range(d)]
+    return amen_solve(diag_operator(a), b, x0=x0, z0=z0, nswp=nswp, eps=eps, rmax=max(caps), max_full=max_full,
+                      kickrank=kickrank, kick2=kick2, local_iterations=local_iterations + 1, resets=resets,
+                      rng=rng, trace=trace, preconditioner=preconditioner, python_rules=True, rank_caps=caps)
 
 
 # --------------------------------------------------------------------------- TT helpers

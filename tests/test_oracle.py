@@ -201,3 +201,53 @@ def test_preconditioner_inverts_block_diagonal_operator(mode):
     # a wrong orientation of the blocks (transposed J) must fail this check
     Jt = np.swapaxes(J, -1, -2) if mode == "c" else J.reshape(r1, n * r2, n * r2).transpose(0, 2, 1).reshape(J.shape)
     assert np.linalg.norm(O.prec_apply(mode, Jt, Bx) - x) > 1e-3 * np.linalg.norm(x)
+
+
+# ----------------------------------------------------------------- elementwise division
+def _division_case():
+    """The operands of the reference's division test (tests/test_algebra_2.py:85-102):
+    x = x0 + x1 + x2 + x3 + x1 x2 + (1 - x3) x2 + 1 on a 7x8x9x10 grid of [0,1]^4, y = 1."""
+    N = [7, 8, 9, 10]
+    g = [np.linspace(0, 1, n) for n in N]
+    ones = [np.ones(n) for n in N]
+
+    def rank1(vs):
+        return [v.reshape(1, -1, 1) for v in vs]
+    terms = []
+    for i in range(4):
+        v = list(ones)
+        v[i] = g[i]
+        terms.append(rank1(v))
+    v = list(ones); v[1] = g[1]; v[2] = g[2]; terms.append(rank1(v))
+    v = list(ones); v[3] = 1 - g[3]; v[2] = g[2]; terms.append(rank1(v))
+    terms.append(rank1(ones))
+    x = terms[0]
+    for t in terms[1:]:
+        x = O.tt_add(x, t)
+    return O.round_tt(x, 1e-15, 1000), rank1(ones)
+
+
+def test_oracle_divide_against_closed_form():
+    """amen_divide's restatement: y / x equals the dense quotient to the reference test's bar."""
+    x, y = _division_case()
+    rng = np.random.default_rng(0)
+    rz = [1, 4, 4, 4, 1]
+    z0 = [rng.standard_normal((rz[k], c.shape[1], rz[k + 1])) for k, c in enumerate(x)]
+    q = O.amen_divide(x, y, z0=z0, nswp=50, eps=1e-12, rmax=500)
+    want = O.tt_full(y) / O.tt_full(x)
+    assert np.linalg.norm(O.tt_full(q) - want) / np.linalg.norm(want) < 1e-11
+
+
+def test_oracle_divide_truncation_floor():
+    """The Python rule keeps rank 2 where the C++ rule drops to 1: dividing a rank-1 tensor by a
+    rank-1 tensor gives a rank-1 quotient, which the Python-rule sweep still stores at rank 2."""
+    rng = np.random.default_rng(1)
+    N = [5, 6, 7]
+    a = [1.5 + rng.random((1, n, 1)) for n in N]
+    b = [rng.standard_normal((1, n, 1)) for n in N]
+    rz = [1, 4, 4, 1]
+    z0 = [rng.standard_normal((rz[k], N[k], rz[k + 1])) for k in range(3)]
+    q = O.amen_divide(a, b, z0=z0, nswp=10, eps=1e-12, rmax=50)
+    want = O.tt_full(b) / O.tt_full(a)
+    assert np.linalg.norm(O.tt_full(q) - want) / np.linalg.norm(want) < 1e-11
+    assert [c.shape[2] for c in q][:-1] == [2, 2]

@@ -5,7 +5,7 @@ and the same TT return type, with the solve running in hand-written HIP kernels 
 gfx950 (libttamen.so, C ABI in include/ttamen.h).
 """
 from ._tt import TT
-from ._extras import eye, ones, zeros, random, randn, kron, dot
+from ._extras import eye, ones, zeros, random, randn, kron, dot, meshgrid, elementwise_divide
 from . import errors
 from . import solvers
 from . import workloads

@@ -79,3 +79,30 @@ def dot(a, b):
     for ca, cb in zip(a.cores, b.cores):
         phi = torch.einsum("ab,aic,bid->cd", phi, ca, cb)
     return phi.squeeze()
+
+
+def meshgrid(vectors):
+    """Rank-1 TTs X_i with vectors[i] on mode i and ones elsewhere (reference
+    torchtt/_extras.py:455-474)."""
+    dtype = vectors[0].dtype
+    out = []
+    for i in range(len(vectors)):
+        cores = [torch.ones((1, v.shape[0], 1), dtype=dtype, device=v.device) for v in vectors]
+        cores[i] = vectors[i].reshape(1, -1, 1)
+        out.append(TT(cores))
+    return out
+
+
+def elementwise_divide(x, y, eps=1e-12, starting_tensor=None, nswp=50, kick=4, local_iterations=40, resets=2,
+                       preconditioner=None, verbose=False):
+    """x / y elementwise by AMEn on the GPU with adjustable arguments (reference
+    torchtt/_extras.py:588-612): amen_divide(y, x, ...) with rank cap 1000."""
+    from .solvers import amen_divide
+    if not isinstance(x, TT):
+        o = TT([torch.ones((1, n, 1), dtype=y.cores[0].dtype, device=y.cores[0].device) for n in y.N])
+        o.cores[0] *= x
+        x = o
+    cores = amen_divide(y, x, nswp, starting_tensor, eps, rmax=1000, kickrank=kick,
+                        local_iterations=local_iterations, resets=resets, verbose=verbose,
+                        preconditioner=preconditioner)
+    return TT(cores)

@@ -214,9 +214,27 @@ class TT:
     __rmul__ = __mul__
 
     def __truediv__(self, other):
+        """x / y: by a scalar, or elementwise by a TT through AMEn on the GPU with the reference's
+        fixed arguments (torchtt/_tt_base.py:898-934: 50 sweeps, eps 1e-12, rank cap 500)."""
         if isinstance(other, (int, float)) or (torch.is_tensor(other) and other.numel() == 1):
             return self * (1.0 / other)
-        raise NotImplementedError("Elementwise division of TTs is not part of this package.")
+        if isinstance(other, TT):
+            if self._is_ttm != other.is_ttm:
+                raise IncompatibleTypes('Operands should be either TT or TTM.')
+            if list(self.N) != list(other.N) or (self._is_ttm and list(self.M) != list(other.M)):
+                raise ShapeMismatch('Both operands should have the same shape.')
+            from .solvers import amen_divide
+            return TT(amen_divide(other, self, 50, None, 1e-12, 500, verbose=False))
+        raise InvalidArguments('Operand not permitted. A TT-object can be divided only with scalars.')
+
+    def __rtruediv__(self, other):
+        """scalar / x through AMEn on the GPU (torchtt/_tt_base.py:936-970)."""
+        if isinstance(other, (int, float)) or (torch.is_tensor(other) and other.numel() == 1):
+            o = TT([torch.ones((1, n, 1), dtype=self.cores[0].dtype, device=self.cores[0].device) for n in self.N])
+            o.cores[0] *= other
+            from .solvers import amen_divide
+            return TT(amen_divide(self, o, 50, None, 1e-12, 500, verbose=False))
+        raise InvalidArguments('The first operand must be int, float or 1d torch.tensor.')
 
     def __matmul__(self, other):
         if not (self._is_ttm and isinstance(other, TT)):

@@ -194,7 +194,8 @@ static int64_t sat_mul(int64_t a, int64_t b) {
 }
 
 static Plan make_plan(int d, const int64_t* N, const int64_t* rA, const int64_t* rb,
-                      const int64_t* rx0, int64_t rmax, int64_t kickrank, int64_t kick2) {
+                      const int64_t* rx0, int64_t rmax, int64_t kickrank, int64_t kick2,
+                      const int64_t* rmax_list = nullptr) {
   TT_CHECK(d >= 1, ERR_INVALID, "d must be >= 1");
   Plan p;
   p.d = d;
@@ -207,11 +208,13 @@ static Plan make_plan(int d, const int64_t* N, const int64_t* rA, const int64_t*
   TT_CHECK(rA[0] == 1 && rA[d] == 1 && rb[0] == 1 && rb[d] == 1, ERR_INVALID,
            "boundary ranks must be 1");
   p.rzcap = std::max<int64_t>(0, kickrank) + std::max<int64_t>(0, kick2);
-  const int64_t rcap = std::max<int64_t>(rmax, 1) + p.rzcap;
+  if (rmax_list)
+    for (int k = 1; k < d; ++k) TT_CHECK(rmax_list[k] >= 1, ERR_INVALID, "rank caps must be positive");
   std::vector<int64_t> right(d + 1, 1);
   for (int k = d - 1; k >= 0; --k) right[k] = sat_mul(right[k + 1], N[k]);
   p.Rb.assign(d + 1, 1);
   for (int k = 0; k < d; ++k) {
+    const int64_t rcap = std::max<int64_t>(rmax_list ? rmax_list[k + 1] : rmax, 1) + p.rzcap;
     int64_t b = std::min(sat_mul(p.Rb[k], N[k]), std::min(rcap, right[k + 1] + p.rzcap));
     if (k + 1 == d) b = 1;
     if (rx0) b = std::max(b, rx0[k + 1]);
@@ -239,6 +242,12 @@ struct Options {
           resets = 2;
   int verbose = 0, prec = 0, precision = 0;
   uint64_t seed = 0;
+  // Rank chosen when every truncation candidate meets the threshold: 1 in the reference's C++
+  // solver (amen_solve.h:385-402), 2 in its Python solvers, whose candidate loop stops at rank 1
+  // and then adds one (e.g. _division.py:283-296).
+  int trunc_floor = 1;
+  std::vector<int64_t> rmax_list;  // per-interface rank caps (d+1 entries), else rmax for all
+  int64_t rank_cap(int k) const { return rmax_list.empty() ? rmax : rmax_list[k]; }
 };
 
 class Solver {
@@ -679,7 +688,7 @@ class Solver {
       double* V = ws_.alloc<double>(r2 * kc);
       svd(sol, r2, 1, rows, r2, U, sv, V, ws_, st_);
       r = truncation_rank(op, U, kc, sv, V, e, Nl, rows, r2, real_tol, damp, use_full);
-      r = std::min<int64_t>(r, std::min<int64_t>(kc, O.rmax));
+      r = std::min<int64_t>(r, std::min<int64_t>(kc, O.rank_cap(k + 1)));
       r = std::max<int64_t>(r, 1);
       if (O.verbose) sync_print_local(k, Nl, r);
       u = U; ldu = kc;
@@ -997,7 +1006,8 @@ class Solver {
       }
       rtop -= nv;
     }
-    if (result < 0) result = 1;  // no candidate exceeded the threshold: r = 0 + 1
+    // no candidate exceeded the threshold: the last one (rank 1) was accepted
+    if (result < 0) result = std::min<int64_t>(O.trunc_floor, kc);
     ws_.release(mk);
     return result;
   }
@@ -1059,6 +1069,20 @@ static Options make_options(int64_t nswp, double eps, int64_t rmax, int64_t max_
 }
 
 // One solve on one stream with one workspace (shared by the single and batched entries).
+static Options divide_options(int64_t nswp, double eps, const int64_t* rmax, int d, int64_t max_full,
+                              int64_t kickrank, int64_t kick2, int64_t local_iterations,
+                              int64_t resets, int verbose, int preconditioner) {
+  TT_CHECK(rmax, ERR_INVALID, "null rank caps");
+  int64_t rmx = 1;
+  for (int k = 0; k <= d; ++k) rmx = std::max(rmx, rmax[k]);
+  TT_CHECK(local_iterations >= 1, ERR_INVALID, "invalid solver parameters");
+  Options O = make_options(nswp, eps, rmx, max_full, kickrank, kick2, local_iterations + 1, resets,
+                           verbose, 0, 0, preconditioner);
+  O.trunc_floor = 2;
+  O.rmax_list.assign(rmax, rmax + d + 1);
+  return O;
+}
+
 static void solve_one(int d, const int64_t* N, const int64_t* rA, const int64_t* rb,
                       const double* const* A_cores, const double* const* b_cores,
                       const double* const* x0_cores, const int64_t* rx0,
@@ -1068,7 +1092,8 @@ static void solve_one(int d, const int64_t* N, const int64_t* rA, const int64_t*
   auto t0 = std::chrono::high_resolution_clock::now();
   TT_CHECK(A_cores && b_cores && N && rA && rb && x_out && rx_out && offsets_out && workspace,
            ERR_INVALID, "null argument");
-  Plan P = make_plan(d, N, rA, rb, x0_cores ? rx0 : nullptr, O.rmax, O.kickrank, O.kick2);
+  Plan P = make_plan(d, N, rA, rb, x0_cores ? rx0 : nullptr, O.rmax, O.kickrank, O.kick2,
+                     O.rmax_list.empty() ? nullptr : O.rmax_list.data());
   TT_CHECK(P.maxR <= kMaxSmall, ERR_UNSUPPORTED,
            "rank bound " + std::to_string(P.maxR) + " exceeds " + std::to_string(kMaxSmall) +
                " (lower rmax)");
@@ -1085,6 +1110,37 @@ static void solve_one(int d, const int64_t* N, const int64_t* rA, const int64_t*
 }
 
 static size_t batch_slice(size_t need) { return (need + 255) / 256 * 256; }
+
+// ---------------------------------------------------------------- elementwise division
+// x = b ./ a as the AMEn solution of diag(a) x = b (reference torchtt/_division.py:76-398). The
+// TT-matrix cores of diag(a), D[s, m, n, S] = a[s, m, S] if m == n else 0, are built in the
+// workspace and the solver runs with the semantics of the reference's Python AMEn: GMRES
+// restarts of local_iterations + 1 steps (_division.py:253), a truncation floor of rank 2
+// (:283-296) and per-interface rank caps (:94-95, :296).
+struct KDiagExpand {
+  struct Args { const double* a; double* D; int s, n, S; };
+  static constexpr int kThreads = 256;
+  static __device__ void run(const Args& p, const Blk& bk) {
+    const int64_t total = (int64_t)p.s * p.n * p.n * p.S;
+    for (int64_t t = bk.x * (int64_t)blockDim.x + threadIdx.x; t < total; t += (int64_t)bk.gx * blockDim.x) {
+      const int64_t Si = t % p.S, rest = t / p.S;
+      const int64_t nn = rest % p.n, rest2 = rest / p.n;
+      const int64_t m = rest2 % p.n, si = rest2 / p.n;
+      p.D[t] = (m == nn) ? p.a[(si * p.n + m) * p.S + Si] : 0.0;
+    }
+  }
+};
+
+static size_t divide_operator_bytes(int d, const int64_t* N, const int64_t* ra) {
+  size_t b = 0;
+  for (int k = 0; k < d; ++k) b += batch_slice(sizeof(double) * (size_t)(ra[k] * N[k] * N[k] * ra[k + 1]));
+  return b;
+}
+
+static Options divide_options(int64_t nswp, double eps, const int64_t* rmax, int d, int64_t max_full,
+                              int64_t kickrank, int64_t kick2, int64_t local_iterations,
+                              int64_t resets, int verbose, int preconditioner);
+
 
 // First-fit allocator of byte ranges of one workspace (the serving memory budget).
 class FirstFit {
@@ -1401,6 +1457,82 @@ int ttamen_amen_solve(int d, const int64_t* N, const int64_t* rA, const int64_t*
     if (preconditioner) prec_init();
     solve_one(d, N, rA, rb, A_cores, b_cores, x0_cores, rx0, z0_cores, O, workspace,
               workspace_bytes, x_out, x_out_elems, rx_out, offsets_out, stats, (hipStream_t)stream);
+  });
+}
+
+size_t ttamen_amen_divide_workspace_size(int d, const int64_t* N, const int64_t* ra,
+                                         const int64_t* rb, const int64_t* rx0, const int64_t* rmax,
+                                         int64_t kickrank, int64_t kick2, int64_t local_iterations,
+                                         int64_t max_full, int preconditioner) {
+  size_t out = 0;
+  int rc = guarded([&] {
+    TT_CHECK(d >= 1 && N && ra && rb && rmax, ERR_INVALID, "null argument");
+    TT_CHECK(preconditioner == 0 || preconditioner == 1, ERR_INVALID,
+             "preconditioner must be 0 (none) or 1 ('c')");
+    Options O = divide_options(22, 1e-10, rmax, d, max_full, kickrank, kick2, local_iterations, 2, 0,
+                               preconditioner);
+    Plan P = make_plan(d, N, ra, rb, rx0, O.rmax, kickrank, kick2, rmax);
+    out = divide_operator_bytes(d, N, ra) + solve_workspace_bound(P, O);
+    if (preconditioner) {
+      size_t mx = 0;
+      for (int k = 0; k < d; ++k)
+        mx = std::max(mx, prec_elems(1, (int)P.Rb[k], (int)P.N[k], (int)P.Rb[k + 1]) +
+                              (size_t)(P.Rb[k] * P.N[k] * P.Rb[k + 1]));
+      out += mx * sizeof(double) + 4096;
+    }
+  });
+  return rc == 0 ? out : 0;
+}
+
+size_t ttamen_amen_divide_output_elems(int d, const int64_t* N, const int64_t* rx0,
+                                       const int64_t* rmax, int64_t kickrank, int64_t kick2) {
+  size_t out = 0;
+  int rc = guarded([&] {
+    TT_CHECK(d >= 1 && N && rmax, ERR_INVALID, "null argument");
+    std::vector<int64_t> ones(d + 1, 1);
+    int64_t rmx = 1;
+    for (int k = 0; k <= d; ++k) rmx = std::max(rmx, rmax[k]);
+    Plan P = make_plan(d, N, ones.data(), ones.data(), rx0, rmx, kickrank, kick2, rmax);
+    for (int k = 0; k < d; ++k) out += (size_t)(P.Rb[k] * P.N[k] * P.Rb[k + 1]);
+  });
+  return rc == 0 ? out : 0;
+}
+
+int ttamen_amen_divide(int d, const int64_t* N, const int64_t* ra, const int64_t* rb,
+                       const double* const* a_cores, const double* const* b_cores,
+                       const double* const* x0_cores, const int64_t* rx0,
+                       const double* const* z0_cores, int64_t nswp, double eps, const int64_t* rmax,
+                       int64_t max_full, int64_t kickrank, int64_t kick2, int64_t local_iterations,
+                       int64_t resets, int verbose, int preconditioner, void* workspace,
+                       size_t workspace_bytes, double* x_out, size_t x_out_elems, int64_t* rx_out,
+                       int64_t* offsets_out, ttamen_stats* stats, void* stream) {
+  return guarded([&] {
+    TT_CHECK(d >= 1 && N && ra && rb && a_cores && b_cores && rmax && workspace, ERR_INVALID,
+             "null argument");
+    TT_CHECK(preconditioner == 0 || preconditioner == 1, ERR_INVALID,
+             "preconditioner must be 0 (none) or 1 ('c')");
+    for (int k = 0; k < d; ++k) TT_CHECK(a_cores[k] && N[k] >= 1, ERR_INVALID, "bad operand cores");
+    Options O = divide_options(nswp, eps, rmax, d, max_full, kickrank, kick2, local_iterations, resets,
+                               verbose, preconditioner);
+    linalg_init();
+    lu_init();
+    if (preconditioner) prec_init();
+    hipStream_t st = (hipStream_t)stream;
+    const size_t opb = divide_operator_bytes(d, N, ra);
+    TT_CHECK(workspace_bytes > opb, ERR_WORKSPACE, "workspace too small");
+    std::vector<const double*> D(d);
+    char* w = (char*)workspace;
+    for (int k = 0; k < d; ++k) {
+      double* Dk = (double*)w;
+      const int64_t total = ra[k] * N[k] * N[k] * ra[k + 1];
+      const int grid = (int)std::max<int64_t>(1, std::min<int64_t>((total + 255) / 256, 4096));
+      launch<KDiagExpand>(dim3(grid), dim3(256), 0, st, {a_cores[k], Dk, (int)ra[k], (int)N[k], (int)ra[k + 1]});
+      TT_LAUNCH_CHECK();
+      D[k] = Dk;
+      w += batch_slice(sizeof(double) * (size_t)total);
+    }
+    solve_one(d, N, ra, rb, D.data(), b_cores, x0_cores, rx0, z0_cores, O, w, workspace_bytes - opb,
+              x_out, x_out_elems, rx_out, offsets_out, stats, st);
   });
 }
 

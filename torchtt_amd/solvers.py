@@ -156,6 +156,83 @@ def amen_solve(A, b, nswp=22, x0=None, eps=1e-10, rmax=32768, max_full=500, kick
     return TT(cores)
 
 
+def amen_divide(a, b, nswp=22, x0=None, eps=1e-10, rmax=100, max_full=500, kickrank=4, kick2=0,
+                trunc_norm='res', local_iterations=40, resets=2, verbose=True, preconditioner=None):
+    """Elementwise division x = b ./ a of TT tensors by AMEn on the GPU (reference
+    torchtt/_division.py:76-398, amen_divide).
+
+    Args follow the reference: `rmax` is an int or a list of d+1 rank caps; x0 defaults to the
+    all-ones TT of ranks 1, 2, ..., 2, 1; each GMRES restart runs up to local_iterations + 1
+    steps; `preconditioner` is None or 'c'; only `trunc_norm='res'` exists (the reference's
+    'fro' branch leaves the rank undefined).
+
+    Returns:
+        list[torch.Tensor]: the solution cores, on the device and dtype of a.cores[0].
+    """
+    global last_stats
+    if not (isinstance(a, TT) and isinstance(b, TT)) or a.is_ttm or b.is_ttm:
+        raise InvalidArguments('a and b must be TT tensors (not TT-matrices).')
+    if list(a.N) != list(b.N):
+        raise ShapeMismatch('Both operands should have the same shape.')
+    if trunc_norm != 'res':
+        raise InvalidArguments("Only trunc_norm='res' is supported.")
+    if preconditioner is None:
+        prec = 0
+    elif preconditioner == 'c':
+        prec = 1
+    else:
+        raise InvalidArguments('Preconditioner ' + str(preconditioner) + ' not defined.')
+    lib = _lib.load()
+    out_device, out_dtype = a.cores[0].device, a.cores[0].dtype
+    dev = _solver_device(a)
+    N = list(b.N)
+    d = len(N)
+    caps = [1] + (d - 1) * [int(rmax)] + [1] if isinstance(rmax, int) else [int(v) for v in rmax]
+    if len(caps) != d + 1:
+        raise InvalidArguments('rmax must be an int or a list of d+1 ranks.')
+    if x0 is None:
+        rx0 = [1] + (d - 1) * [2] + [1]
+        x0c = [torch.ones((rx0[k], N[k], rx0[k + 1]), dtype=out_dtype, device=out_device) for k in range(d)]
+    else:
+        rx0, x0c = list(x0.R), list(x0.cores)
+    # residual-proxy draws after the initial guess, with the options of a (_division.py:98-100)
+    z_cores = _draw_z(N, kickrank, kick2, out_device)
+    with torch.cuda.device(dev):
+        ac = [c.to(device=dev, dtype=torch.float64).contiguous() for c in a.cores]
+        bc = [c.to(device=dev, dtype=torch.float64).contiguous() for c in b.cores]
+        xc = [c.to(device=dev, dtype=torch.float64).contiguous() for c in x0c]
+        if z_cores is not None:
+            z_cores = [z.to(device=dev, dtype=torch.float64).contiguous() for z in z_cores]
+        Na, raa, rba = _lib.i64_array(N), _lib.i64_array(a.R), _lib.i64_array(b.R)
+        rx0a, capsa = _lib.i64_array(rx0), _lib.i64_array(caps)
+        ws_bytes = lib.ttamen_amen_divide_workspace_size(d, Na, raa, rba, rx0a, capsa, int(kickrank), int(kick2),
+                                                         int(local_iterations), int(max_full), prec)
+        if ws_bytes == 0:
+            _lib.check("ttamen_amen_divide_workspace_size", 1)
+        out_elems = lib.ttamen_amen_divide_output_elems(d, Na, rx0a, capsa, int(kickrank), int(kick2))
+        ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dev)
+        xout = torch.empty(max(out_elems, 1), dtype=torch.float64, device=dev)
+        rx_out = (ctypes.c_int64 * (d + 1))()
+        off_out = (ctypes.c_int64 * d)()
+        stats = _lib.TTAmenStats()
+        code = lib.ttamen_amen_divide(
+            d, Na, raa, rba, _lib.ptr_array(ac), _lib.ptr_array(bc), _lib.ptr_array(xc), rx0a,
+            _lib.ptr_array(z_cores) if z_cores is not None else None, int(nswp), float(eps), capsa,
+            int(max_full), int(kickrank), int(kick2), int(local_iterations), int(resets), int(bool(verbose)),
+            prec, ctypes.c_void_p(ws.data_ptr()), ws_bytes, ctypes.c_void_p(xout.data_ptr()), out_elems,
+            rx_out, off_out, ctypes.byref(stats), ctypes.c_void_p(_lib.stream_handle(dev)))
+        _lib.check("ttamen_amen_divide", code)
+        cores = []
+        for k in range(d):
+            n = rx_out[k] * N[k] * rx_out[k + 1]
+            c = xout[off_out[k]:off_out[k] + n].reshape(rx_out[k], N[k], rx_out[k + 1]).clone()
+            cores.append(c.to(device=out_device, dtype=out_dtype))
+        last_stats = stats.as_dict()
+        last_stats["workspace_bytes"] = int(ws_bytes)
+        del ws, xout
+    return cores
+
+
 #: per-system statistics of the most recent batch (list of dicts)
 last_batch_stats = None
 
