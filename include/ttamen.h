@@ -196,6 +196,48 @@ size_t ttamen_amen_divide_workspace_size(int d, const int64_t* N, const int64_t*
 size_t ttamen_amen_divide_output_elems(int d, const int64_t* N, const int64_t* rx0,
                                        const int64_t* rmax, int64_t kickrank, int64_t kick2);
 
+/*
+ * AMEn approximation of TT products (reference torchtt/_amen.py; Python only, no native binding):
+ * y ~= A b (amen_mv, :55-101) for a TT-matrix A with cores rA[k] x M[k] x K[k] x rA[k+1] and a TT
+ * vector b with cores rb[k] x K[k] x rb[k+1]; the sweep of _amen_mm_python (:126-413): projected
+ * product per core, rank_chop truncation at ||w|| eps / d^1.5 (d^0.5 in the last sweep), residual
+ * proxy z of kickrank (+ kick2 random) columns, stop one sweep after max dx < eps.
+ *  x0_cores/rx0: initial guess (the reference's default: zero cores of rank 1, when NULL)
+ *  z0_cores: residual-proxy cores rz x M[k] x rz' (rz = kickrank + kick2; the reference draws a
+ *            random TT, _amen.py:146-151); rmax: d+1 per-interface rank caps (:142-143)
+ *  seed: random enrichment columns when kick2 > 0
+ * Output as ttamen_amen_solve: cores rx[k] x M[k] x rx[k+1] at offsets_out in x_out.
+ */
+int ttamen_amen_mv(int d, const int64_t* M, const int64_t* K, const int64_t* rA, const int64_t* rb,
+                   const double* const* A_cores, const double* const* b_cores,
+                   const double* const* x0_cores, const int64_t* rx0, const double* const* z0_cores,
+                   int64_t nswp, double eps, const int64_t* rmax, int64_t kickrank, int64_t kick2,
+                   uint64_t seed, int verbose, void* workspace, size_t workspace_bytes, double* x_out,
+                   size_t x_out_elems, int64_t* rx_out, int64_t* offsets_out, ttamen_stats* stats,
+                   void* stream);
+size_t ttamen_amen_mv_workspace_size(int d, const int64_t* M, const int64_t* K, const int64_t* rA,
+                                     const int64_t* rb, const int64_t* rx0, const int64_t* rmax,
+                                     int64_t kickrank, int64_t kick2);
+size_t ttamen_amen_mv_output_elems(int d, const int64_t* M, const int64_t* K, const int64_t* rx0,
+                                   const int64_t* rmax, int64_t kickrank, int64_t kick2);
+/*
+ * C ~= A B for TT-matrices A (rA[k] x M[k] x K[k] x rA[k+1]) and B (rB[k] x K[k] x N[k] x rB[k+1])
+ * (reference amen_mm, torchtt/_amen.py:104-123): the ttamen_amen_mv sweep on the operator
+ * A (x) I_N (formed in the workspace) and B read as a vector of modes K[k] N[k]; the result cores
+ * rx[k] x (M[k] N[k]) x rx[k+1] are those of C, [r, m, n, r'] as the reference keeps them.
+ * z0_cores: rz x (M[k] N[k]) x rz'. Output sizes: ttamen_amen_mv_output_elems with M[k] N[k].
+ */
+int ttamen_amen_mm(int d, const int64_t* M, const int64_t* K, const int64_t* N, const int64_t* rA,
+                   const int64_t* rB, const double* const* A_cores, const double* const* B_cores,
+                   const double* const* X0_cores, const int64_t* rx0, const double* const* z0_cores,
+                   int64_t nswp, double eps, const int64_t* rmax, int64_t kickrank, int64_t kick2,
+                   uint64_t seed, int verbose, void* workspace, size_t workspace_bytes, double* x_out,
+                   size_t x_out_elems, int64_t* rx_out, int64_t* offsets_out, ttamen_stats* stats,
+                   void* stream);
+size_t ttamen_amen_mm_workspace_size(int d, const int64_t* M, const int64_t* K, const int64_t* N,
+                                     const int64_t* rA, const int64_t* rB, const int64_t* rx0,
+                                     const int64_t* rmax, int64_t kickrank, int64_t kick2);
+
 size_t ttamen_tt_round_workspace_size(int d, const int64_t* N, const int64_t* R);
 int ttamen_tt_round(int d, const int64_t* N, const int64_t* R, const double* const* cores,
                     double eps, int64_t rmax, void* workspace, size_t workspace_bytes, double* out,

@@ -485,6 +485,171 @@ def amen_divide(a, b, x0=None, z0=None, nswp=22, eps=1e-10, rmax=100, max_full=5
                       rng=rng, trace=trace, preconditioner=preconditioner, python_rules=True, rank_caps=caps)
 
 
+# --------------------------------------------------------------------------- AMEn products
+def local_AB(PhiL, PhiR, a, b):
+    """w[r,m,R] = sum PhiL[r,a,b] A[a,m,k,A] b[b,k,B] PhiR[R,A,B] (_amen.py:36-52, vector b)."""
+    return np.einsum("rab,amkA,bkB,RAB->rmR", PhiL, a, b, PhiR, optimize=True)
+
+
+def amen_mv(A, b, x0=None, z0=None, nswp=22, eps=1e-10, rmax=1024, kickrank=4, kick2=0, rng=None,
+            trace=None):
+    """y ~= A b by AMEn (torchtt/_amen.py:55-101 -> _amen_mm_python :126-413, vector case): the
+    solution cores are recomputed from the projected product, truncated by rank_chop at
+    ||w|| eps / d^(1.5, or 0.5 in the last sweep) (:297-300), enriched with the residual proxy z;
+    the sweep stops one sweep after max dx < eps (:391-397; a NaN dx, 0/0 for a zero product,
+    never stops it). x0 defaults to zero cores of rank 1 (:134-137)."""
+    rng = np.random.default_rng(0) if rng is None else rng
+    d = len(A)
+    P = [a.shape[1] for a in A]
+    if x0 is None:
+        x = [np.zeros((1, p, 1)) for p in P]
+    else:
+        x = [c.copy() for c in x0]
+    rx = [1] + [c.shape[2] for c in x]
+    caps = [1] + (d - 1) * [int(rmax)] + [1] if np.isscalar(rmax) else [int(v) for v in rmax]
+    rz = [1] + (d - 1) * [kickrank + kick2] + [1]
+    z = [c.copy() for c in z0] if z0 is not None else [rng.standard_normal((rz[k], P[k], rz[k + 1])) for k in range(d)]
+    rz = rl_orthogonal(z)                                          # :146-153
+    rz = [1] + rz[1:-1] + [1]
+    one2, one3 = np.ones((1, 1)), np.ones((1, 1, 1))
+    Phiz = [one2] + [None] * (d - 1) + [one2]
+    Phiz_rhs = [one3] + [None] * (d - 1) + [one3]
+    Phis = [one2] + [None] * (d - 1) + [one2]
+    Phis_rhs = [one3] + [None] * (d - 1) + [one3]
+    normA = np.ones(max(d - 1, 0))                                 # stays 1 (:201-205 disabled)
+    normb = np.ones(max(d - 1, 0))
+    normx = np.ones(max(d - 1, 0))
+    nrmsc = 1.0
+    last = False
+    if trace is not None:
+        trace.setdefault("ranks", [])
+        trace.setdefault("max_dx", [])
+    bx = lambda Phi, l, r: np.einsum("LR,lpL,rpR->lr", Phi, l, r)          # _compute_phi_bck_x
+    fx = lambda Phi, l, r: np.einsum("lr,lpL,rpR->LR", Phi, l, r)          # _compute_phi_fwd_x
+    bAB = lambda Phi, a, bb, c: np.einsum("RAB,amkA,bkB,rmR->rab", Phi, a, bb, c, optimize=True)
+    fAB = lambda Phi, a, bb, c: np.einsum("rab,amkA,bkB,rmR->RAB", Phi, a, bb, c, optimize=True)
+    proj = lambda L, c, R: np.einsum("zr,rpR,ZR->zpZ", L, c, R)
+    swp = 0
+    for swp in range(nswp):
+        for k in range(d - 1, 0, -1):                              # :199-262
+            if not last:
+                if swp > 0:
+                    cz_new = local_AB(Phiz_rhs[k], Phiz_rhs[k + 1], A[k], b[k]) * nrmsc - proj(Phiz[k], x[k], Phiz[k + 1])
+                    _, _, vt = np.linalg.svd(cz_new.reshape(cz_new.shape[0], -1), full_matrices=False)
+                    cz_new = vt[:min(kickrank, vt.shape[0]), :].T
+                    if k < d - 1 and kick2 > 0:
+                        cz_new = np.concatenate([cz_new, rng.standard_normal((cz_new.shape[0], kick2))], 1)
+                else:
+                    cz_new = z[k].reshape(rz[k], -1).T
+                qz, _ = _qr(cz_new)
+                rz[k] = qz.shape[1]
+                z[k] = qz.T.reshape(rz[k], P[k], rz[k + 1])
+            if swp > 0:
+                nrmsc = nrmsc * normA[k - 1] * normx[k - 1] / normb[k - 1]
+            q, r = _qr(x[k].reshape(rx[k], -1).T)
+            core_prev = np.tensordot(x[k - 1], r.T, axes=([2], [0]))
+            rx[k] = q.shape[1]
+            cn = np.linalg.norm(core_prev)
+            if cn > 0:
+                core_prev = core_prev / cn
+            else:
+                cn = 1.0
+            normx[k - 1] *= cn
+            x[k] = q.T.reshape(rx[k], P[k], rx[k + 1])
+            x[k - 1] = core_prev
+            Phis[k] = bx(Phis[k + 1], x[k], x[k])
+            Phis_rhs[k] = bAB(Phis_rhs[k + 1], A[k], b[k], x[k])
+            nrm = np.linalg.norm(Phis_rhs[k])
+            nrm = nrm if nrm > 0 else 1.0
+            normb[k - 1] = nrm
+            Phis_rhs[k] = Phis_rhs[k] / nrm
+            nrmsc = nrmsc * normb[k - 1] / (normA[k - 1] * normx[k - 1])
+            if not last:
+                Phiz[k] = bx(Phiz[k + 1], z[k], x[k]) / normA[k - 1]
+                Phiz_rhs[k] = bAB(Phiz_rhs[k + 1], A[k], b[k], z[k]) / normb[k - 1]
+        max_dx = 0.0
+        for k in range(d):                                         # :265-388
+            prev = x[k]
+            sol = local_AB(Phis_rhs[k], Phis_rhs[k + 1], A[k], b[k]) * nrmsc
+            nsol = np.linalg.norm(sol)
+            with np.errstate(invalid="ignore", divide="ignore"):
+                dx = np.linalg.norm(sol - prev) / nsol
+            max_dx = max(dx, max_dx)                               # Python max: a NaN dx stays
+            sol = sol.reshape(rx[k] * P[k], rx[k + 1])
+            if k < d - 1:
+                u, s, vt = np.linalg.svd(sol, full_matrices=False)
+                r = rank_chop(s, nsol * eps / (d ** (0.5 if last else 1.5)))
+                r = min(r, s.shape[0], caps[k + 1])
+            else:
+                u, vt = _qr(sol)
+                r = u.shape[1]
+                s = np.ones(r)
+            u = u[:, :r]
+            v = (np.diag(s[:r]) @ vt[:r, :]).T
+            if not last:
+                cz_new = (local_AB(Phiz_rhs[k], Phiz_rhs[k + 1], A[k], b[k]) * nrmsc
+                          - proj(Phiz[k], (u @ v.T).reshape(rx[k], P[k], rx[k + 1]), Phiz[k + 1]))
+                uz, _, _ = np.linalg.svd(cz_new.reshape(rz[k] * P[k], rz[k + 1]), full_matrices=False)
+                cz_new = uz[:, :min(kickrank, uz.shape[1])]
+                if k < d - 1 and kick2 > 0:
+                    cz_new = np.concatenate([cz_new, rng.standard_normal((cz_new.shape[0], kick2))], 1)
+                qz, _ = _qr(cz_new)
+                rz[k + 1] = qz.shape[1]
+                z[k] = qz.reshape(rz[k], P[k], rz[k + 1])
+            if k < d - 1:
+                if not last:
+                    uk = (local_AB(Phis_rhs[k], Phiz_rhs[k + 1], A[k], b[k]) * nrmsc
+                          - proj(Phis[k], (u @ v.T).reshape(rx[k], P[k], rx[k + 1]), Phiz[k + 1]))
+                    u, Rm = _qr(np.concatenate([u, uk.reshape(u.shape[0], -1)], 1))
+                    v = np.concatenate([v, np.zeros((rx[k + 1], uk.shape[-1]))], 1) @ Rm.T
+                r = u.shape[1]
+                v = np.tensordot(v, x[k + 1], axes=([0], [0]))
+                nrmsc = nrmsc * normA[k] * normx[k] / normb[k]
+                nn = np.linalg.norm(v)
+                if nn > 0:
+                    v = v / nn
+                else:
+                    nn = 1.0
+                normx[k] *= nn
+                x[k] = u.reshape(rx[k], P[k], r)
+                x[k + 1] = v.reshape(r, P[k + 1], rx[k + 2])
+                rx[k + 1] = r
+                Phis[k + 1] = fx(Phis[k], x[k], x[k])
+                Phis_rhs[k + 1] = fAB(Phis_rhs[k], A[k], b[k], x[k])
+                nrm = np.linalg.norm(Phis_rhs[k + 1])
+                nrm = nrm if nrm > 0 else 1.0
+                normb[k] = nrm
+                Phis_rhs[k + 1] = Phis_rhs[k + 1] / nrm
+                nrmsc = nrmsc * normb[k] / (normA[k] * normx[k])
+                if not last:
+                    Phiz[k + 1] = fx(Phiz[k], z[k], x[k]) / normA[k]
+                    Phiz_rhs[k + 1] = fAB(Phiz_rhs[k], A[k], b[k], z[k]) / normb[k]
+            else:
+                x[k] = (u @ np.diag(s[:r]) @ v[:r, :].T).reshape(rx[k], P[k], rx[k + 1])
+        if trace is not None:
+            trace["ranks"].append(list(rx))
+            trace["max_dx"].append(max_dx)
+        if last:
+            break
+        if max_dx < eps:
+            last = True
+    if trace is not None:
+        trace["sweeps"] = swp + 1
+    nx = np.exp(np.sum(np.log(normx)) / d) if d > 1 else 1.0
+    return [c * nx for c in x]
+
+
+def kron_eye_operator(A, N):
+    """Cores of A (x) I_N per mode, so that amen_mm(A, B) = amen_mv(A (x) I, vec B) with x cores
+    indexed [r, (m, n), r'] as _amen_mm_python keeps them (:134-137, :393-411)."""
+    out = []
+    for a, n in zip(A, N):
+        ra, m, k, ra2 = a.shape
+        D = np.einsum("amkA,ij->amikjA", a, np.eye(n)).reshape(ra, m * n, k * n, ra2)
+        out.append(D)
+    return out
+
+
 # --------------------------------------------------------------------------- TT helpers
 def tt_full(cores):
     """Dense tensor of a TT vector (cores r x n x r') -- reference cpp/full.h:3-12."""

@@ -251,3 +251,59 @@ def test_oracle_divide_truncation_floor():
     want = O.tt_full(b) / O.tt_full(a)
     assert np.linalg.norm(O.tt_full(q) - want) / np.linalg.norm(want) < 1e-11
     assert [c.shape[2] for c in q][:-1] == [2, 2]
+
+
+# ----------------------------------------------------------------- AMEn products
+def _randn_tt(rng, shape, R):
+    """Cores scaled as the reference's torchtt.randn (_extras.py:306-330)."""
+    d = len(shape)
+    v = (1.0 / np.prod(R)) ** (1.0 / d)
+    return [rng.standard_normal((R[k],) + tuple(shape[k]) + (R[k + 1],)) * np.sqrt(v) for k in range(d)]
+
+
+def _add4(X, Y):
+    sh = [c.shape for c in X]
+    Z = O.tt_add([c.reshape(c.shape[0], -1, c.shape[-1]) for c in X], [c.reshape(c.shape[0], -1, c.shape[-1]) for c in Y])
+    return [z.reshape((z.shape[0],) + s[1:-1] + (z.shape[-1],)) for z, s in zip(Z, sh)]
+
+
+def test_oracle_amen_mv_product():
+    """The reference's test (tests/test_algebra_2.py:122-138): 5A times 5x equals 25 A x."""
+    rng = np.random.default_rng(0)
+    A = _randn_tt(rng, [(3, 4), (5, 6), (7, 8), (2, 3)], [1, 2, 2, 3, 1])
+    x = _randn_tt(rng, [(4,), (6,), (8,), (3,)], [1, 4, 3, 3, 1])
+    want = 25 * O.tt_full(O.tt_matvec(A, x))
+    A5, x5 = A, x
+    for _ in range(4):
+        A5, x5 = _add4(A5, A), O.tt_add(x5, x)
+    y = O.amen_mv(A5, x5)
+    assert np.linalg.norm(O.tt_full(y) - want) / np.linalg.norm(want) < 1e-11
+
+
+def test_oracle_amen_mv_special_operands():
+    """Zero operator, zero vector and identity (tests/test_algebra_2.py:141-185)."""
+    rng = np.random.default_rng(1)
+    N = [3, 4, 2]
+    x = _randn_tt(rng, [(n,) for n in N], [1, 2, 2, 1])
+    Z = [np.zeros((1, n, n, 1)) for n in N]
+    I = [np.eye(n).reshape(1, n, n, 1) for n in N]
+    assert np.linalg.norm(O.tt_full(O.amen_mv(Z, x, nswp=4, eps=1e-12, kickrank=2))) < 1e-12
+    zero = [np.zeros((1, n, 1)) for n in N]
+    assert np.linalg.norm(O.tt_full(O.amen_mv(I, zero, nswp=4, eps=1e-12, kickrank=2))) < 1e-12
+    y = O.amen_mv(I, x, nswp=8, eps=1e-12, kickrank=2)
+    fx = O.tt_full(x)
+    assert np.linalg.norm(O.tt_full(y) - fx) / np.linalg.norm(fx) < 1e-11
+
+
+def test_oracle_amen_mm_by_kron_identity():
+    """amen_mm(A, B) as amen_mv(A (x) I, vec B) (tests/test_algebra_2.py:188-204)."""
+    rng = np.random.default_rng(2)
+    A = _randn_tt(rng, [(3, 4), (5, 6), (7, 8), (2, 3)], [1, 2, 2, 3, 1])
+    B = _randn_tt(rng, [(4, 2), (6, 4), (8, 5), (3, 7)], [1, 4, 3, 3, 1])
+    full = 25 * O.tt_full([np.einsum("amkA,bknB->abmnAB", a, b).reshape(a.shape[0] * b.shape[0], a.shape[1] * b.shape[2],
+                                                                        a.shape[3] * b.shape[3]) for a, b in zip(A, B)])
+    A5, B5 = A, B
+    for _ in range(4):
+        A5, B5 = _add4(A5, A), _add4(B5, B)
+    y = O.amen_mv(O.kron_eye_operator(A5, [2, 4, 5, 7]), [c.reshape(c.shape[0], -1, c.shape[-1]) for c in B5])
+    assert np.linalg.norm(O.tt_full(y) - full) / np.linalg.norm(full) < 1e-11

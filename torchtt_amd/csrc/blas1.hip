@@ -120,14 +120,17 @@ void cast_f32_f64(const float* x, double* y, int64_t n, hipStream_t st) {
   TT_LAUNCH_CHECK();
 }
 
-void permute_core_A(const double* A, double* P, int s, int n, int S, hipStream_t st) {
-  int64_t total = (int64_t)s * n * n * S;
-  launch<KPermA<double>>(dim3(grid_for(total)), dim3(256), 0, st, {A, P, s, n, S});
+void permute_core_A_rect(const double* A, double* P, int s, int nm, int nk, int S, hipStream_t st) {
+  int64_t total = (int64_t)s * nm * nk * S;
+  launch<KPermA<double>>(dim3(grid_for(total)), dim3(256), 0, st, {A, P, s, nm, nk, S});
   TT_LAUNCH_CHECK();
+}
+void permute_core_A(const double* A, double* P, int s, int n, int S, hipStream_t st) {
+  permute_core_A_rect(A, P, s, n, n, S, st);
 }
 void permute_core_A_f32(const double* A, float* P, int s, int n, int S, hipStream_t st) {
   int64_t total = (int64_t)s * n * n * S;
-  launch<KPermA<float>>(dim3(grid_for(total)), dim3(256), 0, st, {A, P, s, n, S});
+  launch<KPermA<float>>(dim3(grid_for(total)), dim3(256), 0, st, {A, P, s, n, n, S});
   TT_LAUNCH_CHECK();
 }
 

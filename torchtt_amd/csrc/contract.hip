@@ -17,13 +17,13 @@ namespace ttamen {
 //   stage 3 (one GEMM):   y[(v,l,m)][L]     = sum_{S,R} t2[(v,l,m)][(S,R)] PhiR[L,(S,R)]
 template <typename T>
 static void matvec_impl(const T* PhiL, const T* Apm, const T* PhiR, int l1, int s, int r1, int n,
-                        int S, int l2, int r2, const T* x, T* y, int nvec, Arena& ws,
+                        int nm, int S, int l2, int r2, const T* x, T* y, int nvec, Arena& ws,
                         hipStream_t st, const int* skip,
                         void (*gemm)(const GemmDesc&, Arena&, hipStream_t)) {
+  // n: mode of x (columns of the operator core), nm: mode of y (rows; nm == n when square)
   const int64_t nx = (int64_t)r1 * n * r2;
-  const int64_t ny = (int64_t)l1 * n * l2;
   const int64_t t1v = (int64_t)l1 * s * n * r2;  // per vector
-  const int64_t t2v = (int64_t)l1 * n * S * r2;
+  const int64_t t2v = (int64_t)l1 * nm * S * r2;
   size_t mk = ws.mark();
   T* t1 = ws.alloc<T>(t1v * nvec);
   T* t2 = ws.alloc<T>(t2v * nvec);
@@ -38,73 +38,72 @@ static void matvec_impl(const T* PhiL, const T* Apm, const T* PhiR, int l1, int 
   }
   {
     GemmDesc g;
-    g.M = n * S; g.N = r2; g.K = s * n; g.B1 = nvec * l1; g.B2 = 1;
+    g.M = nm * S; g.N = r2; g.K = s * n; g.B1 = nvec * l1; g.B2 = 1;
     g.a.p = Apm; g.a.r = (int64_t)s * n; g.a.c = 1; g.a.b1 = 0;
     g.b.p = t1; g.b.r = r2; g.b.c = 1; g.b.b1 = (int64_t)s * n * r2;
-    g.c.p = t2; g.c.r = r2; g.c.c = 1; g.c.b1 = (int64_t)n * S * r2;
+    g.c.p = t2; g.c.r = r2; g.c.c = 1; g.c.b1 = (int64_t)nm * S * r2;
     g.skip = skip;
     gemm(g, ws, st);
   }
   {
     GemmDesc g;
-    g.M = nvec * l1 * n; g.N = l2; g.K = S * r2;
+    g.M = nvec * l1 * nm; g.N = l2; g.K = S * r2;
     g.a.p = t2; g.a.r = (int64_t)S * r2; g.a.c = 1;
     g.b.p = PhiR; g.b.r = 1; g.b.c = (int64_t)S * r2;
     g.c.p = y; g.c.r = l2; g.c.c = 1;
     g.skip = skip;
     gemm(g, ws, st);
   }
-  (void)ny;
   ws.release(mk);
 }
 
 void local_matvec(const LocalOp& op, const double* x, double* y, int nvec, Arena& ws,
                   hipStream_t st, const int* skip) {
-  matvec_impl<double>(op.PhiL, op.Apm, op.PhiR, op.l1, op.s, op.r1, op.n, op.S, op.l2, op.r2, x,
-                      y, nvec, ws, st, skip, gemm_f64);
+  matvec_impl<double>(op.PhiL, op.Apm, op.PhiR, op.l1, op.s, op.r1, op.n, op.n_out ? op.n_out : op.n,
+                      op.S, op.l2, op.r2, x, y, nvec, ws, st, skip, gemm_f64);
 }
 
 void local_matvec_f32(const LocalOpF32& op, const float* x, float* y, int nvec, Arena& ws,
                       hipStream_t st, const int* skip) {
-  matvec_impl<float>(op.PhiL, op.Apm, op.PhiR, op.l1, op.s, op.r1, op.n, op.S, op.l2, op.r2, x,
-                     y, nvec, ws, st, skip, op.bf16 ? gemm_bf16 : gemm_f32);
+  matvec_impl<float>(op.PhiL, op.Apm, op.PhiR, op.l1, op.s, op.r1, op.n, op.n, op.S, op.l2, op.r2,
+                     x, y, nvec, ws, st, skip, op.bf16 ? gemm_bf16 : gemm_f32);
 }
 
 size_t local_matvec_ws(const LocalOp& op, int nvec) {
   Arena a;
   a.alloc<double>((size_t)op.l1 * op.s * op.n * op.r2 * nvec);
-  a.alloc<double>((size_t)op.l1 * op.n * op.S * op.r2 * nvec);
+  a.alloc<double>((size_t)op.l1 * (op.n_out ? op.n_out : op.n) * op.S * op.r2 * nvec);
   return a.peak() + 64 * 1024 * 1024;  // generous split-K headroom
 }
 
 // Phi forward (reference amen_solve.h:58-68, 'lsr,lML,sMNS,rNR->LSR'):
 //   t2[l][(M,S)][R] = stages 1-2 of the local operator with PhiL = Phi, x = xR
 //   Pn[L][(S,R)]    = sum_{(l,M)} xL[(l,M)][L] t2[(l,M)][(S,R)]
-void phi_fwd_A(const double* Phi, const double* xL, const double* Apm, const double* xR,
-               double* Pn, int l, int s, int r, int n, int S, int L, int R, Arena& ws,
-               hipStream_t st) {
+void phi_fwd_A_rect(const double* Phi, const double* xL, const double* Apm, const double* xR,
+                    double* Pn, int l, int s, int r, int nm, int nk, int S, int L, int R,
+                    Arena& ws, hipStream_t st) {
   size_t mk = ws.mark();
-  double* t1 = ws.alloc<double>((size_t)l * s * n * R);
-  double* t2 = ws.alloc<double>((size_t)l * n * S * R);
+  double* t1 = ws.alloc<double>((size_t)l * s * nk * R);
+  double* t2 = ws.alloc<double>((size_t)l * nm * S * R);
   {
     GemmDesc g;
-    g.M = l; g.N = n * R; g.K = r; g.B1 = 1; g.B2 = s;
+    g.M = l; g.N = nk * R; g.K = r; g.B1 = 1; g.B2 = s;
     g.a.p = Phi; g.a.r = (int64_t)s * r; g.a.c = 1; g.a.b2 = r;
-    g.b.p = xR; g.b.r = (int64_t)n * R; g.b.c = 1;
-    g.c.p = t1; g.c.r = (int64_t)s * n * R; g.c.c = 1; g.c.b2 = (int64_t)n * R;
+    g.b.p = xR; g.b.r = (int64_t)nk * R; g.b.c = 1;
+    g.c.p = t1; g.c.r = (int64_t)s * nk * R; g.c.c = 1; g.c.b2 = (int64_t)nk * R;
     gemm_f64(g, ws, st);
   }
   {
     GemmDesc g;
-    g.M = n * S; g.N = R; g.K = s * n; g.B1 = l;
-    g.a.p = Apm; g.a.r = (int64_t)s * n; g.a.c = 1;
-    g.b.p = t1; g.b.r = R; g.b.c = 1; g.b.b1 = (int64_t)s * n * R;
-    g.c.p = t2; g.c.r = R; g.c.c = 1; g.c.b1 = (int64_t)n * S * R;
+    g.M = nm * S; g.N = R; g.K = s * nk; g.B1 = l;
+    g.a.p = Apm; g.a.r = (int64_t)s * nk; g.a.c = 1;
+    g.b.p = t1; g.b.r = R; g.b.c = 1; g.b.b1 = (int64_t)s * nk * R;
+    g.c.p = t2; g.c.r = R; g.c.c = 1; g.c.b1 = (int64_t)nm * S * R;
     gemm_f64(g, ws, st);
   }
   {
     GemmDesc g;
-    g.M = L; g.N = S * R; g.K = l * n;
+    g.M = L; g.N = S * R; g.K = l * nm;
     g.a.p = xL; g.a.r = 1; g.a.c = L;
     g.b.p = t2; g.b.r = (int64_t)S * R; g.b.c = 1;
     g.c.p = Pn; g.c.r = (int64_t)S * R; g.c.c = 1;
@@ -113,19 +112,25 @@ void phi_fwd_A(const double* Phi, const double* xL, const double* Apm, const dou
   ws.release(mk);
 }
 
+void phi_fwd_A(const double* Phi, const double* xL, const double* Apm, const double* xR,
+               double* Pn, int l, int s, int r, int n, int S, int L, int R, Arena& ws,
+               hipStream_t st) {
+  phi_fwd_A_rect(Phi, xL, Apm, xR, Pn, l, s, r, n, n, S, L, R, ws, st);
+}
+
 // Phi backward (reference amen_solve.h:38-47, 'LSR,lML,sMNS,rNR->lsr'):
 //   u1[(r,N)][S][L] = sum_R xR[(r,N)][R] Phi[L][S][R]            (batch S)
 //   u2[r][(s,M)][L] = sum_{(N,S)} A[(s,M)][(N,S)] u1[r][(N,S)][L] (batch r)
 //   Pp[l][s][r]     = sum_{(M,L)} xL[l][(M,L)] u2[r][s][(M,L)]    (batch s)
-void phi_bck_A(const double* Phi, const double* xL, const double* A, const double* xR,
-               double* Pp, int l, int s, int r, int n, int S, int L, int R, Arena& ws,
-               hipStream_t st) {
+void phi_bck_A_rect(const double* Phi, const double* xL, const double* A, const double* xR,
+                    double* Pp, int l, int s, int r, int nm, int nk, int S, int L, int R,
+                    Arena& ws, hipStream_t st) {
   size_t mk = ws.mark();
-  double* u1 = ws.alloc<double>((size_t)r * n * S * L);
-  double* u2 = ws.alloc<double>((size_t)r * s * n * L);
+  double* u1 = ws.alloc<double>((size_t)r * nk * S * L);
+  double* u2 = ws.alloc<double>((size_t)r * s * nm * L);
   {
     GemmDesc g;
-    g.M = r * n; g.N = L; g.K = R; g.B2 = S;
+    g.M = r * nk; g.N = L; g.K = R; g.B2 = S;
     g.a.p = xR; g.a.r = R; g.a.c = 1;
     g.b.p = Phi; g.b.r = 1; g.b.c = (int64_t)S * R; g.b.b2 = R;
     g.c.p = u1; g.c.r = (int64_t)S * L; g.c.c = 1; g.c.b2 = L;
@@ -133,21 +138,27 @@ void phi_bck_A(const double* Phi, const double* xL, const double* A, const doubl
   }
   {
     GemmDesc g;
-    g.M = s * n; g.N = L; g.K = n * S; g.B1 = r;
-    g.a.p = A; g.a.r = (int64_t)n * S; g.a.c = 1;
-    g.b.p = u1; g.b.r = L; g.b.c = 1; g.b.b1 = (int64_t)n * S * L;
-    g.c.p = u2; g.c.r = L; g.c.c = 1; g.c.b1 = (int64_t)s * n * L;
+    g.M = s * nm; g.N = L; g.K = nk * S; g.B1 = r;
+    g.a.p = A; g.a.r = (int64_t)nk * S; g.a.c = 1;
+    g.b.p = u1; g.b.r = L; g.b.c = 1; g.b.b1 = (int64_t)nk * S * L;
+    g.c.p = u2; g.c.r = L; g.c.c = 1; g.c.b1 = (int64_t)s * nm * L;
     gemm_f64(g, ws, st);
   }
   {
     GemmDesc g;
-    g.M = l; g.N = r; g.K = n * L; g.B2 = s;
-    g.a.p = xL; g.a.r = (int64_t)n * L; g.a.c = 1;
-    g.b.p = u2; g.b.r = 1; g.b.c = (int64_t)s * n * L; g.b.b2 = (int64_t)n * L;
+    g.M = l; g.N = r; g.K = nm * L; g.B2 = s;
+    g.a.p = xL; g.a.r = (int64_t)nm * L; g.a.c = 1;
+    g.b.p = u2; g.b.r = 1; g.b.c = (int64_t)s * nm * L; g.b.b2 = (int64_t)nm * L;
     g.c.p = Pp; g.c.r = (int64_t)s * r; g.c.c = 1; g.c.b2 = r;
     gemm_f64(g, ws, st);
   }
   ws.release(mk);
+}
+
+void phi_bck_A(const double* Phi, const double* xL, const double* A, const double* xR,
+               double* Pp, int l, int s, int r, int n, int S, int L, int R, Arena& ws,
+               hipStream_t st) {
+  phi_bck_A_rect(Phi, xL, A, xR, Pp, l, s, r, n, n, S, L, R, ws, st);
 }
 
 // Phib forward (reference amen_solve.h:93-98, 'br,bnB,rnR->BR').

@@ -183,22 +183,21 @@ struct KCast {
   }
 };
 
-// A[s][m][n][S] -> P[m][S][s][n]
 template <typename T>
 struct KPermA {
-  struct Args { const double* A; T* P; int s, n, S; };
+  struct Args { const double* A; T* P; int s, nm, nk, S; };  // A[s][nm][nk][S] -> P[nm][S][s][nk]
   static constexpr int kThreads = 256;
   static __device__ __forceinline__ void run(const Args& p, const Blk& bk) {
-    const int s = p.s, n = p.n, S = p.S;
-    const int64_t total = (int64_t)s * n * n * S;
+    const int s = p.s, nm = p.nm, nk = p.nk, S = p.S;
+    const int64_t total = (int64_t)s * nm * nk * S;
     TT_GRID_LOOP(t, total) {
-      // t indexes P: ((m*S + Si)*s + si)*n + ni
+      // t indexes P: ((m*S + Si)*s + si)*nk + ki
       int64_t q = t;
-      int ni = (int)(q % n); q /= n;
+      int ki = (int)(q % nk); q /= nk;
       int si = (int)(q % s); q /= s;
       int Si = (int)(q % S); q /= S;
       int mi = (int)q;
-      p.P[t] = (T)p.A[(((int64_t)si * n + mi) * n + ni) * S + Si];
+      p.P[t] = (T)p.A[(((int64_t)si * nm + mi) * nk + ki) * S + Si];
     }
   }
 };

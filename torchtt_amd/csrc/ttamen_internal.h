@@ -161,6 +161,8 @@ void cast_f32_f64(const float* x, double* y, int64_t n, hipStream_t st);
 // Permute an operator core A[s][m][n][S] -> P[m][S][s][n] (the layout the local
 // matvec's middle contraction reads as a row-major (n*S) x (s*n) matrix).
 void permute_core_A(const double* A, double* P, int s, int n, int S, hipStream_t st);
+// Rectangular operator core A[s][nm][nk][S] -> P[nm][S][s][nk].
+void permute_core_A_rect(const double* A, double* P, int s, int nm, int nk, int S, hipStream_t st);
 void permute_core_A_f32(const double* A, float* P, int s, int n, int S, hipStream_t st);
 // Dense local operator B[(l,m,L),(r,n,R)] = sum_{s,S} PhiL[l,s,r] A[s,m,n,S] PhiR[L,S,R]
 // (row-major with leading dimension ld >= r1*n*r2)
@@ -215,6 +217,7 @@ struct LocalOp {
   const double* Apm = nullptr;
   const double* PhiR = nullptr;
   int l1 = 1, s = 1, r1 = 1, n = 1, S = 1, l2 = 1, r2 = 1;
+  int n_out = 0;  // mode size of y for a rectangular operator core A[s][n_out][n][S] (0: n)
 };
 void local_matvec(const LocalOp& op, const double* x, double* y, int nvec, Arena& ws,
                   hipStream_t st, const int* skip = nullptr);
@@ -239,6 +242,14 @@ void phi_fwd_A(const double* Phi, const double* xL, const double* Apm, const dou
 void phi_bck_A(const double* Phi, const double* xL, const double* A, const double* xR,
                double* Pp, int l, int s, int r, int n, int S, int L, int R, Arena& ws,
                hipStream_t st);
+// The same for a rectangular operator core (M of size nm on the xL side, N of size nk on the
+// xR side; Apm from permute_core_A_rect, A unpermuted).
+void phi_fwd_A_rect(const double* Phi, const double* xL, const double* Apm, const double* xR,
+                    double* Pn, int l, int s, int r, int nm, int nk, int S, int L, int R,
+                    Arena& ws, hipStream_t st);
+void phi_bck_A_rect(const double* Phi, const double* xL, const double* A, const double* xR,
+                    double* Pp, int l, int s, int r, int nm, int nk, int S, int L, int R,
+                    Arena& ws, hipStream_t st);
 // fwd rhs: Pn[B,R] = sum Pb[b,r] bc[b,n,B] x[r,n,R]
 void phi_fwd_rhs(const double* Pb, const double* bc, const double* x, double* Pn, int b, int r,
                  int n, int B, int R, Arena& ws, hipStream_t st);
