@@ -238,6 +238,34 @@ size_t ttamen_amen_mm_workspace_size(int d, const int64_t* M, const int64_t* K, 
                                      const int64_t* rA, const int64_t* rB, const int64_t* rx0,
                                      const int64_t* rmax, int64_t kickrank, int64_t kick2);
 
+/*
+ * DMRG approximation of y = A x (reference torchttcpp.dmrg_mv, cpp/dmrg_mv.h:23-230, bound at
+ * cpp/cpp_ext.cpp:12 and called from torchtt/_dmrg.py:42): A has cores rA[k] x M[k] x N[k] x
+ * rA[k+1], x cores rx[k] x N[k] x rx[k+1]. Sweeps of right-to-left orthogonalisation and
+ * left-to-right two-core updates: the supercore projection W of A x, delta = ||W - y_k y_k+1|| /
+ * ||W|| steering a per-core rank enlargement, rank_chop truncation at ||W|| eps / d^1.5
+ * (d^0.5 in the last sweep), kickrank random columns in every sweep but the final one; the
+ * sweep after max delta < eps is the last.
+ *  y0_cores/ry0: initial guess (NULL: rank-2 normal cores drawn on the device from `seed`, as
+ *                the reference draws torch::randn cores, dmrg_mv.h:43-51)
+ *  rmax: d+1 per-interface rank caps (the reference's scalar rmax on every interface)
+ *  seed: the device's normal stream for the kick columns (and the default guess)
+ *  d >= 2; ranks above 140 (the kick included) are refused with status 4 (unsupported).
+ * Output as ttamen_amen_solve: cores ry[k] x M[k] x ry[k+1] at offsets_out in y_out; stats
+ * report sweeps, the largest rank and the last sweep's max delta (max_res_last).
+ */
+int ttamen_dmrg_mv(int d, const int64_t* M, const int64_t* N, const int64_t* rA, const int64_t* rx,
+                   const double* const* A_cores, const double* const* x_cores,
+                   const double* const* y0_cores, const int64_t* ry0, int64_t nswp, double eps,
+                   const int64_t* rmax, int64_t kickrank, uint64_t seed, int verbose, void* workspace,
+                   size_t workspace_bytes, double* y_out, size_t y_out_elems, int64_t* ry_out,
+                   int64_t* offsets_out, ttamen_stats* stats, void* stream);
+size_t ttamen_dmrg_mv_workspace_size(int d, const int64_t* M, const int64_t* N, const int64_t* rA,
+                                     const int64_t* rx, const int64_t* ry0, const int64_t* rmax,
+                                     int64_t kickrank);
+size_t ttamen_dmrg_mv_output_elems(int d, const int64_t* M, const int64_t* ry0, const int64_t* rmax,
+                                   int64_t kickrank);
+
 size_t ttamen_tt_round_workspace_size(int d, const int64_t* N, const int64_t* R);
 int ttamen_tt_round(int d, const int64_t* N, const int64_t* R, const double* const* cores,
                     double eps, int64_t rmax, void* workspace, size_t workspace_bytes, double* out,

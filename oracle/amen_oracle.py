@@ -23,7 +23,7 @@ import numpy as np
 
 __all__ = ["local_product", "phi_fwd_A", "phi_bck_A", "phi_fwd_rhs", "phi_bck_rhs",
            "gmres", "amen_solve", "rl_orthogonal", "rank_chop", "round_tt", "tt_full",
-           "tt_matvec", "tt_add", "tt_norm", "tt_residual"]
+           "tt_matvec", "tt_add", "tt_norm", "tt_residual", "randn_stream", "dmrg_mv"]
 
 
 # --------------------------------------------------------------------------- contractions
@@ -648,6 +648,132 @@ def kron_eye_operator(A, N):
         D = np.einsum("amkA,ij->amikjA", a, np.eye(n)).reshape(ra, m * n, k * n, ra2)
         out.append(D)
     return out
+
+
+# --------------------------------------------------------------------------- DMRG matvec
+_U64 = np.uint64
+
+
+def _splitmix64(x):
+    """SplitMix64 output function (Steele, Lea, Flood, OOPSLA 2014) on uint64 arrays."""
+    with np.errstate(over="ignore"):
+        x = x + _U64(0x9E3779B97F4A7C15)
+        x = (x ^ (x >> _U64(30))) * _U64(0xBF58476D1CE4E5B9)
+        x = (x ^ (x >> _U64(27))) * _U64(0x94D049BB133111EB)
+    return x ^ (x >> _U64(31))
+
+
+def randn_stream(seed, stream_id, n):
+    """n normal numbers of counter stream `stream_id`: element i is Box-Muller of two SplitMix64
+    words keyed by (seed, stream_id, i) -- the rule of the device's normal generator, restated
+    so that the oracle draws the same kick columns as the GPU sweep."""
+    with np.errstate(over="ignore"):
+        i = np.arange(n, dtype=np.uint64)
+        key = _U64(stream_id) * _U64(0x100000001B3) + i * _U64(2)
+        a = _splitmix64(_U64(seed) ^ _splitmix64(key))
+        b = _splitmix64(a + _U64(0x632BE59BD9B4E019))
+    u1 = ((a >> _U64(11)).astype(np.float64) + 1.0) * (1.0 / 9007199254740993.0)
+    u2 = (b >> _U64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+    return np.sqrt(-2.0 * np.log(u1)) * np.cos(6.283185307179586 * u2)
+
+
+def dmrg_mv(A, x, y0, nswp=20, eps=1e-12, rmax=32768, kickrank=4, seed=0, trace=None):
+    """y ~= A x by DMRG sweeps (cpp/dmrg_mv.h:23-230, the reference's native fast_matvec).
+
+    A: operator cores rA x M x N x rA'; x: cores rx x N x rx'; y0: initial cores (the reference
+    draws rank-2 normal cores when none is given, :43-51). Each sweep: right-to-left QR
+    orthogonalisation with the interfaces Phi (:87-108); then for every supercore (k, k+1) the
+    projection W of A x (or y_k y_k+1 in the last sweep), delta = ||W - y_k y_k+1|| / ||W||
+    driving the rank enlargement (:145-158), SVD truncated by rank_chop at ||W|| eps /
+    d^(1.5, 0.5 in the last sweep) plus the enlargement (:160-172), kickrank normal columns
+    appended by QR except in sweep nswp-1 (:177-187), Phi update (:200-209); the sweep after
+    max delta < eps is the last (:212-220). The kick columns come from randn_stream(seed,
+    (1 << 40) | (sweep << 20) | k), row-major m1 x kickrank -- the stream the GPU sweep draws
+    them from. rmax: an int or a list of d+1 caps. `trace`, if a dict, receives per-sweep
+    ranks and deltas."""
+    d = len(x)
+    M = [a.shape[1] for a in A]
+    y = [np.array(c, dtype=np.float64, copy=True) for c in y0]
+    ry = [1] + [c.shape[2] for c in y]
+    caps = [1] + (d - 1) * [int(rmax)] + [1] if isinstance(rmax, int) else list(rmax)
+    Phis = [None] * (d + 1)
+    Phis[0] = np.ones((1, 1, 1))
+    Phis[d] = np.ones((1, 1, 1))
+    delta = [1.0] * (d - 1)
+    delta_prev = [1.0] * (d - 1)
+    r_enl = [2] * d
+    last = False
+    sweeps = 0
+    if trace is not None:
+        trace.setdefault("ranks", [])
+        trace.setdefault("deltas", [])
+    for swp in range(nswp):
+        for k in range(d - 1, 0, -1):
+            core = y[k].transpose(1, 2, 0).reshape(M[k] * ry[k + 1], ry[k])
+            q, r = _qr(core)
+            rnew = min(core.shape)
+            y[k] = q.T.reshape(rnew, M[k], ry[k + 1])
+            ry[k] = rnew
+            y[k - 1] = (y[k - 1].reshape(-1, y[k - 1].shape[2]) @ r.T).reshape(-1, M[k - 1], rnew)
+            phi = np.tensordot(Phis[k + 1], x[k], axes=([2], [2]))          # Y A' x n   (:99-101)
+            phi = np.tensordot(A[k], phi, axes=([2, 3], [3, 1]))            # a m Y x
+            Phis[k] = np.tensordot(y[k], phi, axes=([1, 2], [1, 2]))        # y a x
+        for k in range(d - 1):
+            W_prev = np.tensordot(y[k], y[k + 1], axes=([2], [0]))
+            if not last:
+                W1 = np.tensordot(Phis[k], x[k], axes=([2], [0]))              # y a n X   (:121-130)
+                W1 = np.tensordot(A[k], W1, axes=([0, 2], [1, 2]))             # m A' y X
+                W2 = np.tensordot(Phis[k + 2], x[k + 1], axes=([2], [2]))      # Y B X n'
+                W2 = np.tensordot(A[k + 1], W2, axes=([2, 3], [3, 1]))         # A' m' Y X
+                W = np.tensordot(W1, W2, axes=([1, 3], [0, 3])).transpose(1, 0, 2, 3)  # y m m' Y
+            else:
+                W = W_prev
+            b = float(np.linalg.norm(W))
+            if b > 0:
+                delta[k] = float(np.linalg.norm(W - W_prev)) / b
+            else:
+                delta[k] = 0.0
+            with np.errstate(divide="ignore", invalid="ignore"):
+                ratio = np.float64(delta[k]) / np.float64(delta_prev[k])
+            if ratio >= 1 and delta[k] > eps:
+                r_enl[k] += 1
+            if ratio < 0.1 and delta[k] < eps:
+                r_enl[k] = max(r_enl[k] - 1, 1)
+            m1 = W.shape[0] * W.shape[1]
+            U, S, Vt = np.linalg.svd(W.reshape(m1, -1), full_matrices=False)
+            r_new = rank_chop(S, b * eps / d ** (0.5 if last else 1.5))
+            if not last:
+                r_new += r_enl[k]
+            r_new = max(min(r_new, S.shape[0], caps[k + 1]), 1)
+            W1 = U[:, :r_new]
+            W2 = Vt[:r_new, :].T * S[:r_new]
+            if swp < nswp - 1:
+                sid = (1 << 40) | (swp << 20) | k
+                kick = randn_stream(seed, sid, m1 * kickrank).reshape(m1, kickrank)
+                W1, Rm = _qr(np.concatenate([W1, kick], axis=1))
+                W2 = np.concatenate([W2, np.zeros((W2.shape[0], kickrank))], axis=1)
+                W2 = Rm @ W2.T
+                r_new = W1.shape[1]
+            else:
+                W2 = W2.T
+            ry[k + 1] = r_new
+            y[k] = W1.reshape(ry[k], M[k], r_new)
+            y[k + 1] = W2.reshape(r_new, M[k + 1], ry[k + 2])
+            phi = np.tensordot(Phis[k], x[k], axes=([2], [0]))              # y a n X   (:200-203)
+            phi = np.tensordot(phi, A[k], axes=([1, 2], [0, 2]))            # y X m A'
+            Phis[k + 1] = np.tensordot(y[k], phi, axes=([0, 1], [0, 2])).transpose(0, 2, 1)  # Y A' X
+        sweeps = swp + 1
+        if trace is not None:
+            trace["ranks"].append(list(ry))
+            trace["deltas"].append(list(delta))
+        if last:
+            break
+        if max(delta) < eps:
+            last = True
+        delta_prev = list(delta)
+    if trace is not None:
+        trace["sweeps"] = sweeps
+    return y
 
 
 # --------------------------------------------------------------------------- TT helpers

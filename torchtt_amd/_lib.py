@@ -107,6 +107,13 @@ _SIGNATURES = {
                                       c_dp]),
     "ttamen_amen_mm_workspace_size": (c_size, [ctypes.c_int, c_i64p, c_i64p, c_i64p, c_i64p, c_i64p, c_i64p,
                                                c_i64p, c_i64, c_i64]),
+    "ttamen_dmrg_mv": (ctypes.c_int, [ctypes.c_int, c_i64p, c_i64p, c_i64p, c_i64p, ctypes.POINTER(c_dp),
+                                      ctypes.POINTER(c_dp), ctypes.POINTER(c_dp), c_i64p, c_i64, ctypes.c_double,
+                                      c_i64p, c_i64, ctypes.c_uint64, ctypes.c_int, c_dp, c_size, c_dp, c_size,
+                                      c_i64p, c_i64p, ctypes.POINTER(TTAmenStats), c_dp]),
+    "ttamen_dmrg_mv_workspace_size": (c_size, [ctypes.c_int, c_i64p, c_i64p, c_i64p, c_i64p, c_i64p, c_i64p,
+                                               c_i64]),
+    "ttamen_dmrg_mv_output_elems": (c_size, [ctypes.c_int, c_i64p, c_i64p, c_i64p, c_i64]),
     "ttamen_local_matvec_f32": (ctypes.c_int, [c_dp, c_dp, c_dp, c_dp, c_dp] + [c_i64] * 8 + [c_dp, c_size, c_dp]),
     "ttamen_local_matvec_bf16": (ctypes.c_int, [c_dp, c_dp, c_dp, c_dp, c_dp] + [c_i64] * 8 + [c_dp, c_size, c_dp]),
     "ttamen_local_matvec_workspace_size": (c_size, [c_i64] * 8),

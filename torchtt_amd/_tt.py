@@ -251,6 +251,15 @@ class TT:
                 cores.append(c.reshape(a.shape[0] * b.shape[0], a.shape[1], a.shape[3] * b.shape[2]))
         return TT(cores)
 
+    def fast_matvec(self, other, eps=1e-12, initial=None, nswp=20, verb=False, use_cpp=True):
+        """A @ x by DMRG sweeps on the GPU (reference torchtt/_tt_base.py:847-873)."""
+        if not isinstance(other, TT):
+            raise InvalidArguments('Second operand has to be TT object.')
+        if not self._is_ttm or other.is_ttm:
+            raise IncompatibleTypes('First operand should be a TT matrix and second a TT vector.')
+        from ._dmrg import dmrg_matvec
+        return dmrg_matvec(self, other, y0=initial, eps=eps, verb=verb, nswp=nswp, use_cpp=use_cpp)
+
     def __pow__(self, other):
         """Kronecker product (reference _tt_base.py:1341)."""
         if not isinstance(other, TT) or self._is_ttm != other._is_ttm:

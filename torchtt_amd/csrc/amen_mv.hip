@@ -29,23 +29,6 @@ namespace {
 
 int grid_n(int64_t n) { return (int)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 4096)); }
 
-// rank_chop of cpp/ortho.h:16-44 / torchtt/_decomposition.py:289-324: the smallest rank whose
-// discarded tail has 2-norm below eps; 1 for a zero vector, n - 1 for eps <= 0.
-int rank_chop(const double* s, int n, double eps) {
-  double nrm = 0.0;
-  for (int i = 0; i < n; ++i) nrm += s[i] * s[i];
-  int r = n - 1;
-  if (std::sqrt(nrm) == 0.0) return 1;
-  if (eps <= 0.0) return r;
-  while (r > 0) {
-    double sum = 0.0;
-    for (int k = r; k < n; ++k) sum += s[k] * s[k];
-    if (sum >= eps * eps) break;
-    --r;
-  }
-  return r + 1;
-}
-
 // D[a][(m,n')][(k,n)][A] = A[a][m][k][A] if n == n' (the operator A (x) I_N of amen_mm).
 struct KKronEye {
   struct Args { const double* A; double* D; int ra, M, K, N, rA; };

@@ -79,12 +79,10 @@ size_t local_matvec_ws(const LocalOp& op, int nvec) {
 // Phi forward (reference amen_solve.h:58-68, 'lsr,lML,sMNS,rNR->LSR'):
 //   t2[l][(M,S)][R] = stages 1-2 of the local operator with PhiL = Phi, x = xR
 //   Pn[L][(S,R)]    = sum_{(l,M)} xL[(l,M)][L] t2[(l,M)][(S,R)]
-void phi_fwd_A_rect(const double* Phi, const double* xL, const double* Apm, const double* xR,
-                    double* Pn, int l, int s, int r, int nm, int nk, int S, int L, int R,
-                    Arena& ws, hipStream_t st) {
+void phi_fwd_half_rect(const double* Phi, const double* Apm, const double* xR, double* t2, int l, int s,
+                       int r, int nm, int nk, int S, int R, Arena& ws, hipStream_t st) {
   size_t mk = ws.mark();
   double* t1 = ws.alloc<double>((size_t)l * s * nk * R);
-  double* t2 = ws.alloc<double>((size_t)l * nm * S * R);
   {
     GemmDesc g;
     g.M = l; g.N = nk * R; g.K = r; g.B1 = 1; g.B2 = s;
@@ -101,6 +99,15 @@ void phi_fwd_A_rect(const double* Phi, const double* xL, const double* Apm, cons
     g.c.p = t2; g.c.r = R; g.c.c = 1; g.c.b1 = (int64_t)nm * S * R;
     gemm_f64(g, ws, st);
   }
+  ws.release(mk);
+}
+
+void phi_fwd_A_rect(const double* Phi, const double* xL, const double* Apm, const double* xR,
+                    double* Pn, int l, int s, int r, int nm, int nk, int S, int L, int R,
+                    Arena& ws, hipStream_t st) {
+  size_t mk = ws.mark();
+  double* t2 = ws.alloc<double>((size_t)l * nm * S * R);
+  phi_fwd_half_rect(Phi, Apm, xR, t2, l, s, r, nm, nk, S, R, ws, st);
   {
     GemmDesc g;
     g.M = L; g.N = S * R; g.K = l * nm;

@@ -19,7 +19,7 @@ void linalg_init();
 
 // Same rule as the reference's rank_chop (cpp/ortho.h:16-44), including its results for a
 // zero vector (1) and for eps <= 0 (n - 1).
-static int rank_chop_host(const double* s, int n, double eps) {
+int rank_chop(const double* s, int n, double eps) {
   double nrm = 0.0;
   for (int i = 0; i < n; ++i) nrm += s[i] * s[i];
   int r = n - 1;
@@ -75,7 +75,7 @@ static void tt_round_impl(int d, const int64_t* N, std::vector<int64_t>& R, std:
       read_to_host(host, s, sizeof(double) * k, st);
       double nrm = 0.0;
       for (int64_t j = 0; j < k; ++j) nrm += host[j] * host[j];
-      const int rc = rank_chop_host(host, (int)k, eps * std::sqrt(nrm));
+      const int rc = rank_chop(host, (int)k, eps * std::sqrt(nrm));
       rnew = std::min<int64_t>(rmax, rc);
     }
     // core_{i-1} <- core_{i-1} ((R[i-1] N[i-1]) x m) * U[:, :rnew] diag(s[:rnew])

@@ -201,6 +201,10 @@ int qr(const double* A, int64_t sr, int64_t sc, int64_t m, int64_t n, double* Q,
 // row-major, ld k), k = min(m, n). Singular values descending.
 int svd(const double* M, int64_t sr, int64_t sc, int64_t m, int64_t n, double* U, double* s,
         double* V, Arena& ws, hipStream_t st);
+// The reference's truncation rule (cpp/ortho.h:16-44, torchtt/_decomposition.py:289-324): the
+// smallest rank whose discarded tail of the descending singular values s[0..n) has 2-norm below
+// eps; 1 for a zero vector, n - 1 for eps <= 0. Host arrays.
+int rank_chop(const double* s, int n, double eps);
 // Solve B x = rhs for the augmented n x (n+1) row-major matrix Aug = [B | rhs]
 // (overwritten) by blocked LU with partial pivoting; x receives the solution.
 void lu_solve_aug(double* Aug, int n, double* x, Arena& ws, hipStream_t st);
@@ -250,6 +254,10 @@ void phi_fwd_A_rect(const double* Phi, const double* xL, const double* Apm, cons
 void phi_bck_A_rect(const double* Phi, const double* xL, const double* A, const double* xR,
                     double* Pp, int l, int s, int r, int nm, int nk, int S, int L, int R,
                     Arena& ws, hipStream_t st);
+// The forward update before its contraction with xL: t2[l][M][S][R] = sum Phi[l,s,r]
+// A[s,M,N,S] xR[r,N,R] (l x nm x S x R; Apm from permute_core_A_rect).
+void phi_fwd_half_rect(const double* Phi, const double* Apm, const double* xR, double* t2, int l, int s,
+                       int r, int nm, int nk, int S, int R, Arena& ws, hipStream_t st);
 // fwd rhs: Pn[B,R] = sum Pb[b,r] bc[b,n,B] x[r,n,R]
 void phi_fwd_rhs(const double* Pb, const double* bc, const double* x, double* Pn, int b, int r,
                  int n, int B, int R, Arena& ws, hipStream_t st);
