@@ -360,6 +360,31 @@ size_t ttamen_phi_workspace_size(int64_t l, int64_t s, int64_t r, int64_t n, int
                                  int64_t L, int64_t R);
 
 /*
+ * Local right-hand side rhs[r,n,R] = scale * sum Pb_L[b,r] bc[b,n,B] Pb_R[B,R] (reference
+ * amen_solve.h:303-304, scale = the sweep's nrmsc). Workspace: ttamen_phi_workspace_size(b, 1, r,
+ * n, 1, B, R) suffices.
+ */
+int ttamen_rhs_assemble(const double* PL, const double* bc, const double* PR, double* out, int64_t b,
+                        int64_t r, int64_t n, int64_t B, int64_t R, double scale, void* workspace,
+                        size_t workspace_bytes, void* stream);
+/*
+ * Restarted GMRES on the local operator B = PhiL x A x PhiR of ttamen_local_matvec (square: l = r1,
+ * L = r2). Replaces gmres<T> (cpp/gmres.h:15-161) as amen_solve.h:349 calls it: x = B^-1 rhs from
+ * x = 0, local_iterations (<= 127) Arnoldi steps per cycle, at most `resets` cycles, stopping once
+ * |beta_k+1| <= tol ||rhs||. Classical Gram-Schmidt with selective re-orthogonalisation, Givens
+ * rotations on the device; iters_out / flag_out (1 = converged) as the reference's gmres returns
+ * them.
+ */
+int ttamen_gmres_solve(const double* PhiL, const double* A, const double* PhiR, const double* rhs, double* x,
+                       int64_t r1, int64_t s, int64_t n, int64_t S, int64_t r2, double tol,
+                       int64_t local_iterations, int64_t resets, void* workspace, size_t workspace_bytes,
+                       int64_t* iters_out, int* flag_out, void* stream);
+size_t ttamen_gmres_solve_workspace_size(int64_t r1, int64_t s, int64_t n, int64_t S, int64_t r2,
+                                         int64_t local_iterations);
+/* One line describing the current device (name, architecture, CUs, LDS per block, HBM) into buf. */
+int ttamen_device_info(char* buf, size_t len);
+
+/*
  * Thin QR of a row-major m x n matrix (FP64): Q m x k (k = min(m,n)), R k x n, A = Q R,
  * Q^T Q = I. Replaces at::linalg_qr in cpp/amen_solve.h:242,254,407,437,454 and
  * cpp/ortho.h:59. R is not triangular in general (any orthonormal Q is accepted by the

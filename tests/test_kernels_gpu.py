@@ -207,6 +207,78 @@ def test_phi_rhs_updates(dims):
     assert _rel(ob.cpu().numpy(), O.phi_bck_rhs(pr.numpy(), bc.numpy(), x.numpy())) < 1e-13
 
 
+@pytest.mark.parametrize("dims", [(3, 7, 16, 4, 5), (1, 1, 5, 1, 1)])
+def test_rhs_assemble(dims):
+    """rhs[r,n,R] = scale * sum PL[b,r] bc[b,n,B] PR[B,R] ('br,bnB,BR->rnR', amen_solve.h:303-304)."""
+    L, lib = _lib()
+    b, r, n, B, R = dims
+    _say("rhs_assemble", dims)
+    g = torch.Generator().manual_seed(8)
+    pl = torch.randn(b, r, dtype=torch.float64, generator=g)
+    bc = torch.randn(b, n, B, dtype=torch.float64, generator=g)
+    pr = torch.randn(B, R, dtype=torch.float64, generator=g)
+    d = [v.to(DEV) for v in (pl, bc, pr)]
+    out = torch.empty(r, n, R, dtype=torch.float64, device=DEV)
+    ws = _ws(64 << 20)
+    L.check("rhs_assemble", lib.ttamen_rhs_assemble(_p(d[0]), _p(d[1]), _p(d[2]), _p(out), b, r, n, B, R, 0.7,
+                                                   _p(ws), ws.numel(), _stream()))
+    torch.cuda.synchronize()
+    ref = 0.7 * np.tensordot(np.tensordot(pl.numpy(), bc.numpy(), axes=([0], [0])), pr.numpy(), axes=([2], [0]))
+    assert _rel(out.cpu().numpy(), ref) < 1e-13
+
+
+def _gmres_problem(seed, r1, s, n, S, r2):
+    """A local operator dominated by I (x) (4I - tridiag) (x) I, so that GMRES converges."""
+    g = np.random.default_rng(seed)
+    PhiL = 0.05 * g.standard_normal((r1, s, r1))
+    PhiL[:, 0, :] += np.eye(r1)
+    PhiR = 0.05 * g.standard_normal((r2, S, r2))
+    PhiR[:, 0, :] += np.eye(r2)
+    A = 0.05 * g.standard_normal((s, n, n, S))
+    A[0, :, :, 0] += 4 * np.eye(n) - np.eye(n, k=1) - np.eye(n, k=-1)
+    rhs = g.standard_normal((r1, n, r2))
+    return PhiL, A, PhiR, rhs
+
+
+@pytest.mark.parametrize("name,m,resets,tol", [("converges", 30, 3, 1e-12), ("capped", 5, 2, 1e-14)])
+def test_gmres_solve(name, m, resets, tol):
+    """Device GMRES against the oracle's restatement of gmres.h (same Krylov spaces: equal flag,
+    iteration count within one per cycle, equal solutions) and, converged, against a dense solve."""
+    L, lib = _lib()
+    r1, s, n, S, r2 = 6, 2, 8, 2, 5
+    N = r1 * n * r2
+    PhiL, A, PhiR, rhs = _gmres_problem(4, r1, s, n, S, r2)
+    mv = lambda v: O.local_product(PhiR, PhiL, A, v.reshape(r1, n, r2)).reshape(-1)
+    xo, flag_o, it_o = O.gmres(mv, rhs.reshape(-1), np.zeros(N), m, tol, resets)
+    _say("gmres_solve %s: oracle flag %d iters %d" % (name, flag_o, it_o))
+    d = [torch.tensor(v, device=DEV) for v in (PhiL, A, PhiR, rhs)]
+    x = torch.empty(N, dtype=torch.float64, device=DEV)
+    ws = _ws(lib.ttamen_gmres_solve_workspace_size(r1, s, n, S, r2, m))
+    it = ctypes.c_int64(0)
+    flag = ctypes.c_int(0)
+    L.check("gmres_solve", lib.ttamen_gmres_solve(_p(d[0]), _p(d[1]), _p(d[2]), _p(d[3]), _p(x), r1, s, n, S, r2,
+                                                 tol, m, resets, _p(ws), ws.numel(), ctypes.byref(it),
+                                                 ctypes.byref(flag), _stream()))
+    torch.cuda.synchronize()
+    xg = x.cpu().numpy()
+    _say("  gpu flag %d iters %d, diff vs oracle %.3e" % (flag.value, it.value, _rel(xg, xo)))
+    assert flag.value == flag_o
+    assert abs(it.value - it_o) <= resets
+    assert _rel(xg, xo) < 1e-10
+    if flag_o == 1:
+        Bd = np.stack([mv(e) for e in np.eye(N)], axis=1)
+        assert _rel(xg, np.linalg.solve(Bd, rhs.reshape(-1))) < 1e-10
+
+
+def test_device_info():
+    L, lib = _lib()
+    buf = ctypes.create_string_buffer(256)
+    L.check("device_info", lib.ttamen_device_info(buf, 256))
+    txt = buf.value.decode()
+    _say("device_info:", txt, "|", lib.ttamen_version().decode())
+    assert "CUs" in txt and "gfx" in txt
+
+
 # ----------------------------------------------------------------- QR / SVD
 def _qr_case(kind, m, n, seed):
     return qr_case(kind, m, n, seed)

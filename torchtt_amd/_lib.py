@@ -114,6 +114,11 @@ _SIGNATURES = {
     "ttamen_dmrg_mv_workspace_size": (c_size, [ctypes.c_int, c_i64p, c_i64p, c_i64p, c_i64p, c_i64p, c_i64p,
                                                c_i64]),
     "ttamen_dmrg_mv_output_elems": (c_size, [ctypes.c_int, c_i64p, c_i64p, c_i64p, c_i64]),
+    "ttamen_rhs_assemble": (ctypes.c_int, [c_dp] * 4 + [c_i64] * 5 + [ctypes.c_double, c_dp, c_size, c_dp]),
+    "ttamen_gmres_solve": (ctypes.c_int, [c_dp] * 5 + [c_i64] * 5 + [ctypes.c_double, c_i64, c_i64, c_dp, c_size,
+                                          c_i64p, ctypes.POINTER(ctypes.c_int), c_dp]),
+    "ttamen_gmres_solve_workspace_size": (c_size, [c_i64] * 6),
+    "ttamen_device_info": (ctypes.c_int, [ctypes.c_char_p, c_size]),
     "ttamen_local_matvec_f32": (ctypes.c_int, [c_dp, c_dp, c_dp, c_dp, c_dp] + [c_i64] * 8 + [c_dp, c_size, c_dp]),
     "ttamen_local_matvec_bf16": (ctypes.c_int, [c_dp, c_dp, c_dp, c_dp, c_dp] + [c_i64] * 8 + [c_dp, c_size, c_dp]),
     "ttamen_local_matvec_workspace_size": (c_size, [c_i64] * 8),

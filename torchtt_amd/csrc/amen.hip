@@ -1792,6 +1792,93 @@ int ttamen_profile_read(double* out4) {  // 8 values
   });
 }
 
+// Local right-hand side rhs[r,n,R] = scale * sum PL[b,r] bc[b,n,B] PR[B,R]
+// (reference amen_solve.h:303-304).
+int ttamen_rhs_assemble(const double* PL, const double* bc, const double* PR, double* out, int64_t b,
+                        int64_t r, int64_t n, int64_t B, int64_t R, double scale, void* workspace,
+                        size_t workspace_bytes, void* stream) {
+  return guarded([&] {
+    TT_CHECK(PL && bc && PR && out && workspace, ERR_INVALID, "null argument");
+    TT_CHECK(b > 0 && r > 0 && n > 0 && B > 0 && R > 0, ERR_INVALID, "sizes must be positive");
+    Arena ws(workspace, workspace_bytes);
+    hipStream_t st = (hipStream_t)stream;
+    double* slot = ws.alloc<double>(1);
+    fill(slot, 1, scale, st);
+    assemble_rhs(PL, bc, PR, out, (int)b, (int)r, (int)n, (int)B, (int)R, slot, ws, st);
+  });
+}
+
+size_t ttamen_gmres_solve_workspace_size(int64_t r1, int64_t s, int64_t n, int64_t S, int64_t r2,
+                                         int64_t local_iterations) {
+  if (r1 <= 0 || s <= 0 || n <= 0 || S <= 0 || r2 <= 0 || local_iterations <= 0) return 0;
+  const int64_t N = r1 * n * r2;
+  Arena a;
+  a.alloc<double>(s * n * n * S);
+  a.alloc<double>(8);
+  a.alloc<double>(kRedBlocks * 64 + 64);
+  a.alloc<unsigned>(64);
+  gmres_alloc(a, (int)local_iterations, N, sizeof(double));
+  LocalOp op;
+  op.l1 = (int)r1; op.s = (int)s; op.r1 = (int)r1; op.n = (int)n; op.S = (int)S;
+  op.l2 = (int)r2; op.r2 = (int)r2;
+  return a.peak() + local_matvec_ws(op, 1) + 4096;
+}
+
+// Restarted GMRES on the local operator B = PhiL x A x PhiR (reference gmres<T>, cpp/gmres.h:15-161,
+// as amen_solve.h:349 calls it): x = B^-1 rhs from x = 0, `local_iterations` Arnoldi steps per
+// cycle, at most `resets` cycles, stopping once |beta_k+1| <= tol ||rhs||. The Krylov basis, the
+// Hessenberg matrix and the Givens rotations stay on the device.
+int ttamen_gmres_solve(const double* PhiL, const double* A, const double* PhiR, const double* rhs, double* x,
+                       int64_t r1, int64_t s, int64_t n, int64_t S, int64_t r2, double tol,
+                       int64_t local_iterations, int64_t resets, void* workspace, size_t workspace_bytes,
+                       int64_t* iters_out, int* flag_out, void* stream) {
+  return guarded([&] {
+    TT_CHECK(PhiL && A && PhiR && rhs && x && workspace, ERR_INVALID, "null argument");
+    TT_CHECK(r1 > 0 && s > 0 && n > 0 && S > 0 && r2 > 0, ERR_INVALID, "sizes must be positive");
+    TT_CHECK(local_iterations >= 1 && local_iterations + 1 <= kGmresMaxCols && resets >= 1 && tol >= 0,
+             ERR_INVALID, "local_iterations must lie in [1, " + std::to_string(kGmresMaxCols - 1) + "], resets >= 1");
+    const int64_t N = r1 * n * r2;
+    Arena ws(workspace, workspace_bytes);
+    hipStream_t st = (hipStream_t)stream;
+    double* Apm = ws.alloc<double>(s * n * n * S);
+    permute_core_A(A, Apm, (int)s, (int)n, (int)S, st);
+    double* slot = ws.alloc<double>(8);
+    RedScratch rs;
+    rs.partial = ws.alloc<double>(kRedBlocks * 64 + 64);
+    rs.counter = ws.alloc<unsigned>(64);
+    zero_words(rs.counter, 64, st);
+    LocalOp op;
+    op.PhiL = PhiL; op.Apm = Apm; op.PhiR = PhiR;
+    op.l1 = (int)r1; op.s = (int)s; op.r1 = (int)r1; op.n = (int)n; op.S = (int)S;
+    op.l2 = (int)r2; op.r2 = (int)r2;
+    fill(x, N, 0.0, st);
+    norm2(rhs, N, slot, rs, st, true);  // ||rhs|| (1 for a zero rhs: x = 0 is then returned at once)
+    GmresDev g = gmres_alloc(ws, (int)local_iterations, N, sizeof(double));
+    GmresOp<double> gop;
+    gop.apply = [&](const double* xin, double* yout, const int* skip) { local_matvec(op, xin, yout, 1, ws, st, skip); };
+    gop.axpby = [&](double a, const double* xx, double b, double* yy, int64_t nn) { axpby(a, xx, b, yy, nn, st); };
+    gop.norm = [&](const double* xx, int64_t nn, double* out) { norm2(xx, nn, out, rs, st); };
+    GmresResult gr = gmres_run<double>(g, N, x, rhs, slot, slot, tol, (int)resets, gop, rs, host_sync(), st);
+    TT_HIP(hipStreamSynchronize(st));
+    if (iters_out) *iters_out = gr.iters;
+    if (flag_out) *flag_out = gr.flag;
+  });
+}
+
+// One line describing the current device: name, architecture, compute units, LDS per block, HBM.
+int ttamen_device_info(char* buf, size_t len) {
+  return guarded([&] {
+    TT_CHECK(buf && len > 0, ERR_INVALID, "null argument");
+    int dev = 0;
+    TT_HIP(hipGetDevice(&dev));
+    hipDeviceProp_t p;
+    TT_HIP(hipGetDeviceProperties(&p, dev));
+    snprintf(buf, len, "device %d: %s (%s), %d CUs, %zu KiB LDS per block, %.1f GiB HBM", dev, p.name,
+             p.gcnArchName, p.multiProcessorCount, (size_t)p.sharedMemPerBlock / 1024,
+             (double)p.totalGlobalMem / (1024.0 * 1024.0 * 1024.0));
+  });
+}
+
 int ttamen_gemm_f64(int64_t M, int64_t N, int64_t K, int64_t batch, double alpha, const double* A,
                     int64_t sar, int64_t sac, int64_t sab, const double* B, int64_t sbr,
                     int64_t sbc, int64_t sbb, double beta, double* C, int64_t scr, int64_t scc,
