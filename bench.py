@@ -64,6 +64,8 @@ def parse():
     p.add_argument("--duration", type=float, default=2.0)
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-threads", type=int, default=0)
+    p.add_argument("--autotune", type=int, default=0, choices=[0, 1],
+                   help="GEMM tile autotuner for single-system solves (1: tune each shape bucket once)")
     a = p.parse_args()
     if a.precision is None:
         a.precision = "fp64" if a.workload == "potential" else "fp32"
@@ -397,9 +399,16 @@ def main():
     from torchtt_amd import _lib
     lib = _lib.load()
     _lib.check("cohort_set_params", lib.ttamen_cohort_set_params(128, 64, 8, args.lanes, 15))
+    lib.ttamen_gemm_autotune(args.autotune)
     fn = {"batch": run_batch, "potential": run_potential, "precision": run_precision,
           "serve": run_serve, "solo": run_solo}[args.workload]
-    print(json.dumps(fn(args, tt, W, lib, _lib)), flush=True)
+    out = fn(args, tt, W, lib, _lib)
+    if args.autotune:
+        st = (ctypes.c_double * 4)()
+        _lib.check("gemm_autotune_stats", lib.ttamen_gemm_autotune_stats(st))
+        out["gemm_autotune"] = {"buckets_tuned": st[0], "launches_from_cache": st[1], "tuning_s": st[2],
+                                "launches_off_rule": st[3]}
+    print(json.dumps(out), flush=True)
 
 
 if __name__ == "__main__":

@@ -381,6 +381,21 @@ int ttamen_gmres_solve(const double* PhiL, const double* A, const double* PhiR, 
                        int64_t* iters_out, int* flag_out, void* stream);
 size_t ttamen_gmres_solve_workspace_size(int64_t r1, int64_t s, int64_t n, int64_t S, int64_t r2,
                                          int64_t local_iterations);
+/*
+ * GEMM tile autotuner (process-wide; no counterpart in the reference, whose contractions are ATen
+ * tensordots). Mode 0 (default): the tile of a single-system GEMM follows a fixed rule (32x32 when a
+ * 64x64 grid would give under 4 workgroups per CU, else 64x64). Mode 1: the first launch of each
+ * shape bucket (element type, ceil(log2) of M, N, K and batch, unit-stride operand axes) times the
+ * four tile shapes on its own operands into a scratch output of the workspace (skipped when the
+ * workspace lacks the room) and the fastest serves every later launch of the bucket. Batched and
+ * served solves keep their per-item rule. Results can differ in the last bits between modes (the
+ * split-K plan follows the tile). ttamen_gemm_autotune(-1) returns the mode without changing it;
+ * otherwise the previous mode is returned. stats: buckets tuned, launches served from the cache,
+ * seconds spent tuning, served launches whose tile differs from the rule.
+ */
+int ttamen_gemm_autotune(int mode);
+int ttamen_gemm_autotune_stats(double* out4);
+void ttamen_gemm_autotune_clear(void);
 /* One line describing the current device (name, architecture, CUs, LDS per block, HBM) into buf. */
 int ttamen_device_info(char* buf, size_t len);
 

@@ -270,6 +270,45 @@ def test_gmres_solve(name, m, resets, tol):
         assert _rel(xg, np.linalg.solve(Bd, rhs.reshape(-1))) < 1e-10
 
 
+def test_gemm_autotune():
+    """Tuned tiles compute the same products; a solve with the tuner on matches one with it off
+    (same ranks, solutions within the solve tolerance)."""
+    import torchtt_amd as tt
+    L, lib = _lib()
+    prev = tt.solvers.gemm_autotune(1)
+    try:
+        lib.ttamen_gemm_autotune_clear()
+        g = torch.Generator().manual_seed(9)
+        ws = _ws(256 << 20)
+        for (M, N, K, batch) in [(240, 59, 240, 59), (3540, 59, 236, 1), (59, 3540, 59, 4), (56, 56, 5120, 1)]:
+            A = torch.randn(batch, M, K, dtype=torch.float64, generator=g).to(DEV)
+            B = torch.randn(batch, K, N, dtype=torch.float64, generator=g).to(DEV)
+            C = torch.empty(batch, M, N, dtype=torch.float64, device=DEV)
+            for _ in range(2):  # tune, then served from the cache
+                L.check("gemm_f64", lib.ttamen_gemm_f64(M, N, K, batch, ctypes.c_double(1.0), _p(A), K, 1, M * K,
+                                                        _p(B), N, 1, K * N, ctypes.c_double(0.0), _p(C), N, 1,
+                                                        M * N, _p(ws), ws.numel(), _stream()))
+                torch.cuda.synchronize()
+                assert float((C - torch.bmm(A, B)).abs().max() / torch.bmm(A, B).abs().max()) < 1e-13
+        st = tt.solvers.gemm_autotune_stats()
+        _say("autotune stats", st)
+        assert st["buckets_tuned"] == 4 and st["launches_from_cache"] >= 4
+        from torchtt_amd import workloads as W
+        A, b = W.laplace_potential(5, 16, seed=1)
+        Ag, bg = tt.TT([c.to(DEV) for c in A]), tt.TT([c.to(DEV) for c in b])
+        torch.manual_seed(0)
+        x1 = tt.solvers.amen_solve(Ag, bg, eps=1e-10, rmax=30, verbose=False)
+        tt.solvers.gemm_autotune(0)
+        torch.manual_seed(0)
+        x0 = tt.solvers.amen_solve(Ag, bg, eps=1e-10, rmax=30, verbose=False)
+        _say("autotune solve: ranks %s / %s diff %.3e" % (list(x1.R), list(x0.R), float((x1 - x0).norm() / x0.norm())))
+        assert list(x1.R) == list(x0.R)
+        assert float((x1 - x0).norm() / x0.norm()) < 1e-8
+    finally:
+        tt.solvers.gemm_autotune(prev)
+        lib.ttamen_gemm_autotune_clear()
+
+
 def test_device_info():
     L, lib = _lib()
     buf = ctypes.create_string_buffer(256)

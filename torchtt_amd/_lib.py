@@ -119,6 +119,9 @@ _SIGNATURES = {
                                           c_i64p, ctypes.POINTER(ctypes.c_int), c_dp]),
     "ttamen_gmres_solve_workspace_size": (c_size, [c_i64] * 6),
     "ttamen_device_info": (ctypes.c_int, [ctypes.c_char_p, c_size]),
+    "ttamen_gemm_autotune": (ctypes.c_int, [ctypes.c_int]),
+    "ttamen_gemm_autotune_stats": (ctypes.c_int, [ctypes.POINTER(ctypes.c_double)]),
+    "ttamen_gemm_autotune_clear": (None, []),
     "ttamen_local_matvec_f32": (ctypes.c_int, [c_dp, c_dp, c_dp, c_dp, c_dp] + [c_i64] * 8 + [c_dp, c_size, c_dp]),
     "ttamen_local_matvec_bf16": (ctypes.c_int, [c_dp, c_dp, c_dp, c_dp, c_dp] + [c_i64] * 8 + [c_dp, c_size, c_dp]),
     "ttamen_local_matvec_workspace_size": (c_size, [c_i64] * 8),

@@ -1879,6 +1879,23 @@ int ttamen_device_info(char* buf, size_t len) {
   });
 }
 
+int ttamen_gemm_autotune(int mode) {
+  if (mode < -1 || mode > 1) {
+    set_last_error("autotune mode must be -1 (query), 0 (rule) or 1 (tune)");
+    return -ERR_INVALID;
+  }
+  return gemm_autotune_mode(mode);
+}
+
+int ttamen_gemm_autotune_stats(double* out4) {
+  return guarded([&] {
+    TT_CHECK(out4, ERR_INVALID, "null argument");
+    gemm_autotune_stats(out4);
+  });
+}
+
+void ttamen_gemm_autotune_clear(void) { gemm_autotune_clear(); }
+
 int ttamen_gemm_f64(int64_t M, int64_t N, int64_t K, int64_t batch, double alpha, const double* A,
                     int64_t sar, int64_t sac, int64_t sab, const double* B, int64_t sbr,
                     int64_t sbc, int64_t sbb, double beta, double* C, int64_t scr, int64_t scc,

@@ -20,8 +20,56 @@
 #include "ttamen_internal.h"
 #include "kernels_gemm.h"
 #include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <mutex>
+#include <unordered_map>
 
 namespace ttamen {
+
+// ---------------------------------------------------------------- tile autotuner
+// Shape buckets: element type, ceil(log2) of M, N, K and the batch count, and which operand axes
+// have unit stride. The first launch of a bucket (autotune mode 1, outside cohorts, a workspace
+// with room for a scratch output and no device skip flag) runs each of the four tile shapes, with
+// its own split-K plan, once to warm up and three times under HIP events, writing a scratch
+// output; the fastest is cached for the process and used for every later launch of the bucket.
+// Tile shape does not change a result; the split-K plan it implies can (the order of the K sums).
+namespace {
+std::atomic<int> g_tune_mode{0};
+std::mutex g_tune_mu;
+std::unordered_map<uint64_t, int> g_tune_cache;  // bucket -> tile index (0..3)
+double g_tune_stats[4] = {0, 0, 0, 0};  // buckets tuned, launches served, seconds tuning, differ from rule
+
+int lg2(int64_t v) {
+  int l = 0;
+  while ((int64_t(1) << l) < v && l < 40) ++l;
+  return l;
+}
+uint64_t bucket_key(const GemmDesc& g, int type) {
+  uint64_t k = (uint64_t)type;
+  k = k * 64 + lg2(g.M);
+  k = k * 64 + lg2(g.N);
+  k = k * 64 + lg2(std::max(g.K, 1));
+  k = k * 64 + lg2((int64_t)g.B1 * g.B2);
+  k = k * 4 + (g.a.c == 1 ? 2 : 0) + (g.b.c == 1 ? 1 : 0);
+  return k;
+}
+}  // namespace
+
+int gemm_autotune_mode(int mode) {
+  const int prev = g_tune_mode.load();
+  if (mode >= 0) g_tune_mode.store(mode);
+  return prev;
+}
+void gemm_autotune_stats(double* out4) {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  for (int i = 0; i < 4; ++i) out4[i] = g_tune_stats[i];
+}
+void gemm_autotune_clear() {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  g_tune_cache.clear();
+  for (double& v : g_tune_stats) v = 0;
+}
 
 template <typename T, int BM, int BN>
 static void launch_tile(GemmParams<T>& p, int splits, hipStream_t st, bool bf16) {
@@ -53,8 +101,8 @@ static void gemm_impl(const GemmDesc& g, Arena& ws, hipStream_t st, bool bf16 = 
   // share each launch and already fill the device, so the larger tiles stay.
   const int64_t tiles64 = (int64_t)((g.M + 63) / 64) * ((g.N + 63) / 64) * p.B;
   const bool small_grid = tiles64 < 1024 && current_recorder() == nullptr;
-  const int BM = (g.M <= 32 || small_grid) ? 32 : 64;
-  const int BN = (g.N <= 32 || small_grid) ? 32 : 64;
+  int BM = (g.M <= 32 || small_grid) ? 32 : 64;
+  int BN = (g.N <= 32 || small_grid) ? 32 : 64;
   // Split K when the output grid is too small to fill the 256 CUs and K is long.
   auto split_plan = [&](int bm, int bn, int64_t* kchunk_out) {
     const int tiles = ((g.M + bm - 1) / bm) * ((g.N + bn - 1) / bn) * p.B;
@@ -72,6 +120,86 @@ static void gemm_impl(const GemmDesc& g, Arena& ws, hipStream_t st, bool bf16 = 
     *kchunk_out = kchunk;
     return (int)((g.K + kchunk - 1) / kchunk);
   };
+  const bool tunable = !ws.measuring() && current_recorder() == nullptr && g.skip == nullptr && g.K > 0 &&
+                       g_tune_mode.load() == 1;
+  if (tunable) {
+    const uint64_t key = bucket_key(g, bf16 ? 2 : (sizeof(T) == 8 ? 0 : 1));
+    int choice = -1;
+    {
+      std::lock_guard<std::mutex> lk(g_tune_mu);
+      auto it = g_tune_cache.find(key);
+      if (it != g_tune_cache.end()) {
+        choice = it->second;
+        g_tune_stats[1] += 1;
+      }
+    }
+    if (choice < 0) {
+      // time the four tiles into a scratch output (row-major, batch-contiguous)
+      const size_t out_elems = (size_t)p.B * g.M * g.N;
+      size_t need = ((out_elems * sizeof(T) + 255) & ~size_t(255));
+      for (int t = 0; t < 4; ++t) {
+        int64_t kc;
+        const int sp = split_plan(t & 2 ? 64 : 32, t & 1 ? 64 : 32, &kc);
+        if (sp > 1) need = std::max(need, ((out_elems * sizeof(T) + 255) & ~size_t(255)) +
+                                              (((size_t)sp * out_elems * sizeof(T) + 255) & ~size_t(255)));
+      }
+      if (ws.remaining() >= need + 1024) {
+        const auto t0 = std::chrono::steady_clock::now();
+        size_t mk0 = ws.mark();
+        GemmParams<T> q = p;
+        T* cs = ws.alloc<T>(out_elems);
+        q.c = cs; q.cr = g.N; q.cc = 1; q.cb1 = (int64_t)g.B2 * g.M * g.N; q.cb2 = (int64_t)g.M * g.N;
+        q.beta = 0.0;
+        hipEvent_t e0, e1;
+        TT_HIP(hipEventCreate(&e0));
+        TT_HIP(hipEventCreate(&e1));
+        float best_ms = 0;
+        for (int t = 0; t < 4; ++t) {
+          const int bm = t & 2 ? 64 : 32, bn = t & 1 ? 64 : 32;
+          size_t mk1 = ws.mark();
+          const int sp = split_plan(bm, bn, &q.kchunk);
+          q.partial = sp > 1 ? ws.alloc<T>((size_t)sp * out_elems) : nullptr;
+          auto go = [&] {
+            if (bm == 32 && bn == 32) launch_tile<T, 32, 32>(q, sp, st, bf16);
+            else if (bm == 32) launch_tile<T, 32, 64>(q, sp, st, bf16);
+            else if (bn == 32) launch_tile<T, 64, 32>(q, sp, st, bf16);
+            else launch_tile<T, 64, 64>(q, sp, st, bf16);
+            if (sp > 1) {
+              int64_t total = (int64_t)g.M * g.N * q.B;
+              launch<KGemmReduce<T>>(dim3((int)std::min<int64_t>((total + 255) / 256, 4096)), dim3(256), 0, st,
+                                     {q, sp});
+              TT_LAUNCH_CHECK();
+            }
+          };
+          go();
+          TT_HIP(hipEventRecord(e0, st));
+          for (int rep = 0; rep < 3; ++rep) go();
+          TT_HIP(hipEventRecord(e1, st));
+          TT_HIP(hipEventSynchronize(e1));
+          float ms = 0;
+          TT_HIP(hipEventElapsedTime(&ms, e0, e1));
+          if (choice < 0 || ms < best_ms) { choice = t; best_ms = ms; }
+          ws.release(mk1);
+        }
+        TT_HIP(hipEventDestroy(e0));
+        TT_HIP(hipEventDestroy(e1));
+        ws.release(mk0);
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        g_tune_cache[key] = choice;
+        g_tune_stats[0] += 1;
+        g_tune_stats[2] += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+      }
+    }
+    if (choice >= 0) {
+      const int tbm = choice & 2 ? 64 : 32, tbn = choice & 1 ? 64 : 32;
+      if (tbm != BM || tbn != BN) {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        g_tune_stats[3] += 1;
+      }
+      BM = tbm;
+      BN = tbn;
+    }
+  }
   int splits = split_plan(BM, BN, &p.kchunk);
   size_t mk = ws.mark();
   size_t partial_elems = splits > 1 ? (size_t)splits * p.B * g.M * g.N : 0;

@@ -71,6 +71,7 @@ class Arena {
     return (T*)p;
   }
   size_t mark() const { return off_; }
+  size_t remaining() const { return base_ ? cap_ - off_ : ~size_t(0); }
   void release(size_t m) { off_ = m; }
   size_t peak() const { return peak_; }
   bool measuring() const { return base_ == nullptr; }
@@ -107,6 +108,12 @@ struct GemmDesc {
   int scale_mode = 0;
   const int* skip = nullptr;
 };
+
+// Tile autotuner of the single-system GEMMs (outside cohorts): mode 0 uses the fixed rule, mode 1
+// times the four tile shapes on the first launch of each shape bucket and keeps the fastest.
+int gemm_autotune_mode(int mode);  // sets the mode (if mode >= 0), returns the previous one
+void gemm_autotune_stats(double* out4);
+void gemm_autotune_clear();
 
 // Launch a GEMM in FP64 (A, B, C all double). `ws` provides split-K scratch.
 void gemm_f64(const GemmDesc& g, Arena& ws, hipStream_t st);

@@ -340,6 +340,24 @@ def amen_solve_batch(systems, nswp=22, eps=1e-10, max_full=500, kickrank=4, kick
     return result
 
 
+def gemm_autotune(mode=None):
+    """GEMM tile autotuner of single-system solves: mode 1 times the four tile shapes on the first
+    launch of each shape bucket and keeps the fastest, mode 0 (default) uses the fixed rule; None
+    only queries. Returns the previous mode."""
+    lib = _lib.load()
+    return int(lib.ttamen_gemm_autotune(-1 if mode is None else int(mode)))
+
+
+def gemm_autotune_stats():
+    """Buckets tuned, launches served from the cache, seconds spent tuning, launches off the rule."""
+    import ctypes
+    lib = _lib.load()
+    out = (ctypes.c_double * 4)()
+    _lib.check("ttamen_gemm_autotune_stats", lib.ttamen_gemm_autotune_stats(out))
+    return {"buckets_tuned": int(out[0]), "launches_from_cache": int(out[1]), "tuning_s": out[2],
+            "launches_off_rule": int(out[3])}
+
+
 def round_tt(x, eps=1e-12, rmax=sys.maxsize):
     """TT rounding on the GPU (libttamen: ttamen_tt_round), replacing the reference's native
     `round_this` (cpp/compression.h:3-41): left-to-right QR, then right-to-left truncated SVD
