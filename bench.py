@@ -362,6 +362,7 @@ def run_solo(args, tt, W, lib, _lib):
 def run_serve(args, tt, W, lib, _lib):
     dev = torch.device("cuda", 0)
     arrivals = tt.serving.poisson_arrivals(args.rate, args.duration, seed=1)
+    print("serve: building %d requests" % len(arrivals), file=sys.stderr, flush=True)
     systems = W.batch_systems(len(arrivals), seed=7)
     items = [(tt.TT([c.to(dev) for c in s["A"]]), tt.TT([c.to(dev) for c in s["b"]]), s["rmax"]) for s in systems]
     torch.manual_seed(0)
@@ -374,7 +375,14 @@ def run_serve(args, tt, W, lib, _lib):
                                   z0=z0)
     torch.cuda.synchronize()
     total = time.perf_counter() - t0
-    res = np.array([tt.solvers.residual(it[0], x, it[1]) for it, x in zip(items, sols)])
+    print("serve: %d requests served in %.1f s; checking residuals" % (len(items), total), file=sys.stderr,
+          flush=True)
+    res = []
+    for i, (it, x) in enumerate(zip(items, sols)):
+        res.append(tt.solvers.residual(it[0], x, it[1]))
+        if (i + 1) % 2000 == 0:
+            print("serve: %d residuals checked" % (i + 1), file=sys.stderr, flush=True)
+    res = np.array(res)
     ok = int(np.sum(res <= RES_BAR))
     lat = info["latency_s"] * 1e3
     cfg = {"workload": "served AMEn: Poisson arrivals at %g req/s for %g s (%d requests) of systems drawn as in "

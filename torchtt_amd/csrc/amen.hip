@@ -1600,9 +1600,21 @@ int ttamen_serve(ttamen_system* systems, const double* arrival_s, int count, int
       return wait <= 0;
     };
     const int workers = std::max(1, std::min(8, slots));
+    // TTAMEN_SERVE_PROGRESS=n prints a line to stderr after every n completions (long streams)
+    const char* pe = std::getenv("TTAMEN_SERVE_PROGRESS");
+    const long every = pe ? std::max(0L, std::atol(pe)) : 0L;
+    std::atomic<long> completed{0};
     run_cohort(systems, count, nswp, eps, max_full, kickrank, kick2, local_iterations, resets,
                precision, workers, slots, workspace, workspace_bytes, (hipStream_t)stream, ready,
-               [&](int i) { start_s[i] = now_s(); }, [&](int i) { done_s[i] = now_s(); });
+               [&](int i) { start_s[i] = now_s(); },
+               [&](int i) {
+                 done_s[i] = now_s();
+                 const long c = completed.fetch_add(1) + 1;
+                 if (every > 0 && (c % every == 0 || c == count)) {
+                   fprintf(stderr, "serve: %ld of %d requests done at %.1f s\n", c, count, done_s[i]);
+                   fflush(stderr);
+                 }
+               });
   });
 }
 
