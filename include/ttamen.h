@@ -271,6 +271,14 @@ int ttamen_tt_round(int d, const int64_t* N, const int64_t* R, const double* con
                     double eps, int64_t rmax, void* workspace, size_t workspace_bytes, double* out,
                     size_t out_elems, int64_t* r_out, int64_t* offsets_out, void* stream);
 
+/*
+ * Profile of the merged GEMM launches of batched / served solves (no counterpart in the reference):
+ * enable 1 clears and starts it (each merged GEMM launch is bracketed by HIP events on its stream),
+ * 0 stops it, -1 leaves it as it is. out6 (optional): launches timed, their total duration in ms,
+ * the algorithmic flops (2 M N K per batch entry) of their GEMM items in FP64, FP32 and BF16, and
+ * the number of items.
+ */
+int ttamen_cohort_gemm_profile(int enable, double* out6);
 /* Bytes at the front of a batch / serving workspace that hold grouped-launch tables. */
 size_t ttamen_cohort_staging_bytes(void);
 /* Counters of the last batch / serving call (no counterpart in the reference): rounds,

@@ -119,6 +119,7 @@ _SIGNATURES = {
                                           c_i64p, ctypes.POINTER(ctypes.c_int), c_dp]),
     "ttamen_gmres_solve_workspace_size": (c_size, [c_i64] * 6),
     "ttamen_device_info": (ctypes.c_int, [ctypes.c_char_p, c_size]),
+    "ttamen_cohort_gemm_profile": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(ctypes.c_double)]),
     "ttamen_gemm_autotune": (ctypes.c_int, [ctypes.c_int]),
     "ttamen_gemm_autotune_stats": (ctypes.c_int, [ctypes.POINTER(ctypes.c_double)]),
     "ttamen_gemm_autotune_clear": (None, []),

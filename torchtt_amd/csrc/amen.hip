@@ -1891,6 +1891,13 @@ int ttamen_device_info(char* buf, size_t len) {
   });
 }
 
+int ttamen_cohort_gemm_profile(int enable, double* out6) {
+  return guarded([&] {
+    TT_CHECK(enable >= -1 && enable <= 1, ERR_INVALID, "enable must be -1, 0 or 1");
+    cohort_gemm_profile(enable, out6);
+  });
+}
+
 int ttamen_gemm_autotune(int mode) {
   if (mode < -1 || mode > 1) {
     set_last_error("autotune mode must be -1 (query), 0 (rule) or 1 (tune)");

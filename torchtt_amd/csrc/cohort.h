@@ -81,7 +81,9 @@ class Cohort {
   struct Pending {
     GroupFn fn;
     GroupLaunch gl;
+    double flops[3];  // merged GEMM launches: algorithmic flops of the items (FP64, FP32, BF16)
   };
+  std::vector<hipEvent_t> ev_pool_;  // GEMM launch timing (cohort_gemm_profile)
   std::vector<Pending> pending_;
   Counters cnt_;
   bool trace_ = false;  // TTAMEN_COHORT_TRACE set: print a launch histogram at the end
@@ -89,6 +91,11 @@ class Cohort {
   struct Sync;
   std::unique_ptr<Sync> sync_;
 };
+
+// Profile of the merged GEMM launches of every cohort (k_mega_gemm): enable 1 clears and starts
+// it, 0 stops it, -1 leaves it; out6 (optional): launches timed, their total ms (HIP events on
+// the launch's stream), algorithmic flops of their items in FP64, FP32 and BF16, items.
+void cohort_gemm_profile(int enable, double* out6);
 
 // The pinned-host mailbox of the member running on this thread, or of the thread.
 HostSync& member_host_sync();

@@ -2,6 +2,7 @@
 // launches merged into grouped launches, one stream.
 #include "cohort.h"
 #include "gmres.h"
+#include "kernels_gemm.h"
 
 #include <ucontext.h>
 #include <algorithm>
@@ -165,6 +166,7 @@ Cohort::~Cohort() {
   sync_->stop.store(true, std::memory_order_release);
   for (auto& t : sync_->threads) t.join();
   members_.clear();
+  for (hipEvent_t e : ev_pool_) (void)hipEventDestroy(e);
   if (hstage_) (void)hipHostFree(hstage_);
 }
 
@@ -204,13 +206,65 @@ void Cohort::worker_loop(int w) {
   }
 }
 
+namespace {
+struct GemmGroupProfile {
+  std::atomic<bool> on{false};
+  std::mutex mu;
+  double v[6] = {0, 0, 0, 0, 0, 0};  // launches, ms, flops FP64, FP32, BF16, items
+} g_gprof;
+}  // namespace
+
+void cohort_gemm_profile(int enable, double* out6) {
+  std::lock_guard<std::mutex> lk(g_gprof.mu);
+  if (enable == 1)
+    for (double& x : g_gprof.v) x = 0;
+  if (enable >= 0) g_gprof.on.store(enable == 1);
+  if (out6)
+    for (int i = 0; i < 6; ++i) out6[i] = g_gprof.v[i];
+}
+
 void Cohort::flush() {
   if (pending_.empty()) return;
   const auto t0 = std::chrono::steady_clock::now();
   TT_HIP(hipMemcpyAsync(dstage_, hstage_, stage_off_, hipMemcpyHostToDevice, st_));
-  for (auto& p : pending_) p.fn(p.gl);
+  const bool prof = g_gprof.on.load(std::memory_order_relaxed);
+  size_t nev = 0;
+  double fl[3] = {0, 0, 0}, items = 0;
+  for (auto& p : pending_) {
+    const bool timed = prof && p.fn == &mega_gemm_exec;
+    if (timed) {
+      if (ev_pool_.size() < nev + 2) {
+        hipEvent_t e;
+        for (int i = 0; i < 2; ++i) {
+          TT_HIP(hipEventCreate(&e));
+          ev_pool_.push_back(e);
+        }
+      }
+      TT_HIP(hipEventRecord(ev_pool_[nev], st_));
+    }
+    p.fn(p.gl);
+    if (timed) {
+      TT_HIP(hipEventRecord(ev_pool_[nev + 1], st_));
+      nev += 2;
+      for (int i = 0; i < 3; ++i) fl[i] += p.flops[i];
+      items += p.gl.hdr.n;
+    }
+  }
   TT_HIP(hipGetLastError());
   TT_HIP(hipStreamSynchronize(st_));
+  if (nev) {
+    double ms = 0;
+    for (size_t i = 0; i < nev; i += 2) {
+      float t = 0;
+      TT_HIP(hipEventElapsedTime(&t, ev_pool_[i], ev_pool_[i + 1]));
+      ms += t;
+    }
+    std::lock_guard<std::mutex> lk(g_gprof.mu);
+    g_gprof.v[0] += (double)(nev / 2);
+    g_gprof.v[1] += ms;
+    for (int i = 0; i < 3; ++i) g_gprof.v[2 + i] += fl[i];
+    g_gprof.v[5] += items;
+  }
   cnt_.t_gpu += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   pending_.clear();
   stage_off_ = 0;
@@ -283,6 +337,7 @@ void Cohort::dispatch(int max_steps) {
     if (stage_off_ + bytes > stage_cap_) flush();
     Pending p;
     p.fn = best.fn;
+    p.flops[0] = p.flops[1] = p.flops[2] = 0.0;
     GroupLaunch& gl = p.gl;
     gl.hdr.items = dstage_ + stage_off_;
     gl.hdr.stride = (unsigned)stride;
@@ -303,6 +358,18 @@ void Cohort::dispatch(int max_steps) {
       g[2] = op.grid.z;
       g[3] = op.kind;
       std::memcpy(hp + j * stride + 16, m->args.data() + op.off, op.size);
+      if (best.fn == &mega_gemm_exec && (op.kind == 0 || op.kind == 1)) {
+        // algorithmic flops of a GEMM item: 2 M N K per batch entry (split-K slices included once)
+        int dims[5];  // M, N, K, B2, B: the leading ints of GemmParams<double> and <float>
+        std::memcpy(dims, m->args.data() + op.off, sizeof(dims));
+        const double f = 2.0 * dims[0] * dims[1] * dims[2] * dims[4];
+        if (op.kind == 0) p.flops[0] += f;
+        else {
+          GemmParams<float> q;
+          std::memcpy(&q, m->args.data() + op.off, sizeof(q));
+          p.flops[q.tile >= 4 ? 2 : 1] += f;
+        }
+      }
       gl.lds = std::max<size_t>(gl.lds, op.lds);
       m->head += 1;
       m->age = 0;
