@@ -144,43 +144,50 @@ __device__ __forceinline__ void GemmTile<T, BM, BN, BK>::body(const GemmParams<T
   const bool b_nfast = (p.bc == 1);
 
   T ra[AE], rb[BE];
-  auto loadA = [&](int64_t kt) {
+  // The loop-invariant part of every element this thread stages: its address at k = 0, its
+  // k offset within the tile, whether its row (column) exists, and its LDS slot. A K step
+  // then adds one uniform offset (kt times the K stride) to each address.
+  const T* a_at[AE];
+  const T* b_at[BE];
+  int a_k[AE], b_k[BE], a_slot[AE], b_slot[BE];
+  bool a_in[AE], b_in[BE];
 #pragma unroll
-    for (int e = 0; e < AE; ++e) {
-      int idx = tid + 256 * e, i, kk;
-      if (a_kfast) { i = idx / BK; kk = idx - i * BK; }
-      else { kk = idx / BM; i = idx - kk * BM; }
-      int64_t gi = m0 + i, gk = kt + kk;
-      ra[e] = (gi < p.M && gk < k1) ? A[gi * p.ar + gk * p.ac] : T(0);
-    }
+  for (int e = 0; e < AE; ++e) {
+    int idx = tid + 256 * e, i, kk;
+    if (a_kfast) { i = idx / BK; kk = idx - i * BK; }
+    else { kk = idx / BM; i = idx - kk * BM; }
+    a_in[e] = m0 + i < p.M;
+    a_k[e] = kk;
+    a_slot[e] = kk * (BM + PA) + i;
+    a_at[e] = A + (a_in[e] ? (int64_t)(m0 + i) * p.ar : 0) + (int64_t)kk * p.ac;
+  }
+#pragma unroll
+  for (int e = 0; e < BE; ++e) {
+    int idx = tid + 256 * e, j, kk;
+    if (b_nfast) { kk = idx / BN; j = idx - kk * BN; }
+    else { j = idx / BK; kk = idx - j * BK; }
+    b_in[e] = n0 + j < p.N;
+    b_k[e] = kk;
+    b_slot[e] = kk * (BN + PB) + j;
+    b_at[e] = B + (b_in[e] ? (int64_t)(n0 + j) * p.bc : 0) + (int64_t)kk * p.br;
+  }
+  auto loadA = [&](int64_t kt) {
+    const int64_t off = kt * p.ac;
+#pragma unroll
+    for (int e = 0; e < AE; ++e) ra[e] = (a_in[e] && kt + a_k[e] < k1) ? a_at[e][off] : T(0);
   };
   auto storeA = [&](int buf) {
 #pragma unroll
-    for (int e = 0; e < AE; ++e) {
-      int idx = tid + 256 * e, i, kk;
-      if (a_kfast) { i = idx / BK; kk = idx - i * BK; }
-      else { kk = idx / BM; i = idx - kk * BM; }
-      As[buf][kk][i] = ra[e];
-    }
+    for (int e = 0; e < AE; ++e) (&As[buf][0][0])[a_slot[e]] = ra[e];
   };
   auto loadB = [&](int64_t kt) {
+    const int64_t off = kt * p.br;
 #pragma unroll
-    for (int e = 0; e < BE; ++e) {
-      int idx = tid + 256 * e, j, kk;
-      if (b_nfast) { kk = idx / BN; j = idx - kk * BN; }
-      else { j = idx / BK; kk = idx - j * BK; }
-      int64_t gj = n0 + j, gk = kt + kk;
-      rb[e] = (gj < p.N && gk < k1) ? B[gk * p.br + gj * p.bc] : T(0);
-    }
+    for (int e = 0; e < BE; ++e) rb[e] = (b_in[e] && kt + b_k[e] < k1) ? b_at[e][off] : T(0);
   };
   auto storeB = [&](int buf) {
 #pragma unroll
-    for (int e = 0; e < BE; ++e) {
-      int idx = tid + 256 * e, j, kk;
-      if (b_nfast) { kk = idx / BN; j = idx - kk * BN; }
-      else { j = idx / BK; kk = idx - j * BK; }
-      Bs[buf][kk][j] = rb[e];
-    }
+    for (int e = 0; e < BE; ++e) (&Bs[buf][0][0])[b_slot[e]] = rb[e];
   };
 
   acc_t acc[MI][NI];
