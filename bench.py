@@ -29,7 +29,8 @@ import os
 # Hardware queues per process: the batched solver runs 4 cohort lanes on their own streams beside
 # torch's; with the runtime's default of 4 queues two of them would share one and serialize (measured:
 # 4 lanes on 4 queues 20.1 solves/s, on 8 queues 31.1; 3 lanes 30.4 either way).
-os.environ["GPU_MAX_HW_QUEUES"] = "8"
+if int(os.environ.get("GPU_MAX_HW_QUEUES", "4") or 4) < 8:
+    os.environ["GPU_MAX_HW_QUEUES"] = "8"
 
 import argparse  # noqa: E402
 import ctypes  # noqa: E402
@@ -61,6 +62,10 @@ def parse():
     p.add_argument("--workers", type=int, default=8, help="host threads running the systems' host code")
     p.add_argument("--slots", type=int, default=256, help="serve: requests running at once")
     p.add_argument("--lanes", type=int, default=4, help="concurrent cohorts (streams) of the batched solver")
+    p.add_argument("--window", type=int, default=128, help="cohort: grouped launches per round")
+    p.add_argument("--max-age", type=int, default=64, help="cohort: merge steps a system may wait")
+    p.add_argument("--gmres-poll", type=int, default=8, help="cohort: GMRES iterations between convergence polls")
+    p.add_argument("--idle-pct", type=int, default=15, help="cohort: idle weight share that ends a round")
     p.add_argument("--plain-sample", type=int, default=32)
     p.add_argument("--eps", type=float, default=1e-8)
     p.add_argument("--rate", type=float, default=200.0)
@@ -432,7 +437,8 @@ def main():
     from torchtt_amd import workloads as W
     from torchtt_amd import _lib
     lib = _lib.load()
-    _lib.check("cohort_set_params", lib.ttamen_cohort_set_params(128, 64, 8, args.lanes, 15))
+    _lib.check("cohort_set_params", lib.ttamen_cohort_set_params(args.window, args.max_age, args.gmres_poll,
+                                                                 args.lanes, args.idle_pct))
     lib.ttamen_gemm_autotune(args.autotune)
     fn = {"batch": run_batch, "potential": run_potential, "precision": run_precision,
           "serve": run_serve, "solo": run_solo}[args.workload]
