@@ -348,11 +348,19 @@ class DmrgSweep {
       sv = ws_.alloc<double>(kd);
       svd(W, n2, 1, m1, n2, U, sv, V, ws_, st_);
     }
-    std::vector<double> s(kd, 0.0);  // the dense SVD's min(m1, n2) values, zeros past ns
     read_to_host(host(), sv, sizeof(double) * ns, st_);
-    for (int64_t i = 0; i < ns; ++i) s[i] = host()[i];
     const double thr = b * O.eps / std::pow((double)d, last ? 0.5 : 1.5);
-    const int64_t chop = rank_chop(s.data(), (int)kd, thr);
+    // The dense SVD has min(m1, n2) = kd values, zeros past ns. For thr > 0 rank_chop walks down
+    // through zero tails to ns - 1 and then sees the same sums, so the ns values decide; for
+    // thr <= 0 its answer (kd - 1, or 1 for a zero W) needs the padded length.
+    int64_t chop;
+    if (thr > 0) {
+      chop = rank_chop(host(), (int)ns, thr);
+    } else {
+      std::vector<double> s(kd, 0.0);
+      for (int64_t i = 0; i < ns; ++i) s[i] = host()[i];
+      chop = rank_chop(s.data(), (int)kd, thr);
+    }
     int64_t r_new = chop + (last ? 0 : r_enl_[k]);
     r_new = std::min(std::min(r_new, kd), P.caps[k + 1]);
     r_new = std::max<int64_t>(r_new, 1);
