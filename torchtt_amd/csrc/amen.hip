@@ -1208,20 +1208,6 @@ static Cohort::Counters g_cohort_last;
 // ======================================================================== C ABI
 using namespace ttamen;
 
-template <typename F>
-static int guarded(F&& f) {
-  try {
-    f();
-    return 0;
-  } catch (const Error& e) {
-    set_last_error(e.what());
-    return e.code;
-  } catch (const std::exception& e) {
-    set_last_error(e.what());
-    return ERR_INVALID;
-  }
-}
-
 extern "C" size_t ttamen_amen_solve_workspace_size(int d, const int64_t* N, const int64_t* rA,
                                                    const int64_t* rb, const int64_t* rx0, int64_t rmax,
                                                    int64_t kickrank, int64_t kick2,
@@ -1295,7 +1281,7 @@ static void run_cohort(ttamen_system* systems, int count, int64_t nswp, double e
       const size_t bytes = need[i];
       c.add(
           [=, &first_err] {
-            y->status = guarded([&] {
+            y->status = abi_call([&] {
               Options O = make_options(nswp, eps, y->rmax, max_full, kickrank, kick2,
                                        local_iterations, resets, 0, precision, y->seed);
               solve_one(y->d, y->N, y->rA, y->rb, y->A_cores, y->b_cores, y->x0_cores, y->rx0,
@@ -1384,7 +1370,7 @@ size_t ttamen_amen_solve_workspace_size(int d, const int64_t* N, const int64_t* 
                                         int64_t local_iterations, int64_t max_full,
                                         int precision) {
   size_t out = 0;
-  int rc = guarded([&] {
+  int rc = abi_call([&] {
     Plan P = make_plan(d, N, rA, rb, rx0, rmax, kickrank, kick2);
     Options O;
     O.rmax = rmax; O.kickrank = kickrank; O.kick2 = kick2; O.local_iterations = local_iterations;
@@ -1398,7 +1384,7 @@ size_t ttamen_amen_solve_prec_workspace_size(int d, const int64_t* N, const int6
                                              const int64_t* rb, const int64_t* rx0, int64_t rmax,
                                              int64_t kickrank, int64_t kick2, int preconditioner) {
   size_t out = 0;
-  int rc = guarded([&] {
+  int rc = abi_call([&] {
     TT_CHECK(preconditioner >= 0 && preconditioner <= 2, ERR_INVALID, "bad preconditioner");
     if (preconditioner == 0) return;
     Plan P = make_plan(d, N, rA, rb, rx0, rmax, kickrank, kick2);
@@ -1416,7 +1402,7 @@ size_t ttamen_amen_solve_prec_workspace_size(int d, const int64_t* N, const int6
 int ttamen_local_prec(int mode, const double* PhiL, const double* A, const double* PhiR,
                       const double* x, double* y, int64_t r1, int64_t s, int64_t n, int64_t S,
                       int64_t r2, void* workspace, size_t workspace_bytes, void* stream) {
-  return guarded([&] {
+  return abi_call([&] {
     TT_CHECK(PhiL && A && PhiR && x && y && workspace, ERR_INVALID, "null argument");
     TT_CHECK(r1 > 0 && s > 0 && n > 0 && S > 0 && r2 > 0, ERR_INVALID, "sizes must be positive");
     prec_init();
@@ -1432,7 +1418,7 @@ int ttamen_local_prec(int mode, const double* PhiL, const double* A, const doubl
 size_t ttamen_amen_solve_output_elems(int d, const int64_t* N, const int64_t* rx0, int64_t rmax,
                                       int64_t kickrank, int64_t kick2) {
   size_t out = 0;
-  int rc = guarded([&] {
+  int rc = abi_call([&] {
     std::vector<int64_t> ones(d + 1, 1);
     Plan P = make_plan(d, N, ones.data(), ones.data(), rx0, rmax, kickrank, kick2);
     for (int k = 0; k < d; ++k) out += (size_t)(P.Rb[k] * P.N[k] * P.Rb[k + 1]);
@@ -1449,7 +1435,7 @@ int ttamen_amen_solve(int d, const int64_t* N, const int64_t* rA, const int64_t*
                       int precision, void* workspace, size_t workspace_bytes, double* x_out,
                       size_t x_out_elems, int64_t* rx_out, int64_t* offsets_out,
                       ttamen_stats* stats, void* stream) {
-  return guarded([&] {
+  return abi_call([&] {
     Options O = make_options(nswp, eps, rmax, max_full, kickrank, kick2, local_iterations, resets,
                              verbose, precision, seed, preconditioner);
     linalg_init();
@@ -1465,7 +1451,7 @@ size_t ttamen_amen_divide_workspace_size(int d, const int64_t* N, const int64_t*
                                          int64_t kickrank, int64_t kick2, int64_t local_iterations,
                                          int64_t max_full, int preconditioner) {
   size_t out = 0;
-  int rc = guarded([&] {
+  int rc = abi_call([&] {
     TT_CHECK(d >= 1 && N && ra && rb && rmax, ERR_INVALID, "null argument");
     TT_CHECK(preconditioner == 0 || preconditioner == 1, ERR_INVALID,
              "preconditioner must be 0 (none) or 1 ('c')");
@@ -1487,7 +1473,7 @@ size_t ttamen_amen_divide_workspace_size(int d, const int64_t* N, const int64_t*
 size_t ttamen_amen_divide_output_elems(int d, const int64_t* N, const int64_t* rx0,
                                        const int64_t* rmax, int64_t kickrank, int64_t kick2) {
   size_t out = 0;
-  int rc = guarded([&] {
+  int rc = abi_call([&] {
     TT_CHECK(d >= 1 && N && rmax, ERR_INVALID, "null argument");
     std::vector<int64_t> ones(d + 1, 1);
     int64_t rmx = 1;
@@ -1506,7 +1492,7 @@ int ttamen_amen_divide(int d, const int64_t* N, const int64_t* ra, const int64_t
                        int64_t resets, int verbose, int preconditioner, void* workspace,
                        size_t workspace_bytes, double* x_out, size_t x_out_elems, int64_t* rx_out,
                        int64_t* offsets_out, ttamen_stats* stats, void* stream) {
-  return guarded([&] {
+  return abi_call([&] {
     TT_CHECK(d >= 1 && N && ra && rb && a_cores && b_cores && rmax && workspace, ERR_INVALID,
              "null argument");
     TT_CHECK(preconditioner == 0 || preconditioner == 1, ERR_INVALID,
@@ -1541,7 +1527,7 @@ size_t ttamen_amen_solve_batch_workspace_size(const ttamen_system* systems, int 
                                               int64_t local_iterations, int64_t max_full,
                                               int precision, int workers) {
   size_t out = 0;
-  int rc = guarded([&] {
+  int rc = abi_call([&] {
     TT_CHECK(systems && count >= 0 && workers >= 1, ERR_INVALID, "bad batch arguments");
     std::vector<size_t> need;
     for (int i = 0; i < count; ++i) {
@@ -1564,7 +1550,7 @@ int ttamen_amen_solve_batch(ttamen_system* systems, int count, int64_t nswp, dou
                             int64_t max_full, int64_t kickrank, int64_t kick2,
                             int64_t local_iterations, int64_t resets, int precision, int workers,
                             void* workspace, size_t workspace_bytes, void* stream) {
-  return guarded([&] {
+  return abi_call([&] {
     TT_CHECK(systems && count >= 0 && workers >= 1 && workspace, ERR_INVALID, "bad batch arguments");
     if (count == 0) return;
     run_cohort(systems, count, nswp, eps, max_full, kickrank, kick2, local_iterations, resets,
@@ -1579,7 +1565,7 @@ int ttamen_serve(ttamen_system* systems, const double* arrival_s, int count, int
                  int64_t local_iterations, int64_t resets, int precision, int slots,
                  void* workspace, size_t workspace_bytes, double* start_s, double* done_s,
                  void* stream) {
-  return guarded([&] {
+  return abi_call([&] {
     TT_CHECK(systems && arrival_s && start_s && done_s && workspace && slots >= 1 && count >= 0,
              ERR_INVALID, "bad serve arguments");
     if (count == 0) return;
@@ -1630,7 +1616,7 @@ int ttamen_local_matvec(const double* PhiL, const double* A, const double* PhiR,
                         double* y, int64_t l1, int64_t s, int64_t r1, int64_t n, int64_t S,
                         int64_t l2, int64_t r2, int64_t nvec, void* workspace,
                         size_t workspace_bytes, void* stream) {
-  return guarded([&] {
+  return abi_call([&] {
     TT_CHECK(PhiL && A && PhiR && x && y && workspace, ERR_INVALID, "null argument");
     TT_CHECK(l1 > 0 && s > 0 && r1 > 0 && n > 0 && S > 0 && l2 > 0 && r2 > 0 && nvec > 0,
              ERR_INVALID, "sizes must be positive");
@@ -1650,7 +1636,7 @@ static int local_matvec_f32_abi(const float* PhiL, const float* A, const float* 
                                 float* y, int64_t l1, int64_t s, int64_t r1, int64_t n, int64_t S,
                                 int64_t l2, int64_t r2, int64_t nvec, void* workspace,
                                 size_t workspace_bytes, void* stream, bool bf16) {
-  return guarded([&] {
+  return abi_call([&] {
     TT_CHECK(PhiL && A && PhiR && x && y && workspace, ERR_INVALID, "null argument");
     Arena ws(workspace, workspace_bytes);
     hipStream_t st = (hipStream_t)stream;
@@ -1696,7 +1682,7 @@ size_t ttamen_phi_workspace_size(int64_t l, int64_t s, int64_t r, int64_t n, int
 int ttamen_phi_fwd_A(const double* Phi, const double* xL, const double* A, const double* xR,
                      double* Pn, int64_t l, int64_t s, int64_t r, int64_t n, int64_t S, int64_t L,
                      int64_t R, void* workspace, size_t workspace_bytes, void* stream) {
-  return guarded([&] {
+  return abi_call([&] {
     Arena ws(workspace, workspace_bytes);
     hipStream_t st = (hipStream_t)stream;
     double* Apm = ws.alloc<double>(s * n * n * S);
@@ -1708,7 +1694,7 @@ int ttamen_phi_fwd_A(const double* Phi, const double* xL, const double* A, const
 int ttamen_phi_bck_A(const double* Phi, const double* xL, const double* A, const double* xR,
                      double* Pp, int64_t l, int64_t s, int64_t r, int64_t n, int64_t S, int64_t L,
                      int64_t R, void* workspace, size_t workspace_bytes, void* stream) {
-  return guarded([&] {
+  return abi_call([&] {
     Arena ws(workspace, workspace_bytes);
     phi_bck_A(Phi, xL, A, xR, Pp, (int)l, (int)s, (int)r, (int)n, (int)S, (int)L, (int)R, ws,
               (hipStream_t)stream);
@@ -1718,7 +1704,7 @@ int ttamen_phi_bck_A(const double* Phi, const double* xL, const double* A, const
 int ttamen_phi_fwd_rhs(const double* Pb, const double* bc, const double* x, double* Pn, int64_t b,
                        int64_t r, int64_t n, int64_t B, int64_t R, void* workspace,
                        size_t workspace_bytes, void* stream) {
-  return guarded([&] {
+  return abi_call([&] {
     Arena ws(workspace, workspace_bytes);
     phi_fwd_rhs(Pb, bc, x, Pn, (int)b, (int)r, (int)n, (int)B, (int)R, ws, (hipStream_t)stream);
   });
@@ -1727,7 +1713,7 @@ int ttamen_phi_fwd_rhs(const double* Pb, const double* bc, const double* x, doub
 int ttamen_phi_bck_rhs(const double* Pb, const double* bc, const double* x, double* Pp, int64_t b,
                        int64_t r, int64_t n, int64_t B, int64_t R, void* workspace,
                        size_t workspace_bytes, void* stream) {
-  return guarded([&] {
+  return abi_call([&] {
     Arena ws(workspace, workspace_bytes);
     phi_bck_rhs(Pb, bc, x, Pp, (int)b, (int)r, (int)n, (int)B, (int)R, ws, (hipStream_t)stream);
   });
@@ -1740,7 +1726,7 @@ size_t ttamen_dense_workspace_size(int64_t m, int64_t n) {
 
 int ttamen_qr(const double* A, int64_t m, int64_t n, double* Q, double* R, void* workspace,
               size_t workspace_bytes, void* stream) {
-  return guarded([&] {
+  return abi_call([&] {
     linalg_init();
     Arena ws(workspace, workspace_bytes);
     qr(A, n, 1, m, n, Q, R, ws, (hipStream_t)stream);
@@ -1750,7 +1736,7 @@ int ttamen_qr(const double* A, int64_t m, int64_t n, double* Q, double* R, void*
 
 int ttamen_svd(const double* A, int64_t m, int64_t n, double* U, double* s, double* V,
                void* workspace, size_t workspace_bytes, void* stream) {
-  return guarded([&] {
+  return abi_call([&] {
     linalg_init();
     Arena ws(workspace, workspace_bytes);
     svd(A, n, 1, m, n, U, s, V, ws, (hipStream_t)stream);
@@ -1760,7 +1746,7 @@ int ttamen_svd(const double* A, int64_t m, int64_t n, double* U, double* s, doub
 
 int ttamen_dense_solve(const double* B, const double* rhs, double* x, int64_t n, void* workspace,
                        size_t workspace_bytes, void* stream) {
-  return guarded([&] {
+  return abi_call([&] {
     TT_CHECK(B && rhs && x && workspace, ERR_INVALID, "null argument");
     TT_CHECK(n > 0 && n <= lu_max_order(), ERR_INVALID, "dense_solve: bad order");
     lu_init();
@@ -1778,7 +1764,7 @@ size_t ttamen_dense_solve_workspace_size(int64_t n) {
 }
 
 int ttamen_profile_enable(int enable) {
-  return guarded([&] {
+  return abi_call([&] {
     std::lock_guard<std::mutex> lk(g_prof.mu);
     g_prof.flush();
     g_prof.tagged.clear();
@@ -1789,7 +1775,7 @@ int ttamen_profile_enable(int enable) {
 }
 
 int ttamen_profile_read(double* out4) {  // 8 values
-  return guarded([&] {
+  return abi_call([&] {
     TT_CHECK(out4, ERR_INVALID, "null argument");
     std::lock_guard<std::mutex> lk(g_prof.mu);
     g_prof.flush();
@@ -1809,7 +1795,7 @@ int ttamen_profile_read(double* out4) {  // 8 values
 int ttamen_rhs_assemble(const double* PL, const double* bc, const double* PR, double* out, int64_t b,
                         int64_t r, int64_t n, int64_t B, int64_t R, double scale, void* workspace,
                         size_t workspace_bytes, void* stream) {
-  return guarded([&] {
+  return abi_call([&] {
     TT_CHECK(PL && bc && PR && out && workspace, ERR_INVALID, "null argument");
     TT_CHECK(b > 0 && r > 0 && n > 0 && B > 0 && R > 0, ERR_INVALID, "sizes must be positive");
     Arena ws(workspace, workspace_bytes);
@@ -1844,7 +1830,7 @@ int ttamen_gmres_solve(const double* PhiL, const double* A, const double* PhiR, 
                        int64_t r1, int64_t s, int64_t n, int64_t S, int64_t r2, double tol,
                        int64_t local_iterations, int64_t resets, void* workspace, size_t workspace_bytes,
                        int64_t* iters_out, int* flag_out, void* stream) {
-  return guarded([&] {
+  return abi_call([&] {
     TT_CHECK(PhiL && A && PhiR && rhs && x && workspace, ERR_INVALID, "null argument");
     TT_CHECK(r1 > 0 && s > 0 && n > 0 && S > 0 && r2 > 0, ERR_INVALID, "sizes must be positive");
     TT_CHECK(local_iterations >= 1 && local_iterations + 1 <= kGmresMaxCols && resets >= 1 && tol >= 0,
@@ -1879,7 +1865,7 @@ int ttamen_gmres_solve(const double* PhiL, const double* A, const double* PhiR, 
 
 // One line describing the current device: name, architecture, compute units, LDS per block, HBM.
 int ttamen_device_info(char* buf, size_t len) {
-  return guarded([&] {
+  return abi_call([&] {
     TT_CHECK(buf && len > 0, ERR_INVALID, "null argument");
     int dev = 0;
     TT_HIP(hipGetDevice(&dev));
@@ -1892,7 +1878,7 @@ int ttamen_device_info(char* buf, size_t len) {
 }
 
 int ttamen_cohort_gemm_profile(int enable, double* out6) {
-  return guarded([&] {
+  return abi_call([&] {
     TT_CHECK(enable >= -1 && enable <= 1, ERR_INVALID, "enable must be -1, 0 or 1");
     cohort_gemm_profile(enable, out6);
   });
@@ -1907,7 +1893,7 @@ int ttamen_gemm_autotune(int mode) {
 }
 
 int ttamen_gemm_autotune_stats(double* out4) {
-  return guarded([&] {
+  return abi_call([&] {
     TT_CHECK(out4, ERR_INVALID, "null argument");
     gemm_autotune_stats(out4);
   });
@@ -1919,7 +1905,7 @@ int ttamen_gemm_f64(int64_t M, int64_t N, int64_t K, int64_t batch, double alpha
                     int64_t sar, int64_t sac, int64_t sab, const double* B, int64_t sbr,
                     int64_t sbc, int64_t sbb, double beta, double* C, int64_t scr, int64_t scc,
                     int64_t scb, void* workspace, size_t workspace_bytes, void* stream) {
-  return guarded([&] {
+  return abi_call([&] {
     Arena ws(workspace, workspace_bytes);
     GemmDesc g;
     g.M = (int)M; g.N = (int)N; g.K = (int)K; g.B1 = (int)batch;

@@ -27,8 +27,6 @@ void linalg_init();
 
 namespace {
 
-int grid_n(int64_t n) { return (int)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 4096)); }
-
 // D[a][(m,n')][(k,n)][A] = A[a][m][k][A] if n == n' (the operator A (x) I_N of amen_mm).
 struct KKronEye {
   struct Args { const double* A; double* D; int ra, M, K, N, rA; };
@@ -271,7 +269,7 @@ class ProductSweep {
     size_t mk = ws_.mark();
     const int64_t nc = ld - c0;
     double* t = ws_.alloc<double>(rows * nc);
-    launch<KRandn>(dim3(grid_n(rows * nc)), dim3(256), 0, st_,
+    launch<KRandn>(dim3(grid_1d(rows * nc)), dim3(256), 0, st_,
                    {t, rows * nc, O.seed, (uint64_t)(1000003 * (++rand_calls_) + k)});
     TT_LAUNCH_CHECK();
     copy2d(t, nc, 1, buf + c0, ld, rows, nc, st_);
@@ -408,7 +406,7 @@ class ProductSweep {
       r = std::min<int64_t>(r, std::min<int64_t>(kc, P.caps[k + 1]));
       u = U; ldu = kc;
       vmat = ws_.alloc<double>(r2 * r);
-      launch<KScaleCols>(dim3(grid_n(r2 * r)), dim3(256), 0, st_, {V, kc, sv, vmat, r, r2, (int)r});
+      launch<KScaleCols>(dim3(grid_1d(r2 * r)), dim3(256), 0, st_, {V, kc, sv, vmat, r, r2, (int)r});
       TT_LAUNCH_CHECK();
     } else {
       const int64_t kq = std::min(rows, r2);
@@ -561,20 +559,6 @@ void product_run(const ProductPlan& P, const ProductOptions& O, const double* co
   }
 }
 
-template <class F>
-int guarded_call(F&& f) {
-  try {
-    f();
-    return 0;
-  } catch (const Error& e) {
-    set_last_error(e.what());
-    return e.code;
-  } catch (const std::exception& e) {
-    set_last_error(e.what());
-    return ERR_INVALID;
-  }
-}
-
 }  // namespace
 }  // namespace ttamen
 
@@ -586,7 +570,7 @@ size_t ttamen_amen_mv_workspace_size(int d, const int64_t* M, const int64_t* K, 
                                      const int64_t* rb, const int64_t* rx0, const int64_t* rmax,
                                      int64_t kickrank, int64_t kick2) {
   size_t out = 0;
-  int rc = guarded_call([&] {
+  int rc = abi_call([&] {
     ProductPlan P = make_product_plan(d, M, K, rA, rb, rx0, rmax, kickrank, kick2);
     ProductOptions O;
     O.kickrank = kickrank; O.kick2 = kick2;
@@ -598,7 +582,7 @@ size_t ttamen_amen_mv_workspace_size(int d, const int64_t* M, const int64_t* K, 
 size_t ttamen_amen_mv_output_elems(int d, const int64_t* M, const int64_t* K, const int64_t* rx0,
                                    const int64_t* rmax, int64_t kickrank, int64_t kick2) {
   size_t out = 0;
-  int rc = guarded_call([&] {
+  int rc = abi_call([&] {
     std::vector<int64_t> ones(d + 1, 1);
     ProductPlan P = make_product_plan(d, M, K, ones.data(), ones.data(), rx0, rmax, kickrank, kick2);
     out = product_output_elems(P);
@@ -613,7 +597,7 @@ int ttamen_amen_mv(int d, const int64_t* M, const int64_t* K, const int64_t* rA,
                    uint64_t seed, int verbose, void* workspace, size_t workspace_bytes, double* x_out,
                    size_t x_out_elems, int64_t* rx_out, int64_t* offsets_out, ttamen_stats* stats,
                    void* stream) {
-  return guarded_call([&] {
+  return abi_call([&] {
     TT_CHECK(nswp >= 0 && eps >= 0, ERR_INVALID, "invalid solver parameters");
     ProductPlan P = make_product_plan(d, M, K, rA, rb, x0_cores ? rx0 : nullptr, rmax, kickrank, kick2);
     ProductOptions O;
@@ -627,7 +611,7 @@ size_t ttamen_amen_mm_workspace_size(int d, const int64_t* M, const int64_t* K, 
                                      const int64_t* rA, const int64_t* rB, const int64_t* rx0,
                                      const int64_t* rmax, int64_t kickrank, int64_t kick2) {
   size_t out = 0;
-  int rc = guarded_call([&] {
+  int rc = abi_call([&] {
     TT_CHECK(d >= 1 && M && K && N && rA, ERR_INVALID, "null argument");
     std::vector<int64_t> MN(d), KN(d);
     size_t ops = 0;
@@ -651,7 +635,7 @@ int ttamen_amen_mm(int d, const int64_t* M, const int64_t* K, const int64_t* N, 
                    uint64_t seed, int verbose, void* workspace, size_t workspace_bytes, double* x_out,
                    size_t x_out_elems, int64_t* rx_out, int64_t* offsets_out, ttamen_stats* stats,
                    void* stream) {
-  return guarded_call([&] {
+  return abi_call([&] {
     TT_CHECK(d >= 1 && M && K && N && rA && rB && A_cores && B_cores && workspace, ERR_INVALID, "null argument");
     TT_CHECK(nswp >= 0 && eps >= 0, ERR_INVALID, "invalid solver parameters");
     hipStream_t st = (hipStream_t)stream;
@@ -672,7 +656,7 @@ int ttamen_amen_mm(int d, const int64_t* M, const int64_t* K, const int64_t* N, 
       const size_t bytes = ((size_t)total * sizeof(double) + 255) / 256 * 256;
       TT_CHECK(used + bytes < workspace_bytes, ERR_WORKSPACE, "workspace too small");
       double* Dk = (double*)(w + used);
-      launch<KKronEye>(dim3(grid_n(total)), dim3(256), 0, st,
+      launch<KKronEye>(dim3(grid_1d(total)), dim3(256), 0, st,
                        {A_cores[k], Dk, (int)rA[k], (int)M[k], (int)K[k], (int)N[k], (int)rA[k + 1]});
       TT_LAUNCH_CHECK();
       D[k] = Dk;

@@ -43,8 +43,6 @@ void linalg_init();
 
 namespace {
 
-int grid_n(int64_t n) { return (int)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 4096)); }
-
 constexpr uint64_t kKickStream = uint64_t(1) << 40;
 constexpr uint64_t kGuessStream = uint64_t(2) << 40;
 
@@ -142,7 +140,7 @@ class DmrgSweep {
       for (int k = 1; k < d; ++k) ry_[k] = 2;  // dmrg_mv.h:43-51
       for (int k = 0; k < d; ++k) {
         const int64_t n = ry_[k] * P.M[k] * ry_[k + 1];
-        launch<KRandn>(dim3(grid_n(n)), dim3(256), 0, st_, {y_[k], n, O.seed, kGuessStream | (uint64_t)k});
+        launch<KRandn>(dim3(grid_1d(n)), dim3(256), 0, st_, {y_[k], n, O.seed, kGuessStream | (uint64_t)k});
         TT_LAUNCH_CHECK();
       }
     }
@@ -375,7 +373,7 @@ class DmrgSweep {
     }
     const int64_t rr = std::min(r_new, ns);
     double* W2s = ws_.alloc<double>(n2 * rr);  // V[:, :rr] diag(s[:rr])
-    launch<KScaleCols>(dim3(grid_n(n2 * rr)), dim3(256), 0, st_, {V, ns, sv, W2s, rr, n2, (int)rr});
+    launch<KScaleCols>(dim3(grid_1d(n2 * rr)), dim3(256), 0, st_, {V, ns, sv, W2s, rr, n2, (int)rr});
     TT_LAUNCH_CHECK();
     if (need_qr) {
       // [U_r, 0, kick] = Q Rc; y_k = Q, y_k+1 = Rc [W2, 0, 0]^T (dmrg_mv.h:177-185)
@@ -385,7 +383,7 @@ class DmrgSweep {
       copy2d(U, ns, 1, cat, ncol, m1, rr, st_);
       if (kick > 0) {
         double* t = ws_.alloc<double>(m1 * kick);
-        launch<KRandn>(dim3(grid_n(m1 * kick)), dim3(256), 0, st_,
+        launch<KRandn>(dim3(grid_1d(m1 * kick)), dim3(256), 0, st_,
                        {t, m1 * kick, O.seed, kKickStream | ((uint64_t)swp << 20) | (uint64_t)k});
         TT_LAUNCH_CHECK();
         copy2d(t, kick, 1, cat + r_new, ncol, m1, kick, st_);
@@ -468,20 +466,6 @@ size_t dmrg_output_elems(const DmrgPlan& P) {
   return out;
 }
 
-template <class F>
-int guarded_call(F&& f) {
-  try {
-    f();
-    return 0;
-  } catch (const Error& e) {
-    set_last_error(e.what());
-    return e.code;
-  } catch (const std::exception& e) {
-    set_last_error(e.what());
-    return ERR_INVALID;
-  }
-}
-
 }  // namespace
 }  // namespace ttamen
 
@@ -493,7 +477,7 @@ size_t ttamen_dmrg_mv_workspace_size(int d, const int64_t* M, const int64_t* N, 
                                      const int64_t* rx, const int64_t* ry0, const int64_t* rmax,
                                      int64_t kickrank) {
   size_t out = 0;
-  int rc = guarded_call([&] {
+  int rc = abi_call([&] {
     DmrgPlan P = make_dmrg_plan(d, M, N, rA, rx, ry0, rmax, kickrank);
     DmrgOptions O;
     O.kickrank = kickrank;
@@ -505,7 +489,7 @@ size_t ttamen_dmrg_mv_workspace_size(int d, const int64_t* M, const int64_t* N, 
 size_t ttamen_dmrg_mv_output_elems(int d, const int64_t* M, const int64_t* ry0, const int64_t* rmax,
                                    int64_t kickrank) {
   size_t out = 0;
-  int rc = guarded_call([&] {
+  int rc = abi_call([&] {
     TT_CHECK(d >= 2 && M, ERR_INVALID, "null argument");
     std::vector<int64_t> ones(d + 1, 1);
     DmrgPlan P = make_dmrg_plan(d, M, M, ones.data(), ones.data(), ry0, rmax, kickrank);
@@ -520,7 +504,7 @@ int ttamen_dmrg_mv(int d, const int64_t* M, const int64_t* N, const int64_t* rA,
                    const int64_t* rmax, int64_t kickrank, uint64_t seed, int verbose, void* workspace,
                    size_t workspace_bytes, double* y_out, size_t y_out_elems, int64_t* ry_out,
                    int64_t* offsets_out, ttamen_stats* stats, void* stream) {
-  return guarded_call([&] {
+  return abi_call([&] {
     auto t0 = std::chrono::high_resolution_clock::now();
     TT_CHECK(A_cores && x_cores && workspace && y_out && ry_out && offsets_out, ERR_INVALID, "null argument");
     TT_CHECK(nswp >= 0 && eps >= 0, ERR_INVALID, "invalid solver parameters");

@@ -8,6 +8,7 @@
 
 #include <hip/hip_runtime.h>
 #include "launch.h"
+#include <algorithm>
 #include <cstdint>
 #include <cstddef>
 #include <string>
@@ -32,6 +33,27 @@ enum Status : int {
 };
 
 void set_last_error(const std::string& msg);
+
+// Body of a C ABI entry point: 0 when f returns; a thrown Error gives its status code, any other
+// exception ERR_INVALID; the message is kept for ttamen_last_error().
+template <class F>
+int abi_call(F&& f) {
+  try {
+    f();
+    return 0;
+  } catch (const Error& e) {
+    set_last_error(e.what());
+    return e.code;
+  } catch (const std::exception& e) {
+    set_last_error(e.what());
+    return ERR_INVALID;
+  }
+}
+
+// Blocks of 256 threads for a grid-stride loop over n elements: one element per thread, at most cap blocks.
+inline int grid_1d(int64_t n, int64_t cap = 4096) {
+  return (int)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, cap));
+}
 
 #define TT_HIP(call)                                                               \
   do {                                                                             \
