@@ -1,8 +1,9 @@
 """CPU oracle for the AMEn linear-solver path -- TEST INFRASTRUCTURE ONLY.
 
 A numpy FP64 restatement of the reference's native AMEn solver
-(cpp/amen_solve.h:122-549, cpp/gmres.h:15-161, cpp/ortho.h:16-73), written to be
-read line against line with it. Only tests/, __graft_entry__.smoke() and bench.py's
+(cpp/amen_solve.h:122-549, cpp/gmres.h:15-161, cpp/ortho.h:16-73) and of the sweeps around
+it (preconditioners, amen_divide, the AMEn TT products, the DMRG matvec, TT rounding),
+written to be read line against line with them. Only tests/, __graft_entry__.smoke() and bench.py's
 cpu_baseline leg may import this module, and only as the checker / baseline; the
 product path (torchtt_amd) never does.
 
@@ -14,7 +15,14 @@ Pinning (see tests/test_oracle.py):
     relation A Q_k = Q_{k+1} Hbar_k;
   * local_product / phi_*: against brute-force dense contractions and the
     invariants phi_d = x^T A x and phib_d = <b, x>;
-  * rank_chop / round_tt: against the closed-form tail rule and dense SVD.
+  * rank_chop / round_tt: against the closed-form tail rule and dense SVD;
+  * prec_blocks / prec_apply (cpp/matvecs.h:153-194): exact inverse of a block-diagonal operator;
+  * amen_divide (torchtt/_division.py:76-398): the dense quotient and the rank-2 truncation floor;
+  * amen_mv / kron_eye_operator (torchtt/_amen.py:55-413): exact products, zero and identity
+    operands, amen_mm through A (x) I_N;
+  * dmrg_mv (cpp/dmrg_mv.h:23-230, tests/test_dmrg_oracle.py): dense products, the reference
+    test's structure, identity and zero operands, the final-rank rule; randn_stream against the
+    published SplitMix64 outputs and the N(0,1) moments.
 The reference's Python twin (torchtt/solvers.py:290-688) differs in three places
 (GMRES iterations m+1, truncation start r = full-1, zero-norm guard 1.0); this
 module follows the C++ version.
@@ -23,7 +31,8 @@ import numpy as np
 
 __all__ = ["local_product", "phi_fwd_A", "phi_bck_A", "phi_fwd_rhs", "phi_bck_rhs",
            "gmres", "amen_solve", "rl_orthogonal", "rank_chop", "round_tt", "tt_full",
-           "tt_matvec", "tt_add", "tt_norm", "tt_residual", "randn_stream", "dmrg_mv"]
+           "tt_matvec", "tt_add", "tt_norm", "tt_residual", "prec_blocks", "prec_apply", "diag_operator",
+           "amen_divide", "local_AB", "amen_mv", "kron_eye_operator", "randn_stream", "dmrg_mv"]
 
 
 # --------------------------------------------------------------------------- contractions
