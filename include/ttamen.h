@@ -240,7 +240,7 @@ size_t ttamen_amen_mm_workspace_size(int d, const int64_t* M, const int64_t* K, 
 
 /*
  * DMRG approximation of y = A x (reference torchttcpp.dmrg_mv, cpp/dmrg_mv.h:23-230, bound at
- * cpp/cpp_ext.cpp:12 and called from torchtt/_dmrg.py:42): A has cores rA[k] x M[k] x N[k] x
+ * cpp/cpp_ext.cpp:11 and called from torchtt/_dmrg.py:42): A has cores rA[k] x M[k] x N[k] x
  * rA[k+1], x cores rx[k] x N[k] x rx[k+1]. Sweeps of right-to-left orthogonalisation and
  * left-to-right two-core updates: the supercore projection W of A x, delta = ||W - y_k y_k+1|| /
  * ||W|| steering a per-core rank enlargement, rank_chop truncation at ||W|| eps / d^1.5
@@ -265,6 +265,15 @@ size_t ttamen_dmrg_mv_workspace_size(int d, const int64_t* M, const int64_t* N, 
                                      int64_t kickrank);
 size_t ttamen_dmrg_mv_output_elems(int d, const int64_t* M, const int64_t* ry0, const int64_t* rmax,
                                    int64_t kickrank);
+
+/*
+ * Full tensor of a TT vector with cores R[k] x N[k] x R[k+1] (reference torchttcpp.tt_full,
+ * cpp/full.h:3-12, bound at cpp/cpp_ext.cpp:8): out receives the N[0] x ... x N[d-1] tensor
+ * (row-major, out_elems >= its size). One GEMM per core; the products must stay below 2^31 elements.
+ */
+int ttamen_tt_full(int d, const int64_t* N, const int64_t* R, const double* const* cores, void* workspace,
+                   size_t workspace_bytes, double* out, size_t out_elems, void* stream);
+size_t ttamen_tt_full_workspace_size(int d, const int64_t* N, const int64_t* R);
 
 size_t ttamen_tt_round_workspace_size(int d, const int64_t* N, const int64_t* R);
 int ttamen_tt_round(int d, const int64_t* N, const int64_t* R, const double* const* cores,

@@ -87,3 +87,20 @@ def test_round_errors():
     x = tt.TT([torch.ones(1, 3, 1, dtype=torch.float64)])
     with pytest.raises(RuntimeError):
         tt.solvers.round_tt(x)  # CPU cores: the native path runs on the GPU only
+
+
+@pytest.mark.parametrize("N,R", [([5, 4, 6, 3], [1, 3, 4, 2, 1]), ([7], [1, 1]), ([2, 3], [1, 5, 1]),
+                                 ([8, 8, 8, 8, 8], [1, 6, 9, 9, 6, 1])])
+def test_tt_full_matches_oracle(N, R):
+    """Native full tensor of a TT vector (replacing torchttcpp.tt_full, cpp/full.h:3-12) against the
+    oracle's contraction and the reference Python TT.full's einsum chain."""
+    import torchtt_amd as tt
+    rng = np.random.default_rng(len(N))
+    cores = [rng.standard_normal((R[k], N[k], R[k + 1])) for k in range(len(N))]
+    x = tt.TT([torch.tensor(c, device="cuda") for c in cores])
+    print("tt_full", N, R, flush=True)
+    full = tt.solvers.tt_full(x).cpu().numpy()
+    ref = O.tt_full(cores)
+    assert full.shape == tuple(N)
+    assert np.linalg.norm(full - ref) <= 1e-13 * np.linalg.norm(ref)
+    assert np.linalg.norm(full - x.full().cpu().numpy()) <= 1e-13 * np.linalg.norm(ref)

@@ -62,6 +62,9 @@ _SIGNATURES = {
     "ttamen_local_prec": (ctypes.c_int, [ctypes.c_int, c_dp, c_dp, c_dp, c_dp, c_dp, c_i64, c_i64, c_i64,
                                          c_i64, c_i64, c_dp, c_size, c_dp]),
     "ttamen_tt_round_workspace_size": (c_size, [ctypes.c_int, c_i64p, c_i64p]),
+    "ttamen_tt_full": (ctypes.c_int, [ctypes.c_int, c_i64p, c_i64p, ctypes.POINTER(c_dp), c_dp, c_size, c_dp,
+                                      c_size, c_dp]),
+    "ttamen_tt_full_workspace_size": (c_size, [ctypes.c_int, c_i64p, c_i64p]),
     "ttamen_tt_round": (ctypes.c_int, [ctypes.c_int, c_i64p, c_i64p, ctypes.POINTER(c_dp), ctypes.c_double,
                                        c_i64, c_dp, c_size, c_dp, c_size, c_i64p, c_i64p, c_dp]),
     "ttamen_cohort_stats": (ctypes.c_int, [ctypes.POINTER(ctypes.c_double)]),

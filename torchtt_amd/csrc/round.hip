@@ -174,4 +174,62 @@ int ttamen_tt_round(int d, const int64_t* N, const int64_t* R, const double* con
   }
 }
 
+// Full tensor of a TT vector (reference cpp/full.h:3-12, exported as torchttcpp.tt_full): the cores
+// are multiplied left to right, t_k = t_k-1 (n_0..n_k-1 x r_k) times core_k (r_k x n_k r_k+1), one GEMM
+// per core, ping-ponging between two workspace buffers; the last product is written to out.
+static int64_t full_stage_elems(int d, const int64_t* N, const int64_t* R) {
+  int64_t worst = 1, rows = 1;
+  for (int k = 0; k + 1 < d; ++k) {
+    rows *= N[k];
+    worst = std::max<int64_t>(worst, rows * R[k + 1]);
+  }
+  return worst;
+}
+
+size_t ttamen_tt_full_workspace_size(int d, const int64_t* N, const int64_t* R) {
+  int64_t stage = 0;
+  int rc = abi_call([&] {
+    TT_CHECK(d >= 1 && N && R, ERR_INVALID, "null argument");
+    stage = full_stage_elems(d, N, R);
+  });
+  return rc == 0 ? (size_t)(2 * stage + 64) * sizeof(double) + (64u << 20) : 0;
+}
+
+int ttamen_tt_full(int d, const int64_t* N, const int64_t* R, const double* const* cores, void* workspace,
+                   size_t workspace_bytes, double* out, size_t out_elems, void* stream) {
+  return abi_call([&] {
+    TT_CHECK(d >= 1 && N && R && cores && workspace && out, ERR_INVALID, "null argument");
+    TT_CHECK(R[0] == 1 && R[d] == 1, ERR_INVALID, "boundary ranks must be 1");
+    double total = 1.0;
+    for (int k = 0; k < d; ++k) {
+      TT_CHECK(N[k] >= 1 && R[k + 1] >= 1 && cores[k], ERR_INVALID, "bad cores");
+      total *= (double)N[k];
+    }
+    TT_CHECK(total <= (double)out_elems, ERR_WORKSPACE, "output buffer too small");
+    TT_CHECK(total * (double)std::max<int64_t>(1, *std::max_element(R, R + d + 1)) < 2147483647.0,
+             ERR_UNSUPPORTED, "full tensor too large");
+    hipStream_t st = (hipStream_t)stream;
+    if (d == 1) {
+      copy(cores[0], out, N[0], st);
+      TT_HIP(hipStreamSynchronize(st));
+      return;
+    }
+    Arena ws(workspace, workspace_bytes);
+    const int64_t stage = full_stage_elems(d, N, R);
+    double* buf[2] = {ws.alloc<double>(stage), ws.alloc<double>(stage)};
+    const double* t = cores[0];  // n_0 x r_1
+    int64_t rows = N[0];
+    for (int k = 1; k < d; ++k) {
+      double* dst = (k == d - 1) ? out : buf[k & 1];
+      GemmDesc g;
+      g.M = (int)rows; g.N = (int)(N[k] * R[k + 1]); g.K = (int)R[k];
+      g.a = rm(t, R[k]); g.b = rm(cores[k], N[k] * R[k + 1]); g.c = orm(dst, N[k] * R[k + 1]);
+      gemm_f64(g, ws, st);
+      t = dst;
+      rows *= N[k];
+    }
+    TT_HIP(hipStreamSynchronize(st));
+  });
+}
+
 }  // extern "C"

@@ -395,6 +395,36 @@ def round_tt(x, eps=1e-12, rmax=sys.maxsize):
     return TT(res)
 
 
+def tt_full(x):
+    """Full tensor of a TT vector on the GPU (libttamen: ttamen_tt_full), replacing the reference's
+    native `tt_full` (cpp/full.h:3-12): the cores multiplied left to right, one GEMM per core.
+    FP64 cores on one device; returns an N[0] x ... x N[d-1] tensor."""
+    if not isinstance(x, TT):
+        raise InvalidArguments("Input must be a TT.")
+    if x.is_ttm:
+        raise IncompatibleTypes("tt_full expects a TT vector (the reference's native full is for 3-D cores).")
+    lib = _lib.load()
+    dev = x.cores[0].device
+    if dev.type != "cuda":
+        raise RuntimeError("tt_full runs on a ROCm GPU; the input cores are on %s" % dev)
+    d = len(x.cores)
+    N, R = list(x.N), list(x.R)
+    cores = [c.to(torch.float64).contiguous() for c in x.cores]
+    Na, Ra = _lib.i64_array(N), _lib.i64_array(R)
+    total = int(np.prod(N))
+    with torch.cuda.device(dev):
+        ws_bytes = lib.ttamen_tt_full_workspace_size(d, Na, Ra)
+        if ws_bytes == 0:
+            _lib.check("ttamen_tt_full_workspace_size", 1)
+        ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dev)
+        out = torch.empty(total, dtype=torch.float64, device=dev)
+        code = lib.ttamen_tt_full(d, Na, Ra, _lib.ptr_array(cores), ctypes.c_void_p(ws.data_ptr()), ws_bytes,
+                                  ctypes.c_void_p(out.data_ptr()), total, ctypes.c_void_p(_lib.stream_handle(dev)))
+        _lib.check("ttamen_tt_full", code)
+        del ws
+    return out.reshape(N).to(x.cores[0].dtype)
+
+
 #: counters of the most recent batched / served call (see cohort_stats)
 last_cohort_stats = None
 
