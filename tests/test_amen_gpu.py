@@ -202,3 +202,59 @@ def test_edge_cases():
         tt.solvers.amen_solve(tt.TT(A3), b3)
     with pytest.raises(tt.errors.IncompatibleTypes):
         tt.solvers.amen_solve(tt.TT(b3), tt.TT(b3))
+
+
+def test_contract_of_the_return_value():
+    """The reference's return contract (torchtt/solvers.py:285, amen_solve.h:157): the solution comes
+    back on the device and dtype of A.cores[0] (CPU or FP32 inputs included), rank caps hold, nswp
+    bounds the sweeps, mode sizes may differ per core, and b = 0 gives x = 0 with finite values."""
+    import torchtt_amd as tt
+    rng = np.random.default_rng(21)
+    N = [3, 7, 2, 5]
+    d = len(N)
+    A = [rng.standard_normal((1 if k == 0 else 2, n, n, 1 if k == d - 1 else 2)) * 0.1 for k, n in enumerate(N)]
+    for k, n in enumerate(N):  # a dominant diagonal keeps the system well conditioned
+        A[k][0, :, :, 0] += np.eye(n) * (2.0 if k == 0 else 1.0)
+    b = [rng.standard_normal((1 if k == 0 else 3, n, 1 if k == d - 1 else 3)) for k, n in enumerate(N)]
+    _say("contract: CPU inputs")
+    x = tt.solvers.amen_solve(_tt(A, "cpu"), _tt(b, "cpu"), eps=1e-12)
+    assert x.cores[0].device.type == "cpu" and x.cores[0].dtype == torch.float64
+    assert list(x.N) == N
+    assert tt.solvers.residual(_tt(A), x.to(device="cuda"), _tt(b)) < 1e-10
+    _say("contract: FP32 inputs")
+    A32 = tt.TT([torch.tensor(c, dtype=torch.float32, device="cuda") for c in A])
+    b32 = tt.TT([torch.tensor(c, dtype=torch.float32, device="cuda") for c in b])
+    x32 = tt.solvers.amen_solve(A32, b32, eps=1e-10)
+    assert x32.cores[0].dtype == torch.float32 and x32.cores[0].is_cuda
+    _say("contract: rank cap and nswp")
+    from torchtt_amd import workloads as W
+    Al, bl = W.laplace_potential(6, 12, seed=2, rhs_rank=3)
+    xr = tt.solvers.amen_solve(tt.TT(Al).cuda(), tt.TT(bl).cuda(), eps=1e-12, rmax=3, nswp=4)
+    # truncation caps at rmax; enrichment then adds kickrank (4) in every sweep but the last one
+    # (amen_solve.h:443-460), and a solve stopped by nswp never reached its last sweep
+    assert max(xr.R) <= 3 + 4
+    assert tt.solvers.last_stats["sweeps"] <= 4
+    xs = tt.solvers.amen_solve(tt.TT(Al).cuda(), tt.TT(bl).cuda(), eps=1e-12, nswp=1)
+    assert tt.solvers.last_stats["sweeps"] == 1
+    assert all(torch.isfinite(c).all() for c in xs.cores)
+    _say("contract: zero right-hand side")
+    b0 = tt.TT([torch.zeros_like(torch.as_tensor(c)).cuda() for c in b])
+    x0 = tt.solvers.amen_solve(_tt(A), b0, eps=1e-10, nswp=3)
+    assert all(torch.isfinite(c).all() for c in x0.cores)
+    assert float(x0.norm()) == 0.0
+
+
+def test_rank_above_the_lds_limit_is_a_status():
+    """With the default rmax the plan is bounded at 140 ranks; a solve whose solution needs more
+    stops with status 4 and a message instead of being refused before it starts. Here the right-hand
+    side has rank 200 at the middle interface and the warm start rank 140 there, so the first
+    left-to-right step keeps 140 directions and its enrichment asks for 144."""
+    import torchtt_amd as tt
+    g = torch.Generator().manual_seed(5)
+    N, rb, r0 = [16] * 4, [1, 16, 200, 16, 1], [1, 16, 140, 16, 1]
+    b = tt.TT([torch.randn(rb[k], N[k], rb[k + 1], dtype=torch.float64, generator=g).cuda() for k in range(4)])
+    x0 = tt.TT([torch.randn(r0[k], N[k], r0[k + 1], dtype=torch.float64, generator=g).cuda() for k in range(4)])
+    A = tt.eye(N, device="cuda")
+    _say("rank above the LDS limit")
+    with pytest.raises(Exception, match="not supported"):
+        tt.solvers.amen_solve(A, b, x0=x0, eps=1e-13, nswp=2)

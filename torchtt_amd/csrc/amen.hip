@@ -216,6 +216,9 @@ static Plan make_plan(int d, const int64_t* N, const int64_t* rA, const int64_t*
   for (int k = 0; k < d; ++k) {
     const int64_t rcap = std::max<int64_t>(rmax_list ? rmax_list[k + 1] : rmax, 1) + p.rzcap;
     int64_t b = std::min(sat_mul(p.Rb[k], N[k]), std::min(rcap, right[k + 1] + p.rzcap));
+    // Every QR and SVD of the sweep keeps an r x r block in LDS: bound at kMaxSmall, and a sweep
+    // that needs more is stopped with a status when it gets there (not refused up front).
+    b = std::min<int64_t>(b, kMaxSmall);
     if (k + 1 == d) b = 1;
     if (rx0) b = std::max(b, rx0[k + 1]);
     if (k + 1 < d) b = std::max<int64_t>(b, 1);
@@ -780,6 +783,9 @@ class Solver {
         gemm_f64(g, ws_, st_);
         u = Qe; ldu = kq; vmat = v2; r = kq;
       }
+      TT_CHECK(r <= P.Rb[k + 1], ERR_UNSUPPORTED,
+               "the solution needs rank " + std::to_string(r) + " at interface " + std::to_string(k + 1) +
+                   "; ranks above " + std::to_string(P.Rb[k + 1]) + " are not supported (lower rmax)");
       // v = vmat^T x_{k+1}  (r x n' x r3)
       const int64_t n1 = P.N[k + 1], r3 = rx_[k + 2];
       {
@@ -1095,8 +1101,7 @@ static void solve_one(int d, const int64_t* N, const int64_t* rA, const int64_t*
   Plan P = make_plan(d, N, rA, rb, x0_cores ? rx0 : nullptr, O.rmax, O.kickrank, O.kick2,
                      O.rmax_list.empty() ? nullptr : O.rmax_list.data());
   TT_CHECK(P.maxR <= kMaxSmall, ERR_UNSUPPORTED,
-           "rank bound " + std::to_string(P.maxR) + " exceeds " + std::to_string(kMaxSmall) +
-               " (lower rmax)");
+           "initial-guess rank " + std::to_string(P.maxR) + " exceeds " + std::to_string(kMaxSmall));
   Arena ws(workspace, workspace_bytes);
   Solver s(P, O, ws, st);
   s.allocate(A_cores, b_cores, x0_cores, rx0, z0_cores);

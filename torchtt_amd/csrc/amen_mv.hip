@@ -74,6 +74,7 @@ ProductPlan make_product_plan(int d, const int64_t* M, const int64_t* K, const i
   for (int k = 0; k < d; ++k) {
     int64_t b = std::min(std::min<int64_t>(p.Rb[k] * M[k], int64_t(1) << 40),
                          std::min(caps[k + 1] + p.rz, right[k + 1] + p.rz));
+    b = std::min<int64_t>(b, kMaxSmall);  // LDS-resident QR/SVD; a sweep that needs more stops with a status
     if (k + 1 == d) b = 1;
     if (rx0) b = std::max(b, rx0[k + 1]);
     p.Rb[k + 1] = std::max<int64_t>(b, 1);
@@ -90,7 +91,7 @@ ProductPlan make_product_plan(int d, const int64_t* M, const int64_t* K, const i
     p.maxrb = std::max(p.maxrb, rb[k]);
   }
   TT_CHECK(p.maxR <= kMaxSmall, ERR_UNSUPPORTED,
-           "rank bound " + std::to_string(p.maxR) + " exceeds " + std::to_string(kMaxSmall) + " (lower rmax)");
+           "initial-guess rank " + std::to_string(p.maxR) + " exceeds " + std::to_string(kMaxSmall));
   return p;
 }
 
@@ -477,6 +478,9 @@ class ProductSweep {
         gemm_f64(g, ws_, st_);
         u = Qe; ldu = kq; vmat = v2; r = kq;
       }
+      TT_CHECK(r <= P.Rb[k + 1], ERR_UNSUPPORTED,
+               "the product needs rank " + std::to_string(r) + " at interface " + std::to_string(k + 1) +
+                   "; ranks above " + std::to_string(P.Rb[k + 1]) + " are not supported (lower rmax)");
       // v = vmat^T x_{k+1}  (r x M' x r3)
       const int64_t n1 = P.M[k + 1], r3 = rx_[k + 2];
       {
