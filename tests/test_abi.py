@@ -50,3 +50,34 @@ def test_workspace_queries_without_gpu():
     bad = _lib.i64_array([1] + [4] * (d - 1) + [2])  # boundary rank must be 1
     assert lib.ttamen_amen_solve_workspace_size(d, N, bad, rb, None, 80, 4, 0, 40, 500, 0) == 0
     assert b"boundary" in lib.ttamen_last_error()
+
+
+def test_complex_cores_are_refused_before_any_device_work():
+    """The native sweeps compute in real FP64; a complex TT is refused with InvalidArguments (a cast
+    would silently drop the imaginary parts). Runs without a GPU: the check precedes device work."""
+    import pytest
+    import torch
+    import torchtt_amd as tt
+    from torchtt_amd.errors import InvalidArguments
+    N = [3, 4, 3]
+    A = tt.TT([torch.eye(n, dtype=torch.complex128).reshape(1, n, n, 1) for n in N])
+    Ar = tt.TT([torch.eye(n, dtype=torch.float64).reshape(1, n, n, 1) for n in N])
+    b = tt.TT([torch.ones(1, n, 1, dtype=torch.complex128) for n in N])
+    br = tt.TT([torch.ones(1, n, 1, dtype=torch.float64) for n in N])
+    with pytest.raises(InvalidArguments):
+        tt.solvers.amen_solve(A, br)
+    with pytest.raises(InvalidArguments):
+        tt.solvers.amen_solve(Ar, b)
+    with pytest.raises(InvalidArguments):
+        tt.solvers.amen_solve(Ar, br, x0=b)
+    with pytest.raises(InvalidArguments):
+        tt.solvers.amen_divide(b, br)
+    with pytest.raises(InvalidArguments):
+        tt.amen_mv(A, br)
+    with pytest.raises(InvalidArguments):
+        tt.solvers.round_tt(b)
+    with pytest.raises(InvalidArguments):
+        tt.solvers.tt_full(b)
+    from torchtt_amd._dmrg import dmrg_matvec
+    with pytest.raises(InvalidArguments):
+        dmrg_matvec(A, br)

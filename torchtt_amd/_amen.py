@@ -9,7 +9,7 @@ import torch
 from ._tt import TT
 from .errors import InvalidArguments, IncompatibleTypes, ShapeMismatch
 from . import _lib
-from .solvers import _solver_device
+from .solvers import _solver_device, _require_real
 
 #: statistics of the most recent call (dict)
 last_stats = None
@@ -89,6 +89,7 @@ def amen_mv(A, b, nswp=22, x0=None, eps=1e-10, rmax=1024, kickrank=4, kick2=0, v
         raise IncompatibleTypes('A must be TT-matrix and b must be vector.')
     if list(A.N) != list(b.N):
         raise ShapeMismatch('Dimension mismatch.')
+    _require_real(A, b, x0)
     d = len(A.M)
     M, K = list(A.M), list(A.N)
     Ma, Ka, rAa, rba = _lib.i64_array(M), _lib.i64_array(K), _lib.i64_array(A.R), _lib.i64_array(b.R)
@@ -116,6 +117,7 @@ def amen_mm(A, B, nswp=22, X0=None, eps=1e-10, rmax=1024, kickrank=4, kick2=0, v
         raise InvalidArguments('A and B must be TT-matrices.')
     if list(A.N) != list(B.M):
         raise ShapeMismatch('Dimension mismatch.')
+    _require_real(A, B, X0)
     d = len(A.M)
     M, K, N = list(A.M), list(A.N), list(B.N)
     P = [m * n for m, n in zip(M, N)]

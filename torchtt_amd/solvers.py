@@ -40,6 +40,14 @@ def _solver_device(A):
     return torch.device("cuda", torch.cuda.current_device())
 
 
+def _require_real(*tts):
+    """The native sweeps compute in real FP64: refuse complex cores (a cast would drop their
+    imaginary parts)."""
+    for t in tts:
+        if t is not None and any(c.is_complex() for c in t.cores):
+            raise InvalidArguments('Complex cores are not supported: the solver computes in real FP64.')
+
+
 def _check_system(A, b):
     if not (isinstance(A, TT) and isinstance(b, TT)):
         raise InvalidArguments('A and b must be TT instances.')
@@ -49,6 +57,7 @@ def _check_system(A, b):
         raise ShapeMismatch('A is not quadratic.')
     if A.N != b.N:
         raise ShapeMismatch('Dimension mismatch.')
+    _require_real(A, b)
 
 
 def _draw_z(N, kickrank, kick2, device):
@@ -95,6 +104,7 @@ def amen_solve(A, b, nswp=22, x0=None, eps=1e-10, rmax=32768, max_full=500, kick
     if prec != 0 and precision != 'fp64':
         raise InvalidArguments("Local preconditioners are available with FP64 local solves.")
 
+    _require_real(x0)
     lib = _lib.load()
     out_device, out_dtype = A.cores[0].device, A.cores[0].dtype
     dev = _solver_device(A)
@@ -182,6 +192,7 @@ def amen_divide(a, b, nswp=22, x0=None, eps=1e-10, rmax=100, max_full=500, kickr
         prec = 1
     else:
         raise InvalidArguments('Preconditioner ' + str(preconditioner) + ' not defined.')
+    _require_real(a, b, x0)
     lib = _lib.load()
     out_device, out_dtype = a.cores[0].device, a.cores[0].dtype
     dev = _solver_device(a)
@@ -365,6 +376,7 @@ def round_tt(x, eps=1e-12, rmax=sys.maxsize):
     FP64 cores on one device; ranks up to 140. Returns a new TT."""
     if not isinstance(x, TT):
         raise InvalidArguments("Input must be a TT.")
+    _require_real(x)
     lib = _lib.load()
     dev = x.cores[0].device
     if dev.type != "cuda":
@@ -403,6 +415,7 @@ def tt_full(x):
         raise InvalidArguments("Input must be a TT.")
     if x.is_ttm:
         raise IncompatibleTypes("tt_full expects a TT vector (the reference's native full is for 3-D cores).")
+    _require_real(x)
     lib = _lib.load()
     dev = x.cores[0].device
     if dev.type != "cuda":
