@@ -22,7 +22,9 @@ Workloads (--workload):
 for every result in FP64 TT arithmetic), divided by the wall time of the timed steps, with
 inputs resident in HBM. The roofline is that of the local operator, timed with HIP events
 inside the library on the solver's streams (FP32 applications inside GMRES for fp32, FP64
-otherwise). The CPU baseline is the reference's own C++ solver on this host's cores.
+otherwise) during a separate single-system solve, so that the timed steps run without the
+events; beside it the batch's merged GEMM launches (MFMA) and the GMRES orthogonalisation (HBM).
+The CPU baseline is the reference's own C++ solver on this host's cores.
 """
 import os
 
@@ -48,6 +50,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "solves/sec at rel. residual ≤1e-8, mixed vs FP64 plain; p50/p99 solve latency"
 PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "bf16": 2500.0}
+PEAK_HBM_GBS = 8000.0
 RES_BAR = 1e-8
 
 
@@ -169,6 +172,22 @@ def system_cost(sy):
     return d * n * r * r
 
 
+def ortho_roofline(lib, _lib, precision):
+    """HBM roofline of the GMRES orthogonalisation (CGS dots + update + close per Arnoldi step), from
+    the library's HIP events and algorithmic bytes ((2k + 7) N w per step k), read after a
+    profiled single-system solve."""
+    o = (ctypes.c_double * 3)()
+    _lib.check("profile_read_ortho", lib.ttamen_profile_read_ortho(o))
+    steps, ms, by = list(o)
+    if steps <= 0 or ms <= 0:
+        return None
+    ach = by / (ms / 1e3) / 1e9
+    return {"kernel": "GMRES orthogonalisation: CGS dots, update (+ second-pass dots) and close launches of an "
+                      "Arnoldi step (%s basis)" % ("FP64" if precision == "fp64" else "FP32"),
+            "bound": "hbm", "achieved": ach, "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": ach / PEAK_HBM_GBS,
+            "steps": int(steps), "avg_us": ms * 1e3 / steps, "bytes_per_step": by / steps}
+
+
 def solo_roofline(tt, lib, _lib, item, z, precision, eps):
     """Local-matvec roofline from HIP events around every application in one single-system
     solve (on the solver's stream)."""
@@ -178,8 +197,10 @@ def solo_roofline(tt, lib, _lib, item, z, precision, eps):
     torch.cuda.synchronize()
     prof = (ctypes.c_double * 8)()
     _lib.check("profile_read", lib.ttamen_profile_read(prof))
+    orth = ortho_roofline(lib, _lib, precision)
     lib.ttamen_profile_enable(0)
-    return roofline(prof, precision)
+    out = roofline(prof, precision)
+    return out, orth
 
 
 def run_batch(args, tt, W, lib, _lib):
@@ -248,7 +269,7 @@ def run_batch(args, tt, W, lib, _lib):
                   "measured_on": "a repeat of the timed pass with HIP events around every merged GEMM launch"}
 
     big = int(np.argmax([system_cost(sy) for sy in systems]))
-    roof = solo_roofline(tt, lib, _lib, items[big], z0[big], args.precision, args.eps)
+    roof, roof_orth = solo_roofline(tt, lib, _lib, items[big], z0[big], args.precision, args.eps)
     roof["measured_on"] = ("single solve of the largest batch system (index %d: d=%d, n=%d, rank cap %d); "
                            "inside the batch the GEMMs are items of merged launches"
                            % (big, systems[big]["d"], systems[big]["n"], systems[big]["rmax"]))
@@ -272,6 +293,7 @@ def run_batch(args, tt, W, lib, _lib):
         "error_vs_plain_fp64_median": float(np.median(err)) if err else None,
         "roofline": roof,
         "roofline_batched_gemm": roof_b,
+        "roofline_orthogonalisation": roof_orth,
         "cohort": cohort,
     })
     if not args.no_cpu_baseline:
@@ -289,7 +311,6 @@ def run_potential(args, tt, W, lib, _lib):
         torch.manual_seed(0)
         tt.solvers.amen_solve(Ag, bg, eps=args.eps, rmax=80, precision=args.precision)
     torch.cuda.synchronize()
-    _lib.check("profile_enable", lib.ttamen_profile_enable(1))
     lat, sols, stats = [], [], []
     t_start = time.perf_counter()
     for _ in range(args.steps):
@@ -302,8 +323,14 @@ def run_potential(args, tt, W, lib, _lib):
         stats.append(dict(tt.solvers.last_stats))
     torch.cuda.synchronize()
     total = time.perf_counter() - t_start
+    # rooflines from one more solve with the library's timing events on (kept out of the timed steps)
+    _lib.check("profile_enable", lib.ttamen_profile_enable(1))
+    torch.manual_seed(0)
+    tt.solvers.amen_solve(Ag, bg, eps=args.eps, rmax=80, precision=args.precision)
+    torch.cuda.synchronize()
     prof = (ctypes.c_double * 8)()
     _lib.check("profile_read", lib.ttamen_profile_read(prof))
+    roof_orth = ortho_roofline(lib, _lib, args.precision)
     lib.ttamen_profile_enable(0)
     res = np.array([tt.solvers.residual(Ag, x, bg) for x in sols])
     ok = int(np.sum(res <= RES_BAR))
@@ -315,7 +342,7 @@ def run_potential(args, tt, W, lib, _lib):
     out.update({"p50_ms": float(np.percentile(lat_ms, 50)), "p99_ms": float(np.percentile(lat_ms, 99)),
                 "residual_max": float(res.max()), "solved_le_1e-8": ok, "sweeps": stats[-1]["sweeps"],
                 "gmres_iters": stats[-1]["gmres_iters"], "max_rank": stats[-1]["max_rank"],
-                "roofline": roofline(prof, args.precision)})
+                "roofline": roofline(prof, args.precision), "roofline_orthogonalisation": roof_orth})
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline([(A, b, 80)], args.eps, args, budget_s=0.0)
     return out
@@ -337,11 +364,14 @@ def run_precision(args, tt, W, lib, _lib):
         for _ in range(args.warmup):
             tt.solvers.amen_solve(Ag, bg, eps=args.eps, rmax=128, precision=prec, _z0=z)
         torch.cuda.synchronize()
-        _lib.check("profile_enable", lib.ttamen_profile_enable(1))
         t0 = time.perf_counter()
         x = tt.solvers.amen_solve(Ag, bg, eps=args.eps, rmax=128, precision=prec, _z0=z)
         torch.cuda.synchronize()
         dt = time.perf_counter() - t0
+        # roofline from one more solve with the library's timing events on
+        _lib.check("profile_enable", lib.ttamen_profile_enable(1))
+        tt.solvers.amen_solve(Ag, bg, eps=args.eps, rmax=128, precision=prec, _z0=z)
+        torch.cuda.synchronize()
         prof = (ctypes.c_double * 8)()
         _lib.check("profile_read", lib.ttamen_profile_read(prof))
         lib.ttamen_profile_enable(0)
@@ -381,12 +411,13 @@ def run_solo(args, tt, W, lib, _lib):
     # the same residual-proxy draws as in the batched run (drawn system by system)
     z0 = [tt.solvers._draw_z([c.shape[1] for c in s["b"]], 4, 0, "cpu") for s in systems[:big + 1]][big]
     t0 = time.perf_counter()
-    roof = solo_roofline(tt, lib, _lib, item, z0, args.precision, args.eps)
+    roof, roof_orth = solo_roofline(tt, lib, _lib, item, z0, args.precision, args.eps)
     total = time.perf_counter() - t0
     cfg = {"workload": "single solve of the largest batch system (index %d: d=%d, n=%d, rank cap %d), %s"
                        % (big, sy["d"], sy["n"], sy["rmax"], args.precision), "calibration_copy_bytes": 64 << 20}
     out = base_record(args, 1.0 / total, total, cfg, dtype_of(args.precision))
     out["roofline"] = roof
+    out["roofline_orthogonalisation"] = roof_orth
     return out
 
 

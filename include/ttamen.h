@@ -460,6 +460,10 @@ int ttamen_gemm_f64(int64_t M, int64_t N, int64_t K, int64_t batch, double alpha
  * Process-global and lock-protected (batched solves serialise their timed launches).
  */
 int ttamen_profile_enable(int enable);
+/* Orthogonalisation of the local GMRES while profiling is enabled (no counterpart in the
+ * reference): out3 = Arnoldi steps timed (HIP events around the CGS dots, update and close
+ * launches of a step), their total ms, and their algorithmic bytes ((2k+7) N w for step k). */
+int ttamen_profile_read_ortho(double* out3);
 int ttamen_profile_read(double* out4);
 
 #ifdef __cplusplus

@@ -123,6 +123,7 @@ _SIGNATURES = {
     "ttamen_gmres_solve_workspace_size": (c_size, [c_i64] * 6),
     "ttamen_device_info": (ctypes.c_int, [ctypes.c_char_p, c_size]),
     "ttamen_cohort_gemm_profile": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(ctypes.c_double)]),
+    "ttamen_profile_read_ortho": (ctypes.c_int, [ctypes.POINTER(ctypes.c_double)]),
     "ttamen_gemm_autotune": (ctypes.c_int, [ctypes.c_int]),
     "ttamen_gemm_autotune_stats": (ctypes.c_int, [ctypes.POINTER(ctypes.c_double)]),
     "ttamen_gemm_autotune_clear": (None, []),

@@ -1776,6 +1776,14 @@ int ttamen_profile_enable(int enable) {
     g_prof.enabled = enable != 0;
     g_prof.total_ms = 0.0; g_prof.flops = 0.0; g_prof.bytes = 0.0; g_prof.count = 0;
     g_prof.total_ms32 = 0.0; g_prof.flops32 = 0.0; g_prof.bytes32 = 0.0; g_prof.count32 = 0;
+    ortho_profile_enable(enable != 0);
+  });
+}
+
+int ttamen_profile_read_ortho(double* out3) {
+  return abi_call([&] {
+    TT_CHECK(out3, ERR_INVALID, "null argument");
+    ortho_profile_read(out3);
   });
 }
 

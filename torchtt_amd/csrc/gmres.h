@@ -77,6 +77,11 @@ class HostSync {
 
 GmresDev gmres_alloc(Arena& ws, int m, int64_t N, int esize);
 
+// Orthogonalisation timing of single-system solves: enable clears the counters; read gives
+// Arnoldi steps timed, their total ms and their algorithmic bytes.
+void ortho_profile_enable(bool on);
+void ortho_profile_read(double* out3);
+
 // Solve Op dx = rhs from dx = 0 (x must be zeroed by the caller), restarted
 // `resets` times, stopping when |beta_{k+1}| / ||rhs|| <= real_tol*norm_rhs/dnorm.
 template <typename T>

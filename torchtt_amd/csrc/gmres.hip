@@ -18,6 +18,8 @@
 // inner solve); dot products and the Hessenberg are accumulated in double.
 #include "ttamen_internal.h"
 #include "gmres.h"
+#include <mutex>
+#include <vector>
 #include "cohort.h"
 #include "kernels_gmres.h"
 #include <algorithm>
@@ -42,6 +44,43 @@ GmresDev gmres_alloc(Arena& ws, int m, int64_t N, int esize) {
   return g;
 }
 
+// Timing of the orthogonalisation (outside cohorts, while enabled): HIP events around the three
+// launches of each Arnoldi step (CGS dots, update with the second pass's dots, close), and their
+// algorithmic bytes (2 k + 7) N w for step k: the dots read k+1 basis vectors and w, the update
+// reads them and w and writes w, the close reads w and writes the new basis vector. Steps enqueued
+// after convergence (no-ops) are dropped once the cycle's executed step count is read back.
+namespace {
+struct OrthoProfile {
+  std::mutex mu;
+  bool on = false;
+  double steps = 0, ms = 0, bytes = 0;
+  std::vector<hipEvent_t> pool;
+  hipEvent_t get() {
+    if (pool.empty()) {
+      hipEvent_t e;
+      TT_HIP(hipEventCreate(&e));
+      return e;
+    }
+    hipEvent_t e = pool.back();
+    pool.pop_back();
+    return e;
+  }
+} g_oprof;
+}  // namespace
+
+void ortho_profile_enable(bool on) {
+  std::lock_guard<std::mutex> lk(g_oprof.mu);
+  g_oprof.on = on;
+  g_oprof.steps = g_oprof.ms = g_oprof.bytes = 0;
+}
+
+void ortho_profile_read(double* out3) {
+  std::lock_guard<std::mutex> lk(g_oprof.mu);
+  out3[0] = g_oprof.steps;
+  out3[1] = g_oprof.ms;
+  out3[2] = g_oprof.bytes;
+}
+
 template <typename T>
 GmresResult gmres_run(GmresDev g, int64_t N, T* x, const T* rhs, const double* norm_rhs,
                       const double* dnorm, double real_tol, int resets, const GmresOp<T>& op,
@@ -63,6 +102,13 @@ GmresResult gmres_run(GmresDev g, int64_t N, T* x, const T* rhs, const double* n
   TT_LAUNCH_CHECK();
   const bool cohort = current_recorder() != nullptr;
   const int poll = std::max(1, cohort_params().gmres_poll);
+  bool prof;
+  {
+    std::lock_guard<std::mutex> lk(g_oprof.mu);
+    prof = g_oprof.on && !cohort;
+  }
+  struct Tag { hipEvent_t e0, e1; int k; };
+  std::vector<Tag> tags;
   for (int cyc = 0; cyc < resets; ++cyc) {
     // r = rhs - Op(x); first cycle starts from x = 0 so r = rhs.
     const T* r = rhs;
@@ -86,11 +132,22 @@ GmresResult gmres_run(GmresDev g, int64_t N, T* x, const T* rhs, const double* n
       double* Hcol = g.H + (int64_t)k * (g.m + 1);
       const double* Pw = P1 + (size_t)(k + 1) * kGmresMaxBlocks;  // row kc of P1: w.w
       // pass 1: project out the basis
+      Tag tag{nullptr, nullptr, k};
+      if (prof) {
+        std::lock_guard<std::mutex> lk(g_oprof.mu);
+        tag.e0 = g_oprof.get();
+        tag.e1 = g_oprof.get();
+        TT_HIP(hipEventRecord(tag.e0, st));
+      }
       launch<KCgsDots<T>>(dim3(nb), dim3(kGB), 0, st, {Q, N, k + 1, w, P1, done});
       launch<KCgsUpdate<T>>(dim3(nb), dim3(kGB), 0, st, {Q, N, k + 1, w, P1, Hcol, Pn1, P2, done});
       launch<KGmresClose<T>>(dim3(nb), dim3(kGB), 0, st,
                              {Q, N, k + 1, w, qn, Pw, Pn1, P2, Hcol, g, k, done, g.istate + GM_REORTHS});
       TT_LAUNCH_CHECK();
+      if (prof) {
+        TT_HIP(hipEventRecord(tag.e1, st));
+        tags.push_back(tag);
+      }
       hs.post(done, k, st);
       if (cohort) {
         // Inside a cohort a host read suspends the member until its queue has run: poll
@@ -111,6 +168,21 @@ GmresResult gmres_run(GmresDev g, int64_t N, T* x, const T* rhs, const double* n
     for (int i = 0; i < 8; ++i) state[i] = hs.host_int()[i];
     have_state = true;
     res.cycle_steps.push_back(state[GM_KCOLS]);
+    if (prof) {  // the stream has reached the state read: every event of this cycle is complete
+      std::lock_guard<std::mutex> lk(g_oprof.mu);
+      for (const Tag& t : tags) {
+        if (t.k < state[GM_KCOLS]) {
+          float ms = 0;
+          TT_HIP(hipEventElapsedTime(&ms, t.e0, t.e1));
+          g_oprof.ms += ms;
+          g_oprof.steps += 1;
+          g_oprof.bytes += (2.0 * t.k + 7.0) * (double)N * sizeof(T);
+        }
+        g_oprof.pool.push_back(t.e0);
+        g_oprof.pool.push_back(t.e1);
+      }
+      tags.clear();
+    }
     res.cycles = cyc + 1;
     if (state[GM_FLAG]) break;
   }
