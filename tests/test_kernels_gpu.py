@@ -47,7 +47,10 @@ def _rel(a, b):
 # ----------------------------------------------------------------- GEMM
 @pytest.mark.parametrize("M,N,K,batch,trans", [(64, 64, 64, 1, "nn"), (100, 37, 260, 3, "tn"),
                                                (5, 7, 3, 2, "nt"), (84, 84, 5376, 1, "tn"),
-                                               (300, 20, 1000, 1, "tt"), (1, 1, 1, 1, "nn")])
+                                               (300, 20, 1000, 1, "tt"), (1, 1, 1, 1, "nn"),
+                                               (64, 48, 96, 2, "tn"), (64, 48, 96, 2, "nt"),
+                                               (64, 48, 96, 2, "tt"), (130, 66, 34, 3, "nn"),
+                                               (130, 67, 34, 2, "tt"), (129, 66, 35, 2, "nt")])
 def test_gemm_f64(M, N, K, batch, trans):
     L, lib = _lib()
     _say("gemm", M, N, K, batch, trans)
@@ -67,6 +70,34 @@ def test_gemm_f64(M, N, K, batch, trans):
     torch.cuda.synchronize()
     want = 1.5 * torch.bmm(A, B) - 0.5 * C0
     assert _rel(Cd.cpu().numpy(), want.numpy()) < 1e-13
+
+
+@pytest.mark.parametrize("trans", ["nn", "tn", "nt", "tt"])
+def test_gemm_f64_unaligned_base(trans):
+    """Operands starting one element past a 16-byte boundary (even sizes and strides): the tile
+    must stage them element by element."""
+    L, lib = _lib()
+    M, N, K, batch = 64, 32, 48, 2
+    _say("gemm unaligned base", trans)
+    g = torch.Generator().manual_seed(5)
+    A = torch.randn(batch, M, K, dtype=torch.float64, generator=g)
+    B = torch.randn(batch, K, N, dtype=torch.float64, generator=g)
+    Ab = torch.zeros(batch * M * K + 1, dtype=torch.float64, device=DEV)
+    Bb = torch.zeros(batch * K * N + 1, dtype=torch.float64, device=DEV)
+    Ab[1:] = (A.transpose(1, 2).contiguous() if trans[0] == "t" else A).reshape(-1).to(DEV)
+    Bb[1:] = (B.transpose(1, 2).contiguous() if trans[1] == "t" else B).reshape(-1).to(DEV)
+    Cd = torch.zeros(batch, M, N, dtype=torch.float64, device=DEV)
+    sa = (1, M) if trans[0] == "t" else (K, 1)
+    sb = (1, K) if trans[1] == "t" else (N, 1)
+    ws = _ws(16 << 20)
+    pa = Ab.data_ptr() + 8
+    pb = Bb.data_ptr() + 8
+    code = lib.ttamen_gemm_f64(M, N, K, batch, 1.0, ctypes.c_void_p(pa), sa[0], sa[1], M * K,
+                               ctypes.c_void_p(pb), sb[0], sb[1], K * N, 0.0, _p(Cd), N, 1, M * N,
+                               _p(ws), ws.numel(), _stream())
+    L.check("gemm", code)
+    torch.cuda.synchronize()
+    assert _rel(Cd.cpu().numpy(), torch.bmm(A, B).numpy()) < 1e-13
 
 
 # ----------------------------------------------------------------- local operator

@@ -11,8 +11,8 @@ namespace ttamen {
 constexpr int kGB = 256;  // threads per block (kGB >= kGmresMaxBlocks)
 static_assert(kGB >= kGmresMaxBlocks, "one thread per partial in the block reductions");
 
-inline int gm_blocks(int64_t N) {
-  int64_t b = (N + kGB * 4 - 1) / (kGB * 4);
+inline int gm_blocks(int64_t N) {  // about two elements per thread, at most kGmresMaxBlocks blocks
+  int64_t b = (N + kGB * 2 - 1) / (kGB * 2);
   return (int)std::max<int64_t>(1, std::min<int64_t>(b, kGmresMaxBlocks));
 }
 
@@ -25,20 +25,19 @@ __device__ __forceinline__ double gm_wave_sum(double v) {
 __device__ __forceinline__ bool gm_skip(const int* a) { return a && *a; }
 
 // out[i] = sum_b P[i * kGmresMaxBlocks + b] over b < nb, for i < rows; every thread of
-// the block gets the results in LDS. Fixed reduction tree: deterministic.
+// the block gets the results in LDS. Wave wv reduces rows wv, wv + 4, ...; lane l adds the
+// partials l, l + 64, l + 128, l + 192 (four independent loads), then the wave's xor tree.
+// Fixed order: deterministic, and the same in every block.
 __device__ __forceinline__ void gm_reduce_rows(const double* P, int rows, int nb, double* out) {
-  __shared__ double red[kGB / 64][kGmresMaxCols + 1];
+  static_assert(kGmresMaxBlocks <= 4 * 64, "four partials per lane");
   const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
-  for (int i = 0; i < rows; ++i) {
-    double v = (t < nb) ? P[(int64_t)i * kGmresMaxBlocks + t] : 0.0;
-    v = gm_wave_sum(v);
-    if (lane == 0) red[wv][i] = v;
-  }
-  __syncthreads();
-  for (int i = t; i < rows; i += kGB) {
-    double s = 0.0;
-    for (int w = 0; w < kGB / 64; ++w) s += red[w][i];
-    out[i] = s;
+  for (int i = wv; i < rows; i += kGB / 64) {
+    const double* row = P + (int64_t)i * kGmresMaxBlocks;
+    double p[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) p[q] = (lane + 64 * q < nb) ? row[lane + 64 * q] : 0.0;
+    const double v = gm_wave_sum((p[0] + p[1]) + (p[2] + p[3]));
+    if (lane == 0) out[i] = v;
   }
   __syncthreads();
 }
@@ -55,20 +54,47 @@ __device__ __forceinline__ bool gm_need_reorth(const double* Pw, const double* P
 }
 
 // Per-block partial dot products of the rows Q_0..Q_{rows-1} (and optionally of w itself)
-// with the block's chunk [lo, hi) of w: P[i * kGmresMaxBlocks + blk]. Each wave streams whole
-// rows over the chunk (coalesced); the chunk of w stays in L1.
+// with the block's chunk [lo, hi) of w: P[i * kGmresMaxBlocks + blk]. The block walks the chunk
+// once per group of kDotRows rows, each thread holding one accumulator per row of the group, so
+// that a thread has the group's loads in flight together (the chunk of w is re-read from cache).
+constexpr int kDotRows = 8;
 template <typename T>
 __device__ __forceinline__ void gm_chunk_dots(const T* Q, int64_t N, int rows, const T* w,
                                               bool with_ww, int64_t lo, int64_t hi, double* P,
                                               unsigned blk) {
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  __shared__ double red[kGB / 64][kDotRows];
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
   const int nr = rows + (with_ww ? 1 : 0);
-  for (int i = wv; i < nr; i += kGB / 64) {
-    const T* row = (i < rows) ? Q + (int64_t)i * N : w;
-    double acc = 0.0;
-    for (int64_t j = lo + lane; j < hi; j += 64) acc += (double)row[j] * (double)w[j];
-    acc = gm_wave_sum(acc);
-    if (lane == 0) P[(int64_t)i * kGmresMaxBlocks + blk] = acc;
+  for (int i0 = 0; i0 < nr; i0 += kDotRows) {
+    const T* src[kDotRows];
+#pragma unroll
+    for (int q = 0; q < kDotRows; ++q) {
+      const int i = min(i0 + q, nr - 1);  // rows past the last repeat it; not stored
+      src[q] = (i < rows) ? Q + (int64_t)i * N : w;
+    }
+    double acc[kDotRows];
+#pragma unroll
+    for (int q = 0; q < kDotRows; ++q) acc[q] = 0.0;
+    for (int64_t j = lo + t; j < hi; j += kGB) {
+      const double wj = (double)w[j];
+      double v[kDotRows];
+#pragma unroll
+      for (int q = 0; q < kDotRows; ++q) v[q] = (double)src[q][j];
+#pragma unroll
+      for (int q = 0; q < kDotRows; ++q) acc[q] += v[q] * wj;
+    }
+#pragma unroll
+    for (int q = 0; q < kDotRows; ++q) {
+      const double s = gm_wave_sum(acc[q]);
+      if (lane == 0) red[wv][q] = s;
+    }
+    __syncthreads();
+    if (t < kDotRows && i0 + t < nr) {
+      double s = 0.0;
+      for (int w4 = 0; w4 < kGB / 64; ++w4) s += red[w4][t];
+      P[(int64_t)(i0 + t) * kGmresMaxBlocks + blk] = s;
+    }
+    __syncthreads();
   }
 }
 
@@ -79,7 +105,15 @@ __device__ __forceinline__ double gm_chunk_update(const T* Q, int64_t N, int kc,
   double acc = 0.0;
   for (int64_t j = lo + threadIdx.x; j < hi; j += kGB) {
     double v = (double)w[j];
-    for (int i = 0; i < kc; ++i) v -= h[i] * (double)Q[(int64_t)i * N + j];
+    int i = 0;
+    for (; i + kDotRows <= kc; i += kDotRows) {  // a group's loads in flight together, same order
+      double q[kDotRows];
+#pragma unroll
+      for (int u = 0; u < kDotRows; ++u) q[u] = (double)Q[(int64_t)(i + u) * N + j];
+#pragma unroll
+      for (int u = 0; u < kDotRows; ++u) v -= h[i + u] * q[u];
+    }
+    for (; i < kc; ++i) v -= h[i] * (double)Q[(int64_t)i * N + j];
     w[j] = (T)v;
     acc += v * v;
   }
