@@ -110,11 +110,11 @@ __device__ void KCholesky::run(const Args& a, const Blk&) {
     }
     for (int c = j + 1 + tid; c < n; c += nt) W[j * n + c] = isdead ? 0.0 : W[j * n + c] / piv;
     __syncthreads();
-    if (!isdead) {
-      const int rem = n - j - 1;
-      for (int t = tid; t < rem * rem; t += nt) {
-        int a = j + 1 + t / rem, c = j + 1 + t % rem;
-        if (a <= c) W[a * n + c] -= W[j * n + a] * W[j * n + c];
+    if (!isdead) {  // upper triangle of the trailing block, the threads as a 32 x 32 grid
+      for (int a = j + 1 + (tid >> 5); a < n; a += nt >> 5) {
+        const double wa = W[j * n + a];
+        for (int c = j + 1 + (tid & 31); c < n; c += 32)
+          if (a <= c) W[a * n + c] -= wa * W[j * n + c];
       }
     }
     __syncthreads();
@@ -131,15 +131,17 @@ __device__ void KCholesky::run(const Args& a, const Blk&) {
 
 // ---------------------------------------------------------------- triangular solve
 // X R = A for X (m x n row-major), R upper triangular (n x n row-major). One thread per
-// row; R is staged in LDS in packed upper-triangular form and read by broadcast, and each
-// thread keeps the part of its row solved so far in LDS (column-major over the block's 64
-// rows, so the reads are conflict-free) instead of re-reading it from X.
+// row, 64 rows (one wave) per block; R is staged in LDS in packed upper-triangular form by
+// all kTrsmThreads threads (the other waves leave after the staging) and read by broadcast,
+// and each thread keeps the part of its row solved so far in LDS (column-major over the
+// block's 64 rows, so the reads are conflict-free) instead of re-reading it from X.
 // In-place (X == A with unit column stride and row stride n) is allowed.
 __device__ __forceinline__ int packed_idx(int i, int j, int n) {
   return i * n - (i * (i - 1)) / 2 + (j - i);
 }
 
 constexpr int kTrsmRowsPerBlock = 64;
+constexpr int kTrsmThreads = 256;
 
 inline size_t trsm_lds_bytes(int64_t n) {
   return ((size_t)n * (n + 1) / 2 + (size_t)kTrsmRowsPerBlock * n) * sizeof(double);
@@ -149,7 +151,7 @@ struct KTrsmRows {
   struct Args {
     const double* A; int64_t sr, sc, m; int n; const double* R; double* X; const int* skip;
   };
-  static constexpr int kThreads = kTrsmRowsPerBlock;
+  static constexpr int kThreads = kTrsmThreads;
   static __device__ void run(const Args& a, const Blk& bk);
 };
 __device__ void KTrsmRows::run(const Args& args, const Blk& bk) {
@@ -158,29 +160,27 @@ __device__ void KTrsmRows::run(const Args& args, const Blk& bk) {
   if (skip && *skip) return;
   extern __shared__ double P[];
   double* Xs = P + (size_t)n * (n + 1) / 2;  // Xs[i * 64 + t]: solved x[i] of thread t's row
-  // Stage R with 8 independent loads in flight per thread (one wave: a plain strided loop
-  // would wait one memory round trip per element it stages).
+  // Stage the upper triangle of R with 8 independent loads in flight per thread (a plain
+  // strided loop would wait one memory round trip per element it stages).
   const int nn = n * n;
   for (int t0 = threadIdx.x; t0 < nn; t0 += 8 * blockDim.x) {
     double v[8];
+    int at[8];
 #pragma unroll
     for (int u = 0; u < 8; ++u) {
       const int t = t0 + u * (int)blockDim.x;
-      v[u] = (t < nn) ? R[t] : 0.0;
+      const int i = t / n, j = t - i * n;
+      at[u] = (t < nn && i <= j) ? packed_idx(i, j, n) : -1;
+      v[u] = (at[u] >= 0) ? R[t] : 0.0;
     }
 #pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      const int t = t0 + u * (int)blockDim.x;
-      if (t < nn) {
-        const int i = t / n, j = t - i * n;
-        if (i <= j) P[packed_idx(i, j, n)] = v[u];
-      }
-    }
+    for (int u = 0; u < 8; ++u)
+      if (at[u] >= 0) P[at[u]] = v[u];
   }
   __syncthreads();
   const int t = threadIdx.x;
   const int64_t row = (int64_t)bk.x * kTrsmRowsPerBlock + t;
-  if (row >= m) return;
+  if (t >= kTrsmRowsPerBlock || row >= m) return;
   const double* a = A + row * sr;
   double* x = X + row * n;
   for (int J = 0; J < n; J += 16) {
@@ -188,7 +188,24 @@ __device__ void KTrsmRows::run(const Args& args, const Blk& bk) {
     double acc[16];
 #pragma unroll
     for (int c = 0; c < 16; ++c) acc[c] = (c < jb) ? a[(int64_t)(J + c) * sc] : 0.0;
-    for (int i = 0; i < J; ++i) {
+    int i = 0;
+    for (; i + 1 < J; i += 2) {  // two rows of R per round of LDS reads, applied in order
+      const double x0 = Xs[i * kTrsmRowsPerBlock + t], x1 = Xs[(i + 1) * kTrsmRowsPerBlock + t];
+      const int b0 = packed_idx(i, J, n), b1 = packed_idx(i + 1, J, n);
+      double p0[16], p1[16];
+#pragma unroll
+      for (int c = 0; c < 16; ++c) {
+        p0[c] = (c < jb) ? P[b0 + c] : 0.0;
+        p1[c] = (c < jb) ? P[b1 + c] : 0.0;
+      }
+#pragma unroll
+      for (int c = 0; c < 16; ++c)
+        if (c < jb) {
+          acc[c] -= x0 * p0[c];
+          acc[c] -= x1 * p1[c];
+        }
+    }
+    for (; i < J; ++i) {
       const double xi = Xs[i * kTrsmRowsPerBlock + t];
       const int base = packed_idx(i, J, n);
 #pragma unroll
@@ -502,7 +519,7 @@ int qr(const double* A, int64_t sr, int64_t sc, int64_t m, int64_t n, double* Q,
                         {G, (int)n, shift_c, dead_rel, (int)(pass == kQrPasses - 1), Rk, dead, ndead,
                          skip, flags + pass + 1});
       TT_LAUNCH_CHECK();
-      launch<KTrsmRows>(dim3(trsm_blocks), dim3(kTrsmRowsPerBlock), trsm_lds, st,
+      launch<KTrsmRows>(dim3(trsm_blocks), dim3(kTrsmThreads), trsm_lds, st,
                         {src, ssr, ssc, m, (int)n, Rk, Q, skip});
       TT_LAUNCH_CHECK();
     }
