@@ -159,13 +159,26 @@ __device__ void KTruncPrefix::run(const Args& a, const Blk& bk) {
   __syncthreads();
   if (!last) return;
   __threadfence();
-  if (threadIdx.x < nv) {
-    volatile double* vp = partial;
-    double s = 0.0;
-    for (unsigned b = 0; b < bk.gx; ++b) s += vp[b * 32 + threadIdx.x];
-    out[threadIdx.x] = sqrt(s) / (*nrhs);
+  // The last block: the threads fetch the partials of 32 blocks at a time (four loads each),
+  // thread c < nv adds candidate c's partials in block order.
+  __shared__ double parts[32][32];
+  volatile double* vp = partial;
+  double s = 0.0;
+  const int pr = threadIdx.x >> 5, pc = threadIdx.x & 31;
+  for (unsigned b0 = 0; b0 < bk.gx; b0 += 32) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const unsigned b = b0 + pr + 8 * q;
+      if (b < bk.gx) parts[pr + 8 * q][pc] = vp[b * 32 + pc];
+    }
+    __syncthreads();
+    if (threadIdx.x < nv) {
+      const unsigned nb = min(32u, bk.gx - b0);
+      for (unsigned r = 0; r < nb; ++r) s += parts[r][threadIdx.x];
+    }
+    __syncthreads();
   }
-  __syncthreads();
+  if (threadIdx.x < nv) out[threadIdx.x] = sqrt(s) / (*nrhs);
   if (threadIdx.x == 0) *counter = 0u;
 }
 

@@ -42,11 +42,24 @@ struct KSumsq {
     if (p.skip && *p.skip) return;
     __shared__ double sh[4];
     __shared__ bool last;
+    __shared__ double parts[256];
     double acc = 0.0;
-    for (int64_t i = bk.x * 256 + threadIdx.x; i < p.n; i += (int64_t)bk.gx * 256) {
-      double v = (double)p.x[i];
-      if (p.y) v -= (double)p.y[i];
-      acc += v * v;
+    // Grid-stride order, four strides' loads in flight at once.
+    const int64_t stride = (int64_t)bk.gx * 256;
+    for (int64_t i0 = bk.x * 256 + threadIdx.x; i0 < p.n; i0 += 4 * stride) {
+      double v[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int64_t i = i0 + u * stride;
+        v[u] = 0.0;
+        if (i < p.n) {
+          v[u] = (double)p.x[i];
+          if (p.y) v[u] -= (double)p.y[i];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        if (i0 + u * stride < p.n) acc += v[u] * v[u];
     }
     double bs = block_sum256(acc, sh);
     if (threadIdx.x == 0) {
@@ -58,10 +71,19 @@ struct KSumsq {
     __syncthreads();
     if (!last) return;
     __threadfence();
+    // The last block: the threads fetch 256 partials at a time, thread 0 adds them in block order.
+    volatile double* vp = p.partial;
+    double s = 0.0;
+    for (unsigned b0 = 0; b0 < bk.gx; b0 += 256) {
+      if (b0 + threadIdx.x < bk.gx) parts[threadIdx.x] = vp[b0 + threadIdx.x];
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        const unsigned nb = min(256u, bk.gx - b0);
+        for (unsigned b = 0; b < nb; ++b) s += parts[b];
+      }
+      __syncthreads();
+    }
     if (threadIdx.x == 0) {
-      double s = 0.0;
-      volatile double* vp = p.partial;
-      for (unsigned b = 0; b < bk.gx; ++b) s += vp[b];
       double r = sqrt(s);
       if (p.den) r = r / (*p.den);
       if (p.guard && !(r > 0.0)) r = 1.0;
