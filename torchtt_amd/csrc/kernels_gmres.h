@@ -31,26 +31,41 @@ __device__ __forceinline__ bool gm_skip(const int* a) { return a && *a; }
 __device__ __forceinline__ void gm_reduce_rows(const double* P, int rows, int nb, double* out) {
   static_assert(kGmresMaxBlocks <= 4 * 64, "four partials per lane");
   const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
-  for (int i = wv; i < rows; i += kGB / 64) {
-    const double* row = P + (int64_t)i * kGmresMaxBlocks;
-    double p[4];
+  constexpr int W = kGB / 64;
+  for (int i = wv; i < rows; i += 2 * W) {  // rows i and i + W: eight loads in flight
+    const bool two = i + W < rows;
+    const double* r0 = P + (int64_t)i * kGmresMaxBlocks;
+    const double* r1 = P + (int64_t)(two ? i + W : i) * kGmresMaxBlocks;
+    double p[4], q[4];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) p[q] = (lane + 64 * q < nb) ? row[lane + 64 * q] : 0.0;
-    const double v = gm_wave_sum((p[0] + p[1]) + (p[2] + p[3]));
-    if (lane == 0) out[i] = v;
+    for (int u = 0; u < 4; ++u) {
+      const bool in = lane + 64 * u < nb;
+      p[u] = in ? r0[lane + 64 * u] : 0.0;
+      q[u] = in ? r1[lane + 64 * u] : 0.0;
+    }
+    const double v0 = gm_wave_sum((p[0] + p[1]) + (p[2] + p[3]));
+    const double v1 = gm_wave_sum((q[0] + q[1]) + (q[2] + q[3]));
+    if (lane == 0) {
+      out[i] = v0;
+      if (two) out[i + W] = v1;
+    }
   }
   __syncthreads();
 }
 
-// Whether the second (re-orthogonalisation) pass runs: the first pass cancelled more
-// than 1 - 1/sqrt(2) of ||w|| (Daniel, Gragg, Kaufman and Stewart, Math. Comp. 30 (1976)).
-// Pw: row kc of the first dot partials (w.w); Pn1: ||w'||^2 partials after pass 1.
-__device__ __forceinline__ bool gm_need_reorth(const double* Pw, const double* Pn1, int nb) {
-  __shared__ double r2[2];
-  gm_reduce_rows(Pw, 1, nb, r2);
-  __shared__ double r1[2];
-  gm_reduce_rows(Pn1, 1, nb, r1);
-  return !(sqrt(r1[0]) > 0.7071067811865476 * sqrt(r2[0]));
+// out[0] = sum of the partials of row Pa, out[1] = of row Pb (each as gm_reduce_rows does it),
+// in one round: wave 0 reduces Pa, wave 1 reduces Pb.
+__device__ __forceinline__ void gm_reduce_pair(const double* Pa, const double* Pb, int nb, double* out) {
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  if (wv < 2) {
+    const double* row = wv == 0 ? Pa : Pb;
+    double p[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) p[u] = (lane + 64 * u < nb) ? row[lane + 64 * u] : 0.0;
+    const double v = gm_wave_sum((p[0] + p[1]) + (p[2] + p[3]));
+    if (lane == 0) out[wv] = v;
+  }
+  __syncthreads();
 }
 
 // Per-block partial dot products of the rows Q_0..Q_{rows-1} (and optionally of w itself)
@@ -185,9 +200,11 @@ struct KGmresClose {
     __shared__ double h2[kGmresMaxCols + 1];
     __shared__ double nn[2];
     const int kc = a.kc;
-    const bool reorth = gm_need_reorth(a.Pw, a.Pn1, bk.gx);
-    gm_reduce_rows(a.Pn1, 1, bk.gx, nn);
-    double hn2 = nn[0];
+    // Whether the second (re-orthogonalisation) pass runs: the first pass cancelled more than
+    // 1 - 1/sqrt(2) of ||w|| (Daniel, Gragg, Kaufman and Stewart, Math. Comp. 30 (1976)).
+    gm_reduce_pair(a.Pw, a.Pn1, bk.gx, nn);  // ||w||^2, ||w'||^2
+    const bool reorth = !(sqrt(nn[1]) > 0.7071067811865476 * sqrt(nn[0]));
+    double hn2 = nn[1];
     const int64_t chunk = (a.N + bk.gx - 1) / bk.gx;
     const int64_t lo = bk.x * chunk, hi = min(a.N, lo + chunk);
     if (reorth) {
@@ -205,37 +222,54 @@ struct KGmresClose {
     const double hn = sqrt(hn2 > 0.0 ? hn2 : 0.0);
     const double inv = (hn > 0.0) ? 1.0 / hn : 0.0;
     for (int64_t j = lo + threadIdx.x; j < hi; j += kGB) a.qn[j] = (T)((double)a.w[j] * inv);
-    if (bk.x == 0 && threadIdx.x == 0) {
-      const GmresDev g = a.g;
-      const int k = a.k;
-      const int ld = g.m + 1;
-      double* h = g.H + (int64_t)k * ld;
-      h[k + 1] = hn;
+    if (bk.x != 0) return;
+    // Givens rotations of column k (block 0): its entries, the earlier rotations and the
+    // scalars are staged in LDS by all threads at once, then one thread applies them in order.
+    __shared__ double hc[kGmresMaxCols + 1], cs[kGmresMaxCols], sn[kGmresMaxCols], sc[3];
+    const GmresDev g = a.g;
+    const int k = a.k;
+    double* h = g.H + (int64_t)k * (g.m + 1);
+    for (int i = threadIdx.x; i < k; i += kGB) {
+      hc[i] = h[i];
+      cs[i] = g.cs[i];
+      sn[i] = g.sn[i];
+    }
+    if (threadIdx.x == kGB - 1) hc[k] = h[k];
+    if (threadIdx.x == kGB - 2) sc[0] = g.beta[k];
+    if (threadIdx.x == kGB - 3) sc[1] = g.dstate[GM_BNORM];
+    if (threadIdx.x == kGB - 4) sc[2] = g.dstate[GM_THR];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      hc[k + 1] = hn;
       for (int i = 0; i < k; ++i) {
-        double t = g.cs[i] * h[i] + g.sn[i] * h[i + 1];
-        h[i + 1] = -g.sn[i] * h[i] + g.cs[i] * h[i + 1];
-        h[i] = t;
+        double t = cs[i] * hc[i] + sn[i] * hc[i + 1];
+        hc[i + 1] = -sn[i] * hc[i] + cs[i] * hc[i + 1];
+        hc[i] = t;
       }
-      double den = sqrt(h[k] * h[k] + h[k + 1] * h[k + 1]);
-      double c = 1.0, sn = 0.0;
-      if (den > 0.0) { c = h[k] / den; sn = h[k + 1] / den; }
-      h[k] = c * h[k] + sn * h[k + 1];
-      h[k + 1] = 0.0;
+      double den = sqrt(hc[k] * hc[k] + hc[k + 1] * hc[k + 1]);
+      double c = 1.0, s = 0.0;
+      if (den > 0.0) { c = hc[k] / den; s = hc[k + 1] / den; }
+      hc[k] = c * hc[k] + s * hc[k + 1];
+      hc[k + 1] = 0.0;
       g.cs[k] = c;
-      g.sn[k] = sn;
-      g.beta[k + 1] = -sn * g.beta[k];
-      g.beta[k] = c * g.beta[k];
-      const double err = fabs(g.beta[k + 1]) / g.dstate[GM_BNORM];
+      g.sn[k] = s;
+      const double bk0 = sc[0];
+      const double bk1 = -s * bk0;
+      g.beta[k + 1] = bk1;
+      g.beta[k] = c * bk0;
+      const double err = fabs(bk1) / sc[1];
       g.istate[GM_KCOLS] = k + 1;
       g.istate[GM_ITERS] += 1;
       g.dstate[GM_ERR] = err;
-      if (err <= g.dstate[GM_THR]) {
+      if (err <= sc[2]) {
         g.istate[GM_FLAG] = 1;
         g.istate[GM_DONE] = 1;
       } else if (k + 1 >= g.m) {
         g.istate[GM_DONE] = 1;  // cycle exhausted (flag stays 0)
       }
     }
+    __syncthreads();
+    for (int i = threadIdx.x; i <= k + 1; i += kGB) h[i] = hc[i];
   }
 };
 
