@@ -32,9 +32,18 @@ __device__ void KLuPanel::run(const Args& a, const Blk&) {
   __shared__ int s_p;
   const int tid = threadIdx.x, nt = blockDim.x;
   const int rows = n - j0;
-  for (int t = tid; t < rows * jb; t += nt) {
-    int c = t / rows, i = t - c * rows;
-    P[t] = Aug[(int64_t)(j0 + i) * ld + j0 + c];
+  const int tot = rows * jb;
+  for (int t0 = tid; t0 < tot; t0 += 8 * nt) {  // 8 independent loads in flight per thread
+    double v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int t = t0 + u * nt;
+      const int c = t / rows, i = t - c * rows;
+      v[u] = (t < tot) ? Aug[(int64_t)(j0 + i) * ld + j0 + c] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      if (t0 + u * nt < tot) P[t0 + u * nt] = v[u];
   }
   __syncthreads();
   for (int jj = 0; jj < jb; ++jj) {
@@ -75,10 +84,11 @@ __device__ void KLuPanel::run(const Args& a, const Blk&) {
     const double d = P[jj * rows + jj];
     for (int i = jj + 1 + tid; i < rows; i += nt) P[jj * rows + i] /= d;
     __syncthreads();
-    const int rem_r = rows - jj - 1, rem_c = jb - jj - 1;
-    for (int t = tid; t < rem_r * rem_c; t += nt) {
-      int c = jj + 1 + t / rem_r, i = jj + 1 + t % rem_r;
-      P[c * rows + i] -= P[jj * rows + i] * P[c * rows + jj];
+    // rank-1 update of the panel's trailing part: lanes along the rows (conflict-free), waves
+    // over the columns
+    for (int c = jj + 1 + (tid >> 6); c < jb; c += nt >> 6) {
+      const double u = P[c * rows + jj];
+      for (int i = jj + 1 + (tid & 63); i < rows; i += 64) P[c * rows + i] -= P[jj * rows + i] * u;
     }
     __syncthreads();
   }
@@ -144,14 +154,16 @@ __device__ void KLuBacksolve::run(const Args& a, const Blk& bk) {
   double* Aug = a.Aug; const int n = a.n, ld = a.ld, j0 = a.j0, jb = a.jb; double* x = a.x;
   __shared__ double U[kLuNb][kLuNb];
   __shared__ double xb[kLuNb];
+  __shared__ double yb[kLuNb];
   for (int t = threadIdx.x; t < jb * jb; t += blockDim.x) {
     int i = t / jb, c = t - i * jb;
     U[i][c] = Aug[(int64_t)(j0 + i) * ld + j0 + c];
   }
+  for (int i = threadIdx.x; i < jb; i += blockDim.x) yb[i] = Aug[(int64_t)(j0 + i) * ld + n];
   __syncthreads();
   if (threadIdx.x == 0) {
     for (int i = jb - 1; i >= 0; --i) {
-      double v = Aug[(int64_t)(j0 + i) * ld + n];
+      double v = yb[i];
       for (int t = i + 1; t < jb; ++t) v -= U[i][t] * xb[t];
       xb[i] = v / U[i][i];
     }
@@ -159,8 +171,14 @@ __device__ void KLuBacksolve::run(const Args& a, const Blk& bk) {
   __syncthreads();
   for (int i = threadIdx.x; i < jb; i += blockDim.x) x[j0 + i] = xb[i];
   for (int r = bk.x * blockDim.x + threadIdx.x; r < j0; r += bk.gx * blockDim.x) {
-    double v = Aug[(int64_t)r * ld + n];
-    for (int t = 0; t < jb; ++t) v -= Aug[(int64_t)r * ld + j0 + t] * xb[t];
+    const double* row = Aug + (int64_t)r * ld;
+    double u[kLuNb];  // the row's block entries loaded together, applied in order
+#pragma unroll
+    for (int t = 0; t < kLuNb; ++t) u[t] = (t < jb) ? row[j0 + t] : 0.0;
+    double v = row[n];
+#pragma unroll
+    for (int t = 0; t < kLuNb; ++t)
+      if (t < jb) v -= u[t] * xb[t];
     Aug[(int64_t)r * ld + n] = v;
   }
 }
