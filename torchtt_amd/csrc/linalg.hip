@@ -131,17 +131,17 @@ __device__ void KCholesky::run(const Args& a, const Blk&) {
 
 // ---------------------------------------------------------------- triangular solve
 // X R = A for X (m x n row-major), R upper triangular (n x n row-major). One thread per
-// row, 64 rows (one wave) per block; R is staged in LDS in packed upper-triangular form by
-// all kTrsmThreads threads (the other waves leave after the staging) and read by broadcast,
-// and each thread keeps the part of its row solved so far in LDS (column-major over the
-// block's 64 rows, so the reads are conflict-free) instead of re-reading it from X.
+// row and half of each 16-column block, 64 rows and two waves per block; R is staged in LDS in
+// packed upper-triangular form and read by broadcast, and the solved part of each row is kept
+// in LDS (column-major over the block's 64 rows, so the reads are conflict-free) instead of
+// being re-read from X.
 // In-place (X == A with unit column stride and row stride n) is allowed.
 __device__ __forceinline__ int packed_idx(int i, int j, int n) {
   return i * n - (i * (i - 1)) / 2 + (j - i);
 }
 
 constexpr int kTrsmRowsPerBlock = 64;
-constexpr int kTrsmThreads = 256;
+constexpr int kTrsmThreads = 128;
 
 inline size_t trsm_lds_bytes(int64_t n) {
   return ((size_t)n * (n + 1) / 2 + (size_t)kTrsmRowsPerBlock * n) * sizeof(double);
@@ -178,56 +178,91 @@ __device__ void KTrsmRows::run(const Args& args, const Blk& bk) {
       if (at[u] >= 0) P[at[u]] = v[u];
   }
   __syncthreads();
-  const int t = threadIdx.x;
-  const int64_t row = (int64_t)bk.x * kTrsmRowsPerBlock + t;
-  if (t >= kTrsmRowsPerBlock || row >= m) return;
-  const double* a = A + row * sr;
-  double* x = X + row * n;
+  // Wave h of the two solves columns 8h .. 8h+7 of each block of 16 for all 64 rows. Wave 1 applies
+  // the block's first 8 solved columns (in column order, as a one-thread solve would) after wave 0
+  // has published them. Rows past m keep running (zeros) so that every thread meets the barriers.
+  const int r = threadIdx.x & (kTrsmRowsPerBlock - 1), h = threadIdx.x / kTrsmRowsPerBlock;
+  const int c0 = 8 * h;
+  const int64_t row = (int64_t)bk.x * kTrsmRowsPerBlock + r;
+  const bool valid = row < m;
+  const double* a = A + (valid ? row : 0) * sr;
+  double* x = X + (valid ? row : 0) * n;
   for (int J = 0; J < n; J += 16) {
     const int jb = min(16, n - J);
-    double acc[16];
+    double acc[8];
 #pragma unroll
-    for (int c = 0; c < 16; ++c) acc[c] = (c < jb) ? a[(int64_t)(J + c) * sc] : 0.0;
+    for (int u = 0; u < 8; ++u) acc[u] = (valid && c0 + u < jb) ? a[(int64_t)(J + c0 + u) * sc] : 0.0;
     int i = 0;
     for (; i + 1 < J; i += 2) {  // two rows of R per round of LDS reads, applied in order
-      const double x0 = Xs[i * kTrsmRowsPerBlock + t], x1 = Xs[(i + 1) * kTrsmRowsPerBlock + t];
-      const int b0 = packed_idx(i, J, n), b1 = packed_idx(i + 1, J, n);
-      double p0[16], p1[16];
+      const double x0 = Xs[i * kTrsmRowsPerBlock + r], x1 = Xs[(i + 1) * kTrsmRowsPerBlock + r];
+      const int b0 = packed_idx(i, J, n) + c0, b1 = packed_idx(i + 1, J, n) + c0;
+      double p0[8], p1[8];
 #pragma unroll
-      for (int c = 0; c < 16; ++c) {
-        p0[c] = (c < jb) ? P[b0 + c] : 0.0;
-        p1[c] = (c < jb) ? P[b1 + c] : 0.0;
+      for (int u = 0; u < 8; ++u) {
+        p0[u] = (c0 + u < jb) ? P[b0 + u] : 0.0;
+        p1[u] = (c0 + u < jb) ? P[b1 + u] : 0.0;
       }
 #pragma unroll
-      for (int c = 0; c < 16; ++c)
-        if (c < jb) {
-          acc[c] -= x0 * p0[c];
-          acc[c] -= x1 * p1[c];
+      for (int u = 0; u < 8; ++u)
+        if (c0 + u < jb) {
+          acc[u] -= x0 * p0[u];
+          acc[u] -= x1 * p1[u];
         }
     }
     for (; i < J; ++i) {
-      const double xi = Xs[i * kTrsmRowsPerBlock + t];
-      const int base = packed_idx(i, J, n);
+      const double xi = Xs[i * kTrsmRowsPerBlock + r];
+      const int base = packed_idx(i, J, n) + c0;
 #pragma unroll
-      for (int c = 0; c < 16; ++c)
-        if (c < jb) acc[c] -= xi * P[base + c];
+      for (int u = 0; u < 8; ++u)
+        if (c0 + u < jb) acc[u] -= xi * P[base + u];
     }
+    if (h == 0) {  // wave 0 solves and publishes its columns before the barrier below
 #pragma unroll
-    for (int c = 0; c < 16; ++c) {
-      if (c < jb) {
-        const int base = packed_idx(J + c, J + c, n);
-        acc[c] /= P[base];
+      for (int c = 0; c < 8; ++c) {
+        if (c < jb) {
+          const int base = packed_idx(J + c, J + c, n);
+          acc[c] /= P[base];
 #pragma unroll
-        for (int c2 = c + 1; c2 < 16; ++c2)
-          if (c2 < jb) acc[c2] -= acc[c] * P[base + (c2 - c)];
+          for (int c2 = c + 1; c2 < 8; ++c2)
+            if (c2 < jb) acc[c2] -= acc[c] * P[base + (c2 - c)];
+        }
       }
+#pragma unroll
+      for (int c = 0; c < 8; ++c)
+        if (c < jb) Xs[(J + c) * kTrsmRowsPerBlock + r] = acc[c];
     }
+    __syncthreads();
+    if (h == 1) {
 #pragma unroll
-    for (int c = 0; c < 16; ++c)
-      if (c < jb) {
-        x[J + c] = acc[c];
-        Xs[(J + c) * kTrsmRowsPerBlock + t] = acc[c];
+      for (int c = 0; c < 8; ++c) {  // columns J .. J+7, in order
+        if (c < jb) {
+          const double xc = Xs[(J + c) * kTrsmRowsPerBlock + r];
+          const int base = packed_idx(J + c, J + c, n) + (8 - c);
+#pragma unroll
+          for (int u = 0; u < 8; ++u)
+            if (8 + u < jb) acc[u] -= xc * P[base + u];
+        }
       }
+#pragma unroll
+      for (int c = 0; c < 8; ++c) {
+        if (8 + c < jb) {
+          const int base = packed_idx(J + 8 + c, J + 8 + c, n);
+          acc[c] /= P[base];
+#pragma unroll
+          for (int c2 = c + 1; c2 < 8; ++c2)
+            if (8 + c2 < jb) acc[c2] -= acc[c] * P[base + (c2 - c)];
+        }
+      }
+#pragma unroll
+      for (int c = 0; c < 8; ++c)
+        if (8 + c < jb) Xs[(J + 8 + c) * kTrsmRowsPerBlock + r] = acc[c];
+    }
+    if (valid) {
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+        if (c0 + u < jb) x[J + c0 + u] = acc[u];
+    }
+    __syncthreads();  // the block's 16 solved columns are in Xs before the next block reads them
   }
 }
 
