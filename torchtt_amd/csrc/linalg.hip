@@ -753,23 +753,38 @@ __device__ void KJacobiV::run(const Args& a, const Blk& bk) {
   const int npairs = K / 2;
   for (int j = lane; j < K; j += 64) v[j] = (j == rho) ? 1.0 : 0.0;
   __syncthreads();
-  const int nsw = *nsweeps;
-  for (int sw = 0; sw < nsw; ++sw) {
-    for (int t = 0; t < K - 1; ++t) {
-      for (int pi = lane; pi < npairs; pi += 64) {
+  // Round r = sweep * (K-1) + t holds npairs <= 70 (k <= 140) rotations: at most two per lane. Each
+  // lane fetches its entries of round r+1 while it applies round r.
+  static_assert((kMaxSmall + 1) / 2 <= 2 * 64, "two rotations per lane and round");
+  const int rounds = *nsweeps * (K - 1);
+  double2 cur[2], nxt[2];
+  auto fetch = [&](int r, double2* out) {
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      const int pi = lane + 64 * u;
+      out[u] = (r < rounds && pi < npairs) ? log[(size_t)r * npairs + pi] : make_double2(1.0, 0.0);
+    }
+  };
+  fetch(0, cur);
+  for (int r = 0; r < rounds; ++r) {
+    const int t = r % (K - 1);
+    fetch(r + 1, nxt);
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      const int pi = lane + 64 * u;
+      if (pi < npairs) {
         int p, q;
         rr_pair(t, pi, K, p, q);
-        if (q < k) {
-          double2 cs = log[((size_t)sw * (K - 1) + t) * npairs + pi];
-          if (cs.y != 0.0) {
-            double a = v[p], b = v[q];
-            v[p] = cs.x * a - cs.y * b;
-            v[q] = cs.y * a + cs.x * b;
-          }
+        if (q < k && cur[u].y != 0.0) {
+          double a = v[p], b = v[q];
+          v[p] = cur[u].x * a - cur[u].y * b;
+          v[q] = cur[u].y * a + cur[u].x * b;
         }
       }
-      __syncthreads();
     }
+    __syncthreads();
+    cur[0] = nxt[0];
+    cur[1] = nxt[1];
   }
   for (int j = lane; j < k; j += 64) V[(int64_t)rho * ldv + rank_of[j]] = v[j];
 }
