@@ -358,6 +358,8 @@ def _qr_case(kind, m, n, seed):
                                       ("zero_cols", 100, 12), ("dependent", 80, 10), ("graded", 300, 40),
                                       ("tiny_tail", 1000, 40), ("rand", 1, 1), ("rand", 256, 4),
                                       ("rand", 8448, 136),
+                                      # TRSM column blocks: the second wave idle (6, 8), a ragged block (17)
+                                      ("rand", 100, 6), ("graded", 300, 8), ("dependent", 500, 17),
                                       # single-workgroup path (up to 4 columns)
                                       ("zero_cols", 300, 4), ("dependent", 64, 3), ("graded", 256, 4),
                                       ("rand", 4352, 4), ("rand", 7, 2), ("zero_cols", 40, 2),
