@@ -8,8 +8,7 @@
 
 namespace ttamen {
 
-constexpr int kGB = 256;  // threads per block (kGB >= kGmresMaxBlocks)
-static_assert(kGB >= kGmresMaxBlocks, "one thread per partial in the block reductions");
+constexpr int kGB = 256;  // threads per block of the streaming kernels
 
 inline int gm_blocks(int64_t N) {  // about two elements per thread, at most kGmresMaxBlocks blocks
   int64_t b = (N + kGB * 2 - 1) / (kGB * 2);
