@@ -73,6 +73,8 @@ def parse():
     p.add_argument("--eps", type=float, default=1e-8)
     p.add_argument("--rate", type=float, default=200.0)
     p.add_argument("--duration", type=float, default=2.0)
+    p.add_argument("--serve-duration", type=float, default=10.0,
+                   help="batch: seconds of Poisson arrivals in the serving leg after the batch (0: no leg)")
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-threads", type=int, default=0)
     p.add_argument("--autotune", type=int, default=0, choices=[0, 1],
@@ -93,33 +95,59 @@ def load_ref():
     return mod
 
 
-def cpu_baseline(items, eps, args, budget_s=25.0):
+def cpu_baseline(items, eps, args, budget_s=25.0, rng_states=None, gpu=None):
     """The reference's own C++ AMEn (oracle/_ref, built from its sources) on this host's cores,
     solving the given (A, b, rmax) CPU systems one after another until `budget_s` is spent;
-    the CPU oracle restatement if that build is absent."""
+    the CPU oracle restatement if that build is absent.
+
+    rng_states[i] (optional): torch's CPU generator state from which the GPU run drew system i's
+    residual-proxy cores; restored before the reference's solve, so that the reference draws the
+    same cores (cpp/amen_solve.h:174-176) and solves the same problem the GPU solved.
+    gpu (optional): per-system (residual, ranks) of the GPU's results, recorded beside the
+    reference's own residual and ranks (computed outside the timed solves)."""
     threads = args.cpu_threads or min(16, os.cpu_count() or 1)
     torch.set_num_threads(threads)
     ref = load_ref()
-    done, t0 = 0, time.perf_counter()
-    for A, b, rmax in items:
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import amen_oracle as O
+    done, t_solve, sols = 0, 0.0, []
+    for i, (A, b, rmax) in enumerate(items):
         d = len(A)
         rA = [1] + [a.shape[3] for a in A]
         rb = [1] + [c.shape[2] for c in b]
-        torch.manual_seed(0)
-        if ref is not None:
-            ref.amen_solve(A, b, [], [a.shape[2] for a in A], rA, rb, [1] * (d + 1), 22, eps, rmax, 500,
-                           4, 0, 40, 2, False, 0)
+        if rng_states is not None:
+            torch.set_rng_state(rng_states[i])
         else:
-            sys.path.insert(0, os.path.join(ROOT, "oracle"))
-            import amen_oracle as O
-            O.amen_solve([a.numpy() for a in A], [c.numpy() for c in b], eps=eps, rmax=rmax)
+            torch.manual_seed(0)
+        t0 = time.perf_counter()
+        if ref is not None:
+            x = ref.amen_solve(A, b, [], [a.shape[2] for a in A], rA, rb, [1] * (d + 1), 22, eps, rmax, 500,
+                               4, 0, 40, 2, False, 0)
+            x = [c.numpy() for c in x]
+        else:
+            x = O.amen_solve([a.numpy() for a in A], [c.numpy() for c in b], eps=eps, rmax=rmax)
+        t_solve += time.perf_counter() - t0
+        sols.append(x)
         done += 1
-        if time.perf_counter() - t0 > budget_s:
+        if t_solve > budget_s:
             break
-    dt = time.perf_counter() - t0
-    return {"value": done / dt, "unit": "solves/s", "cores": threads,
-            "kind": "reference" if ref is not None else "port",
-            "sample": "first %d system(s) of the workload solved one after another (%.1f s)" % (done, dt)}
+    out = {"value": done / t_solve, "unit": "solves/s", "cores": threads,
+           "kind": "reference" if ref is not None else "port",
+           "sample": "first %d system(s) of the workload solved one after another (%.1f s)" % (done, t_solve)}
+    # accuracy of each reference solve (FP64 TT residual by the oracle's QR-based norm), beside the GPU's
+    res = [O.tt_residual([a.numpy() for a in A], x, [c.numpy() for c in b]) for (A, b, _), x in zip(items, sols)]
+    out["residuals"] = res
+    out["solved_le_1e-8"] = int(sum(r <= RES_BAR for r in res))
+    out["max_ranks"] = [max(c.shape[2] for c in x) for x in sols]
+    if gpu is not None:
+        g_res, g_ranks = gpu
+        out["gpu_residuals"] = [float(v) for v in g_res[:done]]
+        out["gpu_solved_le_1e-8"] = int(sum(v <= RES_BAR for v in g_res[:done]))
+        out["final_ranks_equal_to_gpu"] = int(sum(list(g_ranks[i]) == [1] + [c.shape[2] for c in x]
+                                                  for i, x in enumerate(sols)))
+        out["stalled_in_both"] = int(sum(g_res[i] > RES_BAR and res[i] > RES_BAR for i in range(done)))
+        out["stalled_on_gpu_only"] = int(sum(g_res[i] > RES_BAR and res[i] <= RES_BAR for i in range(done)))
+    return out
 
 
 def pmc_traffic(precision):
@@ -208,7 +236,10 @@ def run_batch(args, tt, W, lib, _lib):
     systems = W.batch_systems(args.systems, seed=0)
     items = [(tt.TT([c.to(dev) for c in s["A"]]), tt.TT([c.to(dev) for c in s["b"]]), s["rmax"]) for s in systems]
     torch.manual_seed(0)
-    z0 = [tt.solvers._draw_z(it[1].N, 4, 0, "cpu") for it in items]
+    z0, rng_states = [], []
+    for it in items:  # the generator state before each system's draws (the CPU baseline replays them)
+        rng_states.append(torch.get_rng_state())
+        z0.append(tt.solvers._draw_z(it[1].N, 4, 0, "cpu"))
     kw = dict(eps=args.eps, workers=args.workers)
     for _ in range(args.warmup):
         tt.solvers.amen_solve_batch(items[:16], precision=args.precision, z0=z0[:16], **kw)
@@ -245,6 +276,8 @@ def run_batch(args, tt, W, lib, _lib):
     ok_plain = int(np.sum(res_plain <= RES_BAR))
     plain_rate = ok_plain / tplain if ns else None
     err = [float((x - y).norm() / y.norm()) for x, y in zip(sols[:ns], plain)]
+    costs = [system_cost(sy) for sy in systems]
+    sample_share = float(sum(costs[:ns]) / sum(costs))
 
     # the merged GEMM launches of the batch: a repeat of the timed pass with HIP events around each
     gp = (ctypes.c_double * 6)()
@@ -287,7 +320,11 @@ def run_batch(args, tt, W, lib, _lib):
         "residual_max_of_solved": float(res[res <= RES_BAR].max()) if ok else None,
         "batched_" + other: {"time_s": t_o, "solves_per_s": ok_o / t_o, "solved_le_1e-8": ok_o},
         "plain_fp64_sequential": {"sample_systems": ns, "time_s": tplain, "solves_per_s": plain_rate,
-                                  "solved_le_1e-8": ok_plain},
+                                  "solved_le_1e-8": ok_plain,
+                                  "sample_cost_share": sample_share,
+                                  "cost_share_note": "the sample's share of the batch's work estimate (sum over "
+                                                     "systems of d n (rmax + 4)^2); %d of %d systems"
+                                                     % (ns, len(items))},
         "speedup_vs_plain_fp64_sequential": value / plain_rate if plain_rate else None,
         "error_vs_plain_fp64_max": max(err) if err else None,
         "error_vs_plain_fp64_median": float(np.median(err)) if err else None,
@@ -296,10 +333,42 @@ def run_batch(args, tt, W, lib, _lib):
         "roofline_orthogonalisation": roof_orth,
         "cohort": cohort,
     })
+    if args.serve_duration > 0:
+        out["serve"] = serve_leg(args, tt, W, args.serve_duration)
     if not args.no_cpu_baseline:
         cpu_items = [(s["A"], s["b"], s["rmax"]) for s in systems]
-        out["cpu_baseline"] = cpu_baseline(cpu_items, args.eps, args)
+        g_ranks = [x.R for x in plain]
+        out["cpu_baseline"] = cpu_baseline(cpu_items[:ns], args.eps, args, rng_states=rng_states[:ns],
+                                           gpu=(res_plain, g_ranks))
+        out["cpu_baseline"]["gpu_result_compared"] = "plain FP64 GPU solve of the same system and draws"
     return out
+
+
+def serve_leg(args, tt, W, duration):
+    """Latency under arrivals: Poisson arrivals at --rate requests/s for `duration` seconds of systems
+    drawn as the batch's, served by the continuous-batching loop in --precision."""
+    dev = torch.device("cuda", 0)
+    arrivals = tt.serving.poisson_arrivals(args.rate, duration, seed=1)
+    systems = W.batch_systems(len(arrivals), seed=7)
+    items = [(tt.TT([c.to(dev) for c in s["A"]]), tt.TT([c.to(dev) for c in s["b"]]), s["rmax"]) for s in systems]
+    torch.manual_seed(0)
+    z0 = [tt.solvers._draw_z(it[1].N, 4, 0, "cpu") for it in items]
+    print("serve leg: %d requests over %g s" % (len(items), duration), file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    sols, info = tt.serving.serve(items, arrivals, slots=args.slots, eps=args.eps, precision=args.precision, z0=z0)
+    torch.cuda.synchronize()
+    total = time.perf_counter() - t0
+    res = np.array([tt.solvers.residual(it[0], x, it[1]) for it, x in zip(items, sols)])
+    lat = info["latency_s"] * 1e3
+    drained = float(info["done_s"].max())
+    return {"rate_req_s": args.rate, "duration_s": duration, "requests": len(items), "slots": args.slots,
+            "precision": args.precision, "makespan_s": total, "drain_s": drained,
+            "served_per_s": len(items) / drained, "solves_per_s_le_1e-8": int(np.sum(res <= RES_BAR)) / drained,
+            "solved_le_1e-8": int(np.sum(res <= RES_BAR)),
+            "serve_p50_ms": float(np.percentile(lat, 50)), "serve_p99_ms": float(np.percentile(lat, 99)),
+            "queue_wait_p50_ms": float(np.percentile((info["start_s"] - info["arrival_s"]) * 1e3, 50)),
+            "note": "latency = completion - arrival under Poisson arrivals; drain_s counts from the stream's start"}
 
 
 def run_potential(args, tt, W, lib, _lib):

@@ -3,13 +3,22 @@
 Runs in the build container only (it needs oracle/_ref, built from the reference sources
 by `make -C oracle`). For each case it stores the inputs (operator and rhs cores, the
 residual-proxy initial draws), the reference's solution cores and its trajectory summary
-(final ranks, sweep count), as .npz next to this script.
+(final ranks, sweep count), as .npz next to this script. The two large workloads store a
+summary instead of the solution (their inputs are regenerated from the seeded generators of
+torchtt_amd.workloads; a checksum of the inputs guards against a changed generator):
 
-    python tests/golden/make_golden.py
+  precision_d12_n64   Laplacian + potential d=12, n=64, rank cap 128, eps 1e-8: final ranks,
+                      sweep count, relative residual, norm and inner products with rank-1 probes
+  batch_first32       the first 32 systems of the 256-system batched workload (seed 0), eps 1e-8:
+                      per system the relative residual, final ranks and sweep count
+
+    python tests/golden/make_golden.py [case ...]     (default: every case)
 """
 import importlib.util
 import os
+import re
 import sys
+import tempfile
 
 import numpy as np
 import torch
@@ -59,9 +68,102 @@ def run_case(ref, name, A, b, seed, nswp, eps, rmax, kickrank=4, kick2=0, x0=Non
     print("wrote", path, "ranks", out["ranks"].tolist())
 
 
-def main():
+def solve_verbose(ref, A, b, seed, nswp, eps, rmax, kickrank=4):
+    """The reference's solve with verbose output captured (file descriptor 1 redirected), so
+    that its sweep count ("Finished after N sweeps", cpp/amen_solve.h) can be recorded."""
+    d = len(A)
+    N = [a.shape[2] for a in A]
+    rA = [1] + [a.shape[3] for a in A]
+    rb = [1] + [c.shape[2] for c in b]
+    torch.manual_seed(seed)
+    sys.stdout.flush()
+    saved = os.dup(1)
+    with tempfile.TemporaryFile(mode="w+") as f:
+        os.dup2(f.fileno(), 1)
+        try:
+            x = ref.amen_solve(A, b, [], N, rA, rb, [1] * (d + 1), nswp, eps, rmax, 500, kickrank, 0, 40, 2,
+                               True, 0)
+        finally:
+            sys.stdout.flush()
+            os.dup2(saved, 1)
+            os.close(saved)
+        f.seek(0)
+        log = f.read()
+    m = re.findall(r"Finished after (\d+) sweeps", log)
+    sweeps = int(m[-1]) if m else -1
+    return [c.numpy() for c in x], sweeps
+
+
+def rank1_probes(N, count, seed):
+    """Rank-1 probe tensors (one N(0,1) vector per mode), as lists of 1 x n x 1 cores."""
+    rng = np.random.default_rng(seed)
+    return [[rng.standard_normal((1, n, 1)) for n in N] for _ in range(count)]
+
+
+def tt_inner(x, y):
+    """<x, y> of two TT vectors (numpy cores), left to right."""
+    cur = np.ones((1, 1))
+    for a, c in zip(x, y):
+        cur = np.einsum("ij,inp,jnq->pq", cur, a, c)
+    return float(cur[0, 0])
+
+
+def input_checksum(A, b):
+    return float(sum(float(np.sum(np.abs(np.asarray(c)) * (1 + k))) for k, c in enumerate(list(A) + list(b))))
+
+
+def precision_case(ref):
+    """Laplacian + potential d=12, n=64, rank cap 128, eps 1e-8 (the precision study's workload)."""
+    import amen_oracle as O
+    A, b = W.laplace_potential(12, 64, seed=0)
+    x, sweeps = solve_verbose(ref, A, b, seed=0, nswp=22, eps=1e-8, rmax=128)
+    An, bn = [a.numpy() for a in A], [c.numpy() for c in b]
+    probes = rank1_probes([64] * 12, 8, seed=12)
+    out = {"d": 12, "n": 64, "seed": 0, "eps": 1e-8, "rmax": 128, "sweeps": sweeps,
+           "ranks": np.array([1] + [c.shape[2] for c in x]),
+           "residual": O.tt_residual(An, x, bn), "norm": O.tt_norm(x),
+           "probe_seed": 12, "probes": np.array([tt_inner(x, p) for p in probes]),
+           "probe_norms": np.array([O.tt_norm(p) for p in probes]),
+           "input_checksum": input_checksum(An, bn)}
+    path = os.path.join(HERE, "precision_d12_n64.npz")
+    np.savez_compressed(path, **out)
+    print("wrote", path, "ranks", out["ranks"].tolist(), "sweeps", sweeps, "residual", out["residual"])
+
+
+def batch_case(ref, count=32):
+    """The first `count` systems of the batched workload (256 systems, seed 0) at eps 1e-8."""
+    import amen_oracle as O
+    systems = W.batch_systems(256, seed=0)[:count]
+    res, sw, ranks, caps, ds, ns, sums = [], [], [], [], [], [], []
+    for i, y in enumerate(systems):
+        A, b = y["A"], y["b"]
+        x, sweeps = solve_verbose(ref, A, b, seed=i, nswp=22, eps=1e-8, rmax=y["rmax"])
+        An, bn = [a.numpy() for a in A], [c.numpy() for c in b]
+        r = O.tt_residual(An, x, bn)
+        rk = [1] + [c.shape[2] for c in x]
+        res.append(r); sw.append(sweeps); caps.append(y["rmax"]); ds.append(y["d"]); ns.append(y["n"])
+        ranks.append(rk + [0] * (13 - len(rk)))
+        sums.append(input_checksum(An, bn))
+        print("system %d: d=%d n=%d cap %d residual %.3e sweeps %d max rank %d" %
+              (i, y["d"], y["n"], y["rmax"], r, sweeps, max(rk)), flush=True)
+    out = {"count": count, "eps": 1e-8, "residual": np.array(res), "sweeps": np.array(sw),
+           "ranks": np.array(ranks), "rmax": np.array(caps), "d": np.array(ds), "n": np.array(ns),
+           "input_checksum": np.array(sums)}
+    path = os.path.join(HERE, "batch_first32.npz")
+    np.savez_compressed(path, **out)
+    print("wrote", path)
+
+
+def main(cases):
     torch.set_num_threads(8)
     ref = load_ref()
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    if "precision_d12_n64" in cases:
+        precision_case(ref)
+    if "batch_first32" in cases:
+        batch_case(ref)
+    if "small" not in cases:
+        return
     # Laplacian d=6, n=16, h=1/17, b = ones, rank cap 20, 10 sweeps (two tolerances)
     A, b = W.laplace_ones(6, 16)
     run_case(ref, "laplace_d6_n16_eps1e-8", A, b, 0, 10, 1e-8, 20)
@@ -89,4 +191,4 @@ def main():
 
 
 if __name__ == "__main__":
-    main()
+    main(sys.argv[1:] or ["small", "precision_d12_n64", "batch_first32"])

@@ -180,6 +180,18 @@ class TTAmenError(RuntimeError):
         self.code = code
 
 
+class BatchError(TTAmenError):
+    """A batched or served call in which at least one system failed.
+
+    `statuses[i]` is system i's own status (0 for a solved system), `results[i]` its solution
+    (a TT) or None where it failed; `code` is the first failing system's status."""
+
+    def __init__(self, fn, code, msg, statuses, results):
+        super().__init__(fn, code, msg)
+        self.statuses = list(statuses)
+        self.results = list(results)
+
+
 def check(fn, code):
     if code != 0:
         msg = load().ttamen_last_error()

@@ -1281,6 +1281,7 @@ static void run_cohort(ttamen_system* systems, int count, int64_t nswp, double e
   for (auto s : streams) TT_HIP(hipStreamWaitEvent(s, ready_ev, 0));
   std::mutex adm_mu;  // admission state: next, running, mem
   std::string first_err;
+  int first_status = 0;
   int next = 0, running = 0;
   std::vector<std::unique_ptr<Cohort>> co;
   for (int l = 0; l < lanes; ++l)
@@ -1298,7 +1299,7 @@ static void run_cohort(ttamen_system* systems, int count, int64_t nswp, double e
       char* slice = base + off;
       const size_t bytes = need[i];
       c.add(
-          [=, &first_err] {
+          [=, &first_err, &first_status] {
             y->status = abi_call([&] {
               Options O = make_options(nswp, eps, y->rmax, max_full, kickrank, kick2,
                                        local_iterations, resets, 0, precision, y->seed);
@@ -1308,7 +1309,10 @@ static void run_cohort(ttamen_system* systems, int count, int64_t nswp, double e
             });
             if (y->status != 0) {
               std::lock_guard<std::mutex> lk2(g_err_mu);
-              if (first_err.empty()) first_err = "system " + std::to_string(i) + ": " + g_last_error;
+              if (first_err.empty()) {
+                first_err = "system " + std::to_string(i) + ": " + g_last_error;
+                first_status = y->status;
+              }
             }
           },
           [=, &mem, &adm_mu, &running] {
@@ -1351,7 +1355,8 @@ static void run_cohort(ttamen_system* systems, int count, int64_t nswp, double e
     g_cohort_last = sum;
   }
   TT_CHECK(lane_err.empty(), ERR_HIP, lane_err);
-  TT_CHECK(first_err.empty(), ERR_NUMERIC, first_err);
+  // The first failing system's own status (every system's is in its `status` field).
+  TT_CHECK(first_err.empty(), first_status, first_err);
 }
 
 extern "C" {

@@ -33,7 +33,7 @@ namespace ttamen {
 // with room for a scratch output and no device skip flag) runs each of the four tile shapes, with
 // its own split-K plan, once to warm up and three times under HIP events, writing a scratch
 // output; the fastest is cached for the process and used for every later launch of the bucket.
-// Tile shape does not change a result; the split-K plan it implies can (the order of the K sums).
+// Neither the tile shape nor the split-K plan (a function of the shape alone) changes a result.
 namespace {
 std::atomic<int> g_tune_mode{0};
 std::mutex g_tune_mu;
@@ -103,8 +103,12 @@ static void gemm_impl(const GemmDesc& g, Arena& ws, hipStream_t st, bool bf16 = 
   const bool small_grid = tiles64 < 1024 && current_recorder() == nullptr;
   int BM = (g.M <= 32 || small_grid) ? 32 : 64;
   int BN = (g.N <= 32 || small_grid) ? 32 : 64;
-  // Split K when the output grid is too small to fill the 256 CUs and K is long.
-  auto split_plan = [&](int bm, int bn, int64_t* kchunk_out) {
+  // Split K when the output grid is too small to fill the 256 CUs and K is long. The plan
+  // depends on the shape only (the tile count of the cohort's rule: 64, or 32 for an extent of
+  // at most 32), never on the tile that runs, so a GEMM sums its K slices in the same order
+  // alone, inside a cohort and under the autotuner: each result is bitwise the same.
+  auto split_plan = [&](int, int, int64_t* kchunk_out) {
+    const int bm = g.M <= 32 ? 32 : 64, bn = g.N <= 32 ? 32 : 64;
     const int tiles = ((g.M + bm - 1) / bm) * ((g.N + bn - 1) / bn) * p.B;
     int splits = 1;
     if (g.K == 0) {
@@ -202,14 +206,7 @@ static void gemm_impl(const GemmDesc& g, Arena& ws, hipStream_t st, bool bf16 = 
   }
   int splits = split_plan(BM, BN, &p.kchunk);
   size_t mk = ws.mark();
-  size_t partial_elems = splits > 1 ? (size_t)splits * p.B * g.M * g.N : 0;
-  if (ws.measuring()) {
-    // A workspace is sized without a cohort but may be used inside one: reserve the split-K
-    // partials of both tile rules.
-    int64_t kc;
-    const int s64 = split_plan(g.M <= 32 ? 32 : 64, g.N <= 32 ? 32 : 64, &kc);
-    if (s64 > 1) partial_elems = std::max(partial_elems, (size_t)s64 * p.B * g.M * g.N);
-  }
+  const size_t partial_elems = splits > 1 ? (size_t)splits * p.B * g.M * g.N : 0;
   if (partial_elems) p.partial = ws.alloc<T>(partial_elems);
   if (!ws.measuring()) {
     if (g.K == 0) {

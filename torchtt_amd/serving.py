@@ -13,7 +13,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from .solvers import _PRECISIONS, _build_systems, _collect, _solver_device
+from .solvers import _PRECISIONS, _build_systems, _check_batch, _collect, _solver_device
 from .errors import InvalidArguments
 
 
@@ -30,7 +30,7 @@ def poisson_arrivals(rate, duration, seed=0):
 
 def serve(requests, arrivals, slots=64, memory_bytes=None, nswp=22, eps=1e-10, max_full=500, kickrank=4,
           kick2=0, local_iterations=40, resets=2, precision='fp64', z0=None):
-    """Serve `requests` ((A, b[, rmax]) items) arriving at `arrivals` (seconds, sorted).
+    """Serve `requests` ((A, b[, rmax[, x0]]) items) arriving at `arrivals` (seconds, sorted).
 
     slots: requests running at once (at most 256).
     memory_bytes: serving memory (HBM) budget shared by the running requests; default
@@ -56,7 +56,7 @@ def serve(requests, arrivals, slots=64, memory_bytes=None, nswp=22, eps=1e-10, m
             need = 0
             for i in range(count):
                 y = arr[i]
-                w = lib.ttamen_amen_solve_workspace_size(y.d, y.N, y.rA, y.rb, None, y.rmax, int(kickrank),
+                w = lib.ttamen_amen_solve_workspace_size(y.d, y.N, y.rA, y.rb, y.rx0, y.rmax, int(kickrank),
                                                          int(kick2), int(local_iterations), int(max_full),
                                                          _PRECISIONS[precision])
                 if w == 0:
@@ -72,7 +72,7 @@ def serve(requests, arrivals, slots=64, memory_bytes=None, nswp=22, eps=1e-10, m
                                 int(local_iterations), int(resets), _PRECISIONS[precision], int(slots),
                                 ctypes.c_void_p(ws.data_ptr()), ws_bytes, start, done,
                                 ctypes.c_void_p(_lib.stream_handle(dev)))
-        _lib.check("ttamen_serve", code)
+        _check_batch("ttamen_serve", code, arr, outs)
         sols = _collect(outs)
         arrival = np.array(list(arrivals), dtype=float)
         info = {

@@ -249,35 +249,44 @@ last_batch_stats = None
 
 
 def _build_systems(systems, dev, kickrank, kick2, z0, lib):
-    """ctypes array of ttamen_system for (A, b[, rmax]) items, with device copies and outputs."""
+    """ctypes array of ttamen_system for (A, b[, rmax[, x0]]) items, with device copies and outputs."""
     count = len(systems)
     keep, outs = [], []
     arr = (_lib.TTAmenSystem * count)()
     stats = (_lib.TTAmenStats * count)()
     for i, item in enumerate(systems):
         A, b = item[0], item[1]
-        rmax = int(item[2]) if len(item) > 2 else 32768
+        rmax = int(item[2]) if len(item) > 2 and item[2] is not None else 32768
+        x0 = item[3] if len(item) > 3 else None
         _check_system(A, b)
+        _require_real(x0)
         N = list(b.N)
         d = len(N)
         Ac = [c.to(device=dev, dtype=torch.float64).contiguous() for c in A.cores]
         bc = [c.to(device=dev, dtype=torch.float64).contiguous() for c in b.cores]
+        if x0 is not None:
+            if list(x0.N) != N:
+                raise ShapeMismatch('x0 does not match the shape of b.')
+            xc = [c.to(device=dev, dtype=torch.float64).contiguous() for c in x0.cores]
+            rx0a, px0 = _lib.i64_array(x0.R), _lib.ptr_array(xc)
+        else:
+            xc, rx0a, px0 = None, None, None
         zc = z0[i] if z0 is not None else _draw_z(N, kickrank, kick2, A.cores[0].device)
         if zc is not None:
             zc = [z.to(device=dev, dtype=torch.float64).contiguous() for z in zc]
         Na, rAa, rba = _lib.i64_array(N), _lib.i64_array(A.R), _lib.i64_array(b.R)
-        out_elems = lib.ttamen_amen_solve_output_elems(d, Na, None, rmax, int(kickrank), int(kick2))
+        out_elems = lib.ttamen_amen_solve_output_elems(d, Na, rx0a, rmax, int(kickrank), int(kick2))
         xout = torch.empty(max(out_elems, 1), dtype=torch.float64, device=dev)
         rx_out = (ctypes.c_int64 * (d + 1))()
         off_out = (ctypes.c_int64 * d)()
         pa, pb = _lib.ptr_array(Ac), _lib.ptr_array(bc)
         pz = _lib.ptr_array(zc) if zc is not None else None
-        keep.extend([Ac, bc, zc, Na, rAa, rba, pa, pb, pz, xout, rx_out, off_out])
+        keep.extend([Ac, bc, zc, xc, rx0a, px0, Na, rAa, rba, pa, pb, pz, xout, rx_out, off_out])
         y = arr[i]
         y.d = d
         y.N, y.rA, y.rb = Na, rAa, rba
         y.A_cores, y.b_cores = pa, pb
-        y.x0_cores, y.rx0 = None, None
+        y.x0_cores, y.rx0 = px0, rx0a
         y.z0_cores = pz
         y.seed = 0
         y.rmax = rmax
@@ -290,9 +299,12 @@ def _build_systems(systems, dev, kickrank, kick2, z0, lib):
     return arr, stats, outs, keep
 
 
-def _collect(outs):
+def _collect(outs, statuses=None):
     result = []
-    for (A, N, xout, rx_out, off_out) in outs:
+    for i, (A, N, xout, rx_out, off_out) in enumerate(outs):
+        if statuses is not None and statuses[i] != 0:
+            result.append(None)
+            continue
         d = len(N)
         cores = []
         for k in range(d):
@@ -307,11 +319,21 @@ def _collect(outs):
 last_batch_stats = None
 
 
+def _check_batch(fn, code, arr, outs):
+    """Raise BatchError (every system's status, the solved systems' results) when `code` != 0."""
+    if code == 0:
+        return
+    statuses = [int(arr[i].status) for i in range(len(outs))]
+    msg = _lib.load().ttamen_last_error()
+    raise _lib.BatchError(fn, code, msg.decode() if msg else "", statuses, _collect(outs, statuses))
+
+
 def amen_solve_batch(systems, nswp=22, eps=1e-10, max_full=500, kickrank=4, kick2=0, local_iterations=40,
                      resets=2, precision='fp64', workers=8, z0=None):
     """Solve many independent TT systems concurrently (no counterpart in the reference).
 
-    systems: list of (A, b) or (A, b, rmax) with TT A, b. Each system runs the same solver as
+    systems: list of (A, b), (A, b, rmax) or (A, b, rmax, x0) with TT A, b (and warm start x0).
+    Each system runs the same solver as
     `amen_solve` with the same options; the systems run together as one cohort
     (libttamen: ttamen_amen_solve_batch): at every host synchronisation point the kernels
     that all running systems have queued are merged into grouped launches. `workers` host
@@ -321,6 +343,11 @@ def amen_solve_batch(systems, nswp=22, eps=1e-10, max_full=500, kickrank=4, kick
 
     Returns:
         list[TT]: the solutions, on each A's device and dtype.
+
+    Raises:
+        BatchError: when a system fails; it carries every system's status (`statuses`) and the
+        solutions of the systems that succeeded (`results`, None where a system failed). Its
+        code is the first failing system's status.
     """
     global last_batch_stats
     if precision not in _PRECISIONS:
@@ -343,9 +370,9 @@ def amen_solve_batch(systems, nswp=22, eps=1e-10, max_full=500, kickrank=4, kick
                                            int(kick2), int(local_iterations), int(resets),
                                            _PRECISIONS[precision], workers, ctypes.c_void_p(ws.data_ptr()),
                                            ws_bytes, ctypes.c_void_p(_lib.stream_handle(dev)))
-        _lib.check("ttamen_amen_solve_batch", code)
-        result = _collect(outs)
         last_batch_stats = [stats[i].as_dict() for i in range(count)]
+        _check_batch("ttamen_amen_solve_batch", code, arr, outs)
+        result = _collect(outs)
         cohort_stats()
         del ws, keep
     return result
