@@ -352,6 +352,22 @@ int ttamen_local_matvec_bf16(const float* PhiL, const float* A, const float* Phi
                              size_t workspace_bytes, void* stream);
 size_t ttamen_local_matvec_workspace_size(int64_t l1, int64_t s, int64_t r1, int64_t n,
                                           int64_t S, int64_t l2, int64_t r2, int64_t nvec);
+/*
+ * Allocation-free variant of ttamen_local_matvec (same arguments and results, no workspace).
+ * No counterpart in the reference: it is for callers that cannot allocate device memory (code
+ * under graph capture, tight serving loops, harnesses that forbid hipMalloc). The scratch the
+ * kernels need (permuted operator core, stage intermediates, split-K partials) lives in a
+ * file-scope device array of ttamen_local_matvec_noalloc_limit() bytes (64 MiB) that each GPU
+ * holds from the library's load until the process ends; sizes needing more are refused with
+ * status 2 before any device work. The call allocates nothing (no hipMalloc, hipFree or
+ * allocating library routine). All calls on one GPU share its array: calls that overlap on one
+ * GPU (two host threads, or two streams without ordering between them) race.
+ */
+int ttamen_local_matvec_noalloc(const double* PhiL, const double* A, const double* PhiR,
+                                const double* x, double* y, int64_t l1, int64_t s, int64_t r1,
+                                int64_t n, int64_t S, int64_t l2, int64_t r2, int64_t nvec,
+                                void* stream);
+size_t ttamen_local_matvec_noalloc_limit(void);
 
 /*
  * Interface updates (FP64). Replace compute_phi_fwd_A / compute_phi_bck_A /

@@ -56,6 +56,11 @@ uint64_t bucket_key(const GemmDesc& g, int type) {
 }
 }  // namespace
 
+bool& gemm_tuning_off() {
+  static thread_local bool off = false;
+  return off;
+}
+
 int gemm_autotune_mode(int mode) {
   const int prev = g_tune_mode.load();
   if (mode >= 0) g_tune_mode.store(mode);
@@ -125,7 +130,7 @@ static void gemm_impl(const GemmDesc& g, Arena& ws, hipStream_t st, bool bf16 = 
     return (int)((g.K + kchunk - 1) / kchunk);
   };
   const bool tunable = !ws.measuring() && current_recorder() == nullptr && g.skip == nullptr && g.K > 0 &&
-                       g_tune_mode.load() == 1;
+                       g_tune_mode.load() == 1 && !gemm_tuning_off();
   if (tunable) {
     const uint64_t key = bucket_key(g, bf16 ? 2 : (sizeof(T) == 8 ? 0 : 1));
     int choice = -1;

@@ -134,6 +134,9 @@ struct GemmDesc {
 // Tile autotuner of the single-system GEMMs (outside cohorts): mode 0 uses the fixed rule, mode 1
 // times the four tile shapes on the first launch of each shape bucket and keeps the fastest.
 int gemm_autotune_mode(int mode);  // sets the mode (if mode >= 0), returns the previous one
+// While set on the calling thread, GEMMs use the fixed tile rule whatever the autotune mode (the
+// tuner creates events and a scratch output; the allocation-free entry point must do neither).
+bool& gemm_tuning_off();
 void gemm_autotune_stats(double* out4);
 void gemm_autotune_clear();
 
