@@ -426,6 +426,15 @@ size_t ttamen_gmres_solve_workspace_size(int64_t r1, int64_t s, int64_t n, int64
  * otherwise the previous mode is returned. stats: buckets tuned, launches served from the cache,
  * seconds spent tuning, served launches whose tile differs from the rule.
  */
+/*
+ * Form of the local operator (no counterpart in the reference, whose local operator is three ATen
+ * tensordots): 0 (default) three strided-batched MFMA GEMMs; 1 one fused kernel per application
+ * (stage intermediates in LDS, partial sums over blocks of the right rank added by the last
+ * workgroup in a fixed order) wherever its tiles fit the LDS, else the GEMMs. Both give the local
+ * product to rounding; mode 1 measured slower on the solver's shapes and is there to be measured.
+ * mode -1 only queries. Returns the previous mode (-1 on a bad argument).
+ */
+int ttamen_matvec_mode(int mode);
 int ttamen_gemm_autotune(int mode);
 int ttamen_gemm_autotune_stats(double* out4);
 void ttamen_gemm_autotune_clear(void);

@@ -258,3 +258,26 @@ def test_rank_above_the_lds_limit_is_a_status():
     _say("rank above the LDS limit")
     with pytest.raises(Exception, match="not supported"):
         tt.solvers.amen_solve(A, b, x0=x0, eps=1e-13, nswp=2)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_fused_local_operator_solve(precision):
+    """The solve with the fused local operator (ttamen_matvec_mode(1)): the reference fixture's
+    ranks and solution, as with the three-GEMM form."""
+    import torchtt_amd as tt
+    lib = tt._lib.load()
+    c = load_golden("potential_d4_n16")
+    prev = lib.ttamen_matvec_mode(1)
+    try:
+        _say("amen golden potential_d4_n16, fused local operator,", precision)
+        torch.manual_seed(c["seed"])
+        x = tt.solvers.amen_solve(_tt(c["A"], "cpu"), _tt(c["b"], "cpu"), nswp=c["nswp"], eps=c["eps"],
+                                  rmax=c["rmax"], kickrank=c["kickrank"], precision=precision)
+    finally:
+        lib.ttamen_matvec_mode(prev)
+    xr = _tt(c["x"], "cpu")
+    diff = _rel_tt(x, xr)
+    res = tt.solvers.residual(_tt(c["A"], "cpu"), x, _tt(c["b"], "cpu"))
+    _say("  ranks %s (ref %s) rel diff %.3e residual %.3e" % (x.R, c["ranks"].tolist(), diff, res))
+    assert x.R == c["ranks"].tolist()
+    assert diff <= 1e-8 and res <= 50 * c["eps"]

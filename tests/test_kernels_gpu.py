@@ -102,7 +102,11 @@ def test_gemm_f64_unaligned_base(trans):
 
 # ----------------------------------------------------------------- local operator
 SHAPES = [(84, 4, 84, 64, 4, 84, 84, 1), (7, 2, 5, 16, 2, 6, 4, 3), (4, 4, 20, 64, 4, 4, 24, 1),
-          (1, 1, 1, 16, 2, 5, 5, 2), (20, 2, 20, 8, 1, 1, 1, 1)]
+          (1, 1, 1, 16, 2, 5, 5, 2), (20, 2, 20, 8, 1, 1, 1, 1),
+          # fused local operator: ragged l and R blocks, a rank-3 operator, 17 partial sums of R,
+          # a residual-proxy product, and a long mode whose FP64 tiles exceed the LDS (three GEMMs)
+          (13, 3, 11, 17, 3, 9, 23, 1), (132, 4, 132, 64, 4, 132, 132, 1), (30, 4, 30, 33, 4, 4, 29, 2),
+          (5, 4, 6, 160, 4, 3, 8, 1)]
 
 
 def _rand_local(shape, dtype=torch.float64, seed=0):
@@ -115,11 +119,20 @@ def _rand_local(shape, dtype=torch.float64, seed=0):
     return pl, a, pr, x
 
 
+@pytest.fixture(params=[0, 1], ids=["gemms", "fused"])
+def matvec_mode(request):
+    """Both forms of the local operator: three GEMMs (default) and the fused kernel."""
+    L, lib = _lib()
+    prev = lib.ttamen_matvec_mode(request.param)
+    yield request.param
+    lib.ttamen_matvec_mode(prev)
+
+
 @pytest.mark.parametrize("shape", SHAPES)
-def test_local_matvec_f64(shape):
+def test_local_matvec_f64(shape, matvec_mode):
     L, lib = _lib()
     l1, s, r1, n, S, l2, r2, nv = shape
-    _say("local_matvec f64", shape)
+    _say("local_matvec f64", shape, "mode", matvec_mode)
     pl, a, pr, x = _rand_local(shape)
     y = torch.empty(nv, l1, n, l2, dtype=torch.float64, device=DEV)
     ws = _ws(lib.ttamen_local_matvec_workspace_size(l1, s, r1, n, S, l2, r2, nv))
@@ -133,11 +146,11 @@ def test_local_matvec_f64(shape):
         assert _rel(y[v].cpu().numpy(), want) < 1e-13, v
 
 
-@pytest.mark.parametrize("shape", SHAPES[:3])
-def test_local_matvec_f32(shape):
+@pytest.mark.parametrize("shape", SHAPES)
+def test_local_matvec_f32(shape, matvec_mode):
     L, lib = _lib()
     l1, s, r1, n, S, l2, r2, nv = shape
-    _say("local_matvec f32", shape)
+    _say("local_matvec f32", shape, "mode", matvec_mode)
     pl, a, pr, x = _rand_local(shape, torch.float32, seed=1)
     y = torch.empty(nv, l1, n, l2, dtype=torch.float32, device=DEV)
     ws = _ws(lib.ttamen_local_matvec_workspace_size(l1, s, r1, n, S, l2, r2, nv) * 2)

@@ -89,6 +89,7 @@ _SIGNATURES = {
     "ttamen_local_matvec": (ctypes.c_int, [c_dp, c_dp, c_dp, c_dp, c_dp] + [c_i64] * 8 + [c_dp, c_size, c_dp]),
     "ttamen_local_matvec_noalloc": (ctypes.c_int, [c_dp, c_dp, c_dp, c_dp, c_dp] + [c_i64] * 8 + [c_dp]),
     "ttamen_local_matvec_noalloc_limit": (c_size, []),
+    "ttamen_matvec_mode": (ctypes.c_int, [ctypes.c_int]),
     "ttamen_amen_divide": (ctypes.c_int, [ctypes.c_int, c_i64p, c_i64p, c_i64p, ctypes.POINTER(c_dp),
                                           ctypes.POINTER(c_dp), ctypes.POINTER(c_dp), c_i64p,
                                           ctypes.POINTER(c_dp), c_i64, ctypes.c_double, c_i64p, c_i64, c_i64,

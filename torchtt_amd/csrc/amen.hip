@@ -292,10 +292,18 @@ class Solver {
     b_.assign(b, b + d);
     Apm_.resize(d);
     for (int k = 0; k < d; ++k) Apm_[k] = ws_.alloc<double>(P.rA[k] * P.N[k] * P.N[k] * P.rA[k + 1]);
+    // the permuted cores packed in MFMA fragment order, for the fused local operator
+    Apk_.resize(d);
+    for (int k = 0; k < d; ++k)
+      Apk_[k] = ws_.alloc<double>(frag_a_elems((int)(P.N[k] * P.rA[k + 1]), (int)(P.rA[k] * P.N[k])));
     if (O.precision >= 1) {
       Apm32_.resize(d);
+      Apk32_.resize(d);
       for (int k = 0; k < d; ++k) Apm32_[k] = ws_.alloc<float>(P.rA[k] * P.N[k] * P.N[k] * P.rA[k + 1]);
+      for (int k = 0; k < d; ++k)
+        Apk32_[k] = ws_.alloc<float>(frag_a_elems((int)(P.N[k] * P.rA[k + 1]), (int)(P.rA[k] * P.N[k])));
     }
+    mv_cnt_ = ws_.alloc<unsigned>(kMvCounters);
     x_.resize(d);
     for (int k = 0; k < d; ++k) x_[k] = ws_.alloc<double>(P.maxcore);
     spare_ = ws_.alloc<double>(P.maxcore);
@@ -324,10 +332,16 @@ class Solver {
   void run() {
     if (ws_.measuring()) return;
     zero_words(rs_.counter, 64, st_);
-    for (int k = 0; k < d; ++k) permute_core_A(A_[k], Apm_[k], (int)P.rA[k], (int)P.N[k], (int)P.rA[k + 1], st_);
+    zero_words(mv_cnt_, kMvCounters, st_);
+    for (int k = 0; k < d; ++k) {
+      permute_core_A(A_[k], Apm_[k], (int)P.rA[k], (int)P.N[k], (int)P.rA[k + 1], st_);
+      pack_frag_a(Apm_[k], Apk_[k], (int)(P.N[k] * P.rA[k + 1]), (int)(P.rA[k] * P.N[k]), st_);
+    }
     if (O.precision >= 1)
-      for (int k = 0; k < d; ++k)
+      for (int k = 0; k < d; ++k) {
         permute_core_A_f32(A_[k], Apm32_[k], (int)P.rA[k], (int)P.N[k], (int)P.rA[k + 1], st_);
+        pack_frag_a_f32(Apm32_[k], Apk32_[k], (int)(P.N[k] * P.rA[k + 1]), (int)(P.rA[k] * P.N[k]), st_);
+      }
     TT_LAUNCH_CHECK();
     // initial guess (reference amen_solve.h:161-167)
     rx_.assign(d + 1, 1);
@@ -423,7 +437,10 @@ class Solver {
   int d;
   std::vector<const double*> A_, b_;
   std::vector<double*> Apm_, x_, z_, Phi_, Phib_, Phiz_, Phizb_;
-  std::vector<float*> Apm32_;
+  std::vector<float*> Apm32_, Apk32_;
+  std::vector<double*> Apk_;
+  static constexpr int kMvCounters = 1024;  // zeroed counters of the fused local operator
+  unsigned* mv_cnt_ = nullptr;
   double* spare_ = nullptr;
   double* S_ = nullptr;
   int nscal_ = 0;
@@ -453,7 +470,10 @@ class Solver {
       e1 = g_prof.get();
       TT_HIP(hipEventRecord(e0, st_));
     }
-    local_matvec(op, x, y, nvec, ws_, st_, skip);
+    LocalOp o = op;
+    o.cnt = mv_cnt_;
+    o.cnt_cap = kMvCounters;
+    local_matvec(o, x, y, nvec, ws_, st_, skip);
     if (!skip) count_matvec(op, nvec);
     if (e0) {
       TT_HIP(hipEventRecord(e1, st_));
@@ -507,7 +527,7 @@ class Solver {
       if (swp > 0) {
         const int64_t r1 = rx_[k], r2 = rx_[k + 1];
         LocalOp op;
-        op.PhiL = Phiz_[k]; op.Apm = Apm_[k]; op.PhiR = Phiz_[k + 1];
+        op.PhiL = Phiz_[k]; op.Apm = Apm_[k]; op.Apk = Apk_[k]; op.PhiR = Phiz_[k + 1];
         op.l1 = (int)rzphi_[k]; op.s = (int)s; op.r1 = (int)r1; op.n = (int)n; op.S = (int)S;
         op.l2 = (int)rzphi_[k + 1]; op.r2 = (int)r2;
         const int64_t rows = rzphi_[k], cols = n * rzphi_[k + 1];
@@ -620,7 +640,7 @@ class Solver {
                  (int)r2, sc(NRMSC()), ws_, st_);
     norm2(rhs, Nl, sc(TMP(T_NRHS)), rs_, st_);
     LocalOp op;
-    op.PhiL = Phi_[k]; op.Apm = Apm_[k]; op.PhiR = Phi_[k + 1];
+    op.PhiL = Phi_[k]; op.Apm = Apm_[k]; op.Apk = Apk_[k]; op.PhiR = Phi_[k + 1];
     op.l1 = (int)r1; op.s = (int)s; op.r1 = (int)r1; op.n = (int)n; op.S = (int)S;
     op.l2 = (int)r2; op.r2 = (int)r2;
     const bool use_full = Nl < O.max_full;
@@ -735,7 +755,7 @@ class Solver {
       size_t mk2 = ws_.mark();
       const int64_t rzk = rzphi_[k], rzk1 = rzphi_[k + 1];
       LocalOp zop;
-      zop.PhiL = Phiz_[k]; zop.Apm = Apm_[k]; zop.PhiR = Phiz_[k + 1];
+      zop.PhiL = Phiz_[k]; zop.Apm = Apm_[k]; zop.Apk = Apk_[k]; zop.PhiR = Phiz_[k + 1];
       zop.l1 = (int)rzk; zop.s = (int)s; zop.r1 = (int)r1; zop.n = (int)n; zop.S = (int)S;
       zop.l2 = (int)rzk1; zop.r2 = (int)r2;
       const int64_t zrows = rzk * n, zcols = rzk1;
@@ -771,7 +791,7 @@ class Solver {
       if (!last && P.rzcap > 0) {
         const int64_t radd = rzphi_[k + 1];
         LocalOp eop;
-        eop.PhiL = Phi_[k]; eop.Apm = Apm_[k]; eop.PhiR = Phiz_[k + 1];
+        eop.PhiL = Phi_[k]; eop.Apm = Apm_[k]; eop.Apk = Apk_[k]; eop.PhiR = Phiz_[k + 1];
         eop.l1 = (int)r1; eop.s = (int)s; eop.r1 = (int)r1; eop.n = (int)n; eop.S = (int)S;
         eop.l2 = (int)radd; eop.r2 = (int)r2;
         const int64_t rows = r1 * n;
@@ -868,7 +888,8 @@ class Solver {
     cast_f64_f32(op.PhiL, pl, r1 * s * r1, st_);
     cast_f64_f32(op.PhiR, pr, r2 * S * r2, st_);
     LocalOpF32 op32;
-    op32.PhiL = pl; op32.Apm = Apm32_[k]; op32.PhiR = pr;
+    op32.PhiL = pl; op32.Apm = Apm32_[k]; op32.Apk = Apk32_[k]; op32.PhiR = pr;
+    op32.cnt = mv_cnt_; op32.cnt_cap = kMvCounters;
     op32.l1 = op.l1; op32.s = op.s; op32.r1 = op.r1; op32.n = op.n; op32.S = op.S;
     op32.l2 = op.l2; op32.r2 = op.r2;
     op32.bf16 = (O.precision == 2);
@@ -1054,6 +1075,7 @@ static size_t solve_workspace_bound(const Plan& P, const Options& O) {
   doubles += std::max((double)std::min<int64_t>(P.maxcore, O.max_full) * (std::min<int64_t>(P.maxcore, O.max_full) + 2),
                       Nl * (m + 4) + 3.0 * 256 * 128 + (m + 1) * m + 8 * m);  // dense B | GMRES
   doubles += 2 * R * s * n * R * 2;                    // local matvec t1, t2 (+ z products)
+  doubles += (double)local_matvec_fused_bytes((int)R, (int)s, (int)R, (int)n, (int)s, (int)R, (int)R, 1, 8) / 8;
   doubles += Nl * 3 + 4 * R * R + 48.0 * (R + 2) * (R / 2 + 2) * 2;  // SVD: U, Q, R, Ur, V, log
   doubles += 32.0 * Nl + 32.0 * R * s * n * 2 + 2 * R * s * 32;      // truncation batch
   doubles += 3 * R * n * cw + 2 * cw * cw + R * cw;    // enrichment QR
@@ -1632,7 +1654,8 @@ size_t ttamen_local_matvec_workspace_size(int64_t l1, int64_t s, int64_t r1, int
   LocalOp op;
   op.l1 = (int)l1; op.s = (int)s; op.r1 = (int)r1; op.n = (int)n; op.S = (int)S;
   op.l2 = (int)l2; op.r2 = (int)r2;
-  return local_matvec_ws(op, (int)nvec) + (size_t)s * n * n * S * 8 + 4096;
+  return local_matvec_ws(op, (int)nvec) + (size_t)s * n * n * S * 8 +
+         frag_a_elems((int)(n * S), (int)(s * n)) * 8 + 4096;
 }
 
 int ttamen_local_matvec(const double* PhiL, const double* A, const double* PhiR, const double* x,
@@ -1647,8 +1670,10 @@ int ttamen_local_matvec(const double* PhiL, const double* A, const double* PhiR,
     hipStream_t st = (hipStream_t)stream;
     double* Apm = ws.alloc<double>(s * n * n * S);
     permute_core_A(A, Apm, (int)s, (int)n, (int)S, st);
+    double* Apk = ws.alloc<double>(frag_a_elems((int)(n * S), (int)(s * n)));
+    pack_frag_a(Apm, Apk, (int)(n * S), (int)(s * n), st);
     LocalOp op;
-    op.PhiL = PhiL; op.Apm = Apm; op.PhiR = PhiR;
+    op.PhiL = PhiL; op.Apm = Apm; op.Apk = Apk; op.PhiR = PhiR;
     op.l1 = (int)l1; op.s = (int)s; op.r1 = (int)r1; op.n = (int)n; op.S = (int)S;
     op.l2 = (int)l2; op.r2 = (int)r2;
     local_matvec(op, x, y, (int)nvec, ws, st);
@@ -1670,8 +1695,10 @@ static int local_matvec_f32_abi(const float* PhiL, const float* A, const float* 
       cast_f32_f64(A, Ad, s * n * n * S, st);
       permute_core_A_f32(Ad, Apm, (int)s, (int)n, (int)S, st);
     }
+    float* Apk = ws.alloc<float>(frag_a_elems((int)(n * S), (int)(s * n)));
+    pack_frag_a_f32(Apm, Apk, (int)(n * S), (int)(s * n), st);
     LocalOpF32 op;
-    op.PhiL = PhiL; op.Apm = Apm; op.PhiR = PhiR;
+    op.PhiL = PhiL; op.Apm = Apm; op.Apk = Apk; op.PhiR = PhiR;
     op.l1 = (int)l1; op.s = (int)s; op.r1 = (int)r1; op.n = (int)n; op.S = (int)S;
     op.l2 = (int)l2; op.r2 = (int)r2;
     op.bf16 = bf16;
@@ -1843,6 +1870,8 @@ size_t ttamen_gmres_solve_workspace_size(int64_t r1, int64_t s, int64_t n, int64
   const int64_t N = r1 * n * r2;
   Arena a;
   a.alloc<double>(s * n * n * S);
+  a.alloc<double>(frag_a_elems((int)(n * S), (int)(s * n)));
+  a.alloc<unsigned>(1024);
   a.alloc<double>(8);
   a.alloc<double>(kRedBlocks * 64 + 64);
   a.alloc<unsigned>(64);
@@ -1871,15 +1900,20 @@ int ttamen_gmres_solve(const double* PhiL, const double* A, const double* PhiR, 
     hipStream_t st = (hipStream_t)stream;
     double* Apm = ws.alloc<double>(s * n * n * S);
     permute_core_A(A, Apm, (int)s, (int)n, (int)S, st);
+    double* Apk = ws.alloc<double>(frag_a_elems((int)(n * S), (int)(s * n)));
+    pack_frag_a(Apm, Apk, (int)(n * S), (int)(s * n), st);
+    unsigned* cnt = ws.alloc<unsigned>(1024);
+    zero_words(cnt, 1024, st);
     double* slot = ws.alloc<double>(8);
     RedScratch rs;
     rs.partial = ws.alloc<double>(kRedBlocks * 64 + 64);
     rs.counter = ws.alloc<unsigned>(64);
     zero_words(rs.counter, 64, st);
     LocalOp op;
-    op.PhiL = PhiL; op.Apm = Apm; op.PhiR = PhiR;
+    op.PhiL = PhiL; op.Apm = Apm; op.Apk = Apk; op.PhiR = PhiR;
     op.l1 = (int)r1; op.s = (int)s; op.r1 = (int)r1; op.n = (int)n; op.S = (int)S;
     op.l2 = (int)r2; op.r2 = (int)r2;
+    op.cnt = cnt; op.cnt_cap = 1024;
     fill(x, N, 0.0, st);
     norm2(rhs, N, slot, rs, st, true);  // ||rhs|| (1 for a zero rhs: x = 0 is then returned at once)
     GmresDev g = gmres_alloc(ws, (int)local_iterations, N, sizeof(double));
@@ -1913,6 +1947,14 @@ int ttamen_cohort_gemm_profile(int enable, double* out6) {
     TT_CHECK(enable >= -1 && enable <= 1, ERR_INVALID, "enable must be -1, 0 or 1");
     cohort_gemm_profile(enable, out6);
   });
+}
+
+int ttamen_matvec_mode(int mode) {
+  if (mode < -1 || mode > 1) {
+    set_last_error("matvec mode must be -1 (query), 0 (three GEMMs) or 1 (fused kernel)");
+    return -1;
+  }
+  return matvec_mode(mode);
 }
 
 int ttamen_gemm_autotune(int mode) {

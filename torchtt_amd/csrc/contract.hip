@@ -7,8 +7,127 @@
 //   A-interface        Phi[l, s, r]         (l: test/output side, r: trial/x side)
 //   rhs interface      Phib[b, r]
 #include "ttamen_internal.h"
+#include "kernels_matvec.h"
+
+#include <atomic>
+#include <mutex>
 
 namespace ttamen {
+
+// ---------------------------------------------------------------- fused local operator
+// Launch plan of KLocalMv (kernels_matvec.h) for one shape, or ok = false when the shape is
+// outside what the kernel takes (then the three-GEMM form below runs):
+//   BL rows l per workgroup: 8 / s (at least 1, at most l1), so that stage 1 reuses each loaded x
+//       element for about 8 products; NLS = the PhiL slice width (4, 8 or 16) >= BL s
+//   BR columns R per workgroup: 16 / BL (the 16 stage-2 columns), at most 32 / S (stage 3 keeps
+//       S BR / 4 <= 8 fragments in registers) and at most r2; P = ceil(r2 / BR) <= 18 partial sums
+//       (ranks up to 144 with 8 columns R per workgroup)
+//   LDS: PhiL slice + T1 (s n rows) + T2 (n S rows), within the 160 KB of a CU
+struct MvPlan {
+  bool ok = false;
+  int BL = 1, BR = 1, nlb = 1, P = 1, NLS = 4, KS = 0, MS = 0, KST = 0, RB = 0;
+  size_t lds = 0;
+};
+constexpr size_t kMvMaxLds = 160 * 1024 - 1024;
+
+static MvPlan mv_plan(int l1, int s, int r1, int n, int S, int l2, int r2, size_t elem) {
+  MvPlan p;
+  if (s < 1 || s > 16 || S < 1 || S > 32 || n < 1 || l1 < 1 || l2 < 1 || r1 < 1 || r2 < 1) return p;
+  p.BL = std::max(1, std::min(s <= 8 ? 8 / s : 1, l1));
+  p.NLS = p.BL * s <= 4 ? 4 : (p.BL * s <= 8 ? 8 : 16);
+  p.BR = std::max(1, std::min({kMvCols / p.BL, 32 / S, r2}));
+  p.nlb = (l1 + p.BL - 1) / p.BL;
+  p.P = (r2 + p.BR - 1) / p.BR;
+  if (p.P > 18) return p;
+  p.KS = s * n;
+  p.MS = n * S;
+  p.KST = (p.KS + 3) / 4;
+  p.RB = (p.MS + 15) / 16;
+  p.lds = elem * ((size_t)r1 * p.NLS + (size_t)(4 * p.KST + 16 * p.RB) * kMvPitch);
+  if (p.lds > kMvMaxLds) return p;
+  p.ok = true;
+  return p;
+}
+
+size_t frag_a_elems(int MS, int KS) { return (size_t)((MS + 15) / 16) * ((KS + 3) / 4) * 64; }
+
+template <typename T>
+static void pack_frag_a_t(const T* Apm, T* Apk, int MS, int KS, hipStream_t st) {
+  typename KPackFragA<T>::Args a{Apm, Apk, MS, KS, (KS + 3) / 4, (MS + 15) / 16};
+  launch<KPackFragA<T>>(dim3(grid_1d((int64_t)frag_a_elems(MS, KS))), dim3(256), 0, st, a);
+  TT_LAUNCH_CHECK();
+}
+void pack_frag_a(const double* Apm, double* Apk, int MS, int KS, hipStream_t st) {
+  pack_frag_a_t<double>(Apm, Apk, MS, KS, st);
+}
+void pack_frag_a_f32(const float* Apm, float* Apk, int MS, int KS, hipStream_t st) {
+  pack_frag_a_t<float>(Apm, Apk, MS, KS, st);
+}
+
+template <typename T, int NLS>
+static void launch_mv(const MvArgs<T>& a, const MvPlan& pl, hipStream_t st) {
+  static std::once_flag once;
+  std::call_once(once, [] {
+    if (allow_lds<KLocalMv<T, NLS>>((int)kMvMaxLds) != hipSuccess) (void)hipGetLastError();
+  });
+  launch<KLocalMv<T, NLS>>(dim3(pl.nlb * pl.P, a.nvec), dim3(KLocalMv<T, NLS>::kThreads), pl.lds, st, a);
+  TT_LAUNCH_CHECK();
+}
+
+// Local-operator form (ttamen_matvec_mode): 0 (default) three GEMMs, 1 the fused kernel where the
+// plan allows it. Measured on MI355X (largest batch system, FP32, ranks ~30-60, n = 60): the fused
+// kernel takes 38 us per application against 24 us for the three GEMMs, and the 256-system batch runs
+// at 14.5 against 37.7 solves/s. In-kernel timestamps show why: a workgroup's four stages each wait on
+// one round trip to memory (3.6-11 us each at these shapes, 17-28 us per workgroup end to end), and
+// the stages of one workgroup cannot overlap, while the three GEMMs spread the same flops over many
+// short tiles on every CU.
+static std::atomic<int> g_matvec_mode{0};
+int matvec_mode(int mode) {
+  const int prev = g_matvec_mode.load();
+  if (mode >= 0) g_matvec_mode.store(mode);
+  return prev;
+}
+
+// Runs the fused kernel when it is selected, the plan allows it and the packed core is there
+// (returns true), else returns false. With a measuring arena it allocates what the fused form needs and returns false,
+// so the caller also measures the three-GEMM form (the workspace covers either).
+template <typename T>
+static bool fused_matvec(const T* PhiL, const T* Apk, const T* PhiR, int l1, int s, int r1, int n, int S,
+                         int l2, int r2, const T* x, T* y, int nvec, unsigned* cnt, int cnt_cap, Arena& ws,
+                         hipStream_t st, const int* skip) {
+  if (!ws.measuring() && (!Apk || g_matvec_mode.load() != 1)) return false;
+  const MvPlan pl = mv_plan(l1, s, r1, n, S, l2, r2, sizeof(T));
+  if (!pl.ok) return false;
+  const size_t mk = ws.mark();
+  MvArgs<T> a;
+  a.PhiL = PhiL; a.Apk = Apk; a.PhiR = PhiR; a.x = x; a.y = y; a.part = nullptr; a.cnt = nullptr; a.skip = skip;
+  a.l1 = l1; a.s = s; a.r1 = r1; a.n = n; a.S = S; a.l2 = l2; a.r2 = r2; a.nvec = nvec;
+  a.BL = pl.BL; a.BR = pl.BR; a.nlb = pl.nlb; a.P = pl.P;
+  a.KS = pl.KS; a.MS = pl.MS; a.KST = pl.KST; a.RB = pl.RB; a.pad = 0;
+  const int64_t need_cnt = (int64_t)nvec * pl.nlb;
+  bool zero_cnt = false;
+  if (pl.P > 1) {
+    a.part = ws.alloc<T>((size_t)pl.P * nvec * l1 * n * l2);
+    if (cnt && need_cnt <= cnt_cap) {
+      a.cnt = cnt;
+    } else {
+      a.cnt = ws.alloc<unsigned>((size_t)need_cnt);
+      zero_cnt = true;
+    }
+  }
+  if (ws.measuring()) {
+    ws.release(mk);
+    return false;
+  }
+  if (zero_cnt) zero_words(a.cnt, need_cnt, st);
+  switch (pl.NLS) {
+    case 4: launch_mv<T, 4>(a, pl, st); break;
+    case 8: launch_mv<T, 8>(a, pl, st); break;
+    default: launch_mv<T, 16>(a, pl, st); break;
+  }
+  ws.release(mk);
+  return true;
+}
 
 // Local operator y[v] = PhiL * A * PhiR * x[v]   (reference: matvecs.h:196-212,
 // amen_solve.h:17-27, contraction 'lsr,smnS,LSR,rnR->lmL').
@@ -59,12 +178,19 @@ static void matvec_impl(const T* PhiL, const T* Apm, const T* PhiR, int l1, int 
 
 void local_matvec(const LocalOp& op, const double* x, double* y, int nvec, Arena& ws,
                   hipStream_t st, const int* skip) {
+  const bool square = op.n_out == 0 || op.n_out == op.n;
+  if (square && fused_matvec<double>(op.PhiL, op.Apk, op.PhiR, op.l1, op.s, op.r1, op.n, op.S, op.l2, op.r2,
+                                     x, y, nvec, op.cnt, op.cnt_cap, ws, st, skip))
+    return;
   matvec_impl<double>(op.PhiL, op.Apm, op.PhiR, op.l1, op.s, op.r1, op.n, op.n_out ? op.n_out : op.n,
                       op.S, op.l2, op.r2, x, y, nvec, ws, st, skip, gemm_f64);
 }
 
 void local_matvec_f32(const LocalOpF32& op, const float* x, float* y, int nvec, Arena& ws,
                       hipStream_t st, const int* skip) {
+  if (!op.bf16 && fused_matvec<float>(op.PhiL, op.Apk, op.PhiR, op.l1, op.s, op.r1, op.n, op.S, op.l2, op.r2,
+                                      x, y, nvec, op.cnt, op.cnt_cap, ws, st, skip))
+    return;
   matvec_impl<float>(op.PhiL, op.Apm, op.PhiR, op.l1, op.s, op.r1, op.n, op.n, op.S, op.l2, op.r2,
                      x, y, nvec, ws, st, skip, op.bf16 ? gemm_bf16 : gemm_f32);
 }
@@ -73,7 +199,19 @@ size_t local_matvec_ws(const LocalOp& op, int nvec) {
   Arena a;
   a.alloc<double>((size_t)op.l1 * op.s * op.n * op.r2 * nvec);
   a.alloc<double>((size_t)op.l1 * (op.n_out ? op.n_out : op.n) * op.S * op.r2 * nvec);
-  return a.peak() + 64 * 1024 * 1024;  // generous split-K headroom
+  size_t peak = a.peak();
+  // the fused form: partial sums and counters (on top of the packed core, which callers add)
+  const MvPlan pl = mv_plan(op.l1, op.s, op.r1, op.n, op.S, op.l2, op.r2, sizeof(double));
+  if (pl.ok && pl.P > 1)
+    peak = std::max(peak, (size_t)pl.P * nvec * op.l1 * op.n * op.l2 * sizeof(double) +
+                              (size_t)nvec * pl.nlb * sizeof(unsigned) + 512);
+  return peak + 64 * 1024 * 1024;  // generous split-K headroom
+}
+
+size_t local_matvec_fused_bytes(int l1, int s, int r1, int n, int S, int l2, int r2, int nvec, size_t elem) {
+  const MvPlan pl = mv_plan(l1, s, r1, n, S, l2, r2, elem);
+  if (!pl.ok || pl.P <= 1) return 0;
+  return (size_t)pl.P * nvec * l1 * n * l2 * elem + (size_t)nvec * pl.nlb * sizeof(unsigned) + 512;
 }
 
 // Phi forward (reference amen_solve.h:58-68, 'lsr,lML,sMNS,rNR->LSR'):

@@ -8,8 +8,9 @@
 // stand-in, not a device address): the device address is looked up once per device with
 // hipGetSymbolAddress, with that device current, and kept per device.
 //
-// Every call on a GPU shares that GPU's array and leaves its contents behind (nothing in it must
-// start at zero: every byte a call reads it has written first). Calls that overlap on one GPU
+// Every call on a GPU shares that GPU's array and leaves its contents behind: what must start at
+// zero (the fused kernel's counters) is cleared by a kernel at the start of each call, and every
+// other byte a call reads it has written first. Calls that overlap on one GPU
 // (two host threads, or two streams without ordering between them) race on the array.
 #include "ttamen_internal.h"
 #include "../../include/ttamen.h"
@@ -46,6 +47,7 @@ static char* scratch_on_current_device() {
 static size_t local_matvec_scratch(const LocalOp& shape, int nvec) {
   Arena meas;
   meas.alloc<double>((size_t)shape.s * shape.n * shape.n * shape.S);
+  meas.alloc<double>(frag_a_elems(shape.n * shape.S, shape.s * shape.n));
   local_matvec(shape, nullptr, nullptr, nvec, meas, nullptr);
   return meas.peak();
 }
@@ -86,7 +88,10 @@ int ttamen_local_matvec_noalloc(const double* PhiL, const double* A, const doubl
     } no_tune;
     double* Apm = ws.alloc<double>((size_t)s * n * n * S);
     permute_core_A(A, Apm, (int)s, (int)n, (int)S, st);
-    op.PhiL = PhiL; op.Apm = Apm; op.PhiR = PhiR;
+    double* Apk = ws.alloc<double>(frag_a_elems((int)(n * S), (int)(s * n)));
+    pack_frag_a(Apm, Apk, (int)(n * S), (int)(s * n), st);
+    // the fused kernel's counters come from the scratch and are cleared (by a kernel) first
+    op.PhiL = PhiL; op.Apm = Apm; op.Apk = Apk; op.PhiR = PhiR;
     local_matvec(op, x, y, (int)nvec, ws, st);
   });
 }

@@ -248,12 +248,19 @@ int lu_max_order();
 //   y[v][l,m,L] = sum PhiL[l,s,r] A[s,m,n,S] PhiR[L,S,R] x[v][r,n,R]
 // PhiL is l1 x s x r1, PhiR is l2 x S x r2, x is r1 x n x r2, y is l1 x n x l2.
 // Apm is the permuted core from permute_core_A.
+// With Apk (the permuted core packed by pack_frag_a) the operator runs as one fused kernel
+// (kernels_matvec.h) where the shape allows, else as three GEMMs on Apm. cnt (optional): cnt_cap
+// zeroed counters the fused kernel may use (it leaves them at zero); without them it takes
+// counters from the workspace and clears them first.
 struct LocalOp {
   const double* PhiL = nullptr;
   const double* Apm = nullptr;
   const double* PhiR = nullptr;
   int l1 = 1, s = 1, r1 = 1, n = 1, S = 1, l2 = 1, r2 = 1;
   int n_out = 0;  // mode size of y for a rectangular operator core A[s][n_out][n][S] (0: n)
+  const double* Apk = nullptr;
+  unsigned* cnt = nullptr;
+  int cnt_cap = 0;
 };
 void local_matvec(const LocalOp& op, const double* x, double* y, int nvec, Arena& ws,
                   hipStream_t st, const int* skip = nullptr);
@@ -265,7 +272,19 @@ struct LocalOpF32 {
   const float* PhiR = nullptr;
   int l1 = 1, s = 1, r1 = 1, n = 1, S = 1, l2 = 1, r2 = 1;
   bool bf16 = false;  // BF16 products (operands rounded to BF16), FP32 accumulation
+  const float* Apk = nullptr;
+  unsigned* cnt = nullptr;
+  int cnt_cap = 0;
 };
+// Elements of the operator core Apm (MS x KS, MS = n S, KS = s n) packed in MFMA fragment order.
+size_t frag_a_elems(int MS, int KS);
+void pack_frag_a(const double* Apm, double* Apk, int MS, int KS, hipStream_t st);
+void pack_frag_a_f32(const float* Apm, float* Apk, int MS, int KS, hipStream_t st);
+// Local-operator form: 0 three GEMMs (default), 1 the fused kernel where its plan allows it; sets the
+// mode if mode >= 0 and returns the previous one.
+int matvec_mode(int mode);
+// Workspace of the fused local operator's partial sums and counters at these sizes (0 if none).
+size_t local_matvec_fused_bytes(int l1, int s, int r1, int n, int S, int l2, int r2, int nvec, size_t elem);
 void local_matvec_f32(const LocalOpF32& op, const float* x, float* y, int nvec, Arena& ws,
                       hipStream_t st, const int* skip = nullptr);
 
