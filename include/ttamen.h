@@ -158,6 +158,31 @@ int ttamen_serve(ttamen_system* systems, const double* arrival_s, int count, int
                  void* stream);
 
 /*
+ * Online serving (no counterpart in the reference): the same continuous batching as ttamen_serve,
+ * for requests that arrive while others run. ttamen_server_create starts a server that owns
+ * `workspace` (the first ttamen_cohort_staging_bytes() bytes hold launch tables, the rest is the
+ * serving memory) and runs cohort lanes on host threads and streams of its own. Shared solver
+ * options are fixed at creation; rank cap, seed and x0/z0 come with each request.
+ * ttamen_server_submit queues one request (its descriptor, arrays and device buffers stay the
+ * caller's and must stay valid until the request is done); its inputs may still be in flight on
+ * `stream`: the request starts after the work queued there so far. Requests are admitted in
+ * submission order when a slot is idle and a contiguous region of their footprint is free; a
+ * request larger than the serving memory is refused (status 2). ttamen_server_poll reports the
+ * state (0 queued, 1 running, 2 done) and the submission, admission and completion times (seconds
+ * since creation); ttamen_server_wait blocks until the request is done and returns its status
+ * (the solution is then complete on the device). ttamen_server_destroy lets every submitted request
+ * finish, then stops the server. Each result equals ttamen_amen_solve's on the same inputs.
+ */
+typedef struct ttamen_server ttamen_server;
+int ttamen_server_create(ttamen_server** out, int64_t nswp, double eps, int64_t max_full, int64_t kickrank,
+                         int64_t kick2, int64_t local_iterations, int64_t resets, int precision, int slots,
+                         void* workspace, size_t workspace_bytes);
+int ttamen_server_submit(ttamen_server* server, ttamen_system* system, void* stream, int64_t* id_out);
+int ttamen_server_poll(ttamen_server* server, int64_t id, int* state_out, double* times_out);
+int ttamen_server_wait(ttamen_server* server, int64_t id);
+int ttamen_server_destroy(ttamen_server* server);
+
+/*
  * TT rounding (replaces the reference's native round_this, cpp/compression.h:3-41, bound as
  * torchttcpp.round_this at cpp/cpp_ext.cpp:10, which rounds in place): left-to-right QR, then
  * right-to-left truncated SVD with relative accuracy eps / sqrt(d-1) per bond and rank cap

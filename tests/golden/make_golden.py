@@ -11,6 +11,8 @@ torchtt_amd.workloads; a checksum of the inputs guards against a changed generat
                       sweep count, relative residual, norm and inner products with rank-1 probes
   batch_first32       the first 32 systems of the 256-system batched workload (seed 0), eps 1e-8:
                       per system the relative residual, final ranks and sweep count
+and two solves with the local preconditioners ('c' and 'r', every local problem iterative):
+  prec_c_d4_n12, prec_r_d5_n6
 
     python tests/golden/make_golden.py [case ...]     (default: every case)
 """
@@ -40,7 +42,7 @@ def load_ref():
     return mod
 
 
-def run_case(ref, name, A, b, seed, nswp, eps, rmax, kickrank=4, kick2=0, x0=None):
+def run_case(ref, name, A, b, seed, nswp, eps, rmax, kickrank=4, kick2=0, x0=None, max_full=500, prec=0):
     d = len(A)
     N = [a.shape[2] for a in A]
     rA = [1] + [a.shape[3] for a in A]
@@ -53,9 +55,10 @@ def run_case(ref, name, A, b, seed, nswp, eps, rmax, kickrank=4, kick2=0, x0=Non
         xc, rx = [], [1] * (d + 1)
     else:
         xc, rx = x0, [1] + [c.shape[2] for c in x0]
-    x = ref.amen_solve(A, b, xc, N, rA, rb, rx, nswp, eps, rmax, 500, kickrank, kick2, 40, 2, False, 0)
+    x = ref.amen_solve(A, b, xc, N, rA, rb, rx, nswp, eps, rmax, max_full, kickrank, kick2, 40, 2, False, prec)
     out = {"d": d, "seed": seed, "nswp": nswp, "eps": eps, "rmax": rmax, "kickrank": kickrank,
-           "kick2": kick2, "ranks": np.array([1] + [c.shape[2] for c in x])}
+           "kick2": kick2, "ranks": np.array([1] + [c.shape[2] for c in x]), "max_full": max_full,
+           "prec": prec}
     for k in range(d):
         out["A%d" % k] = A[k].numpy()
         out["b%d" % k] = b[k].numpy()
@@ -162,6 +165,13 @@ def main(cases):
         precision_case(ref)
     if "batch_first32" in cases:
         batch_case(ref)
+    if "prec" in cases:
+        # local preconditioners 'c' (1) and 'r' (2), cpp/matvecs.h:153-194; max_full = 1 so that every
+        # local problem takes the iterative branch, where the preconditioner applies
+        A, b = W.laplace_potential(4, 12, seed=1, rhs_rank=2)
+        run_case(ref, "prec_c_d4_n12", A, b, 0, 22, 1e-10, 40, max_full=1, prec=1)
+        A, b = W.laplace_potential(5, 6, seed=1, rhs_rank=2)
+        run_case(ref, "prec_r_d5_n6", A, b, 0, 22, 1e-10, 40, max_full=1, prec=2)
     if "small" not in cases:
         return
     # Laplacian d=6, n=16, h=1/17, b = ones, rank cap 20, 10 sweeps (two tolerances)
@@ -191,4 +201,4 @@ def main(cases):
 
 
 if __name__ == "__main__":
-    main(sys.argv[1:] or ["small", "precision_d12_n64", "batch_first32"])
+    main(sys.argv[1:] or ["small", "precision_d12_n64", "batch_first32", "prec"])

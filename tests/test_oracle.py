@@ -333,3 +333,17 @@ def test_oracle_matches_reference_on_batch_systems(index):
     ref_res = float(f["residual"][index])
     assert abs(res - ref_res) <= 0.1 * ref_res, (res, ref_res)
     assert (res <= 1e-8) == (ref_res <= 1e-8)
+
+
+@pytest.mark.parametrize("name,mode", [("prec_c_d4_n12", "c"), ("prec_r_d5_n6", "r")])
+def test_oracle_preconditioned_against_reference_fixture(name, mode):
+    """The reference's C++ solver with its local preconditioners (cpp/matvecs.h:153-194) and every
+    local problem iterative (max_full = 1): the oracle's restatement reaches the same ranks and
+    solution from the same draws."""
+    c = load_golden(name)
+    assert int(c["prec"]) == {"c": 1, "r": 2}[mode] and int(c["max_full"]) == 1
+    x = O.amen_solve(c["A"], c["b"], z0=[z.copy() for z in c["z"]], nswp=c["nswp"], eps=c["eps"], rmax=c["rmax"],
+                     max_full=1, preconditioner=mode)
+    assert [1] + [a.shape[2] for a in x] == c["ranks"].tolist()
+    ref = O.tt_full(c["x"])
+    assert np.linalg.norm(O.tt_full(x) - ref) <= 1e-8 * np.linalg.norm(ref)

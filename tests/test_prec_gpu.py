@@ -82,3 +82,26 @@ def test_preconditioner_errors():
         tt.solvers.amen_solve(A, b, preconditioner="x")
     with pytest.raises(Exception):
         tt.solvers.amen_solve(A, b, preconditioner="c", precision="fp32")
+
+
+@pytest.mark.parametrize("name,mode", [("prec_c_d4_n12", "c"), ("prec_r_d5_n6", "r")])
+def test_preconditioned_solve_against_reference_fixture(name, mode):
+    """GPU solve with the reference's preconditioner on the reference's own fixture (written by its
+    C++ solver with the same draws, every local problem iterative): same final ranks, solution
+    within 1e-8, residual within the reference test's bound (tests/test_solvers.py:23-38)."""
+    import torchtt_amd as tt
+    from conftest import load_golden
+    c = load_golden(name)
+    A = tt.TT([torch.from_numpy(a) for a in c["A"]])
+    b = tt.TT([torch.from_numpy(v) for v in c["b"]])
+    z = [torch.from_numpy(v.copy()) for v in c["z"]]
+    x = tt.solvers.amen_solve(A.cuda(), b.cuda(), nswp=int(c["nswp"]), eps=float(c["eps"]), rmax=int(c["rmax"]),
+                              max_full=1, preconditioner=mode, _z0=z)
+    assert list(x.R) == c["ranks"].tolist(), (x.R, c["ranks"].tolist())
+    fo = O.tt_full(c["x"])
+    fg = O.tt_full([v.cpu().numpy() for v in x.cores])
+    diff = np.linalg.norm(fg - fo) / np.linalg.norm(fo)
+    res = tt.solvers.residual(A.cuda(), x, b.cuda())
+    print("%s: rel diff vs reference %.2e residual %.2e" % (name, diff, res), flush=True)
+    assert diff <= 1e-8
+    assert res < 5e-8

@@ -28,6 +28,7 @@
 #include <thread>
 #include <condition_variable>
 #include <map>
+#include <deque>
 
 namespace ttamen {
 
@@ -1381,6 +1382,207 @@ static void run_cohort(ttamen_system* systems, int count, int64_t nswp, double e
   TT_CHECK(first_err.empty(), first_status, first_err);
 }
 
+// ---------------------------------------------------------------- online serving
+// A server owns a workspace and runs cohort lanes on threads of its own until it is destroyed;
+// requests are submitted one at a time while others run (ttamen_server_submit) and admitted, in
+// submission order, whenever a slot is idle and a contiguous region of their footprint is free.
+// Admission, memory and results work as in run_cohort; the difference is that the request list is
+// not known in advance.
+namespace ttamen {
+
+class Server {
+ public:
+  struct Req {
+    ttamen_system* sys = nullptr;
+    size_t need = 0, off = 0;
+    double cost = 0;
+    hipEvent_t ready = nullptr;
+    int state = 0;  // 0 queued, 1 running, 2 done
+    double t_submit = 0, t_start = 0, t_done = 0;
+  };
+
+  Server(int64_t nswp, double eps, int64_t max_full, int64_t kickrank, int64_t kick2,
+         int64_t local_iterations, int64_t resets, int precision, int slots, void* workspace,
+         size_t workspace_bytes)
+      : nswp_(nswp), eps_(eps), max_full_(max_full), kickrank_(kickrank), kick2_(kick2),
+        local_iterations_(local_iterations), resets_(resets), precision_(precision),
+        base_((char*)workspace + kStageBytes), cap_(workspace_bytes - kStageBytes),
+        mem_(workspace_bytes - kStageBytes), t0_(std::chrono::steady_clock::now()) {
+    // validate the shared options once (a request then only brings its own sizes)
+    (void)make_options(nswp, eps, 1, max_full, kickrank, kick2, local_iterations, resets, 0, precision, 0);
+    linalg_init();
+    lu_init();
+    slots_ = std::max(1, std::min(slots, kMaxLanes * kMaxGroup));
+    const int lanes = std::max(1, std::min({cohort_params().lanes, kMaxLanes, slots_}));
+    lane_slots_ = std::max(1, std::min(kMaxGroup, (slots_ + lanes - 1) / lanes));
+    const int workers = std::max(1, std::min(8, slots_));
+    const int lane_workers = std::max(1, (workers + lanes - 1) / lanes);
+    TT_HIP(hipGetDevice(&dev_));
+    char* stage = (char*)workspace;
+    for (int l = 0; l < lanes; ++l) {
+      hipStream_t st;
+      TT_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+      streams_.push_back(st);
+      co_.emplace_back(new Cohort(st, stage + (size_t)l * kStagePerLane, kStagePerLane, lane_workers));
+    }
+    for (int l = 0; l < lanes; ++l) threads_.emplace_back([this, l] { lane(l); });
+  }
+
+  ~Server() {
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      stop_ = true;
+    }
+    for (auto& t : threads_) t.join();
+    co_.clear();
+    for (auto st : streams_) (void)hipStreamDestroy(st);
+    for (auto& r : reqs_)
+      if (r->ready) (void)hipEventDestroy(r->ready);
+  }
+
+  int64_t submit(ttamen_system* y, hipStream_t caller) {
+    TT_CHECK(y && y->d >= 1 && y->N && y->rA && y->rb && y->A_cores && y->b_cores && y->x_out && y->rx_out &&
+                 y->offsets_out,
+             ERR_INVALID, "bad request descriptor");
+    Plan P = make_plan(y->d, y->N, y->rA, y->rb, y->x0_cores ? y->rx0 : nullptr, y->rmax, kickrank_, kick2_);
+    std::unique_ptr<Req> r(new Req());
+    r->sys = y;
+    for (int k = 0; k < y->d; ++k) r->cost += (double)P.Rb[k] * P.N[k] * P.Rb[k + 1];
+    const size_t w = ttamen_amen_solve_workspace_size(y->d, y->N, y->rA, y->rb, y->x0_cores ? y->rx0 : nullptr,
+                                                      y->rmax, kickrank_, kick2_, local_iterations_, max_full_,
+                                                      precision_);
+    TT_CHECK(w > 0, ERR_INVALID, "request: " + g_last_error);
+    r->need = batch_slice(w);
+    TT_CHECK(r->need <= cap_, ERR_WORKSPACE,
+             "the request needs " + std::to_string(r->need) + " bytes; the server has " + std::to_string(cap_));
+    // the request's inputs may still be in flight on the caller's stream: its lane waits for them
+    TT_HIP(hipEventCreateWithFlags(&r->ready, hipEventDisableTiming));
+    TT_HIP(hipEventRecord(r->ready, caller));
+    y->status = 0;
+    std::lock_guard<std::mutex> lk(mu_);
+    TT_CHECK(!stop_, ERR_INVALID, "the server is shutting down");
+    TT_CHECK(lane_err_.empty(), ERR_HIP, "a server lane failed: " + lane_err_);
+    r->t_submit = now_s();
+    cost_sum_ += r->cost;
+    const int64_t id = (int64_t)reqs_.size();
+    queue_.push_back(id);
+    reqs_.push_back(std::move(r));
+    return id;
+  }
+
+  // state 0 queued, 1 running, 2 done; times: submission, admission, completion (s since creation)
+  void poll(int64_t id, int* state, double* times3) {
+    std::lock_guard<std::mutex> lk(mu_);
+    TT_CHECK(id >= 0 && id < (int64_t)reqs_.size(), ERR_INVALID, "unknown request id");
+    const Req& r = *reqs_[id];
+    if (state) *state = r.state;
+    if (times3) {
+      times3[0] = r.t_submit;
+      times3[1] = r.t_start;
+      times3[2] = r.t_done;
+    }
+  }
+
+  int wait(int64_t id) {
+    std::unique_lock<std::mutex> lk(mu_);
+    TT_CHECK(id >= 0 && id < (int64_t)reqs_.size(), ERR_INVALID, "unknown request id");
+    Req& r = *reqs_[id];
+    cv_.wait(lk, [&] { return r.state == 2 || !lane_err_.empty(); });
+    TT_CHECK(r.state == 2, ERR_HIP, "a server lane failed: " + lane_err_);
+    if (r.sys->status != 0) set_last_error(req_err_.count(id) ? req_err_[id] : std::string("request failed"));
+    return r.sys->status;
+  }
+
+ private:
+  double now_s() const {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0_).count();
+  }
+
+  void lane(int l) {
+    try {
+      (void)hipSetDevice(dev_);
+      Cohort& c = *co_[l];
+      hipStream_t st = streams_[l];
+      auto admit = [&](Cohort& cc) -> bool {
+        std::lock_guard<std::mutex> lk(mu_);
+        while (!queue_.empty() && cc.members() < lane_slots_ && running_ < slots_) {
+          const int64_t id = queue_.front();
+          Req* r = reqs_[id].get();
+          size_t off;
+          if (!mem_.alloc(r->need, &off)) break;
+          queue_.pop_front();
+          ++running_;
+          r->off = off;
+          r->state = 1;
+          r->t_start = now_s();
+          TT_HIP(hipStreamWaitEvent(st, r->ready, 0));
+          const double mean = cost_sum_ / (double)reqs_.size();
+          const double rel = r->cost / std::max(1.0, mean);
+          ttamen_system* y = r->sys;
+          char* slice = base_ + off;
+          const size_t bytes = r->need;
+          cc.add(
+              [=] {
+                y->status = abi_call([&] {
+                  Options O = make_options(nswp_, eps_, y->rmax, max_full_, kickrank_, kick2_, local_iterations_,
+                                           resets_, 0, precision_, y->seed);
+                  solve_one(y->d, y->N, y->rA, y->rb, y->A_cores, y->b_cores, y->x0_cores, y->rx0,
+                            y->z0_cores, O, slice, bytes, y->x_out, y->x_out_elems, y->rx_out,
+                            y->offsets_out, y->stats, st);
+                });
+                if (y->status != 0) {
+                  std::lock_guard<std::mutex> lk2(mu_);
+                  req_err_[id] = g_last_error;
+                }
+              },
+              [=] {
+                std::lock_guard<std::mutex> lk2(mu_);
+                mem_.release(r->off, r->need);
+                --running_;
+                r->state = 2;
+                r->t_done = now_s();
+                cv_.notify_all();
+              },
+              std::min(16.0, std::max(1.0 / 16, rel * rel)));
+        }
+        return !(stop_ && queue_.empty());
+      };
+      c.run(admit);
+    } catch (const std::exception& e) {
+      std::lock_guard<std::mutex> lk(mu_);
+      if (lane_err_.empty()) lane_err_ = e.what();
+      cv_.notify_all();
+    }
+  }
+
+  int64_t nswp_;
+  double eps_;
+  int64_t max_full_, kickrank_, kick2_, local_iterations_, resets_;
+  int precision_;
+  char* base_;
+  size_t cap_;
+  FirstFit mem_;
+  std::chrono::steady_clock::time_point t0_;
+  int slots_ = 1, lane_slots_ = 1, running_ = 0, dev_ = 0;
+  double cost_sum_ = 0;
+  bool stop_ = false;
+  std::string lane_err_;
+  std::map<int64_t, std::string> req_err_;
+  std::vector<std::unique_ptr<Req>> reqs_;
+  std::deque<int64_t> queue_;
+  std::mutex mu_;
+  std::condition_variable cv_;
+  std::vector<hipStream_t> streams_;
+  std::vector<std::unique_ptr<Cohort>> co_;
+  std::vector<std::thread> threads_;
+};
+
+}  // namespace ttamen
+
+struct ttamen_server {
+  std::unique_ptr<ttamen::Server> impl;
+};
+
 extern "C" {
 
 const char* ttamen_version(void) { return "ttamen 0.1.0 (gfx950)"; }
@@ -1646,6 +1848,50 @@ int ttamen_serve(ttamen_system* systems, const double* arrival_s, int count, int
                    fflush(stderr);
                  }
                });
+  });
+}
+
+int ttamen_server_create(ttamen_server** out, int64_t nswp, double eps, int64_t max_full, int64_t kickrank,
+                         int64_t kick2, int64_t local_iterations, int64_t resets, int precision, int slots,
+                         void* workspace, size_t workspace_bytes) {
+  return abi_call([&] {
+    TT_CHECK(out && workspace && slots >= 1, ERR_INVALID, "bad server arguments");
+    TT_CHECK(workspace_bytes > kStageBytes, ERR_WORKSPACE, "workspace smaller than the staging area");
+    *out = nullptr;
+    std::unique_ptr<ttamen_server> s(new ttamen_server());
+    s->impl.reset(new Server(nswp, eps, max_full, kickrank, kick2, local_iterations, resets, precision, slots,
+                             workspace, workspace_bytes));
+    *out = s.release();
+  });
+}
+
+int ttamen_server_submit(ttamen_server* server, ttamen_system* system, void* stream, int64_t* id_out) {
+  return abi_call([&] {
+    TT_CHECK(server && server->impl && id_out, ERR_INVALID, "null argument");
+    *id_out = server->impl->submit(system, (hipStream_t)stream);
+  });
+}
+
+int ttamen_server_poll(ttamen_server* server, int64_t id, int* state_out, double* times_out) {
+  return abi_call([&] {
+    TT_CHECK(server && server->impl, ERR_INVALID, "null server");
+    server->impl->poll(id, state_out, times_out);
+  });
+}
+
+int ttamen_server_wait(ttamen_server* server, int64_t id) {
+  int code = 0;
+  const int rc = abi_call([&] {
+    TT_CHECK(server && server->impl, ERR_INVALID, "null server");
+    code = server->impl->wait(id);
+  });
+  return rc != 0 ? rc : code;
+}
+
+int ttamen_server_destroy(ttamen_server* server) {
+  return abi_call([&] {
+    TT_CHECK(server, ERR_INVALID, "null server");
+    delete server;
   });
 }
 
