@@ -300,6 +300,21 @@ int ttamen_tt_full(int d, const int64_t* N, const int64_t* R, const double* cons
                    size_t workspace_bytes, double* out, size_t out_elems, void* stream);
 size_t ttamen_tt_full_workspace_size(int d, const int64_t* N, const int64_t* R);
 
+/*
+ * Global residual check in FP64 on the GPU: out2[0] = ||A x - b||, out2[1] = ||b|| (Frobenius).
+ * Replaces the TT algebra the reference's callers use for it, (A @ x - b).norm() / b.norm()
+ * (torchtt/_tt_base.py:780-846, :545, :990-1052): a left-to-right sweep over the residual TT
+ * (ranks rA rx + rb, its cores never formed) with Householder R factors, so that the norm keeps
+ * its accuracy when A x and b cancel to a small residual. Cores row-major on the device:
+ * A[k] rA_k x N_k x N_k x rA_{k+1}, x[k] rx_k x N_k x rx_{k+1}, b[k] rb_k x N_k x rb_{k+1}.
+ * Synchronous (out2 is host memory).
+ */
+int ttamen_tt_residual_norm(int d, const int64_t* N, const int64_t* rA, const int64_t* rx, const int64_t* rb,
+                            const double* const* A_cores, const double* const* x_cores,
+                            const double* const* b_cores, void* workspace, size_t workspace_bytes,
+                            double* out2, void* stream);
+size_t ttamen_tt_residual_norm_workspace_size(int d, const int64_t* N, const int64_t* rA, const int64_t* rx,
+                                              const int64_t* rb);
 size_t ttamen_tt_round_workspace_size(int d, const int64_t* N, const int64_t* R);
 int ttamen_tt_round(int d, const int64_t* N, const int64_t* R, const double* const* cores,
                     double eps, int64_t rmax, void* workspace, size_t workspace_bytes, double* out,

@@ -90,6 +90,10 @@ _SIGNATURES = {
     "ttamen_local_matvec_noalloc": (ctypes.c_int, [c_dp, c_dp, c_dp, c_dp, c_dp] + [c_i64] * 8 + [c_dp]),
     "ttamen_local_matvec_noalloc_limit": (c_size, []),
     "ttamen_matvec_mode": (ctypes.c_int, [ctypes.c_int]),
+    "ttamen_tt_residual_norm": (ctypes.c_int, [ctypes.c_int, c_i64p, c_i64p, c_i64p, c_i64p, ctypes.POINTER(c_dp),
+                                               ctypes.POINTER(c_dp), ctypes.POINTER(c_dp), c_dp, c_size,
+                                               ctypes.POINTER(ctypes.c_double), c_dp]),
+    "ttamen_tt_residual_norm_workspace_size": (c_size, [ctypes.c_int, c_i64p, c_i64p, c_i64p, c_i64p]),
     "ttamen_server_create": (ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p), c_i64, ctypes.c_double, c_i64, c_i64,
                                             c_i64, c_i64, c_i64, ctypes.c_int, ctypes.c_int, c_dp, c_size]),
     "ttamen_server_submit": (ctypes.c_int, [c_dp, ctypes.POINTER(TTAmenSystem), c_dp, ctypes.POINTER(c_i64)]),

@@ -482,6 +482,42 @@ def cohort_stats():
     return last_cohort_stats
 
 
+def residual_norms(A, x, b):
+    """(||A x - b||, ||b||) in FP64 on the GPU (libttamen: ttamen_tt_residual_norm): a left-to-right
+    sweep over the residual TT (ranks rA rx + rb) with Householder R factors, replacing the TT algebra
+    the reference's callers use for this check ((A @ x - b).norm(), torchtt/_tt_base.py:780-846,
+    :545, :990-1052). Inputs on the CPU are copied to the GPU."""
+    if not (isinstance(A, TT) and isinstance(x, TT) and isinstance(b, TT)):
+        raise InvalidArguments('A, x and b must be TT instances.')
+    if not A.is_ttm or x.is_ttm or b.is_ttm:
+        raise IncompatibleTypes('A must be TT-matrix, x and b vectors.')
+    if list(A.N) != list(x.N) or list(A.M) != list(b.N):
+        raise ShapeMismatch('Dimension mismatch.')
+    _require_real(A, x, b)
+    lib = _lib.load()
+    dev = _solver_device(A)
+    N = list(b.N)
+    d = len(N)
+    with torch.cuda.device(dev):
+        Ac = [c.to(device=dev, dtype=torch.float64).contiguous() for c in A.cores]
+        xc = [c.to(device=dev, dtype=torch.float64).contiguous() for c in x.cores]
+        bc = [c.to(device=dev, dtype=torch.float64).contiguous() for c in b.cores]
+        Na, rA, rx, rb = (_lib.i64_array(v) for v in (N, A.R, x.R, b.R))
+        wsb = lib.ttamen_tt_residual_norm_workspace_size(d, Na, rA, rx, rb)
+        if wsb == 0:
+            _lib.check("ttamen_tt_residual_norm_workspace_size", 1)
+        ws = torch.empty(wsb, dtype=torch.uint8, device=dev)
+        out = (ctypes.c_double * 2)()
+        code = lib.ttamen_tt_residual_norm(d, Na, rA, rx, rb, _lib.ptr_array(Ac), _lib.ptr_array(xc),
+                                           _lib.ptr_array(bc), ctypes.c_void_p(ws.data_ptr()), wsb, out,
+                                           ctypes.c_void_p(_lib.stream_handle(dev)))
+        _lib.check("ttamen_tt_residual_norm", code)
+        del ws
+    return float(out[0]), float(out[1])
+
+
 def residual(A, x, b):
-    """Relative residual ||A x - b|| / ||b|| in TT arithmetic (the check every test applies)."""
-    return float((A @ x - b).norm() / b.norm())
+    """Relative residual ||A x - b|| / ||b|| in FP64 on the GPU (the check every test applies);
+    see residual_norms."""
+    r, nb = residual_norms(A, x, b)
+    return r / nb
