@@ -314,6 +314,45 @@ def test_gmres_solve(name, m, resets, tol):
         assert _rel(xg, np.linalg.solve(Bd, rhs.reshape(-1))) < 1e-10
 
 
+@pytest.mark.parametrize("spread", [0.0, 1e-9, 1e-6])
+def test_gmres_near_krylov_breakdown(spread):
+    """An operator with three clusters of eigenvalues (a Kronecker sum of two small spectra, rotated
+    by a random orthogonal matrix; cluster width `spread`): the Krylov space is (nearly) invariant
+    after three steps, so the second Gram-Schmidt pass removes almost all of w and the downdated
+    norm ||w'||^2 - ||h2||^2 cancels; the close kernel then reduces ||w''|| from w'' itself. The
+    solve agrees with the oracle's restatement of gmres.h (explicit norms) and with a dense solve."""
+    L, lib = _lib()
+    n = 64
+    g = np.random.default_rng(17)
+    lam = np.add.outer(np.array([0.0, 1.0, 1.0, 0.0]), np.array([1.0, 2.0, 2.0, 1.0])).reshape(-1)  # {1, 2, 3}
+    lam = np.tile(lam, n // lam.size) * (1.0 + spread * g.standard_normal(n))
+    Q, _ = np.linalg.qr(g.standard_normal((n, n)))
+    M = (Q * lam) @ Q.T
+    PhiL, PhiR = np.ones((1, 1, 1)), np.ones((1, 1, 1))
+    A = M.reshape(1, n, n, 1)
+    rhs = g.standard_normal((1, n, 1))
+    m, resets, tol = 20, 2, 1e-12
+    mv = lambda v: (M @ v.reshape(-1)).reshape(-1)
+    xo, flag_o, it_o = O.gmres(mv, rhs.reshape(-1), np.zeros(n), m, tol, resets)
+    d = [torch.tensor(v, device=DEV) for v in (PhiL, A, PhiR, rhs)]
+    x = torch.empty(n, dtype=torch.float64, device=DEV)
+    ws = _ws(lib.ttamen_gmres_solve_workspace_size(1, 1, n, 1, 1, m))
+    it, flag = ctypes.c_int64(0), ctypes.c_int(0)
+    L.check("gmres_solve", lib.ttamen_gmres_solve(_p(d[0]), _p(d[1]), _p(d[2]), _p(d[3]), _p(x), 1, 1, n, 1, 1,
+                                                 tol, m, resets, _p(ws), ws.numel(), ctypes.byref(it),
+                                                 ctypes.byref(flag), _stream()))
+    torch.cuda.synchronize()
+    xg = x.cpu().numpy()
+    want = np.linalg.solve(M, rhs.reshape(-1))
+    _say("near breakdown (spread %.0e): gpu flag %d iters %d, oracle flag %d iters %d, error vs dense %.2e"
+         % (spread, flag.value, it.value, flag_o, it_o, _rel(xg, want)))
+    assert np.all(np.isfinite(xg))
+    assert flag.value == flag_o == 1
+    assert abs(it.value - it_o) <= resets
+    assert _rel(xg, want) < 1e-10
+    assert _rel(xg, xo) < 1e-10
+
+
 def test_gemm_autotune():
     """Tuned tiles compute the same products; a solve with the tuner on matches one with it off
     (same ranks, solutions within the solve tolerance)."""

@@ -8,7 +8,9 @@ namespace ttamen {
 constexpr int kGmresMaxCols = 128;    // (local_iterations + 1) upper bound
 constexpr int kGmresMaxBlocks = 256;   // blocks of the streaming kernels
 
-enum { GM_DONE = 0, GM_FLAG = 1, GM_KCOLS = 2, GM_ITERS = 3, GM_RSKIP = 4, GM_REORTHS = 5 };  // istate
+enum { GM_DONE = 0, GM_FLAG = 1, GM_KCOLS = 2, GM_ITERS = 3, GM_RSKIP = 4, GM_REORTHS = 5,  // istate
+       GM_EXACT = 6,    // Arnoldi steps whose ||w''|| was reduced exactly (cancellation)
+       GM_CNT = 8 };    // arrival counter of KGmresClose's exact-norm path (kept at zero)
 enum { GM_BNORM = 0, GM_THR = 1, GM_RNORM = 2, GM_ERR = 3, GM_WN2 = 4, GM_WNORM = 5 };      // dstate
 
 // Device buffers of one GMRES instance (all inside the caller's arena).

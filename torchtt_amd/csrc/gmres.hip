@@ -142,7 +142,8 @@ GmresResult gmres_run(GmresDev g, int64_t N, T* x, const T* rhs, const double* n
       launch<KCgsDots<T>>(dim3(nb), dim3(kGB), 0, st, {Q, N, k + 1, w, P1, done});
       launch<KCgsUpdate<T>>(dim3(nb), dim3(kGB), 0, st, {Q, N, k + 1, w, P1, Hcol, Pn1, P2, done});
       launch<KGmresClose<T>>(dim3(nb), dim3(kGB), 0, st,
-                             {Q, N, k + 1, w, qn, Pw, Pn1, P2, Hcol, g, k, done, g.istate + GM_REORTHS});
+                             {Q, N, k + 1, w, qn, Pw, Pn1, P2, Hcol, g, k, done, g.istate + GM_REORTHS,
+                              Pn1 + kGmresMaxBlocks});
       TT_LAUNCH_CHECK();
       if (prof) {
         TT_HIP(hipEventRecord(tag.e1, st));

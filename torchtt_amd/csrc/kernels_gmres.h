@@ -141,7 +141,12 @@ __device__ __forceinline__ double gm_chunk_update(const T* Q, int64_t N, int kc,
 //   KCgsUpdate   h = sum P1, w' = w - Q h, Pn1 = ||w'||^2 partials and P2 = Q^T w' partials
 //                (the second pass's dots, from the chunk the block just updated)
 //   KGmresClose  if ||w'|| < ||w|| / sqrt(2): h2 = sum P2, w'' = w' - Q h2, H(:,k) += h2 and
-//                ||w''||^2 = ||w'||^2 - ||h2||^2; q_{k+1} = w'' / ||w''||; Givens on H(:,k)
+//                ||w''||^2 = ||w'||^2 - ||h2||^2; q_{k+1} = w'' / ||w''||; Givens on H(:,k).
+//                When ||h2||^2 > ||w'||^2 / 2 that difference cancels (w' nearly in span(Q), near
+//                Krylov breakdown): ||w''||^2 is then reduced from w'' itself -- per-block partials,
+//                summed in block order by the last block to finish, which normalises q_{k+1} and
+//                applies the Givens rotations (the reference computes this norm explicitly,
+//                cpp/gmres.h:106).
 template <typename T>
 struct KCgsDots {
   struct Args { const T* Q; int64_t N; int kc; const T* w; double* P; const int* skip; };
@@ -191,6 +196,7 @@ struct KGmresClose {
   struct Args {
     const T* Q; int64_t N; int kc; T* w; T* qn; const double* Pw; const double* Pn1;
     const double* P2; double* Hcol; GmresDev g; int k; const int* done_in; int* reorth_count;
+    double* Pex;  // per-block partials of ||w''||^2 (exact-norm path)
   };
   static constexpr int kThreads = kGB;
   static constexpr int kMega = std::is_same<T, double>::value ? 8 : 9;
@@ -206,22 +212,57 @@ struct KGmresClose {
     double hn2 = nn[1];
     const int64_t chunk = (a.N + bk.gx - 1) / bk.gx;
     const int64_t lo = bk.x * chunk, hi = min(a.N, lo + chunk);
+    bool exact = false;  // the same decision in every block (same partials, same order)
     if (reorth) {
       gm_reduce_rows(a.P2, kc, bk.gx, h2);
       double s2 = 0.0;
       for (int i = 0; i < kc; ++i) s2 += h2[i] * h2[i];
+      exact = s2 > 0.5 * hn2;
       hn2 -= s2;
-      (void)gm_chunk_update(a.Q, a.N, kc, a.w, h2, lo, hi);
+      const double part = gm_chunk_update(a.Q, a.N, kc, a.w, h2, lo, hi);
       if (bk.x == 0) {
         for (int i = threadIdx.x; i < kc; i += kGB) a.Hcol[i] += h2[i];
         if (threadIdx.x == 0 && a.reorth_count) *a.reorth_count += 1;
+      }
+      if (exact) {
+        // ||w''||^2 of this block's chunk; the last block to finish adds the partials
+        __shared__ double sw[kGB / 64];
+        __shared__ bool last;
+        const double v = gm_wave_sum(part);
+        if ((threadIdx.x & 63) == 0) sw[threadIdx.x >> 6] = v;
+        __syncthreads();
+        // release: each wave's stores (w'', H, the partial) have reached memory before the counter
+        if (threadIdx.x == 0) a.Pex[bk.x] = (sw[0] + sw[1]) + (sw[2] + sw[3]);
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+          __threadfence();
+          const int prev = atomicAdd(a.g.istate + GM_CNT, 1);
+          last = (prev == (int)bk.gx - 1);
+        }
+        __syncthreads();
+        if (!last) return;
+        __threadfence();
+        if (threadIdx.x == 0) {
+          double t = 0.0;
+          for (unsigned b = 0; b < bk.gx; ++b) t += ((volatile double*)a.Pex)[b];
+          sw[0] = t;
+          a.g.istate[GM_CNT] = 0;
+          a.g.istate[GM_EXACT] += 1;
+        }
+        __syncthreads();
+        hn2 = sw[0];
       }
       __syncthreads();
     }
     const double hn = sqrt(hn2 > 0.0 ? hn2 : 0.0);
     const double inv = (hn > 0.0) ? 1.0 / hn : 0.0;
-    for (int64_t j = lo + threadIdx.x; j < hi; j += kGB) a.qn[j] = (T)((double)a.w[j] * inv);
-    if (bk.x != 0) return;
+    if (exact) {  // the last block normalises the whole vector and closes the step
+      for (int64_t j = threadIdx.x; j < a.N; j += kGB) a.qn[j] = (T)((double)((volatile T*)a.w)[j] * inv);
+    } else {
+      for (int64_t j = lo + threadIdx.x; j < hi; j += kGB) a.qn[j] = (T)((double)a.w[j] * inv);
+      if (bk.x != 0) return;
+    }
     // Givens rotations of column k (block 0): its entries, the earlier rotations and the
     // scalars are staged in LDS by all threads at once, then one thread applies them in order.
     __shared__ double hc[kGmresMaxCols + 1], cs[kGmresMaxCols], sn[kGmresMaxCols], sc[3];
@@ -320,6 +361,8 @@ __device__ __forceinline__ void KGmresSetup::run(const Args& a, const Blk&) {
   g.istate[GM_DONE] = 0;
   g.istate[GM_KCOLS] = 0;
   g.istate[GM_REORTHS] = 0;
+  g.istate[GM_EXACT] = 0;
+  g.istate[GM_CNT] = 0;
 }
 
 // y = H(0:kc,0:kc) \ beta(0:kc) (upper triangular), one thread.
