@@ -60,7 +60,7 @@ def parse():
     p.add_argument("--steps", type=int, default=1)
     p.add_argument("--warmup", type=int, default=1)
     p.add_argument("--workload", default="batch", choices=["batch", "potential", "precision", "serve", "solo"])
-    p.add_argument("--precision", default=None, choices=["fp64", "fp32", "bf16"])
+    p.add_argument("--precision", default=None, choices=["fp64", "fp32", "fp32-phi", "bf16"])
     p.add_argument("--systems", type=int, default=256)
     p.add_argument("--workers", type=int, default=8, help="host threads running the systems' host code")
     p.add_argument("--slots", type=int, default=256, help="serve: requests running at once")
@@ -169,7 +169,7 @@ def roofline(prof, precision):
     if precision == "bf16" and n32 > 0:
         n, ms, fl, by, peak = n32, ms32, fl32, by32, PEAK_TFLOPS["bf16"]
         kern = "local matvec BF16 products, FP32 accumulation (3 MFMA GEMM launches per application), Arnoldi steps that ran"
-    elif precision == "fp32" and n32 > 0:
+    elif precision in ("fp32", "fp32-phi") and n32 > 0:
         n, ms, fl, by, peak = n32, ms32, fl32, by32, PEAK_TFLOPS["fp32"]
         kern = "local matvec FP32 (3 MFMA GEMM launches per application), Arnoldi steps that ran"
     else:
@@ -191,7 +191,8 @@ def base_record(args, value, total, config, dtype):
 
 
 def dtype_of(p):
-    return {"fp64": "f64", "fp32": "f32-inner+f64-refinement", "bf16": "bf16-products+f32-acc+f64-refinement"}[p]
+    return {"fp64": "f64", "fp32": "f32-inner+f64-refinement", "fp32-phi": "f32-inner+f32-interfaces+f64-refinement",
+            "bf16": "bf16-products+f32-acc+f64-refinement"}[p]
 
 
 def system_cost(sy):
@@ -359,7 +360,7 @@ def serve_leg(args, tt, W, duration):
     for A, b, rm in items:
         need = max(need, lib.ttamen_amen_solve_workspace_size(len(b.N), tt._lib.i64_array(b.N), tt._lib.i64_array(A.R),
                                                               tt._lib.i64_array(b.R), None, rm, 4, 0, 40, 500,
-                                                              {"fp64": 0, "fp32": 1, "bf16": 2}[args.precision]))
+                                                              {"fp64": 0, "fp32": 1, "bf16": 2, "fp32-phi": 3}[args.precision]))
     print("serve leg: %d requests over %g s" % (len(items), duration), file=sys.stderr, flush=True)
     torch.cuda.synchronize()
     with tt.serving.Server(slots=args.slots, memory_bytes=args.slots * (need + 256), eps=args.eps,
@@ -448,7 +449,7 @@ def run_precision(args, tt, W, lib, _lib):
     torch.manual_seed(0)
     z = tt.solvers._draw_z(bg.N, 4, 0, "cpu")
     rows = {}
-    for prec in ("fp64", "fp32", "bf16"):
+    for prec in ("fp64", "fp32", "fp32-phi", "bf16"):
         for _ in range(args.warmup):
             tt.solvers.amen_solve(Ag, bg, eps=args.eps, rmax=128, precision=prec, _z0=z)
         torch.cuda.synchronize()
@@ -468,16 +469,19 @@ def run_precision(args, tt, W, lib, _lib):
                       "gmres_iters": st["gmres_iters"], "max_rank": st["max_rank"],
                       "refinements": st.get("refinements", 0), "roofline": roofline(prof, prec)}
     x64, x32, xbf = rows["fp64"].pop("x"), rows["fp32"].pop("x"), rows["bf16"].pop("x")
+    xphi = rows["fp32-phi"].pop("x")
     err = float((x32 - x64).norm() / x64.norm())
     err_bf = float((xbf - x64).norm() / x64.norm())
+    err_phi = float((xphi - x64).norm() / x64.norm())
     ok = sum(1 for r in rows.values() if r["residual"] <= RES_BAR)
-    total = rows["fp64"]["time_s"] + rows["fp32"]["time_s"] + rows["bf16"]["time_s"]
+    total = sum(r["time_s"] for r in rows.values())
     cfg = {"workload": "AMEn solve, Laplacian + potential, d=12, n=64, TT rank 4, random rank-4 rhs, rank cap "
                        "128, eps %g: FP64, FP32-inner and BF16-products-inner, both + FP64-refinement" % args.eps,
            "d": 12, "n": 64, "rmax": 128, "eps": args.eps}
     out = base_record(args, 1.0 / rows["fp32"]["time_s"] if rows["fp32"]["residual"] <= RES_BAR else 0.0,
                       total, cfg, dtype_of("fp32"))
-    out.update({"fp64": rows["fp64"], "fp32": rows["fp32"], "bf16": rows["bf16"], "error_fp32_vs_fp64": err,
+    out.update({"fp64": rows["fp64"], "fp32": rows["fp32"], "fp32-phi": rows["fp32-phi"], "bf16": rows["bf16"],
+                "error_fp32_vs_fp64": err, "error_fp32_phi_vs_fp64": err_phi,
                 "error_bf16_vs_fp64": err_bf, "solved_le_1e-8": ok,
                 "roofline": rows["fp32"]["roofline"]})
     return out

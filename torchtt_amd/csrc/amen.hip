@@ -297,6 +297,10 @@ class Solver {
     Apk_.resize(d);
     for (int k = 0; k < d; ++k)
       Apk_[k] = ws_.alloc<double>(frag_a_elems((int)(P.N[k] * P.rA[k + 1]), (int)(P.rA[k] * P.N[k])));
+    if (O.precision == 3) {  // FP32 copies of the operator cores for the FP32 interface updates
+      A32_.resize(d);
+      for (int k = 0; k < d; ++k) A32_[k] = ws_.alloc<float>(P.rA[k] * P.N[k] * P.N[k] * P.rA[k + 1]);
+    }
     if (O.precision >= 1) {
       Apm32_.resize(d);
       Apk32_.resize(d);
@@ -343,6 +347,8 @@ class Solver {
         permute_core_A_f32(A_[k], Apm32_[k], (int)P.rA[k], (int)P.N[k], (int)P.rA[k + 1], st_);
         pack_frag_a_f32(Apm32_[k], Apk32_[k], (int)(P.N[k] * P.rA[k + 1]), (int)(P.rA[k] * P.N[k]), st_);
       }
+    if (O.precision == 3)
+      for (int k = 0; k < d; ++k) cast_f64_f32(A_[k], A32_[k], P.rA[k] * P.N[k] * P.N[k] * P.rA[k + 1], st_);
     TT_LAUNCH_CHECK();
     // initial guess (reference amen_solve.h:161-167)
     rx_.assign(d + 1, 1);
@@ -438,7 +444,7 @@ class Solver {
   int d;
   std::vector<const double*> A_, b_;
   std::vector<double*> Apm_, x_, z_, Phi_, Phib_, Phiz_, Phizb_;
-  std::vector<float*> Apm32_, Apk32_;
+  std::vector<float*> Apm32_, Apk32_, A32_;
   std::vector<double*> Apk_;
   static constexpr int kMvCounters = 1024;  // zeroed counters of the fused local operator
   unsigned* mv_cnt_ = nullptr;
@@ -485,6 +491,34 @@ class Solver {
       g_prof.count += nvec;
       if (g_prof.pending.size() >= 256) g_prof.flush();
     }
+  }
+
+  // A-interface update of core k (reference amen_solve.h:38-68): forward Pn[L,S,R] = sum Phi[l,s,r]
+  // xL[l,M,L] A[s,M,N,S] xR[r,N,R], backward its mirror. With precision 3 the operands are rounded to
+  // FP32 and contracted on the FP32 matrix cores; the interface is kept in FP64.
+  void phi_update(bool fwd, int k, const double* Phi, const double* xL, const double* xR, double* out, int l, int s,
+                  int r, int n, int S, int L, int R) {
+    if (O.precision != 3) {
+      if (fwd) phi_fwd_A(Phi, xL, Apm_[k], xR, out, l, s, r, n, S, L, R, ws_, st_);
+      else phi_bck_A(Phi, xL, A_[k], xR, out, l, s, r, n, S, L, R, ws_, st_);
+      return;
+    }
+    const size_t mk = ws_.mark();
+    // forward: Phi is l x s x r, xL l x n x L, xR r x n x R; backward: Phi is L x S x R and the
+    // result l x s x r
+    const int64_t nPhi = fwd ? (int64_t)l * s * r : (int64_t)L * S * R;
+    const int64_t nOut = fwd ? (int64_t)L * S * R : (int64_t)l * s * r;
+    float* P32 = ws_.alloc<float>(nPhi);
+    float* L32 = ws_.alloc<float>((int64_t)l * n * L);
+    float* R32 = (xR == xL) ? L32 : ws_.alloc<float>((int64_t)r * n * R);
+    float* O32 = ws_.alloc<float>(nOut);
+    cast_f64_f32(Phi, P32, nPhi, st_);
+    cast_f64_f32(xL, L32, (int64_t)l * n * L, st_);
+    if (R32 != L32) cast_f64_f32(xR, R32, (int64_t)r * n * R, st_);
+    if (fwd) phi_fwd_A_f32(P32, L32, Apm32_[k], R32, O32, l, s, r, n, S, L, R, ws_, st_);
+    else phi_bck_A_f32(P32, L32, A32_[k], R32, O32, l, s, r, n, S, L, R, ws_, st_);
+    cast_f32_f64(O32, out, nOut, st_);
+    ws_.release(mk);
   }
 
   // Right-to-left orthogonalisation of the z cores (reference ortho.h:46-73).
@@ -592,8 +626,8 @@ class Solver {
     }
     // interfaces (reference :270-291)
     const int64_t r1 = rx_[k], r2 = rx_[k + 1];
-    phi_bck_A(Phi_[k + 1], x_[k], A_[k], x_[k], Phi_[k], (int)r1, (int)s, (int)r1, (int)n, (int)S,
-              (int)r2, (int)r2, ws_, st_);
+    phi_update(false, k, Phi_[k + 1], x_[k], x_[k], Phi_[k], (int)r1, (int)s, (int)r1, (int)n, (int)S,
+               (int)r2, (int)r2);
     phi_bck_rhs(Phib_[k + 1], b_[k], x_[k], Phib_[k], (int)rbk, (int)r1, (int)n, (int)rbk1, (int)r2,
                 ws_, st_);
     norm2(Phi_[k], r1 * s * r1, sc(NORMA(k - 1)), rs_, st_, true);
@@ -603,8 +637,8 @@ class Solver {
     nrmsc_update(S_, NRMSC(), NORMA(k - 1), NORMX(k - 1), NORMB(k - 1), 1, st_);
     if (!last && P.rzcap > 0) {
       const int64_t rzk = rz_[k], rzk1 = rz_[k + 1];
-      phi_bck_A(Phiz_[k + 1], z_[k], A_[k], x_[k], Phiz_[k], (int)rzk, (int)s, (int)r1, (int)n,
-                (int)S, (int)rzk1, (int)r2, ws_, st_);
+      phi_update(false, k, Phiz_[k + 1], z_[k], x_[k], Phiz_[k], (int)rzk, (int)s, (int)r1, (int)n,
+                 (int)S, (int)rzk1, (int)r2);
       scale_inv(Phiz_[k], rzk * s * r1, sc(NORMA(k - 1)), st_);
       phi_bck_rhs(Phizb_[k + 1], b_[k], z_[k], Phizb_[k], (int)rbk, (int)rzk, (int)n, (int)rbk1,
                   (int)rzk1, ws_, st_);
@@ -836,8 +870,8 @@ class Solver {
       rx_[k + 1] = r;
       stats_.max_rank = std::max<int64_t>(stats_.max_rank, r);
       // interfaces (reference :482-503)
-      phi_fwd_A(Phi_[k], x_[k], Apm_[k], x_[k], Phi_[k + 1], (int)r1, (int)s, (int)r1, (int)n,
-                (int)S, (int)r, (int)r, ws_, st_);
+      phi_update(true, k, Phi_[k], x_[k], x_[k], Phi_[k + 1], (int)r1, (int)s, (int)r1, (int)n,
+                 (int)S, (int)r, (int)r);
       phi_fwd_rhs(Phib_[k], b_[k], x_[k], Phib_[k + 1], (int)rbk, (int)r1, (int)n, (int)rbk1,
                   (int)r, ws_, st_);
       norm2(Phi_[k + 1], r * S * r, sc(NORMA(k)), rs_, st_, true);
@@ -847,8 +881,8 @@ class Solver {
       nrmsc_update(S_, NRMSC(), NORMA(k), NORMX(k), NORMB(k), 1, st_);
       if (!last && P.rzcap > 0) {
         const int64_t rzk = rz_[k], rzk1 = rz_[k + 1];
-        phi_fwd_A(Phiz_[k], z_[k], Apm_[k], x_[k], Phiz_[k + 1], (int)rzk, (int)s, (int)r1,
-                  (int)n, (int)S, (int)rzk1, (int)r, ws_, st_);
+        phi_update(true, k, Phiz_[k], z_[k], x_[k], Phiz_[k + 1], (int)rzk, (int)s, (int)r1,
+                   (int)n, (int)S, (int)rzk1, (int)r);
         scale_inv(Phiz_[k + 1], rzk1 * S * r, sc(NORMA(k)), st_);
         phi_fwd_rhs(Phizb_[k], b_[k], z_[k], Phizb_[k + 1], (int)rbk, (int)rzk, (int)n,
                     (int)rbk1, (int)rzk1, ws_, st_);
@@ -1091,8 +1125,9 @@ static Options make_options(int64_t nswp, double eps, int64_t rmax, int64_t max_
                             int64_t kickrank, int64_t kick2, int64_t local_iterations,
                             int64_t resets, int verbose, int precision, uint64_t seed,
                             int preconditioner = 0) {
-  TT_CHECK(precision >= 0 && precision <= 2, ERR_UNSUPPORTED,
-           "precision must be 0 (FP64), 1 (FP32 + FP64 refinement) or 2 (BF16 products, FP32 "
+  TT_CHECK(precision >= 0 && precision <= 3, ERR_UNSUPPORTED,
+           "precision must be 0 (FP64), 1 (FP32 + FP64 refinement), 3 (the same with FP32 interface updates) "
+           "or 2 (BF16 products, FP32 "
            "accumulation + FP64 refinement)");
   TT_CHECK(preconditioner >= 0 && preconditioner <= 2, ERR_INVALID,
            "preconditioner must be 0 (none), 1 ('c') or 2 ('r')");

@@ -214,20 +214,23 @@ size_t local_matvec_fused_bytes(int l1, int s, int r1, int n, int S, int l2, int
   return (size_t)pl.P * nvec * l1 * n * l2 * elem + (size_t)nvec * pl.nlb * sizeof(unsigned) + 512;
 }
 
+typedef void (*GemmFn)(const GemmDesc&, Arena&, hipStream_t);
+
 // Phi forward (reference amen_solve.h:58-68, 'lsr,lML,sMNS,rNR->LSR'):
 //   t2[l][(M,S)][R] = stages 1-2 of the local operator with PhiL = Phi, x = xR
 //   Pn[L][(S,R)]    = sum_{(l,M)} xL[(l,M)][L] t2[(l,M)][(S,R)]
-void phi_fwd_half_rect(const double* Phi, const double* Apm, const double* xR, double* t2, int l, int s,
-                       int r, int nm, int nk, int S, int R, Arena& ws, hipStream_t st) {
+template <typename T>
+static void phi_fwd_half_t(const T* Phi, const T* Apm, const T* xR, T* t2, int l, int s, int r, int nm, int nk,
+                           int S, int R, Arena& ws, hipStream_t st, GemmFn gemm) {
   size_t mk = ws.mark();
-  double* t1 = ws.alloc<double>((size_t)l * s * nk * R);
+  T* t1 = ws.alloc<T>((size_t)l * s * nk * R);
   {
     GemmDesc g;
     g.M = l; g.N = nk * R; g.K = r; g.B1 = 1; g.B2 = s;
     g.a.p = Phi; g.a.r = (int64_t)s * r; g.a.c = 1; g.a.b2 = r;
     g.b.p = xR; g.b.r = (int64_t)nk * R; g.b.c = 1;
     g.c.p = t1; g.c.r = (int64_t)s * nk * R; g.c.c = 1; g.c.b2 = (int64_t)nk * R;
-    gemm_f64(g, ws, st);
+    gemm(g, ws, st);
   }
   {
     GemmDesc g;
@@ -235,51 +238,45 @@ void phi_fwd_half_rect(const double* Phi, const double* Apm, const double* xR, d
     g.a.p = Apm; g.a.r = (int64_t)s * nk; g.a.c = 1;
     g.b.p = t1; g.b.r = R; g.b.c = 1; g.b.b1 = (int64_t)s * nk * R;
     g.c.p = t2; g.c.r = R; g.c.c = 1; g.c.b1 = (int64_t)nm * S * R;
-    gemm_f64(g, ws, st);
+    gemm(g, ws, st);
   }
   ws.release(mk);
 }
 
-void phi_fwd_A_rect(const double* Phi, const double* xL, const double* Apm, const double* xR,
-                    double* Pn, int l, int s, int r, int nm, int nk, int S, int L, int R,
-                    Arena& ws, hipStream_t st) {
+template <typename T>
+static void phi_fwd_A_t(const T* Phi, const T* xL, const T* Apm, const T* xR, T* Pn, int l, int s, int r, int nm,
+                        int nk, int S, int L, int R, Arena& ws, hipStream_t st, GemmFn gemm) {
   size_t mk = ws.mark();
-  double* t2 = ws.alloc<double>((size_t)l * nm * S * R);
-  phi_fwd_half_rect(Phi, Apm, xR, t2, l, s, r, nm, nk, S, R, ws, st);
+  T* t2 = ws.alloc<T>((size_t)l * nm * S * R);
+  phi_fwd_half_t<T>(Phi, Apm, xR, t2, l, s, r, nm, nk, S, R, ws, st, gemm);
   {
     GemmDesc g;
     g.M = L; g.N = S * R; g.K = l * nm;
     g.a.p = xL; g.a.r = 1; g.a.c = L;
     g.b.p = t2; g.b.r = (int64_t)S * R; g.b.c = 1;
     g.c.p = Pn; g.c.r = (int64_t)S * R; g.c.c = 1;
-    gemm_f64(g, ws, st);
+    gemm(g, ws, st);
   }
   ws.release(mk);
-}
-
-void phi_fwd_A(const double* Phi, const double* xL, const double* Apm, const double* xR,
-               double* Pn, int l, int s, int r, int n, int S, int L, int R, Arena& ws,
-               hipStream_t st) {
-  phi_fwd_A_rect(Phi, xL, Apm, xR, Pn, l, s, r, n, n, S, L, R, ws, st);
 }
 
 // Phi backward (reference amen_solve.h:38-47, 'LSR,lML,sMNS,rNR->lsr'):
 //   u1[(r,N)][S][L] = sum_R xR[(r,N)][R] Phi[L][S][R]            (batch S)
 //   u2[r][(s,M)][L] = sum_{(N,S)} A[(s,M)][(N,S)] u1[r][(N,S)][L] (batch r)
 //   Pp[l][s][r]     = sum_{(M,L)} xL[l][(M,L)] u2[r][s][(M,L)]    (batch s)
-void phi_bck_A_rect(const double* Phi, const double* xL, const double* A, const double* xR,
-                    double* Pp, int l, int s, int r, int nm, int nk, int S, int L, int R,
-                    Arena& ws, hipStream_t st) {
+template <typename T>
+static void phi_bck_A_t(const T* Phi, const T* xL, const T* A, const T* xR, T* Pp, int l, int s, int r, int nm,
+                        int nk, int S, int L, int R, Arena& ws, hipStream_t st, GemmFn gemm) {
   size_t mk = ws.mark();
-  double* u1 = ws.alloc<double>((size_t)r * nk * S * L);
-  double* u2 = ws.alloc<double>((size_t)r * s * nm * L);
+  T* u1 = ws.alloc<T>((size_t)r * nk * S * L);
+  T* u2 = ws.alloc<T>((size_t)r * s * nm * L);
   {
     GemmDesc g;
     g.M = r * nk; g.N = L; g.K = R; g.B2 = S;
     g.a.p = xR; g.a.r = R; g.a.c = 1;
     g.b.p = Phi; g.b.r = 1; g.b.c = (int64_t)S * R; g.b.b2 = R;
     g.c.p = u1; g.c.r = (int64_t)S * L; g.c.c = 1; g.c.b2 = L;
-    gemm_f64(g, ws, st);
+    gemm(g, ws, st);
   }
   {
     GemmDesc g;
@@ -287,7 +284,7 @@ void phi_bck_A_rect(const double* Phi, const double* xL, const double* A, const 
     g.a.p = A; g.a.r = (int64_t)nk * S; g.a.c = 1;
     g.b.p = u1; g.b.r = L; g.b.c = 1; g.b.b1 = (int64_t)nk * S * L;
     g.c.p = u2; g.c.r = L; g.c.c = 1; g.c.b1 = (int64_t)s * nm * L;
-    gemm_f64(g, ws, st);
+    gemm(g, ws, st);
   }
   {
     GemmDesc g;
@@ -295,15 +292,43 @@ void phi_bck_A_rect(const double* Phi, const double* xL, const double* A, const 
     g.a.p = xL; g.a.r = (int64_t)nm * L; g.a.c = 1;
     g.b.p = u2; g.b.r = 1; g.b.c = (int64_t)s * nm * L; g.b.b2 = (int64_t)nm * L;
     g.c.p = Pp; g.c.r = (int64_t)s * r; g.c.c = 1; g.c.b2 = r;
-    gemm_f64(g, ws, st);
+    gemm(g, ws, st);
   }
   ws.release(mk);
 }
 
+void phi_fwd_half_rect(const double* Phi, const double* Apm, const double* xR, double* t2, int l, int s,
+                       int r, int nm, int nk, int S, int R, Arena& ws, hipStream_t st) {
+  phi_fwd_half_t<double>(Phi, Apm, xR, t2, l, s, r, nm, nk, S, R, ws, st, gemm_f64);
+}
+void phi_fwd_A_rect(const double* Phi, const double* xL, const double* Apm, const double* xR,
+                    double* Pn, int l, int s, int r, int nm, int nk, int S, int L, int R,
+                    Arena& ws, hipStream_t st) {
+  phi_fwd_A_t<double>(Phi, xL, Apm, xR, Pn, l, s, r, nm, nk, S, L, R, ws, st, gemm_f64);
+}
+void phi_bck_A_rect(const double* Phi, const double* xL, const double* A, const double* xR,
+                    double* Pp, int l, int s, int r, int nm, int nk, int S, int L, int R,
+                    Arena& ws, hipStream_t st) {
+  phi_bck_A_t<double>(Phi, xL, A, xR, Pp, l, s, r, nm, nk, S, L, R, ws, st, gemm_f64);
+}
+void phi_fwd_A(const double* Phi, const double* xL, const double* Apm, const double* xR,
+               double* Pn, int l, int s, int r, int n, int S, int L, int R, Arena& ws,
+               hipStream_t st) {
+  phi_fwd_A_rect(Phi, xL, Apm, xR, Pn, l, s, r, n, n, S, L, R, ws, st);
+}
 void phi_bck_A(const double* Phi, const double* xL, const double* A, const double* xR,
                double* Pp, int l, int s, int r, int n, int S, int L, int R, Arena& ws,
                hipStream_t st) {
   phi_bck_A_rect(Phi, xL, A, xR, Pp, l, s, r, n, n, S, L, R, ws, st);
+}
+// FP32 operands and result, FP32 accumulation on the matrix cores.
+void phi_fwd_A_f32(const float* Phi, const float* xL, const float* Apm, const float* xR, float* Pn, int l, int s,
+                   int r, int n, int S, int L, int R, Arena& ws, hipStream_t st) {
+  phi_fwd_A_t<float>(Phi, xL, Apm, xR, Pn, l, s, r, n, n, S, L, R, ws, st, gemm_f32);
+}
+void phi_bck_A_f32(const float* Phi, const float* xL, const float* A, const float* xR, float* Pp, int l, int s,
+                   int r, int n, int S, int L, int R, Arena& ws, hipStream_t st) {
+  phi_bck_A_t<float>(Phi, xL, A, xR, Pp, l, s, r, n, n, S, L, R, ws, st, gemm_f32);
 }
 
 // Phib forward (reference amen_solve.h:93-98, 'br,bnB,rnR->BR').

@@ -305,6 +305,11 @@ void phi_fwd_A_rect(const double* Phi, const double* xL, const double* Apm, cons
 void phi_bck_A_rect(const double* Phi, const double* xL, const double* A, const double* xR,
                     double* Pp, int l, int s, int r, int nm, int nk, int S, int L, int R,
                     Arena& ws, hipStream_t st);
+// The A-interface updates with FP32 operands and result (FP32 accumulation on the matrix cores).
+void phi_fwd_A_f32(const float* Phi, const float* xL, const float* Apm, const float* xR, float* Pn, int l, int s,
+                   int r, int n, int S, int L, int R, Arena& ws, hipStream_t st);
+void phi_bck_A_f32(const float* Phi, const float* xL, const float* A, const float* xR, float* Pp, int l, int s,
+                   int r, int n, int S, int L, int R, Arena& ws, hipStream_t st);
 // The forward update before its contraction with xL: t2[l][M][S][R] = sum Phi[l,s,r]
 // A[s,M,N,S] xR[r,N,R] (l x nm x S x R; Apm from permute_core_A_rect).
 void phi_fwd_half_rect(const double* Phi, const double* Apm, const double* xR, double* t2, int l, int s,

@@ -19,7 +19,7 @@ from . import _lib
 #: statistics of the most recent solve (dict), see include/ttamen.h: ttamen_stats
 last_stats = None
 
-_PRECISIONS = {"fp64": 0, "fp32": 1, "bf16": 2}
+_PRECISIONS = {"fp64": 0, "fp32": 1, "bf16": 2, "fp32-phi": 3}
 
 
 def cpp_enabled():
@@ -80,8 +80,9 @@ def amen_solve(A, b, nswp=22, x0=None, eps=1e-10, rmax=32768, max_full=500, kick
     are accepted for compatibility; the native path always uses GMRES and the residual-based
     truncation, as the reference's C++ path does. New arguments:
         precision: 'fp64' (default), 'fp32' (FP32 matrix-core local operator with FP64
-            iterative refinement) or 'bf16' (the same with BF16 products and FP32
-            accumulation); `use_single_precision=True` selects 'fp32'.
+            iterative refinement), 'fp32-phi' (the same, and the interface updates contracted in
+            FP32 on the matrix cores, the interfaces kept in FP64) or 'bf16' (BF16 products and
+            FP32 accumulation in the local operator); `use_single_precision=True` selects 'fp32'.
         seed: seed for device-drawn random enrichment columns (only used when kick2 > 0).
 
     Returns:
