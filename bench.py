@@ -77,6 +77,8 @@ def parse():
                    help="batch: seconds of Poisson arrivals in the serving leg after the batch (0: no leg)")
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-threads", type=int, default=0)
+    p.add_argument("--matvec-mode", type=int, default=0, choices=[0, 1, 2],
+                   help="local operator: 0 GEMM + fused stages 2-3, 1 one fused kernel, 2 three GEMMs")
     p.add_argument("--autotune", type=int, default=0, choices=[0, 1],
                    help="GEMM tile autotuner for single-system solves (1: tune each shape bucket once)")
     a = p.parse_args()
@@ -563,6 +565,7 @@ def main():
     _lib.check("cohort_set_params", lib.ttamen_cohort_set_params(args.window, args.max_age, args.gmres_poll,
                                                                  args.lanes, args.idle_pct))
     lib.ttamen_gemm_autotune(args.autotune)
+    lib.ttamen_matvec_mode(args.matvec_mode)
     fn = {"batch": run_batch, "potential": run_potential, "precision": run_precision,
           "serve": run_serve, "solo": run_solo}[args.workload]
     out = fn(args, tt, W, lib, _lib)

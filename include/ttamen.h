@@ -468,11 +468,14 @@ size_t ttamen_gmres_solve_workspace_size(int64_t r1, int64_t s, int64_t n, int64
  */
 /*
  * Form of the local operator (no counterpart in the reference, whose local operator is three ATen
- * tensordots): 0 (default) three strided-batched MFMA GEMMs; 1 one fused kernel per application
- * (stage intermediates in LDS, partial sums over blocks of the right rank added by the last
- * workgroup in a fixed order) wherever its tiles fit the LDS, else the GEMMs. Both give the local
- * product to rounding; mode 1 measured slower on the solver's shapes and is there to be measured.
- * mode -1 only queries. Returns the previous mode (-1 on a bad argument).
+ * tensordots): 0 (default) stage 1 (Phi_L x) as a strided-batched MFMA GEMM and stages 2-3 (the
+ * operator core, then Phi_R) as one kernel whose stage-2 MFMA results feed the stage-3 MFMAs in
+ * registers (FP64 and FP32, right rank l2 <= 144, else GEMMs; BF16 always uses GEMMs); 1 one
+ * fused kernel per application (stage intermediates in LDS, partial sums over blocks of the right
+ * rank added by the last workgroup in a fixed order) wherever its tiles fit the LDS, else the
+ * GEMMs; 2 three strided-batched MFMA GEMMs. All give the local product to rounding; mode 1
+ * measured slower on the solver's shapes and is there to be measured. mode -1 only queries.
+ * Returns the previous mode (-1 on a bad argument).
  */
 int ttamen_matvec_mode(int mode);
 int ttamen_gemm_autotune(int mode);

@@ -100,7 +100,11 @@ def test_divide_matches_oracle(case):
          % (q.R, [1] + [c.shape[2] for c in q_or], d_or, e, tt.solvers.last_stats))
     assert d_or < 1e-9
     assert e < 50 * eps
-    assert list(q.R) == [1] + [c.shape[2] for c in q_or]
+    # Ranks within one of the oracle's at each interface: at eps = 1e-12 the truncation compares local
+    # residuals at the rounding level, and the oracle's own middle rank on the reference grid moves
+    # between 29 and 30 when the operand is perturbed by 2e-16 relative (a summation order does as much).
+    r_or = [1] + [c.shape[2] for c in q_or]
+    assert len(q.R) == len(r_or) and all(abs(u - v) <= 1 for u, v in zip(q.R, r_or)), (q.R, r_or)
     # the stop test max_res < eps compares local residuals that GMRES drives to ~eps/(2 sqrt(d)):
     # one sweep more or less is within the rounding of two implementations (same ranks, same quotient)
     assert abs(tt.solvers.last_stats["sweeps"] - tr["sweeps"]) <= 1

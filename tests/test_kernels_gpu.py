@@ -119,9 +119,10 @@ def _rand_local(shape, dtype=torch.float64, seed=0):
     return pl, a, pr, x
 
 
-@pytest.fixture(params=[0, 1], ids=["gemms", "fused"])
+@pytest.fixture(params=[0, 1, 2], ids=["gemm+tail", "fused", "gemms"])
 def matvec_mode(request):
-    """Both forms of the local operator: three GEMMs (default) and the fused kernel."""
+    """The three forms of the local operator: a GEMM and the fused stages 2-3 kernel (default), the
+    one-launch kernel, and three GEMMs."""
     L, lib = _lib()
     prev = lib.ttamen_matvec_mode(request.param)
     yield request.param

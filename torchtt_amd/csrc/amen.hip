@@ -2231,8 +2231,9 @@ int ttamen_cohort_gemm_profile(int enable, double* out6) {
 }
 
 int ttamen_matvec_mode(int mode) {
-  if (mode < -1 || mode > 1) {
-    set_last_error("matvec mode must be -1 (query), 0 (three GEMMs) or 1 (fused kernel)");
+  if (mode < -1 || mode > 2) {
+    set_last_error("matvec mode must be -1 (query), 0 (GEMM + fused stages 2-3), 1 (one fused kernel) or 2 "
+                   "(three GEMMs)");
     return -1;
   }
   return matvec_mode(mode);

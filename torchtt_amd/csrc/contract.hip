@@ -8,6 +8,7 @@
 //   rhs interface      Phib[b, r]
 #include "ttamen_internal.h"
 #include "kernels_matvec.h"
+#include "kernels_mvtail.h"
 
 #include <atomic>
 #include <mutex>
@@ -74,13 +75,15 @@ static void launch_mv(const MvArgs<T>& a, const MvPlan& pl, hipStream_t st) {
   TT_LAUNCH_CHECK();
 }
 
-// Local-operator form (ttamen_matvec_mode): 0 (default) three GEMMs, 1 the fused kernel where the
-// plan allows it. Measured on MI355X (largest batch system, FP32, ranks ~30-60, n = 60): the fused
-// kernel takes 38 us per application against 24 us for the three GEMMs, and the 256-system batch runs
-// at 14.5 against 37.7 solves/s. In-kernel timestamps show why: a workgroup's four stages each wait on
-// one round trip to memory (3.6-11 us each at these shapes, 17-28 us per workgroup end to end), and
-// the stages of one workgroup cannot overlap, while the three GEMMs spread the same flops over many
-// short tiles on every CU.
+// Local-operator form (ttamen_matvec_mode): 0 (default) stage 1 as a GEMM and stages 2-3 as one kernel
+// (KMvTail, kernels_mvtail.h; FP64 and FP32, l2 <= 144, else the GEMMs); 1 the fused one-launch kernel
+// where its plan allows it; 2 three GEMMs.
+// Measured on MI355X (largest batch system, FP32, ranks ~30-60, n = 60): the one-launch kernel takes
+// 38 us per application against 24 us for the three GEMMs, and the 256-system batch runs at 14.5
+// against 37.7 solves/s. In-kernel timestamps show why: a workgroup's four stages each wait on one
+// round trip to memory (3.6-11 us each at these shapes, 17-28 us per workgroup end to end), and the
+// stages of one workgroup cannot overlap, while the GEMMs spread the same flops over many short tiles
+// on every CU.
 static std::atomic<int> g_matvec_mode{0};
 int matvec_mode(int mode) {
   const int prev = g_matvec_mode.load();
@@ -134,11 +137,29 @@ static bool fused_matvec(const T* PhiL, const T* Apk, const T* PhiR, int l1, int
 //   stage 1 (batch v,s):  t1[v][l][s][n][R] = sum_r PhiL[l,s,r] x[v][r][n][R]
 //   stage 2 (batch v,l):  t2[v][l][m][S][R] = sum_{s,n} Apm[(m,S),(s,n)] t1[v][l][(s,n)][R]
 //   stage 3 (one GEMM):   y[(v,l,m)][L]     = sum_{S,R} t2[(v,l,m)][(S,R)] PhiR[L,(S,R)]
+// Stages 2 and 3 of the local operator as one KMvTail launch (t1[v][l][(s,n)][R] -> y); false when the
+// shape is outside the kernel (l2 above 16 kTailMaxLB).
+template <typename T>
+static bool mv_tail(const T* t1, const T* Apm, const T* PhiR, T* y, int64_t nvl, int K, int r2, int nm, int S,
+                    int l2, Arena& ws, hipStream_t st, const int* skip) {
+  const int LB = (l2 + 15) / 16;
+  // a measuring arena returns false, so that the caller also sizes the GEMM form (either may run)
+  if (LB > kTailMaxLB || nvl > INT32_MAX || (nm + 15) / 16 > 65535 || ws.measuring()) return false;
+  typename KMvTail<T>::Args a;
+  a.t1 = t1; a.Apm = Apm; a.PhiR = PhiR; a.y = y; a.skip = skip;
+  a.K = K; a.r2 = r2; a.nm = nm; a.S = S; a.l2 = l2; a.LB = LB;
+  const int vw = (int)(16 / sizeof(T));
+  a.apv = (K % vw == 0 && reinterpret_cast<uintptr_t>(Apm) % 16 == 0) ? 1 : 0;
+  launch<KMvTail<T>>(dim3((unsigned)nvl, (nm + 15) / 16), dim3(KMvTail<T>::kThreads), 0, st, a);
+  TT_LAUNCH_CHECK();
+  return true;
+}
+
 template <typename T>
 static void matvec_impl(const T* PhiL, const T* Apm, const T* PhiR, int l1, int s, int r1, int n,
                         int nm, int S, int l2, int r2, const T* x, T* y, int nvec, Arena& ws,
                         hipStream_t st, const int* skip,
-                        void (*gemm)(const GemmDesc&, Arena&, hipStream_t)) {
+                        void (*gemm)(const GemmDesc&, Arena&, hipStream_t), bool tail) {
   // n: mode of x (columns of the operator core), nm: mode of y (rows; nm == n when square)
   const int64_t nx = (int64_t)r1 * n * r2;
   const int64_t t1v = (int64_t)l1 * s * n * r2;  // per vector
@@ -154,6 +175,11 @@ static void matvec_impl(const T* PhiL, const T* Apm, const T* PhiR, int l1, int 
     g.c.p = t1; g.c.r = (int64_t)s * n * r2; g.c.c = 1; g.c.b1 = t1v; g.c.b2 = (int64_t)n * r2;
     g.skip = skip;
     gemm(g, ws, st);
+  }
+  if (tail && g_matvec_mode.load() == 0 &&
+      mv_tail<T>(t1, Apm, PhiR, y, (int64_t)nvec * l1, s * n, r2, nm, S, l2, ws, st, skip)) {
+    ws.release(mk);
+    return;
   }
   {
     GemmDesc g;
@@ -183,7 +209,7 @@ void local_matvec(const LocalOp& op, const double* x, double* y, int nvec, Arena
                                      x, y, nvec, op.cnt, op.cnt_cap, ws, st, skip))
     return;
   matvec_impl<double>(op.PhiL, op.Apm, op.PhiR, op.l1, op.s, op.r1, op.n, op.n_out ? op.n_out : op.n,
-                      op.S, op.l2, op.r2, x, y, nvec, ws, st, skip, gemm_f64);
+                      op.S, op.l2, op.r2, x, y, nvec, ws, st, skip, gemm_f64, true);
 }
 
 void local_matvec_f32(const LocalOpF32& op, const float* x, float* y, int nvec, Arena& ws,
@@ -192,7 +218,7 @@ void local_matvec_f32(const LocalOpF32& op, const float* x, float* y, int nvec, 
                                       x, y, nvec, op.cnt, op.cnt_cap, ws, st, skip))
     return;
   matvec_impl<float>(op.PhiL, op.Apm, op.PhiR, op.l1, op.s, op.r1, op.n, op.n, op.S, op.l2, op.r2,
-                     x, y, nvec, ws, st, skip, op.bf16 ? gemm_bf16 : gemm_f32);
+                     x, y, nvec, ws, st, skip, op.bf16 ? gemm_bf16 : gemm_f32, !op.bf16);
 }
 
 size_t local_matvec_ws(const LocalOp& op, int nvec) {

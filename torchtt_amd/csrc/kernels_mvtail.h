@@ -1,0 +1,241 @@
+// Stages 2 and 3 of the local operator in one launch (launch.h form):
+//
+//   t2^T[R][m] = sum_k t1[v,l][k][R] Apm[(m,S)][k]              k = (s,n), one 16x16 block per (R block, S)
+//   y[v,l,m,L] = sum_{S,R} PhiR[L][S][R] t2^T[R][m]               (reference: cpp/matvecs.h:196-212,
+//                                                                 cpp/amen_solve.h:17-27)
+//
+// The stage-2 product is computed transposed so that its MFMA result registers are, as they stand,
+// the B operand of the stage-3 MFMAs: in a 16x16 result block, lane l holds rows R = dR(l/16, v) of
+// column m = l%16 in register v, and the B operand of one 16x16x4 step is row k = l/16 of column
+// l%16, so register v feeds step v with k <-> R = dR(l/16, v). No intermediate leaves the registers.
+//
+// Workgroup = (vector v and row l of the output, block of 16 rows m); 4 waves. An item is one R block
+// with a group of SM values of S: its K loop loads the t1 fragment once and uses it against the SM
+// operator rows (SM independent accumulator chains); its K steps are software-pipelined DEPTH deep
+// (every step's loads are issued DEPTH steps ahead, the step index clamped at the last full step so
+// that the loads are unconditional). The waves take items round-robin and keep partial sums of
+// y^T[L][m] for all L blocks; the four partial sums are added in LDS in a fixed order,
+// (p0 + p2) + (p1 + p3), so the result is deterministic.
+//
+// Operand layouts: t1[v][l][k][R] (k = s n + n', R fastest), Apm[(m, S)][k] row-major (k fastest),
+// PhiR[L][S][R], y[v][l][m][L]. Within a 16-deep K step lane group g's element for MFMA jj is
+// k = kofs(g, jj) (a permutation of the 16: the sum over k is the same, and contiguous per lane so
+// that operator rows load as vectors when K allows).
+#pragma once
+#include "ttamen_internal.h"
+#include "kernels_gemm.h"
+
+namespace ttamen {
+
+constexpr int kTailMaxLB = 9;  // l2 <= 144
+
+template <typename T> __device__ __forceinline__ int tail_kofs(int g, int jj);
+template <> __device__ __forceinline__ int tail_kofs<float>(int g, int jj) { return 4 * g + jj; }
+template <> __device__ __forceinline__ int tail_kofs<double>(int g, int jj) { return 2 * g + (jj & 1) + 8 * (jj >> 1); }
+
+// Row R (within a 16x16 result block) held by lane group g in result register v.
+template <typename T> __device__ __forceinline__ int tail_rrow(int g, int v);
+template <> __device__ __forceinline__ int tail_rrow<double>(int g, int v) { return g + 4 * v; }
+template <> __device__ __forceinline__ int tail_rrow<float>(int g, int v) { return 4 * g + v; }
+
+typedef double tail_d2 __attribute__((ext_vector_type(2)));
+
+// The four k values kofs(g, 0..3) of a lane from a k-contiguous row (16-byte aligned vector loads).
+template <typename T>
+__device__ __forceinline__ void tail_vec4(const T* p, T* out) {
+  if constexpr (std::is_same<T, float>::value) {
+    const f4 v = *reinterpret_cast<const f4*>(p);
+    out[0] = v[0]; out[1] = v[1]; out[2] = v[2]; out[3] = v[3];
+  } else {
+    const tail_d2 v0 = *reinterpret_cast<const tail_d2*>(p);
+    const tail_d2 v1 = *reinterpret_cast<const tail_d2*>(p + 8);
+    out[0] = v0[0]; out[1] = v0[1]; out[2] = v1[0]; out[3] = v1[1];
+  }
+}
+
+// LDS of the partial-sum tree: 2 x kTailMaxLB x 256 values, one buffer for every instantiation.
+__device__ __forceinline__ char* tail_lds() {
+  __shared__ __attribute__((aligned(16))) char smem[2 * kTailMaxLB * 256 * sizeof(double)];
+  return smem;
+}
+
+template <typename T>
+struct KMvTail {
+  struct Args {
+    const T* t1;
+    const T* Apm;
+    const T* PhiR;
+    T* y;
+    const int* skip;
+    int K, r2, nm, S, l2;
+    int LB;   // ceil(l2 / 16) <= kTailMaxLB
+    int apv;  // operator rows load as vectors (K a multiple of the vector width, aligned base)
+  };
+  static constexpr int kThreads = 256;
+  static __device__ __forceinline__ void run(const Args& a, const Blk& bk);
+};
+
+template <typename T, int LB, int SM, int DEPTH, bool APV>
+__device__ __forceinline__ void mvtail_body(const typename KMvTail<T>::Args& a, const Blk& bk) {
+  typedef typename AccT<T>::type acc_t;
+  T* red = reinterpret_cast<T*>(tail_lds());
+  const int64_t vl = bk.x;
+  const int m0 = bk.y * 16;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int r = lane & 15, g = lane >> 4;
+  const T* t1 = a.t1 + vl * a.K * a.r2;
+  const int RB = (a.r2 + 15) / 16;
+  const int SG = (a.S + SM - 1) / SM;
+  const int items = RB * SG;
+  const int mrow = min(m0 + r, a.nm - 1);
+  const int K = a.K;
+  const int64_t r2 = a.r2;
+
+  acc_t yacc[LB];
+#pragma unroll
+  for (int lb = 0; lb < LB; ++lb) yacc[lb] = acc_t{0, 0, 0, 0};
+
+  for (int it = wave; it < items; it += 4) {
+    const int rb = it / SG, s0 = (it - rb * SG) * SM;
+    const T* arow = t1 + min(rb * 16 + r, a.r2 - 1);
+    const T* brow[SM];
+#pragma unroll
+    for (int q = 0; q < SM; ++q) brow[q] = a.Apm + ((int64_t)mrow * a.S + min(s0 + q, a.S - 1)) * K;
+    acc_t d[SM];
+#pragma unroll
+    for (int q = 0; q < SM; ++q) d[q] = acc_t{0, 0, 0, 0};
+    struct Frag { T a[4]; T b[SM][4]; };
+    auto load = [&](Frag& f, int kt) {
+#pragma unroll
+      for (int jj = 0; jj < 4; ++jj) f.a[jj] = arow[(int64_t)(kt + tail_kofs<T>(g, jj)) * r2];
+#pragma unroll
+      for (int q = 0; q < SM; ++q) {
+        if constexpr (APV) {
+          tail_vec4<T>(brow[q] + kt + tail_kofs<T>(g, 0), f.b[q]);
+        } else {
+#pragma unroll
+          for (int jj = 0; jj < 4; ++jj) f.b[q][jj] = brow[q][kt + tail_kofs<T>(g, jj)];
+        }
+      }
+    };
+    auto mma = [&](const Frag& f) {
+#pragma unroll
+      for (int jj = 0; jj < 4; ++jj)
+#pragma unroll
+        for (int q = 0; q < SM; ++q) d[q] = mfma16(f.a[jj], f.b[q][jj], d[q]);
+    };
+    const int nsteps = K / 16;
+    if (nsteps > 0) {
+      Frag f[DEPTH];
+      const int last = nsteps - 1;
+#pragma unroll
+      for (int dd = 0; dd < DEPTH; ++dd) load(f[dd], 16 * min(dd, last));
+      const int full = nsteps / DEPTH * DEPTH;
+      for (int s = 0; s < full; s += DEPTH) {
+#pragma unroll
+        for (int dd = 0; dd < DEPTH; ++dd) {
+          mma(f[dd]);
+          load(f[dd], 16 * min(s + dd + DEPTH, last));
+        }
+      }
+#pragma unroll
+      for (int dd = 0; dd < DEPTH - 1; ++dd)
+        if (full + dd < nsteps) mma(f[dd]);
+    }
+    if (16 * nsteps < K) {  // the last, partial K step: elements past K are zero
+      Frag f;
+#pragma unroll
+      for (int jj = 0; jj < 4; ++jj) {
+        const int k = 16 * nsteps + tail_kofs<T>(g, jj);
+        const bool in = k < K;
+        const int kc = in ? k : 0;
+        const T va = arow[(int64_t)kc * r2];
+        f.a[jj] = in ? va : T(0);
+#pragma unroll
+        for (int q = 0; q < SM; ++q) {
+          const T vb = brow[q][kc];
+          f.b[q][jj] = in ? vb : T(0);
+        }
+      }
+      mma(f);
+    }
+    // stage 3: y^T[L][m] += sum_{R in block rb} PhiR[L][S][R] t2^T[R][m]
+#pragma unroll
+    for (int q = 0; q < SM; ++q) {
+      if (s0 + q < a.S) {
+#pragma unroll
+        for (int lb = 0; lb < LB; ++lb) {
+          const T* prow = a.PhiR + ((int64_t)min(lb * 16 + r, a.l2 - 1) * a.S + s0 + q) * r2;
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            const int R = rb * 16 + tail_rrow<T>(g, v);
+            const T pv = prow[min(R, a.r2 - 1)];
+            yacc[lb] = mfma16(R < a.r2 ? pv : T(0), d[q][v], yacc[lb]);
+          }
+        }
+      }
+    }
+  }
+
+  // (p0 + p2) + (p1 + p3) through LDS, then wave 0 writes its 16 rows m of y.
+  auto slot = [&](int s, int lb, int v) -> T& { return red[((s * LB + lb) * 4 + v) * 64 + lane]; };
+  if (wave >= 2) {
+#pragma unroll
+    for (int lb = 0; lb < LB; ++lb)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) slot(wave - 2, lb, v) = yacc[lb][v];
+  }
+  __syncthreads();
+  if (wave < 2) {
+#pragma unroll
+    for (int lb = 0; lb < LB; ++lb)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) yacc[lb][v] = yacc[lb][v] + slot(wave, lb, v);
+  }
+  __syncthreads();
+  if (wave == 1) {
+#pragma unroll
+    for (int lb = 0; lb < LB; ++lb)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) slot(0, lb, v) = yacc[lb][v];
+  }
+  __syncthreads();
+  if (wave == 0 && m0 + r < a.nm) {
+    T* y = a.y + (vl * a.nm + m0 + r) * a.l2;
+#pragma unroll
+    for (int lb = 0; lb < LB; ++lb)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int L = lb * 16 + acc_row<T>(g, v);
+        if (L < a.l2) y[L] = yacc[lb][v] + slot(0, lb, v);
+      }
+  }
+}
+
+template <typename T, int SM, int DEPTH, bool APV>
+__device__ __forceinline__ void mvtail_lb(const typename KMvTail<T>::Args& a, const Blk& bk) {
+  switch (a.LB) {
+    case 1: mvtail_body<T, 1, SM, DEPTH, APV>(a, bk); break;
+    case 2: mvtail_body<T, 2, SM, DEPTH, APV>(a, bk); break;
+    case 3: mvtail_body<T, 3, SM, DEPTH, APV>(a, bk); break;
+    case 4: mvtail_body<T, 4, SM, DEPTH, APV>(a, bk); break;
+    case 5: mvtail_body<T, 5, SM, DEPTH, APV>(a, bk); break;
+    case 6: mvtail_body<T, 6, SM, DEPTH, APV>(a, bk); break;
+    case 7: mvtail_body<T, 7, SM, DEPTH, APV>(a, bk); break;
+    case 8: mvtail_body<T, 8, SM, DEPTH, APV>(a, bk); break;
+    default: mvtail_body<T, 9, SM, DEPTH, APV>(a, bk); break;
+  }
+}
+
+// S groups of 2 in both precisions (the item order, hence the summation order, is the same wherever the
+// kernel runs); K steps prefetched 2 deep in FP64 (244 VGPRs: 2 waves per SIMD) and 3 deep in FP32
+// (164 VGPRs: 3 waves per SIMD).
+template <typename T>
+__device__ __forceinline__ void KMvTail<T>::run(const Args& a, const Blk& bk) {
+  if (a.skip && *a.skip) return;
+  constexpr int D = std::is_same<T, double>::value ? 2 : 3;
+  if (a.apv) mvtail_lb<T, 2, D, true>(a, bk);
+  else mvtail_lb<T, 2, D, false>(a, bk);
+}
+
+}  // namespace ttamen
