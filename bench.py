@@ -166,17 +166,24 @@ def pmc_traffic(precision):
     return json.loads(blob)["traffic_bytes_per_application"], os.path.relpath(path, ROOT)
 
 
+MATVEC_MODE = [0]  # the local-operator form of this run (--matvec-mode)
+MATVEC_FORM = {0: "a stage-1 MFMA GEMM + one fused stages-2/3 MFMA kernel per application",
+               1: "one fused kernel per application", 2: "3 MFMA GEMM launches per application"}
+
+
 def roofline(prof, precision):
+    mode = MATVEC_MODE[0]
     n64, ms64, fl64, by64, n32, ms32, fl32, by32 = list(prof)
     if precision == "bf16" and n32 > 0:
         n, ms, fl, by, peak = n32, ms32, fl32, by32, PEAK_TFLOPS["bf16"]
-        kern = "local matvec BF16 products, FP32 accumulation (3 MFMA GEMM launches per application), Arnoldi steps that ran"
+        kern = ("local matvec BF16 products, FP32 accumulation (3 MFMA GEMM launches per application), "
+                "Arnoldi steps that ran")
     elif precision in ("fp32", "fp32-phi") and n32 > 0:
         n, ms, fl, by, peak = n32, ms32, fl32, by32, PEAK_TFLOPS["fp32"]
-        kern = "local matvec FP32 (3 MFMA GEMM launches per application), Arnoldi steps that ran"
+        kern = "local matvec FP32 (%s), Arnoldi steps that ran" % MATVEC_FORM[mode]
     else:
         n, ms, fl, by, peak = n64, ms64, fl64, by64, PEAK_TFLOPS["fp64"]
-        kern = "local matvec FP64 (3 MFMA GEMM launches per application)"
+        kern = "local matvec FP64 (%s)" % MATVEC_FORM[mode]
     avg_s = (ms / 1e3) / max(n, 1)
     ach = (fl / max(n, 1)) / avg_s / 1e12 if n else 0.0
     traffic, src = pmc_traffic(precision)
@@ -566,6 +573,7 @@ def main():
                                                                  args.lanes, args.idle_pct))
     lib.ttamen_gemm_autotune(args.autotune)
     lib.ttamen_matvec_mode(args.matvec_mode)
+    MATVEC_MODE[0] = args.matvec_mode
     fn = {"batch": run_batch, "potential": run_potential, "precision": run_precision,
           "serve": run_serve, "solo": run_solo}[args.workload]
     out = fn(args, tt, W, lib, _lib)

@@ -68,12 +68,12 @@ __device__ __forceinline__ char* gemm_lds() {
 }
 
 // FP32 operands, BF16 arithmetic: each element is rounded to BF16 (round to nearest even)
-// when it is staged into LDS, the products run on v_mfma_f32_16x16x16bf16_1k and accumulate
-// in FP32. Fragment layout (16x16xK16 block): lane l holds A[i = l%16][k = 4*(l/16) + j] and
-// B[k = 4*(l/16) + j][n = l%16], j = 0..3; result register v holds D[4*(l/16) + v][l%16].
-// The LDS tiles are stored k-contiguous (A as [m][k], B as [n][k]) so that a lane's four
-// k values are one 8-byte read.
-typedef short s4 __attribute__((ext_vector_type(4)));
+// when it is staged into LDS, the products run on gfx950's v_mfma_f32_16x16x32_bf16 and accumulate
+// in FP32. Fragment layout (16x16xK32 block): lane l holds A[i = l%16][k = 8*(l/16) + j] and
+// B[k = 8*(l/16) + j][n = l%16], j = 0..7; result register v holds D[4*(l/16) + v][l%16].
+// The LDS tiles are stored k-contiguous (A as [m][k], B as [n][k]) so that a lane's eight
+// k values are one 16-byte read.
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 
 __device__ __forceinline__ unsigned short bf16_rne(float f) {
   unsigned u = __float_as_uint(f);
@@ -84,7 +84,7 @@ __device__ __forceinline__ unsigned short bf16_rne(float f) {
 
 template <int BM, int BN>
 struct GemmTileBf16 {
-  static constexpr int BK = 16, PK = 4;  // row of BK + PK bf16 = 40 bytes (8-byte aligned reads)
+  static constexpr int BK = 32, PK = 8;  // row of BK + PK bf16 = 80 bytes (16-byte aligned reads)
   static constexpr size_t kLds = sizeof(unsigned short) * 2 * (BM + BN) * (BK + PK);
   static __device__ __forceinline__ void body(const GemmParams<float>& p, const Blk& bk, char* smem);
 };
@@ -348,18 +348,18 @@ __device__ __forceinline__ void GemmTileBf16<BM, BN>::body(const GemmParams<floa
       loadA(kt + BK);
       loadB(kt + BK);
     }
-    s4 af[MI], bfr[NI];
+    bf16x8 af[MI], bfr[NI];
 #pragma unroll
     for (int mi = 0; mi < MI; ++mi)
-      af[mi] = *reinterpret_cast<const s4*>(&As[cur][wm * (BM / 2) + mi * 16 + lr][4 * lk]);
+      af[mi] = *reinterpret_cast<const bf16x8*>(&As[cur][wm * (BM / 2) + mi * 16 + lr][8 * lk]);
 #pragma unroll
     for (int ni = 0; ni < NI; ++ni)
-      bfr[ni] = *reinterpret_cast<const s4*>(&Bs[cur][wn * (BN / 2) + ni * 16 + lr][4 * lk]);
+      bfr[ni] = *reinterpret_cast<const bf16x8*>(&Bs[cur][wn * (BN / 2) + ni * 16 + lr][8 * lk]);
 #pragma unroll
     for (int mi = 0; mi < MI; ++mi)
 #pragma unroll
       for (int ni = 0; ni < NI; ++ni)
-        acc[mi][ni] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(af[mi], bfr[ni], acc[mi][ni], 0, 0, 0);
+        acc[mi][ni] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[mi], bfr[ni], acc[mi][ni], 0, 0, 0);
     if (more) {
       storeA(cur ^ 1);
       storeB(cur ^ 1);
