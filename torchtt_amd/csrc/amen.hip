@@ -907,6 +907,10 @@ class Solver {
   // r = rhs - B x in FP64, solve B d = r to a relative tolerance tau in FP32, x += d. Stops
   // when ||r|| <= real_tol * ||rhs||, the reference's local GMRES target
   // (amen_solve.h:340-343), or after kMaxRefine (kMaxRefineBf16) steps.
+  // The FP64 update of a step (x_new = x + d, r = rhs - B x_new, ||r||) is enqueued inside the step's
+  // GMRES run, before its cycle state is read back, and ||r|| and ||rhs|| come back in that same read:
+  // the stop test needs no host read of its own. x alternates between two buffers (x_new is formed
+  // from the step's x, so a GMRES restart may form it again).
   // On return sol = x and e = B x - rhs (FP64); res_old is set from the initial residual.
   void refine_solve_f32(int k, const LocalOp& op, const double* prev, const double* rhs,
                         double* sol, double* e, int64_t Nl, double real_tol) {
@@ -953,14 +957,22 @@ class Solver {
     }
     gop.axpby = [&](double a, const float* xx, double b, float* yy, int64_t nn) { axpby_f32(a, xx, b, yy, nn, st_); };
     gop.norm = [&](const float* xx, int64_t nn, double* slot) { norm2_f32(xx, nn, slot, rs_, st_); };
+    double* xb[2] = {sol, ws_.alloc<double>(Nl)};
+    int cur = 0;  // xb[cur] holds x
+    double nrhs = 0.0, rn = 0.0;
     for (int it = 0; it < max_refine; ++it) {
       // The first step always runs (its tolerance is computed on the device); later steps
-      // run while the FP64 residual is above the target.
-      if (it > 0) {
-        read_to_host(host_sync().host_dbl(), S_ + TMP(0), sizeof(double) * 4, st_);
-        const double nrhs = host_sync().host_dbl()[T_NRHS], rn = host_sync().host_dbl()[T_DNORM];
-        if (!(rn > real_tol * nrhs)) break;
-      }
+      // run while the FP64 residual (read with the previous step's GMRES state) is above the target.
+      if (it > 0 && !(rn > real_tol * nrhs)) break;
+      const double* xin = xb[cur];
+      double* xout = xb[cur ^ 1];
+      gop.post = [&, xin, xout](double* xtra) {
+        add_f32_to_f64_into(d32, xin, xout, Nl, st_);
+        matvec(op, xout, e, 1, nullptr, true);
+        sub(rhs, e, r, Nl, st_);
+        norm2(r, Nl, sc(TMP(T_DNORM)), rs_, st_);
+        copy(S_ + TMP(0), xtra, 4, st_);  // T_NRHS and T_DNORM among them
+      };
       cast_f64_f32(r, r32, Nl, st_);
       norm2_f32(r32, Nl, sc(TMP(T_R32)), rs_, st_);
       // tau = min(0.5, max(floor, 0.5 real_tol ||rhs|| / ||r||)); GMRES stops at tau ||r32||
@@ -979,11 +991,11 @@ class Solver {
       stats_.gmres_iters += gr.iters;
       stats_.refinements += 1;
       count_matvec32(op, gr.iters + (gr.cycles - 1));
-      add_f32_to_f64(d32, sol, Nl, st_);
-      matvec(op, sol, e, 1, nullptr, true);
-      sub(rhs, e, r, Nl, st_);
-      norm2(r, Nl, sc(TMP(T_DNORM)), rs_, st_);
+      cur ^= 1;
+      nrhs = gr.xtra[T_NRHS];
+      rn = gr.xtra[T_DNORM];
     }
+    if (cur != 0) copy(xb[cur], sol, Nl, st_);
     axpby(-1.0, r, 0.0, e, Nl, st_);  // e = B x - rhs
     ws_.release(mk);
   }

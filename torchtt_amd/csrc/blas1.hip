@@ -68,7 +68,13 @@ void axpby_f32(double a, const float* x, double b, float* y, int64_t n, hipStrea
 
 void add_f32_to_f64(const float* x, double* y, int64_t n, hipStream_t st) {
   if (n <= 0) return;
-  launch<KAddF32F64>(dim3(grid_for(n)), dim3(256), 0, st, {x, y, n});
+  launch<KAddF32F64>(dim3(grid_for(n)), dim3(256), 0, st, {x, nullptr, y, n});
+  TT_LAUNCH_CHECK();
+}
+
+void add_f32_to_f64_into(const float* x, const double* base, double* y, int64_t n, hipStream_t st) {
+  if (n <= 0) return;
+  launch<KAddF32F64>(dim3(grid_for(n)), dim3(256), 0, st, {x, base, y, n});
   TT_LAUNCH_CHECK();
 }
 

@@ -24,7 +24,8 @@ struct GmresDev {
   double* sn = nullptr;
   double* y = nullptr;
   double* dstate = nullptr;
-  int* istate = nullptr;
+  int* istate = nullptr;      // 16 ints, then the 4 doubles of xtra
+  double* xtra = nullptr;     // caller values read back with the cycle state (GmresOp::post)
   double* partial = nullptr;  // per-block partials, row-major [row][block]
 };
 
@@ -36,10 +37,15 @@ struct GmresOp {
   std::function<void(const T*, T*, const int*, int, int)> apply_step;
   std::function<void(double, const T*, double, T*, int64_t)> axpby;  // y = a x + b y
   std::function<void(const T*, int64_t, double*)> norm;         // *slot = ||x||
+  // Optional: work enqueued after each cycle's solution update and before the cycle's state is read
+  // back; it may store 4 doubles into `xtra`, which come back with the state (GmresResult::xtra).
+  // It must give the same result whether or not another cycle follows (it can run once per cycle).
+  std::function<void(double* xtra)> post;
 };
 
 struct GmresResult {
   int flag = 0, iters = 0, cycles = 0, reorths = 0;
+  double xtra[4] = {0, 0, 0, 0};  // what GmresOp::post stored, read with the last cycle's state
   std::vector<int> cycle_steps;  // Arnoldi steps executed in each cycle
 };
 
@@ -49,7 +55,7 @@ class HostSync {
   static constexpr int NS = 4;
   void init() {
     if (host_) return;
-    TT_HIP(hipHostMalloc((void**)&host_, sizeof(int) * (NS + 16), hipHostMallocDefault));
+    TT_HIP(hipHostMalloc((void**)&host_, sizeof(int) * (NS + 32), hipHostMallocDefault));
     TT_HIP(hipHostMalloc((void**)&hostd_, sizeof(double) * 4096, hipHostMallocDefault));
     for (int i = 0; i < NS; ++i) TT_HIP(hipEventCreateWithFlags(&ev_[i], hipEventDisableTiming));
   }

@@ -24,6 +24,7 @@
 #include "kernels_gmres.h"
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 
 namespace ttamen {
 
@@ -39,7 +40,8 @@ GmresDev gmres_alloc(Arena& ws, int m, int64_t N, int esize) {
   g.sn = ws.alloc<double>(m);
   g.y = ws.alloc<double>(m);
   g.dstate = ws.alloc<double>(16);
-  g.istate = ws.alloc<int>(16);
+  g.istate = ws.alloc<int>(32);  // 16 ints of state, then xtra (4 doubles at byte 64)
+  g.xtra = reinterpret_cast<double*>(g.istate + 16);
   g.partial = ws.alloc<double>((size_t)kGmresMaxBlocks * (2 * kGmresMaxCols + 4));
   return g;
 }
@@ -164,9 +166,12 @@ GmresResult gmres_run(GmresDev g, int64_t N, T* x, const T* rhs, const double* n
     launch<KGmresY>(dim3(1), dim3(1), 0, st, {g});
     launch<KGmresXupd<T>>(dim3(nb), dim3(kGB), 0, st, {Q, N, x, g});
     TT_LAUNCH_CHECK();
-    // Did this cycle converge? (the whole state in one read: flag, columns, iterations, ...)
-    read_to_host(hs.host_int(), g.istate, sizeof(int) * 8, st);
+    if (op.post) op.post(g.xtra);
+    // Did this cycle converge? (the whole state in one read: flag, columns, iterations, ..., and
+    // the caller's xtra values)
+    read_to_host(hs.host_int(), g.istate, op.post ? 64 + sizeof(double) * 4 : sizeof(int) * 8, st);
     for (int i = 0; i < 8; ++i) state[i] = hs.host_int()[i];
+    if (op.post) std::memcpy(res.xtra, hs.host_int() + 16, sizeof(double) * 4);
     have_state = true;
     res.cycle_steps.push_back(state[GM_KCOLS]);
     if (prof) {  // the stream has reached the state read: every event of this cycle is complete

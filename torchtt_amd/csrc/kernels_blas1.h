@@ -126,12 +126,17 @@ struct KAxpbyF32 {
 };
 
 // y (FP64) += x (FP32)
+// y = (base ? base : y) + x, x in FP32
 struct KAddF32F64 {
-  struct Args { const float* x; double* y; int64_t n; };
+  struct Args { const float* x; const double* base; double* y; int64_t n; };
   static constexpr int kThreads = 256;
   static constexpr int kMega = 22;
   static __device__ __forceinline__ void run(const Args& p, const Blk& bk) {
-    TT_GRID_LOOP(i, p.n) p.y[i] += (double)p.x[i];
+    if (p.base) {
+      TT_GRID_LOOP(i, p.n) p.y[i] = p.base[i] + (double)p.x[i];
+    } else {
+      TT_GRID_LOOP(i, p.n) p.y[i] += (double)p.x[i];
+    }
   }
 };
 

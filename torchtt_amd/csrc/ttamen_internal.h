@@ -179,6 +179,8 @@ void axpby_f32(double a, const float* x, double b, float* y, int64_t n, hipStrea
                const int* skip = nullptr);
 // y (FP64) += x (FP32)
 void add_f32_to_f64(const float* x, double* y, int64_t n, hipStream_t st);
+// y (FP64) = base (FP64) + x (FP32)
+void add_f32_to_f64_into(const float* x, const double* base, double* y, int64_t n, hipStream_t st);
 // z = x - y
 void sub(const double* x, const double* y, double* z, int64_t n, hipStream_t st);
 void copy(const double* x, double* y, int64_t n, hipStream_t st);
