@@ -478,6 +478,17 @@ size_t ttamen_gmres_solve_workspace_size(int64_t r1, int64_t s, int64_t n, int64
  * Returns the previous mode (-1 on a bad argument).
  */
 int ttamen_matvec_mode(int mode);
+/*
+ * Rank capacity of the solves planned on the calling thread (no counterpart in the reference, whose
+ * C++ solver grows its tensors as the ranks grow): every interface's buffers are planned at
+ * min(rmax + kickrank + kick2, the product bounds, capacity). Default 140, the largest square block
+ * the LDS-resident Cholesky-QR and Jacobi SVD hold; up to 1024, QR and SVD of more columns run as
+ * Householder QR and one-sided Jacobi in device memory. A sweep whose ranks outgrow the capacity
+ * stops with status 4; solving again at a larger capacity gives the result the solve would have
+ * given (the capacity only sizes buffers). cap <= 0 only queries. Returns the previous capacity, or
+ * -1 for cap above 1024. Workspace sizes must be queried at the capacity the solve runs with.
+ */
+int ttamen_rank_capacity(int cap);
 int ttamen_gemm_autotune(int mode);
 int ttamen_gemm_autotune_stats(double* out4);
 void ttamen_gemm_autotune_clear(void);

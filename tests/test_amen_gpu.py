@@ -244,20 +244,51 @@ def test_contract_of_the_return_value():
     assert float(x0.norm()) == 0.0
 
 
-def test_rank_above_the_lds_limit_is_a_status():
-    """With the default rmax the plan is bounded at 140 ranks; a solve whose solution needs more
-    stops with status 4 and a message instead of being refused before it starts. Here the right-hand
-    side has rank 200 at the middle interface and the warm start rank 140 there, so the first
-    left-to-right step keeps 140 directions and its enrichment asks for 144."""
+def test_rank_above_140_solves_at_a_raised_capacity():
+    """A solve whose ranks outgrow the default rank capacity (140, the LDS-resident QR/SVD) stops with
+    status 4 and is solved again at a larger capacity, where QR and SVD of more than 140 columns run
+    in device memory. Laplacian + potential on 16^4, a random right-hand side of rank 200 at the middle
+    interface and a warm start of rank 140 there: the enrichment asks for 144 at once. Same operands
+    and residual-proxy draws as the oracle: the same ranks (within one: eps 1e-10 truncates at the
+    rounding level of the local residuals) and solutions within 1e-8 of each other."""
     import torchtt_amd as tt
+    from torchtt_amd import workloads as W
     g = torch.Generator().manual_seed(5)
     N, rb, r0 = [16] * 4, [1, 16, 200, 16, 1], [1, 16, 140, 16, 1]
-    b = tt.TT([torch.randn(rb[k], N[k], rb[k + 1], dtype=torch.float64, generator=g).cuda() for k in range(4)])
-    x0 = tt.TT([torch.randn(r0[k], N[k], r0[k + 1], dtype=torch.float64, generator=g).cuda() for k in range(4)])
-    A = tt.eye(N, device="cuda")
-    _say("rank above the LDS limit")
-    with pytest.raises(Exception, match="not supported"):
-        tt.solvers.amen_solve(A, b, x0=x0, eps=1e-13, nswp=2)
+    A, _ = W.laplace_potential(4, 16, seed=2)
+    b = [torch.randn(rb[k], N[k], rb[k + 1], dtype=torch.float64, generator=g) for k in range(4)]
+    x0 = [torch.randn(r0[k], N[k], r0[k + 1], dtype=torch.float64, generator=g) for k in range(4)]
+    torch.manual_seed(7)
+    z0 = tt.solvers._draw_z(N, 4, 0, "cpu")
+    _say("rank above 140: GPU solve")
+    lib = tt._lib.load()
+    assert lib.ttamen_rank_capacity(0) == 140
+    x = tt.solvers.amen_solve(tt.TT([c.cuda() for c in A]), tt.TT([c.cuda() for c in b]),
+                              x0=tt.TT([c.cuda() for c in x0]), eps=1e-10, nswp=2, _z0=z0)
+    stats = dict(tt.solvers.last_stats)
+    assert lib.ttamen_rank_capacity(0) == 140  # restored
+    _say("  ranks %s, capacity %d; oracle" % (x.R, stats["rank_capacity"]))
+    assert max(x.R) > 140 and stats["rank_capacity"] > 140
+    xo = O.amen_solve([c.numpy() for c in A], [c.numpy() for c in b], x0=[c.numpy() for c in x0],
+                      z0=[z.numpy() for z in z0], nswp=2, eps=1e-10)
+    ro = [1] + [c.shape[2] for c in xo]
+    fx, fo = x.full().cpu().numpy(), O.tt_full(xo)
+    diff = np.linalg.norm(fx - fo) / np.linalg.norm(fo)
+    _say("  oracle ranks %s, rel diff %.3e" % (ro, diff))
+    assert all(abs(u - v) <= 1 for u, v in zip(x.R, ro)), (x.R, ro)
+    assert diff < 1e-8
+
+
+def test_rank_capacity_abi():
+    """ttamen_rank_capacity: per calling thread, 140 by default, raised up to 1024, above refused."""
+    import torchtt_amd as tt
+    lib = tt._lib.load()
+    assert lib.ttamen_rank_capacity(0) == 140
+    assert lib.ttamen_rank_capacity(300) == 140
+    assert lib.ttamen_rank_capacity(0) == 300
+    assert lib.ttamen_rank_capacity(2000) == -1
+    assert lib.ttamen_rank_capacity(100) == 300   # below the LDS size: clamped to it
+    assert lib.ttamen_rank_capacity(0) == 140
 
 
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])

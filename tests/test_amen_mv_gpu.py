@@ -55,6 +55,22 @@ def test_amen_mv_special_operands():
     assert e < 1e-11
 
 
+def test_amen_mv_rank_above_140():
+    """A product whose rank outgrows the default rank capacity (140): the sweep stops with status 4
+    and runs again at a larger capacity, its QR and SVD above 140 columns in device memory. Identity
+    times a random TT of rank 180 at the middle interface (16^4: at most 256 there), warm-started from
+    that TT (the plan holds 180 there; the enrichment asks for 184), reproduces it."""
+    import torchtt_amd as tt
+    g = torch.Generator().manual_seed(11)
+    N, r = [16] * 4, [1, 16, 180, 16, 1]
+    x = tt.TT([torch.randn(r[k], N[k], r[k + 1], dtype=torch.float64, generator=g).cuda() for k in range(4)])
+    C = tt.amen_mv(tt.eye(N, device="cuda"), x, x0=x, nswp=2, eps=1e-12)
+    e = _rel(C, x)
+    _say("amen_mv rank 180: rel err %.3e ranks %s" % (e, C.R))
+    assert max(C.R) > 140
+    assert e < 1e-10
+
+
 def test_amen_mm_product():
     """5A times 5B against 25 A B (tests/test_algebra_2.py:188-204)."""
     import torchtt_amd as tt

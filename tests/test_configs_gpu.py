@@ -181,7 +181,7 @@ def test_batch_equals_solo_at_n64_rank_cap_80():
 
 def test_batch_reports_each_systems_status():
     """A system that fails inside a batch (its solution needs rank 144 at an interface, above the
-    140-column limit: status 4) does not hide the others: the call raises BatchError with the first
+    rank capacity batched solves are planned at, 140: status 4) does not hide the others: the call raises BatchError with the first
     failure's own status, every system's status, and the other systems' solutions (equal to solo)."""
     import torchtt_amd as tt
     from torchtt_amd import workloads as W

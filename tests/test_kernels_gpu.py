@@ -419,7 +419,10 @@ def _qr_case(kind, m, n, seed):
                                       # numerically rank-deficient and near-square inputs
                                       ("rank_deficient", 16, 12), ("rank_deficient", 120, 110),
                                       ("rank_deficient", 3000, 60), ("rank_deficient", 64, 4),
-                                      ("square_zero_col", 148, 137), ("square_zero_col", 5, 4)])
+                                      ("square_zero_col", 148, 137), ("square_zero_col", 5, 4),
+                                      # more than 140 columns: Householder QR in device memory
+                                      ("rand", 3000, 150), ("rank_deficient", 400, 200), ("zero_cols", 1000, 160),
+                                      ("square_zero_col", 180, 170), ("graded", 600, 141)])
 def test_qr(kind, m, n):
     L, lib = _lib()
     _say("qr", kind, m, n)
@@ -439,7 +442,11 @@ def test_qr(kind, m, n):
                                       ("graded", 300, 40), ("zero_cols", 60, 9), ("rand", 1, 7), ("rand", 9, 1),
                                       ("rand", 1000, 136), ("rand", 4, 256), ("rand", 256, 4), ("zero_cols", 128, 4),
                                       ("rank_deficient", 16, 12), ("rank_deficient", 12, 16),
-                                      ("rank_deficient", 192, 14), ("square_zero_col", 148, 137)])
+                                      ("rank_deficient", 192, 14), ("square_zero_col", 148, 137),
+                                      # rank above 140: one-sided Jacobi in device memory (zero columns: the
+                                      # completion of zero singular values)
+                                      ("rand", 2000, 150), ("rank_deficient", 300, 180), ("rand", 160, 400),
+                                      ("zero_cols", 500, 150), ("square_zero_col", 170, 170)])
 def test_svd(kind, m, n):
     L, lib = _lib()
     _say("svd", kind, m, n)
@@ -456,8 +463,11 @@ def test_svd(kind, m, n):
     smax = max(s_np[0], 1e-300)
     assert np.all(np.diff(sh) <= 1e-15 * smax)
     assert np.max(np.abs(sh - s_np)) <= 1e-13 * smax * max(1, np.sqrt(k))
-    assert np.linalg.norm(Uh.T @ Uh - np.eye(k)) < 1e-12
-    assert np.linalg.norm(Vh.T @ Vh - np.eye(k)) < 1e-12
+    # Jacobi stops once every pair of columns has |cosine| <= k u, so ||U^T U - I||_F is bounded by
+    # about k^2 u: 1e-12 up to k = 134, k^2 2^-52 above (7.2e-12 at k = 180)
+    orth = max(1e-12, k * k * 2.0 ** -52)
+    assert np.linalg.norm(Uh.T @ Uh - np.eye(k)) < orth
+    assert np.linalg.norm(Vh.T @ Vh - np.eye(k)) < orth
     assert np.linalg.norm((Uh * sh) @ Vh.T - A) <= 1e-13 * np.sqrt(m * n) * np.linalg.norm(A)
 
 

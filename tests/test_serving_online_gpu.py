@@ -63,8 +63,8 @@ def test_online_server_submit_while_running(precision):
 
 
 def test_online_server_failing_request():
-    """A request that fails (its solution needs rank 144 at an interface, above the 140-column limit:
-    status 4) reports its status; requests around it are unaffected."""
+    """A request that fails (its solution needs rank 144 at an interface, above the rank capacity
+    served solves are planned at, 140: status 4) reports its status; requests around it are unaffected."""
     import torchtt_amd as tt
     g = torch.Generator().manual_seed(5)
     N, rb, r0 = [16] * 4, [1, 16, 200, 16, 1], [1, 16, 140, 16, 1]

@@ -33,7 +33,6 @@ def _draw_z(P, kickrank, kick2, device):
 
 
 def _run(entry, sizes, A_cores, b_cores, P, x0, kickrank, kick2, rmax, nswp, eps, verbose, dev, device, dtype):
-    global last_stats
     lib = _lib.load()
     d = len(P)
     caps = _caps(rmax, d)
@@ -50,27 +49,49 @@ def _run(entry, sizes, A_cores, b_cores, P, x0, kickrank, kick2, rmax, nswp, eps
             z = [c.to(device=dev, dtype=torch.float64).contiguous() for c in z]
         capsa = _lib.i64_array(caps)
         rx0a = _lib.i64_array(rx0) if rx0 is not None else None
-        ws_bytes, out_elems, call = sizes(lib, capsa, rx0a)
-        if ws_bytes == 0 or out_elems == 0:
-            _lib.check(entry + "_workspace_size", 1)
-        ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dev)
-        xout = torch.empty(out_elems, dtype=torch.float64, device=dev)
-        rx_out = (ctypes.c_int64 * (d + 1))()
-        off_out = (ctypes.c_int64 * d)()
-        stats = _lib.TTAmenStats()
-        code = call(lib, _lib.ptr_array(Ac), _lib.ptr_array(bc), _lib.ptr_array(xc) if xc is not None else None, rx0a,
-                    _lib.ptr_array(z) if z is not None else None, int(nswp), float(eps), capsa, int(kickrank),
-                    int(kick2), 0, int(bool(verbose)), ctypes.c_void_p(ws.data_ptr()), ws_bytes,
-                    ctypes.c_void_p(xout.data_ptr()), out_elems, rx_out, off_out, ctypes.byref(stats),
-                    ctypes.c_void_p(_lib.stream_handle(dev)))
-        _lib.check(entry, code)
-        cores = []
-        for k in range(d):
-            n = rx_out[k] * P[k] * rx_out[k + 1]
-            cores.append(xout[off_out[k]:off_out[k] + n].reshape(rx_out[k], P[k], rx_out[k + 1]).clone()
-                         .to(device=device, dtype=dtype))
-        last_stats = stats.as_dict()
-        del ws, xout
+        cap = None
+        while True:  # at the default rank capacity, then larger ones if the sweep outgrows it
+            prev_cap = lib.ttamen_rank_capacity(cap) if cap else None
+            try:
+                cores = _run_native(entry, sizes, lib, d, P, Ac, bc, xc, rx0a, z, capsa, kickrank, kick2, nswp, eps,
+                                    verbose, dev, device, dtype)
+                break
+            except _lib.TTAmenError as e:
+                from .solvers import _grown_capacity
+                cap = _grown_capacity(e, cap or lib.ttamen_rank_capacity(0))
+                if cap is None:
+                    raise
+            finally:
+                if prev_cap is not None:
+                    lib.ttamen_rank_capacity(prev_cap)
+    return cores
+
+
+def _run_native(entry, sizes, lib, d, P, Ac, bc, xc, rx0a, z, capsa, kickrank, kick2, nswp, eps, verbose, dev,
+                device, dtype):
+    """One call of the native product at the calling thread's rank capacity; the product's cores."""
+    global last_stats
+    ws_bytes, out_elems, call = sizes(lib, capsa, rx0a)
+    if ws_bytes == 0 or out_elems == 0:
+        _lib.check(entry + "_workspace_size", 1)
+    ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dev)
+    xout = torch.empty(out_elems, dtype=torch.float64, device=dev)
+    rx_out = (ctypes.c_int64 * (d + 1))()
+    off_out = (ctypes.c_int64 * d)()
+    stats = _lib.TTAmenStats()
+    code = call(lib, _lib.ptr_array(Ac), _lib.ptr_array(bc), _lib.ptr_array(xc) if xc is not None else None, rx0a,
+                _lib.ptr_array(z) if z is not None else None, int(nswp), float(eps), capsa, int(kickrank),
+                int(kick2), 0, int(bool(verbose)), ctypes.c_void_p(ws.data_ptr()), ws_bytes,
+                ctypes.c_void_p(xout.data_ptr()), out_elems, rx_out, off_out, ctypes.byref(stats),
+                ctypes.c_void_p(_lib.stream_handle(dev)))
+    _lib.check(entry, code)
+    cores = []
+    for k in range(d):
+        n = rx_out[k] * P[k] * rx_out[k + 1]
+        cores.append(xout[off_out[k]:off_out[k] + n].reshape(rx_out[k], P[k], rx_out[k + 1]).clone()
+                     .to(device=device, dtype=dtype))
+    last_stats = stats.as_dict()
+    del ws, xout
     return cores
 
 

@@ -90,6 +90,7 @@ _SIGNATURES = {
     "ttamen_local_matvec_noalloc": (ctypes.c_int, [c_dp, c_dp, c_dp, c_dp, c_dp] + [c_i64] * 8 + [c_dp]),
     "ttamen_local_matvec_noalloc_limit": (c_size, []),
     "ttamen_matvec_mode": (ctypes.c_int, [ctypes.c_int]),
+    "ttamen_rank_capacity": (ctypes.c_int, [ctypes.c_int]),
     "ttamen_tt_residual_norm": (ctypes.c_int, [ctypes.c_int, c_i64p, c_i64p, c_i64p, c_i64p, ctypes.POINTER(c_dp),
                                                ctypes.POINTER(c_dp), ctypes.POINTER(c_dp), c_dp, c_size,
                                                ctypes.POINTER(ctypes.c_double), c_dp]),

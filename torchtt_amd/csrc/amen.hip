@@ -230,9 +230,10 @@ static Plan make_plan(int d, const int64_t* N, const int64_t* rA, const int64_t*
   for (int k = 0; k < d; ++k) {
     const int64_t rcap = std::max<int64_t>(rmax_list ? rmax_list[k + 1] : rmax, 1) + p.rzcap;
     int64_t b = std::min(sat_mul(p.Rb[k], N[k]), std::min(rcap, right[k + 1] + p.rzcap));
-    // Every QR and SVD of the sweep keeps an r x r block in LDS: bound at kMaxSmall, and a sweep
-    // that needs more is stopped with a status when it gets there (not refused up front).
-    b = std::min<int64_t>(b, kMaxSmall);
+    // Bound at the rank capacity (kMaxSmall, the LDS-resident QR/SVD, unless raised on this thread up to
+    // kMaxRank): a sweep that needs more is stopped with a status when it gets there (not refused up
+    // front), and the caller may solve again with a larger capacity (the result does not depend on it).
+    b = std::min<int64_t>(b, rank_capacity());
     if (k + 1 == d) b = 1;
     if (rx0) b = std::max(b, rx0[k + 1]);
     if (k + 1 < d) b = std::max<int64_t>(b, 1);
@@ -853,7 +854,9 @@ class Solver {
       }
       TT_CHECK(r <= P.Rb[k + 1], ERR_UNSUPPORTED,
                "the solution needs rank " + std::to_string(r) + " at interface " + std::to_string(k + 1) +
-                   "; ranks above " + std::to_string(P.Rb[k + 1]) + " are not supported (lower rmax)");
+                   "; ranks above " + std::to_string(P.Rb[k + 1]) +
+                   " are not supported at this rank capacity (ttamen_rank_capacity raises it up to " +
+                   std::to_string(kMaxRank) + "; or lower rmax)");
       // v = vmat^T x_{k+1}  (r x n' x r3)
       const int64_t n1 = P.N[k + 1], r3 = rx_[k + 2];
       {
@@ -1183,8 +1186,8 @@ static void solve_one(int d, const int64_t* N, const int64_t* rA, const int64_t*
            ERR_INVALID, "null argument");
   Plan P = make_plan(d, N, rA, rb, x0_cores ? rx0 : nullptr, O.rmax, O.kickrank, O.kick2,
                      O.rmax_list.empty() ? nullptr : O.rmax_list.data());
-  TT_CHECK(P.maxR <= kMaxSmall, ERR_UNSUPPORTED,
-           "initial-guess rank " + std::to_string(P.maxR) + " exceeds " + std::to_string(kMaxSmall));
+  TT_CHECK(P.maxR <= kMaxRank, ERR_UNSUPPORTED,
+           "initial-guess rank " + std::to_string(P.maxR) + " exceeds " + std::to_string(kMaxRank));
   Arena ws(workspace, workspace_bytes);
   Solver s(P, O, ws, st);
   s.allocate(A_cores, b_cores, x0_cores, rx0, z0_cores);
@@ -2249,6 +2252,14 @@ int ttamen_matvec_mode(int mode) {
     return -1;
   }
   return matvec_mode(mode);
+}
+
+int ttamen_rank_capacity(int cap) {
+  if (cap > kMaxRank) {
+    set_last_error("rank capacity above " + std::to_string(kMaxRank));
+    return -1;
+  }
+  return set_rank_capacity(cap);
 }
 
 int ttamen_gemm_autotune(int mode) {

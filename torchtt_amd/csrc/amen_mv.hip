@@ -74,7 +74,8 @@ ProductPlan make_product_plan(int d, const int64_t* M, const int64_t* K, const i
   for (int k = 0; k < d; ++k) {
     int64_t b = std::min(std::min<int64_t>(p.Rb[k] * M[k], int64_t(1) << 40),
                          std::min(caps[k + 1] + p.rz, right[k + 1] + p.rz));
-    b = std::min<int64_t>(b, kMaxSmall);  // LDS-resident QR/SVD; a sweep that needs more stops with a status
+    b = std::min<int64_t>(b, rank_capacity());  // a sweep that needs more stops with a status (solve again at
+                                                //   a larger capacity: the result does not depend on it)
     if (k + 1 == d) b = 1;
     if (rx0) b = std::max(b, rx0[k + 1]);
     p.Rb[k + 1] = std::max<int64_t>(b, 1);
@@ -90,8 +91,8 @@ ProductPlan make_product_plan(int d, const int64_t* M, const int64_t* K, const i
     p.maxrA = std::max(p.maxrA, rA[k]);
     p.maxrb = std::max(p.maxrb, rb[k]);
   }
-  TT_CHECK(p.maxR <= kMaxSmall, ERR_UNSUPPORTED,
-           "initial-guess rank " + std::to_string(p.maxR) + " exceeds " + std::to_string(kMaxSmall));
+  TT_CHECK(p.maxR <= kMaxRank, ERR_UNSUPPORTED,
+           "initial-guess rank " + std::to_string(p.maxR) + " exceeds " + std::to_string(kMaxRank));
   return p;
 }
 
@@ -480,7 +481,9 @@ class ProductSweep {
       }
       TT_CHECK(r <= P.Rb[k + 1], ERR_UNSUPPORTED,
                "the product needs rank " + std::to_string(r) + " at interface " + std::to_string(k + 1) +
-                   "; ranks above " + std::to_string(P.Rb[k + 1]) + " are not supported (lower rmax)");
+                   "; ranks above " + std::to_string(P.Rb[k + 1]) +
+                   " are not supported at this rank capacity (ttamen_rank_capacity raises it up to " +
+                   std::to_string(kMaxRank) + "; or lower rmax)");
       // v = vmat^T x_{k+1}  (r x M' x r3)
       const int64_t n1 = P.M[k + 1], r3 = rx_[k + 2];
       {

@@ -517,8 +517,7 @@ int qr(const double* A, int64_t sr, int64_t sc, int64_t m, int64_t n, double* Q,
     ws.release(mk);
     return (int)n;
   }
-  TT_CHECK(n <= kMaxSmall, ERR_UNSUPPORTED,
-           "qr: more than " + std::to_string(kMaxSmall) + " columns is not supported");
+  if (n > kMaxSmall) return qr_large(A, sr, sc, m, n, Q, R, ws, st);
   size_t mk = ws.mark();
   double* G = ws.alloc<double>(n * n);
   double* Rk = ws.alloc<double>(n * n);
@@ -791,8 +790,10 @@ __device__ void KJacobiV::run(const Args& a, const Blk& bk) {
 
 static void jacobi_svd_small(const double* M, int64_t sr, int64_t sc, int k, double* s,
                              double* Ur, double* V, int64_t ldv, Arena& ws, hipStream_t st) {
-  TT_CHECK(k <= kMaxSmall, ERR_UNSUPPORTED,
-           "svd: rank above " + std::to_string(kMaxSmall) + " is not supported");
+  if (k > kMaxSmall) {
+    svd_large(M, sr, sc, k, s, Ur, V, ldv, ws, st);
+    return;
+  }
   const int K = (k % 2) ? k + 1 : k;
   size_t mk = ws.mark();
   int* rank_of = ws.alloc<int>(k + 1);

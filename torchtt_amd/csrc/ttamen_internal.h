@@ -224,6 +224,19 @@ void read_to_host(void* host, const void* dev, size_t bytes, hipStream_t st);
 
 // ---------------------------------------------------------------- dense linear algebra
 constexpr int kMaxSmall = 140;   // largest square block kept in LDS (FP64)
+constexpr int kMaxRank = 1024;   // largest rank of the device-memory QR / SVD (linalg_large.hip)
+
+// The rank every interface of a solve is planned at when rmax allows more (workspace sizing only:
+// results do not depend on it). kMaxSmall unless raised on the calling thread, at most kMaxRank;
+// QR and SVD above kMaxSmall columns run in device memory (qr_large, svd_large).
+int rank_capacity();
+int set_rank_capacity(int cap);  // returns the previous value; cap <= 0 only queries
+// Householder QR (m >= n) and one-sided Jacobi SVD (k x k) in device memory, for n, k > kMaxSmall;
+// same outputs as qr() and the SVD of its square factor.
+int qr_large(const double* A, int64_t sr, int64_t sc, int64_t m, int64_t n, double* Q, double* R, Arena& ws,
+             hipStream_t st);
+void svd_large(const double* M, int64_t sr, int64_t sc, int k, double* s, double* Ur, double* V, int64_t ldv,
+               Arena& ws, hipStream_t st);
 
 // Thin QR of a tall m x n (m >= n) matrix given by strides, via shifted
 // Cholesky-QR with three passes and triangular solves; Q is m x n row-major

@@ -7,6 +7,7 @@ cpp/amen_solve.h:122-549). There is no CPU fallback: without the library or a RO
 the call raises.
 """
 import ctypes
+import re
 import sys
 
 import numpy as np
@@ -129,42 +130,87 @@ def amen_solve(A, b, nswp=22, x0=None, eps=1e-10, rmax=32768, max_full=500, kick
             xc, rx0 = None, None
         Na, rAa, rba = _lib.i64_array(N), _lib.i64_array(rA), _lib.i64_array(rb)
         rx0a = _lib.i64_array(rx0) if rx0 is not None else None
-        ws_bytes = lib.ttamen_amen_solve_workspace_size(d, Na, rAa, rba, rx0a, int(rmax), int(kickrank),
-                                                        int(kick2), int(local_iterations), int(max_full),
-                                                        _PRECISIONS[precision])
-        if ws_bytes == 0:
-            _lib.check("ttamen_amen_solve_workspace_size", 1)
-        if prec:
-            extra = lib.ttamen_amen_solve_prec_workspace_size(d, Na, rAa, rba, rx0a, int(rmax), int(kickrank),
-                                                              int(kick2), prec)
-            if extra == 0:
-                _lib.check("ttamen_amen_solve_prec_workspace_size", 1)
-            ws_bytes += extra
-        out_elems = lib.ttamen_amen_solve_output_elems(d, Na, rx0a, int(rmax), int(kickrank), int(kick2))
-        ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dev)
-        xout = torch.empty(max(out_elems, 1), dtype=torch.float64, device=dev)
-        rx_out = (ctypes.c_int64 * (d + 1))()
-        off_out = (ctypes.c_int64 * d)()
-        stats = _lib.TTAmenStats()
-        code = lib.ttamen_amen_solve(
-            d, Na, rAa, rba, _lib.ptr_array(Ac), _lib.ptr_array(bc),
-            _lib.ptr_array(xc) if xc is not None else None, rx0a,
-            _lib.ptr_array(z_cores) if z_cores is not None else None,
-            int(seed) if seed is not None else 0, int(nswp), float(eps), int(rmax), int(max_full),
-            int(kickrank), int(kick2), int(local_iterations), int(resets), int(bool(verbose)), prec,
-            _PRECISIONS[precision], ctypes.c_void_p(ws.data_ptr()), ws_bytes,
-            ctypes.c_void_p(xout.data_ptr()), out_elems, rx_out, off_out, ctypes.byref(stats),
-            ctypes.c_void_p(_lib.stream_handle(dev)))
-        _lib.check("ttamen_amen_solve", code)
-        cores = []
-        for k in range(d):
-            n = rx_out[k] * N[k] * rx_out[k + 1]
-            c = xout[off_out[k]:off_out[k] + n].reshape(rx_out[k], N[k], rx_out[k + 1]).clone()
-            cores.append(c.to(device=out_device, dtype=out_dtype))
-        last_stats = stats.as_dict()
+        cap = None
+        while True:
+            prev_cap = lib.ttamen_rank_capacity(cap) if cap else None
+            try:
+                x_cores, ws_bytes = _amen_solve_native(
+                    lib, d, N, Na, rAa, rba, Ac, bc, xc, rx0a, z_cores, seed, nswp, eps, rmax, max_full,
+                    kickrank, kick2, local_iterations, resets, verbose, prec, precision, dev, out_device,
+                    out_dtype)
+                break
+            except _lib.TTAmenError as e:
+                cap = _grown_capacity(e, cap or lib.ttamen_rank_capacity(0))
+                if cap is None:
+                    raise
+            finally:
+                if prev_cap is not None:
+                    lib.ttamen_rank_capacity(prev_cap)
         last_stats["workspace_bytes"] = int(ws_bytes)
-        del ws, xout
-    return TT(cores)
+        last_stats["rank_capacity"] = int(cap) if cap else int(lib.ttamen_rank_capacity(0))
+    return TT(x_cores)
+
+
+_RANK_CAPACITY_MAX = 1024
+_CAPACITY_MSG = re.compile(r"needs rank (\d+) at interface \d+; ranks above (\d+) are not supported at this "
+                           r"rank capacity")
+
+
+def _grown_capacity(err, current):
+    """The rank capacity to solve again with after a sweep outgrew `current` (status 4 at an interface
+    planned at the capacity), or None when more capacity would not help."""
+    if err.code != 4:
+        return None
+    m = _CAPACITY_MSG.search(str(err))
+    if m is None:
+        return None
+    need, held = int(m.group(1)), int(m.group(2))
+    if held < current:  # that interface was bounded by rmax or by the mode sizes, not the capacity
+        return None
+    nxt = min(_RANK_CAPACITY_MAX, max(2 * current, need + 16))
+    return nxt if nxt > current else None
+
+
+def _amen_solve_native(lib, d, N, Na, rAa, rba, Ac, bc, xc, rx0a, z_cores, seed, nswp, eps, rmax, max_full,
+                       kickrank, kick2, local_iterations, resets, verbose, prec, precision, dev, out_device,
+                       out_dtype):
+    """One call of ttamen_amen_solve at the calling thread's rank capacity; the solution's cores."""
+    global last_stats
+    ws_bytes = lib.ttamen_amen_solve_workspace_size(d, Na, rAa, rba, rx0a, int(rmax), int(kickrank),
+                                                    int(kick2), int(local_iterations), int(max_full),
+                                                    _PRECISIONS[precision])
+    if ws_bytes == 0:
+        _lib.check("ttamen_amen_solve_workspace_size", 1)
+    if prec:
+        extra = lib.ttamen_amen_solve_prec_workspace_size(d, Na, rAa, rba, rx0a, int(rmax), int(kickrank),
+                                                          int(kick2), prec)
+        if extra == 0:
+            _lib.check("ttamen_amen_solve_prec_workspace_size", 1)
+        ws_bytes += extra
+    out_elems = lib.ttamen_amen_solve_output_elems(d, Na, rx0a, int(rmax), int(kickrank), int(kick2))
+    ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dev)
+    xout = torch.empty(max(out_elems, 1), dtype=torch.float64, device=dev)
+    rx_out = (ctypes.c_int64 * (d + 1))()
+    off_out = (ctypes.c_int64 * d)()
+    stats = _lib.TTAmenStats()
+    code = lib.ttamen_amen_solve(
+        d, Na, rAa, rba, _lib.ptr_array(Ac), _lib.ptr_array(bc),
+        _lib.ptr_array(xc) if xc is not None else None, rx0a,
+        _lib.ptr_array(z_cores) if z_cores is not None else None,
+        int(seed) if seed is not None else 0, int(nswp), float(eps), int(rmax), int(max_full),
+        int(kickrank), int(kick2), int(local_iterations), int(resets), int(bool(verbose)), prec,
+        _PRECISIONS[precision], ctypes.c_void_p(ws.data_ptr()), ws_bytes,
+        ctypes.c_void_p(xout.data_ptr()), out_elems, rx_out, off_out, ctypes.byref(stats),
+        ctypes.c_void_p(_lib.stream_handle(dev)))
+    _lib.check("ttamen_amen_solve", code)
+    cores = []
+    for k in range(d):
+        n = rx_out[k] * N[k] * rx_out[k + 1]
+        c = xout[off_out[k]:off_out[k] + n].reshape(rx_out[k], N[k], rx_out[k + 1]).clone()
+        cores.append(c.to(device=out_device, dtype=out_dtype))
+    last_stats = stats.as_dict()
+    del ws, xout
+    return cores, ws_bytes
 
 
 def amen_divide(a, b, nswp=22, x0=None, eps=1e-10, rmax=100, max_full=500, kickrank=4, kick2=0,
