@@ -211,17 +211,23 @@ def system_cost(sy):
 
 
 def ortho_roofline(lib, _lib, precision):
-    """HBM roofline of the GMRES orthogonalisation (CGS dots + update + close per Arnoldi step), from
-    the library's HIP events and algorithmic bytes ((2k + 7) N w per step k), read after a
-    profiled single-system solve."""
+    """HBM roofline of the GMRES orthogonalisation per Arnoldi step, from the library's HIP events and
+    algorithmic bytes, read after a profiled single-system solve: FP32 basis, CGS dots + update + close
+    ((2k + 7) N w bytes at step k); FP64 basis, update + close ((k + 5) N w: the first pass's dots run inside
+    the local operator's stages-2/3 kernel wherever that kernel's grid allows, and are not timed here)."""
     o = (ctypes.c_double * 3)()
     _lib.check("profile_read_ortho", lib.ttamen_profile_read_ortho(o))
     steps, ms, by = list(o)
     if steps <= 0 or ms <= 0:
         return None
     ach = by / (ms / 1e3) / 1e9
-    return {"kernel": "GMRES orthogonalisation: CGS dots, update (+ second-pass dots) and close launches of an "
-                      "Arnoldi step (%s basis)" % ("FP64" if precision == "fp64" else "FP32"),
+    if precision == "fp64":
+        kern = ("GMRES orthogonalisation: update (+ second-pass dots) and close launches of an Arnoldi step, FP64 "
+                "basis (the first pass's dots fused into the local operator's kernel)")
+    else:
+        kern = ("GMRES orthogonalisation: CGS dots, update (+ second-pass dots) and close launches of an Arnoldi "
+                "step (FP32 basis)")
+    return {"kernel": kern,
             "bound": "hbm", "achieved": ach, "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": ach / PEAK_HBM_GBS,
             "steps": int(steps), "avg_us": ms * 1e3 / steps, "bytes_per_step": by / steps}
 

@@ -141,13 +141,20 @@ static bool fused_matvec(const T* PhiL, const T* Apk, const T* PhiR, int l1, int
 // shape is outside the kernel (l2 above 16 kTailMaxLB).
 template <typename T>
 static bool mv_tail(const T* t1, const T* Apm, const T* PhiR, T* y, int64_t nvl, int K, int r2, int nm, int S,
-                    int l2, Arena& ws, hipStream_t st, const int* skip) {
+                    int l2, Arena& ws, hipStream_t st, const int* skip, MvDots* dots) {
   const int LB = (l2 + 15) / 16;
   // a measuring arena returns false, so that the caller also sizes the GEMM form (either may run)
   if (LB > kTailMaxLB || nvl > INT32_MAX || (nm + 15) / 16 > 65535 || ws.measuring()) return false;
   typename KMvTail<T>::Args a;
   a.t1 = t1; a.Apm = Apm; a.PhiR = PhiR; a.y = y; a.skip = skip;
   a.K = K; a.r2 = r2; a.nm = nm; a.S = S; a.l2 = l2; a.LB = LB;
+  a.Qd = nullptr; a.P = nullptr; a.N = 0; a.kc = 0;
+  const int64_t nblocks = nvl * ((nm + 15) / 16);
+  if (dots && dots->kc > 0 && dots->kc + 1 <= kGmresMaxCols && nblocks <= kGmresMaxBlocks &&
+      dots->N == nvl * nm * l2) {
+    a.Qd = (const T*)dots->Q; a.P = dots->P; a.N = dots->N; a.kc = dots->kc;
+    dots->nb = (int)nblocks;
+  }
   const int vw = (int)(16 / sizeof(T));
   a.apv = (K % vw == 0 && reinterpret_cast<uintptr_t>(Apm) % 16 == 0) ? 1 : 0;
   launch<KMvTail<T>>(dim3((unsigned)nvl, (nm + 15) / 16), dim3(KMvTail<T>::kThreads), 0, st, a);
@@ -159,7 +166,7 @@ template <typename T>
 static void matvec_impl(const T* PhiL, const T* Apm, const T* PhiR, int l1, int s, int r1, int n,
                         int nm, int S, int l2, int r2, const T* x, T* y, int nvec, Arena& ws,
                         hipStream_t st, const int* skip,
-                        void (*gemm)(const GemmDesc&, Arena&, hipStream_t), bool tail) {
+                        void (*gemm)(const GemmDesc&, Arena&, hipStream_t), bool tail, MvDots* dots) {
   // n: mode of x (columns of the operator core), nm: mode of y (rows; nm == n when square)
   const int64_t nx = (int64_t)r1 * n * r2;
   const int64_t t1v = (int64_t)l1 * s * n * r2;  // per vector
@@ -177,7 +184,8 @@ static void matvec_impl(const T* PhiL, const T* Apm, const T* PhiR, int l1, int 
     gemm(g, ws, st);
   }
   if (tail && g_matvec_mode.load() == 0 &&
-      mv_tail<T>(t1, Apm, PhiR, y, (int64_t)nvec * l1, s * n, r2, nm, S, l2, ws, st, skip)) {
+      mv_tail<T>(t1, Apm, PhiR, y, (int64_t)nvec * l1, s * n, r2, nm, S, l2, ws, st, skip,
+                 nvec == 1 ? dots : nullptr)) {
     ws.release(mk);
     return;
   }
@@ -203,22 +211,24 @@ static void matvec_impl(const T* PhiL, const T* Apm, const T* PhiR, int l1, int 
 }
 
 void local_matvec(const LocalOp& op, const double* x, double* y, int nvec, Arena& ws,
-                  hipStream_t st, const int* skip) {
+                  hipStream_t st, const int* skip, MvDots* dots) {
+  if (dots) dots->nb = 0;
   const bool square = op.n_out == 0 || op.n_out == op.n;
   if (square && fused_matvec<double>(op.PhiL, op.Apk, op.PhiR, op.l1, op.s, op.r1, op.n, op.S, op.l2, op.r2,
                                      x, y, nvec, op.cnt, op.cnt_cap, ws, st, skip))
     return;
   matvec_impl<double>(op.PhiL, op.Apm, op.PhiR, op.l1, op.s, op.r1, op.n, op.n_out ? op.n_out : op.n,
-                      op.S, op.l2, op.r2, x, y, nvec, ws, st, skip, gemm_f64, true);
+                      op.S, op.l2, op.r2, x, y, nvec, ws, st, skip, gemm_f64, true, dots);
 }
 
 void local_matvec_f32(const LocalOpF32& op, const float* x, float* y, int nvec, Arena& ws,
-                      hipStream_t st, const int* skip) {
+                      hipStream_t st, const int* skip, MvDots* dots) {
+  if (dots) dots->nb = 0;
   if (!op.bf16 && fused_matvec<float>(op.PhiL, op.Apk, op.PhiR, op.l1, op.s, op.r1, op.n, op.S, op.l2, op.r2,
                                       x, y, nvec, op.cnt, op.cnt_cap, ws, st, skip))
     return;
   matvec_impl<float>(op.PhiL, op.Apm, op.PhiR, op.l1, op.s, op.r1, op.n, op.n, op.S, op.l2, op.r2,
-                     x, y, nvec, ws, st, skip, op.bf16 ? gemm_bf16 : gemm_f32, !op.bf16);
+                     x, y, nvec, ws, st, skip, op.bf16 ? gemm_bf16 : gemm_f32, !op.bf16, dots);
 }
 
 size_t local_matvec_ws(const LocalOp& op, int nvec) {

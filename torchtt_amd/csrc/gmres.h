@@ -32,9 +32,14 @@ struct GmresDev {
 template <typename T>
 struct GmresOp {
   std::function<void(const T*, T*, const int*)> apply;          // y = Op x (skip flag)
+  // Optional: y = Op x with the first orthogonalisation pass fused in (MvDots: the shares of Q_i . y,
+  // i < kc, and y . y in P); returns the number of shares per row, 0 when it did not fuse them (y is
+  // computed either way).
+  std::function<int(const T*, T*, const int*, const T*, int, double*)> apply_dots;
   // Optional: Arnoldi-step application, tagged (cycle, step) so a caller can attribute
   // timing to the steps that actually ran (later steps of a converged cycle are skipped).
-  std::function<void(const T*, T*, const int*, int, int)> apply_step;
+  // Receives the basis (Q, kc rows) and P1 as apply_dots does and returns the same count.
+  std::function<int(const T*, T*, const int*, int, int, const T*, int, double*)> apply_step;
   std::function<void(double, const T*, double, T*, int64_t)> axpby;  // y = a x + b y
   std::function<void(const T*, int64_t, double*)> norm;         // *slot = ||x||
   // Optional: work enqueued after each cycle's solution update and before the cycle's state is read

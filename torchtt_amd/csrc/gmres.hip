@@ -109,7 +109,7 @@ GmresResult gmres_run(GmresDev g, int64_t N, T* x, const T* rhs, const double* n
     std::lock_guard<std::mutex> lk(g_oprof.mu);
     prof = g_oprof.on && !cohort;
   }
-  struct Tag { hipEvent_t e0, e1; int k; };
+  struct Tag { hipEvent_t e0, e1; int k; bool fused; };
   std::vector<Tag> tags;
   for (int cyc = 0; cyc < resets; ++cyc) {
     // r = rhs - Op(x); first cycle starts from x = 0 so r = rhs.
@@ -129,23 +129,25 @@ GmresResult gmres_run(GmresDev g, int64_t N, T* x, const T* rhs, const double* n
     for (k = 0; k < g.m && !stop; ++k) {
       T* qk = Q + (int64_t)k * N;
       T* qn = Q + (int64_t)(k + 1) * N;
-      if (op.apply_step) op.apply_step(qk, w, done, cyc, k);
+      int nb1 = 0;  // > 0: the operator wrote the first pass's dot shares itself
+      if (op.apply_step) nb1 = op.apply_step(qk, w, done, cyc, k, Q, k + 1, P1);
+      else if (op.apply_dots) nb1 = op.apply_dots(qk, w, done, Q, k + 1, P1);
       else op.apply(qk, w, done);
       double* Hcol = g.H + (int64_t)k * (g.m + 1);
       const double* Pw = P1 + (size_t)(k + 1) * kGmresMaxBlocks;  // row kc of P1: w.w
       // pass 1: project out the basis
-      Tag tag{nullptr, nullptr, k};
+      Tag tag{nullptr, nullptr, k, nb1 > 0};
       if (prof) {
         std::lock_guard<std::mutex> lk(g_oprof.mu);
         tag.e0 = g_oprof.get();
         tag.e1 = g_oprof.get();
         TT_HIP(hipEventRecord(tag.e0, st));
       }
-      launch<KCgsDots<T>>(dim3(nb), dim3(kGB), 0, st, {Q, N, k + 1, w, P1, done});
-      launch<KCgsUpdate<T>>(dim3(nb), dim3(kGB), 0, st, {Q, N, k + 1, w, P1, Hcol, Pn1, P2, done});
+      if (nb1 == 0) launch<KCgsDots<T>>(dim3(nb), dim3(kGB), 0, st, {Q, N, k + 1, w, P1, done});
+      launch<KCgsUpdate<T>>(dim3(nb), dim3(kGB), 0, st, {Q, N, k + 1, w, P1, Hcol, Pn1, P2, done, nb1});
       launch<KGmresClose<T>>(dim3(nb), dim3(kGB), 0, st,
                              {Q, N, k + 1, w, qn, Pw, Pn1, P2, Hcol, g, k, done, g.istate + GM_REORTHS,
-                              Pn1 + kGmresMaxBlocks});
+                              Pn1 + kGmresMaxBlocks, nb1});
       TT_LAUNCH_CHECK();
       if (prof) {
         TT_HIP(hipEventRecord(tag.e1, st));
@@ -182,7 +184,8 @@ GmresResult gmres_run(GmresDev g, int64_t N, T* x, const T* rhs, const double* n
           TT_HIP(hipEventElapsedTime(&ms, t.e0, t.e1));
           g_oprof.ms += ms;
           g_oprof.steps += 1;
-          g_oprof.bytes += (2.0 * t.k + 7.0) * (double)N * sizeof(T);
+          // the dots (k + 2 vectors) run inside the operator's kernel when fused
+          g_oprof.bytes += (t.fused ? (double)t.k + 5.0 : 2.0 * t.k + 7.0) * (double)N * sizeof(T);
         }
         g_oprof.pool.push_back(t.e0);
         g_oprof.pool.push_back(t.e1);

@@ -52,12 +52,13 @@ __device__ __forceinline__ void gm_reduce_rows(const double* P, int rows, int nb
   __syncthreads();
 }
 
-// out[0] = sum of the partials of row Pa, out[1] = of row Pb (each as gm_reduce_rows does it),
-// in one round: wave 0 reduces Pa, wave 1 reduces Pb.
-__device__ __forceinline__ void gm_reduce_pair(const double* Pa, const double* Pb, int nb, double* out) {
+// out[0] = sum of the nba partials of row Pa, out[1] = of the nbb partials of row Pb (each as
+// gm_reduce_rows does it), in one round: wave 0 reduces Pa, wave 1 reduces Pb.
+__device__ __forceinline__ void gm_reduce_pair(const double* Pa, int nba, const double* Pb, int nbb, double* out) {
   const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
   if (wv < 2) {
     const double* row = wv == 0 ? Pa : Pb;
+    const int nb = wv == 0 ? nba : nbb;
     double p[4];
 #pragma unroll
     for (int u = 0; u < 4; ++u) p[u] = (lane + 64 * u < nb) ? row[lane + 64 * u] : 0.0;
@@ -137,7 +138,8 @@ __device__ __forceinline__ double gm_chunk_update(const T* Q, int64_t N, int kc,
 // Arnoldi step k, orthogonalisation in three kernels (classical Gram-Schmidt, selective second
 // pass with the DGKS criterion 1/sqrt(2); Daniel, Gragg, Kaufman, Stewart, Math. Comp. 30
 // (1976)), all deterministic (fixed-order partials reduced by every consumer block):
-//   KCgsDots     P1 = [Q^T w ; w.w] partials
+//   KCgsDots     P1 = [Q^T w ; w.w] partials (or the local operator's fused kernel writes them: nb1 partials
+//                per row, one per workgroup of that kernel, instead of one per block of these kernels)
 //   KCgsUpdate   h = sum P1, w' = w - Q h, Pn1 = ||w'||^2 partials and P2 = Q^T w' partials
 //                (the second pass's dots, from the chunk the block just updated)
 //   KGmresClose  if ||w'|| < ||w|| / sqrt(2): h2 = sum P2, w'' = w' - Q h2, H(:,k) += h2 and
@@ -164,7 +166,7 @@ template <typename T>
 struct KCgsUpdate {
   struct Args {
     const T* Q; int64_t N; int kc; T* w; const double* P1; double* Hcol; double* Pn1; double* P2;
-    const int* skip;
+    const int* skip; int nb1;  // partials per row of P1 (0: one per block of this launch)
   };
   static constexpr int kThreads = kGB;
   static constexpr int kMega = std::is_same<T, double>::value ? 6 : 7;
@@ -173,7 +175,7 @@ struct KCgsUpdate {
     __shared__ double hs[kGmresMaxCols + 1];
     __shared__ double sh[kGB / 64];
     const int kc = a.kc;
-    gm_reduce_rows(a.P1, kc, bk.gx, hs);
+    gm_reduce_rows(a.P1, kc, a.nb1 > 0 ? a.nb1 : (int)bk.gx, hs);
     if (bk.x == 0)
       for (int i = threadIdx.x; i < kc; i += kGB) a.Hcol[i] = hs[i];
     const int64_t chunk = (a.N + bk.gx - 1) / bk.gx;
@@ -197,6 +199,7 @@ struct KGmresClose {
     const T* Q; int64_t N; int kc; T* w; T* qn; const double* Pw; const double* Pn1;
     const double* P2; double* Hcol; GmresDev g; int k; const int* done_in; int* reorth_count;
     double* Pex;  // per-block partials of ||w''||^2 (exact-norm path)
+    int nb1;      // partials in Pw (0: one per block of this launch)
   };
   static constexpr int kThreads = kGB;
   static constexpr int kMega = std::is_same<T, double>::value ? 8 : 9;
@@ -207,7 +210,7 @@ struct KGmresClose {
     const int kc = a.kc;
     // Whether the second (re-orthogonalisation) pass runs: the first pass cancelled more than
     // 1 - 1/sqrt(2) of ||w|| (Daniel, Gragg, Kaufman and Stewart, Math. Comp. 30 (1976)).
-    gm_reduce_pair(a.Pw, a.Pn1, bk.gx, nn);  // ||w||^2, ||w'||^2
+    gm_reduce_pair(a.Pw, a.nb1 > 0 ? a.nb1 : (int)bk.gx, a.Pn1, (int)bk.gx, nn);  // ||w||^2, ||w'||^2
     const bool reorth = !(sqrt(nn[1]) > 0.7071067811865476 * sqrt(nn[0]));
     double hn2 = nn[1];
     const int64_t chunk = (a.N + bk.gx - 1) / bk.gx;

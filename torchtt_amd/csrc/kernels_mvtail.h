@@ -24,6 +24,7 @@
 #pragma once
 #include "ttamen_internal.h"
 #include "kernels_gemm.h"
+#include "gmres.h"
 
 namespace ttamen {
 
@@ -70,10 +71,49 @@ struct KMvTail {
     int K, r2, nm, S, l2;
     int LB;   // ceil(l2 / 16) <= kTailMaxLB
     int apv;  // operator rows load as vectors (K a multiple of the vector width, aligned base)
+    // Optional (kc > 0; one vector, grid x grid.y <= kGmresMaxBlocks workgroups): the first pass of the
+    // GMRES orthogonalisation on the output, P[i * kGmresMaxBlocks + blk] = this workgroup's share of Q_i . y for the
+    // basis rows i < kc and of y . y in row kc (blk = bk.x * grid.y + bk.y; Q_i = Qd + i * N).
+    const T* Qd;
+    double* P;
+    int64_t N;
+    int kc;
   };
   static constexpr int kThreads = 256;
   static __device__ __forceinline__ void run(const Args& a, const Blk& bk);
 };
+
+// P[i * kGmresMaxBlocks + blk] = sum_{e < E} Q_i[off + e] y[e] for i < kc, and y . y in row kc, with y (E values) in
+// LDS. Wave w takes the rows w, w + 4, ..., four at a time (their loads in flight together); lane l adds
+// the elements e = l, l + 64, ... of a row in order, then the wave's xor tree gives the row's share: no
+// barrier, a fixed order. Accumulated in FP64, as the separate dot kernel does.
+template <typename T>
+__device__ __forceinline__ void mvtail_dots(const typename KMvTail<T>::Args& a, const T* y, int E, int64_t off,
+                                            int blk) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int nr = a.kc + 1;
+  for (int i0 = w; i0 < nr; i0 += 16) {  // rows i0, i0 + 4, i0 + 8, i0 + 12: w, w + 4, ... over the rounds
+    const T* src[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int i = i0 + 4 * q < nr ? i0 + 4 * q : nr - 1;  // w's rows: i0, i0 + 4, i0 + 8, i0 + 12
+      src[q] = (i < a.kc) ? a.Qd + (int64_t)i * a.N + off : nullptr;
+    }
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int e = lane; e < E; e += 64) {
+      const double ye = (double)y[e];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) acc[q] += (src[q] ? (double)src[q][e] : ye) * ye;
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      double s = acc[q];
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+      if (lane == 0 && i0 + 4 * q < nr) a.P[(int64_t)(i0 + 4 * q) * kGmresMaxBlocks + blk] = s;
+    }
+  }
+}
 
 template <typename T, int LB, int SM, int DEPTH, bool APV>
 __device__ __forceinline__ void mvtail_body(const typename KMvTail<T>::Args& a, const Blk& bk) {
@@ -200,6 +240,9 @@ __device__ __forceinline__ void mvtail_body(const typename KMvTail<T>::Args& a, 
       for (int v = 0; v < 4; ++v) slot(0, lb, v) = yacc[lb][v];
   }
   __syncthreads();
+  // With the fused dots, the output block also goes to the second slot's LDS (free by now), row-major
+  // as in y, for all four waves to read.
+  T* ych = red + LB * 256;
   if (wave == 0 && m0 + r < a.nm) {
     T* y = a.y + (vl * a.nm + m0 + r) * a.l2;
 #pragma unroll
@@ -207,8 +250,17 @@ __device__ __forceinline__ void mvtail_body(const typename KMvTail<T>::Args& a, 
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
         const int L = lb * 16 + acc_row<T>(g, v);
-        if (L < a.l2) y[L] = yacc[lb][v] + slot(0, lb, v);
+        if (L < a.l2) {
+          const T val = yacc[lb][v] + slot(0, lb, v);
+          y[L] = val;
+          if (a.kc > 0) ych[r * a.l2 + L] = val;
+        }
       }
+  }
+  if (a.kc > 0) {
+    __syncthreads();
+    const int rows = min(16, a.nm - m0);
+    mvtail_dots<T>(a, ych, rows * a.l2, (vl * a.nm + m0) * a.l2, (int)(bk.x * bk.gy + bk.y));
   }
 }
 

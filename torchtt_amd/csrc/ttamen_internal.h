@@ -277,8 +277,20 @@ struct LocalOp {
   unsigned* cnt = nullptr;
   int cnt_cap = 0;
 };
+// Optional first pass of the GMRES orthogonalisation fused into the local operator (one vector, the
+// default form, FP64/FP32): each workgroup of the stages-2/3 kernel writes its share of Q_i . y (i < kc,
+// Q_i = Q + i N in the operator's precision) and of y . y (row kc) to P[i * kGmresMaxBlocks + blk]. On
+// return nb is the number of shares per row, or 0 when the form cannot fuse them (the caller then
+// computes the dots itself).
+struct MvDots {
+  const void* Q = nullptr;
+  int64_t N = 0;
+  int kc = 0;
+  double* P = nullptr;
+  int nb = 0;
+};
 void local_matvec(const LocalOp& op, const double* x, double* y, int nvec, Arena& ws,
-                  hipStream_t st, const int* skip = nullptr);
+                  hipStream_t st, const int* skip = nullptr, MvDots* dots = nullptr);
 size_t local_matvec_ws(const LocalOp& op, int nvec);
 
 struct LocalOpF32 {
@@ -301,7 +313,7 @@ int matvec_mode(int mode);
 // Workspace of the fused local operator's partial sums and counters at these sizes (0 if none).
 size_t local_matvec_fused_bytes(int l1, int s, int r1, int n, int S, int l2, int r2, int nvec, size_t elem);
 void local_matvec_f32(const LocalOpF32& op, const float* x, float* y, int nvec, Arena& ws,
-                      hipStream_t st, const int* skip = nullptr);
+                      hipStream_t st, const int* skip = nullptr, MvDots* dots = nullptr);
 
 // Interface updates (FP64).
 // fwd: Pn[L,S,R] = sum Phi[l,s,r] xL[l,M,L] A[s,M,N,S] xR[r,N,R]
