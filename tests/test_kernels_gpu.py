@@ -166,6 +166,59 @@ def test_local_matvec_f32(shape, matvec_mode):
         assert _rel(y[v].cpu().double().numpy(), want) < 2e-5, v
 
 
+def _block_sparse_core(s, n, S, seed):
+    """An operator core with the block structure of a TT Laplacian-plus-potential: identities, a
+    tridiagonal stencil and a diagonal on some (s, S) blocks, zero elsewhere; with S = 5 the last S
+    value has no nonzero at all, and with s = 4 one entry sits alone in the last row and column."""
+    g = np.random.default_rng(seed)
+    a = np.zeros((s, n, n, S))
+    eye = np.eye(n)
+    tri = np.diag(g.standard_normal(n)) + np.diag(g.standard_normal(n - 1), 1) + np.diag(g.standard_normal(n - 1), -1)
+    a[0, :, :, 0] = eye
+    if s > 1:
+        a[1, :, :, 0] = tri
+        a[1, :, :, 1 % S] += eye
+    if s > 3 and S > 3:
+        a[2, :, :, 2] = eye
+        a[3, :, :, 2] = np.diag(g.standard_normal(n))
+        a[3, n - 1, n - 1, 3] = 2.5
+    return a
+
+
+SPARSE_SHAPES = [(13, 4, 11, 33, 5, 9, 23, 1), (30, 4, 30, 64, 4, 30, 29, 2), (6, 2, 7, 17, 2, 5, 4, 1),
+                 # K = s n above 1024: no tile masks, every step
+                 (5, 2, 6, 600, 2, 4, 7, 1)]
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+@pytest.mark.parametrize("shape", SPARSE_SHAPES)
+def test_local_matvec_block_sparse(shape, prec, matvec_mode):
+    """Block-sparse operator cores (the stages-2/3 kernel runs only the K steps with a nonzero operator
+    tile): against the oracle's dense contraction, and an all-zero core gives exactly zero."""
+    L, lib = _lib()
+    l1, s, r1, n, S, l2, r2, nv = shape
+    _say("local_matvec block-sparse", prec, shape, "mode", matvec_mode)
+    dt = torch.float64 if prec == "f64" else torch.float32
+    pl, _, pr, x = _rand_local(shape, dt, seed=5)
+    for zero in (False, True):
+        a = torch.zeros(s, n, n, S, dtype=dt) if zero else torch.tensor(_block_sparse_core(s, n, S, 6), dtype=dt)
+        y = torch.full((nv, l1, n, l2), float("nan"), dtype=dt, device=DEV)
+        ws = _ws(lib.ttamen_local_matvec_workspace_size(l1, s, r1, n, S, l2, r2, nv) * 2)
+        t = [v.to(DEV) for v in (pl, a, pr, x)]
+        fn = lib.ttamen_local_matvec if prec == "f64" else lib.ttamen_local_matvec_f32
+        code = fn(_p(t[0]), _p(t[1]), _p(t[2]), _p(t[3]), _p(y), l1, s, r1, n, S, l2, r2, nv,
+                  _p(ws), ws.numel(), _stream())
+        L.check("local_matvec", code)
+        torch.cuda.synchronize()
+        if zero:
+            assert bool((y == 0).all())
+            continue
+        for v in range(nv):
+            want = O.local_product(pr.double().numpy(), pl.double().numpy(), a.double().numpy(),
+                                   x[v].double().numpy())
+            assert _rel(y[v].cpu().double().numpy(), want) < (1e-13 if prec == "f64" else 2e-5), v
+
+
 def _bf16(a):
     """Round float32 values to BF16 (round to nearest even), returned as float32."""
     u = np.ascontiguousarray(a, dtype=np.float32).view(np.uint32).astype(np.uint64)

@@ -298,6 +298,12 @@ class Solver {
     Apk_.resize(d);
     for (int k = 0; k < d; ++k)
       Apk_[k] = ws_.alloc<double>(frag_a_elems((int)(P.N[k] * P.rA[k + 1]), (int)(P.rA[k] * P.N[k])));
+    // zero-tile masks of the permuted cores, shared by every precision (a value that is zero in FP64 is
+    // zero in FP32 and BF16)
+    tmask_.assign(d, nullptr);
+    for (int k = 0; k < d; ++k)
+      if (P.rA[k] * P.N[k] <= kTileMaskMaxK)
+        tmask_[k] = ws_.alloc<uint64_t>(tile_mask_words(P.N[k], P.rA[k + 1]));
     if (O.precision == 3) {  // FP32 copies of the operator cores for the FP32 interface updates
       A32_.resize(d);
       for (int k = 0; k < d; ++k) A32_[k] = ws_.alloc<float>(P.rA[k] * P.N[k] * P.N[k] * P.rA[k + 1]);
@@ -342,6 +348,8 @@ class Solver {
     for (int k = 0; k < d; ++k) {
       permute_core_A(A_[k], Apm_[k], (int)P.rA[k], (int)P.N[k], (int)P.rA[k + 1], st_);
       pack_frag_a(Apm_[k], Apk_[k], (int)(P.N[k] * P.rA[k + 1]), (int)(P.rA[k] * P.N[k]), st_);
+      if (tmask_[k])
+        op_tile_masks(Apm_[k], (int)P.N[k], (int)P.rA[k + 1], (int)(P.rA[k] * P.N[k]), tmask_[k], st_);
     }
     if (O.precision >= 1)
       for (int k = 0; k < d; ++k) {
@@ -447,6 +455,7 @@ class Solver {
   std::vector<double*> Apm_, x_, z_, Phi_, Phib_, Phiz_, Phizb_;
   std::vector<float*> Apm32_, Apk32_, A32_;
   std::vector<double*> Apk_;
+  std::vector<uint64_t*> tmask_;
   static constexpr int kMvCounters = 1024;  // zeroed counters of the fused local operator
   unsigned* mv_cnt_ = nullptr;
   double* spare_ = nullptr;
@@ -564,6 +573,7 @@ class Solver {
         const int64_t r1 = rx_[k], r2 = rx_[k + 1];
         LocalOp op;
         op.PhiL = Phiz_[k]; op.Apm = Apm_[k]; op.Apk = Apk_[k]; op.PhiR = Phiz_[k + 1];
+        op.tmask = tmask_[k];
         op.l1 = (int)rzphi_[k]; op.s = (int)s; op.r1 = (int)r1; op.n = (int)n; op.S = (int)S;
         op.l2 = (int)rzphi_[k + 1]; op.r2 = (int)r2;
         const int64_t rows = rzphi_[k], cols = n * rzphi_[k + 1];
@@ -676,7 +686,7 @@ class Solver {
                  (int)r2, sc(NRMSC()), ws_, st_);
     norm2(rhs, Nl, sc(TMP(T_NRHS)), rs_, st_);
     LocalOp op;
-    op.PhiL = Phi_[k]; op.Apm = Apm_[k]; op.Apk = Apk_[k]; op.PhiR = Phi_[k + 1];
+    op.PhiL = Phi_[k]; op.Apm = Apm_[k]; op.Apk = Apk_[k]; op.PhiR = Phi_[k + 1]; op.tmask = tmask_[k];
     op.l1 = (int)r1; op.s = (int)s; op.r1 = (int)r1; op.n = (int)n; op.S = (int)S;
     op.l2 = (int)r2; op.r2 = (int)r2;
     const bool use_full = Nl < O.max_full;
@@ -807,6 +817,7 @@ class Solver {
       const int64_t rzk = rzphi_[k], rzk1 = rzphi_[k + 1];
       LocalOp zop;
       zop.PhiL = Phiz_[k]; zop.Apm = Apm_[k]; zop.Apk = Apk_[k]; zop.PhiR = Phiz_[k + 1];
+      zop.tmask = tmask_[k];
       zop.l1 = (int)rzk; zop.s = (int)s; zop.r1 = (int)r1; zop.n = (int)n; zop.S = (int)S;
       zop.l2 = (int)rzk1; zop.r2 = (int)r2;
       const int64_t zrows = rzk * n, zcols = rzk1;
@@ -843,6 +854,7 @@ class Solver {
         const int64_t radd = rzphi_[k + 1];
         LocalOp eop;
         eop.PhiL = Phi_[k]; eop.Apm = Apm_[k]; eop.Apk = Apk_[k]; eop.PhiR = Phiz_[k + 1];
+        eop.tmask = tmask_[k];
         eop.l1 = (int)r1; eop.s = (int)s; eop.r1 = (int)r1; eop.n = (int)n; eop.S = (int)S;
         eop.l2 = (int)radd; eop.r2 = (int)r2;
         const int64_t rows = r1 * n;
@@ -945,7 +957,7 @@ class Solver {
     cast_f64_f32(op.PhiL, pl, r1 * s * r1, st_);
     cast_f64_f32(op.PhiR, pr, r2 * S * r2, st_);
     LocalOpF32 op32;
-    op32.PhiL = pl; op32.Apm = Apm32_[k]; op32.Apk = Apk32_[k]; op32.PhiR = pr;
+    op32.PhiL = pl; op32.Apm = Apm32_[k]; op32.Apk = Apk32_[k]; op32.PhiR = pr; op32.tmask = tmask_[k];
     op32.cnt = mv_cnt_; op32.cnt_cap = kMvCounters;
     op32.l1 = op.l1; op32.s = op.s; op32.r1 = op.r1; op32.n = op.n; op32.S = op.S;
     op32.l2 = op.l2; op32.r2 = op.r2;
@@ -1970,7 +1982,7 @@ size_t ttamen_local_matvec_workspace_size(int64_t l1, int64_t s, int64_t r1, int
   op.l1 = (int)l1; op.s = (int)s; op.r1 = (int)r1; op.n = (int)n; op.S = (int)S;
   op.l2 = (int)l2; op.r2 = (int)r2;
   return local_matvec_ws(op, (int)nvec) + (size_t)s * n * n * S * 8 +
-         frag_a_elems((int)(n * S), (int)(s * n)) * 8 + 4096;
+         frag_a_elems((int)(n * S), (int)(s * n)) * 8 + tile_mask_words(n, S) * 8 + 4096;
 }
 
 int ttamen_local_matvec(const double* PhiL, const double* A, const double* PhiR, const double* x,
@@ -1991,6 +2003,11 @@ int ttamen_local_matvec(const double* PhiL, const double* A, const double* PhiR,
     op.PhiL = PhiL; op.Apm = Apm; op.Apk = Apk; op.PhiR = PhiR;
     op.l1 = (int)l1; op.s = (int)s; op.r1 = (int)r1; op.n = (int)n; op.S = (int)S;
     op.l2 = (int)l2; op.r2 = (int)r2;
+    if (s * n <= kTileMaskMaxK) {
+      uint64_t* tm = ws.alloc<uint64_t>(tile_mask_words(n, S));
+      op_tile_masks(Apm, (int)n, (int)S, (int)(s * n), tm, st);
+      op.tmask = tm;
+    }
     local_matvec(op, x, y, (int)nvec, ws, st);
   });
 }
@@ -2016,6 +2033,11 @@ static int local_matvec_f32_abi(const float* PhiL, const float* A, const float* 
     op.PhiL = PhiL; op.Apm = Apm; op.Apk = Apk; op.PhiR = PhiR;
     op.l1 = (int)l1; op.s = (int)s; op.r1 = (int)r1; op.n = (int)n; op.S = (int)S;
     op.l2 = (int)l2; op.r2 = (int)r2;
+    if (s * n <= kTileMaskMaxK) {
+      uint64_t* tm = ws.alloc<uint64_t>(tile_mask_words(n, S));
+      op_tile_masks(Apm, (int)n, (int)S, (int)(s * n), tm, st);
+      op.tmask = tm;
+    }
     op.bf16 = bf16;
     local_matvec_f32(op, x, y, (int)nvec, ws, st);
   });
@@ -2229,6 +2251,11 @@ int ttamen_gmres_solve(const double* PhiL, const double* A, const double* PhiR, 
     op.l1 = (int)r1; op.s = (int)s; op.r1 = (int)r1; op.n = (int)n; op.S = (int)S;
     op.l2 = (int)r2; op.r2 = (int)r2;
     op.cnt = cnt; op.cnt_cap = 1024;
+    if (s * n <= kTileMaskMaxK) {
+      uint64_t* tm = ws.alloc<uint64_t>(tile_mask_words(n, S));
+      op_tile_masks(Apm, (int)n, (int)S, (int)(s * n), tm, st);
+      op.tmask = tm;
+    }
     fill(x, N, 0.0, st);
     norm2(rhs, N, slot, rs, st, true);  // ||rhs|| (1 for a zero rhs: x = 0 is then returned at once)
     GmresDev g = gmres_alloc(ws, (int)local_iterations, N, sizeof(double));

@@ -140,13 +140,27 @@ static bool fused_matvec(const T* PhiL, const T* Apk, const T* PhiR, int l1, int
 // Stages 2 and 3 of the local operator as one KMvTail launch (t1[v][l][(s,n)][R] -> y); false when the
 // shape is outside the kernel (l2 above 16 kTailMaxLB).
 template <typename T>
+static void op_tile_masks_t(const T* Apm, int nm, int S, int K, uint64_t* out, hipStream_t st) {
+  TT_CHECK(K >= 1 && K <= kTileMaskMaxK && nm >= 1 && S >= 1, ERR_INVALID, "op_tile_masks: shape");
+  launch<KTileMask<T>>(dim3((nm + 15) / 16, S), dim3(64), 0, st, {Apm, out, nm, S, K});
+  TT_LAUNCH_CHECK();
+}
+void op_tile_masks(const double* Apm, int nm, int S, int K, uint64_t* out, hipStream_t st) {
+  op_tile_masks_t<double>(Apm, nm, S, K, out, st);
+}
+void op_tile_masks(const float* Apm, int nm, int S, int K, uint64_t* out, hipStream_t st) {
+  op_tile_masks_t<float>(Apm, nm, S, K, out, st);
+}
+
+template <typename T>
 static bool mv_tail(const T* t1, const T* Apm, const T* PhiR, T* y, int64_t nvl, int K, int r2, int nm, int S,
-                    int l2, Arena& ws, hipStream_t st, const int* skip, MvDots* dots) {
+                    int l2, Arena& ws, hipStream_t st, const int* skip, MvDots* dots, const uint64_t* tmask) {
   const int LB = (l2 + 15) / 16;
   // a measuring arena returns false, so that the caller also sizes the GEMM form (either may run)
   if (LB > kTailMaxLB || nvl > INT32_MAX || (nm + 15) / 16 > 65535 || ws.measuring()) return false;
   typename KMvTail<T>::Args a;
   a.t1 = t1; a.Apm = Apm; a.PhiR = PhiR; a.y = y; a.skip = skip;
+  a.tmask = K <= kTileMaskMaxK ? tmask : nullptr;
   a.K = K; a.r2 = r2; a.nm = nm; a.S = S; a.l2 = l2; a.LB = LB;
   a.Qd = nullptr; a.P = nullptr; a.N = 0; a.kc = 0;
   const int64_t nblocks = nvl * ((nm + 15) / 16);
@@ -166,7 +180,8 @@ template <typename T>
 static void matvec_impl(const T* PhiL, const T* Apm, const T* PhiR, int l1, int s, int r1, int n,
                         int nm, int S, int l2, int r2, const T* x, T* y, int nvec, Arena& ws,
                         hipStream_t st, const int* skip,
-                        void (*gemm)(const GemmDesc&, Arena&, hipStream_t), bool tail, MvDots* dots) {
+                        void (*gemm)(const GemmDesc&, Arena&, hipStream_t), bool tail, MvDots* dots,
+                        const uint64_t* tmask) {
   // n: mode of x (columns of the operator core), nm: mode of y (rows; nm == n when square)
   const int64_t nx = (int64_t)r1 * n * r2;
   const int64_t t1v = (int64_t)l1 * s * n * r2;  // per vector
@@ -185,7 +200,7 @@ static void matvec_impl(const T* PhiL, const T* Apm, const T* PhiR, int l1, int 
   }
   if (tail && g_matvec_mode.load() == 0 &&
       mv_tail<T>(t1, Apm, PhiR, y, (int64_t)nvec * l1, s * n, r2, nm, S, l2, ws, st, skip,
-                 nvec == 1 ? dots : nullptr)) {
+                 nvec == 1 ? dots : nullptr, tmask)) {
     ws.release(mk);
     return;
   }
@@ -218,7 +233,7 @@ void local_matvec(const LocalOp& op, const double* x, double* y, int nvec, Arena
                                      x, y, nvec, op.cnt, op.cnt_cap, ws, st, skip))
     return;
   matvec_impl<double>(op.PhiL, op.Apm, op.PhiR, op.l1, op.s, op.r1, op.n, op.n_out ? op.n_out : op.n,
-                      op.S, op.l2, op.r2, x, y, nvec, ws, st, skip, gemm_f64, true, dots);
+                      op.S, op.l2, op.r2, x, y, nvec, ws, st, skip, gemm_f64, true, dots, op.tmask);
 }
 
 void local_matvec_f32(const LocalOpF32& op, const float* x, float* y, int nvec, Arena& ws,
@@ -228,7 +243,7 @@ void local_matvec_f32(const LocalOpF32& op, const float* x, float* y, int nvec, 
                                       x, y, nvec, op.cnt, op.cnt_cap, ws, st, skip))
     return;
   matvec_impl<float>(op.PhiL, op.Apm, op.PhiR, op.l1, op.s, op.r1, op.n, op.n, op.S, op.l2, op.r2,
-                     x, y, nvec, ws, st, skip, op.bf16 ? gemm_bf16 : gemm_f32, !op.bf16, dots);
+                     x, y, nvec, ws, st, skip, op.bf16 ? gemm_bf16 : gemm_f32, !op.bf16, dots, op.tmask);
 }
 
 size_t local_matvec_ws(const LocalOp& op, int nvec) {

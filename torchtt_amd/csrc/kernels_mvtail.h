@@ -21,6 +21,13 @@
 // PhiR[L][S][R], y[v][l][m][L]. Within a 16-deep K step lane group g's element for MFMA jj is
 // k = kofs(g, jj) (a permutation of the 16: the sum over k is the same, and contiguous per lane so
 // that operator rows load as vectors when K allows).
+//
+// Operator tiles. The operator cores of finite-difference and finite-element operators in TT form are
+// block-sparse: their n x n blocks A[s, :, :, S] are identities, stencils, diagonals or zero. With the
+// optional tile masks (KTileMask), an item runs only the K steps where its 16 operator rows hold a
+// nonzero for one of its S values, and a chain with no nonzero tile skips stage 3. A skipped step would
+// have added exact zero products to the accumulators, so the results are the same as the dense loop's
+// (up to the sign of a zero).
 #pragma once
 #include "ttamen_internal.h"
 #include "kernels_gemm.h"
@@ -68,6 +75,7 @@ struct KMvTail {
     const T* PhiR;
     T* y;
     const int* skip;
+    const uint64_t* tmask;  // tile masks (KTileMask) of Apm, or null: every tile; only for K <= kTileMaskMaxK
     int K, r2, nm, S, l2;
     int LB;   // ceil(l2 / 16) <= kTailMaxLB
     int apv;  // operator rows load as vectors (K a multiple of the vector width, aligned base)
@@ -130,6 +138,8 @@ __device__ __forceinline__ void mvtail_body(const typename KMvTail<T>::Args& a, 
   const int mrow = min(m0 + r, a.nm - 1);
   const int K = a.K;
   const int64_t r2 = a.r2;
+  const bool masked = a.tmask != nullptr;
+  const uint64_t* tmask = a.tmask + (masked ? (int64_t)bk.y * a.S : 0);
 
   acc_t yacc[LB];
 #pragma unroll
@@ -144,8 +154,35 @@ __device__ __forceinline__ void mvtail_body(const typename KMvTail<T>::Args& a, 
     acc_t d[SM];
 #pragma unroll
     for (int q = 0; q < SM; ++q) d[q] = acc_t{0, 0, 0, 0};
+    // Steps of the K loop: bit j of mq[q] marks step j (k in [16 j, 16 j + 16)) as one where chain q's
+    // operator rows hold a nonzero (chains past S have none); the loop runs the steps of any chain.
+    // Without masks, every step.
+    const int nsteps = K / 16;
+    uint64_t mq[SM];
+    uint64_t any = 0;
+#pragma unroll
+    for (int q = 0; q < SM; ++q) {
+      mq[q] = (s0 + q < a.S) ? (masked ? tmask[s0 + q] : ~0ull) : 0ull;
+      any |= mq[q];
+    }
+    uint64_t rem = masked ? (any & (nsteps >= 64 ? ~0ull : ((1ull << nsteps) - 1))) : 0ull;
+    const int cnt = masked ? __popcll(rem) : nsteps;
+    int pos = 0, lastj = 0;
+    auto next = [&]() -> int {  // the next step to load; past the last one, the last one again
+      if (masked) {
+        if (rem) {
+          lastj = __builtin_ctzll(rem);
+          rem &= rem - 1;
+        }
+      } else {
+        lastj = min(pos, nsteps - 1);
+        ++pos;
+      }
+      return lastj;
+    };
     struct Frag { T a[4]; T b[SM][4]; };
-    auto load = [&](Frag& f, int kt) {
+    auto load = [&](Frag& f, int j) {
+      const int kt = 16 * j;
 #pragma unroll
       for (int jj = 0; jj < 4; ++jj) f.a[jj] = arow[(int64_t)(kt + tail_kofs<T>(g, jj)) * r2];
 #pragma unroll
@@ -164,25 +201,24 @@ __device__ __forceinline__ void mvtail_body(const typename KMvTail<T>::Args& a, 
 #pragma unroll
         for (int q = 0; q < SM; ++q) d[q] = mfma16(f.a[jj], f.b[q][jj], d[q]);
     };
-    const int nsteps = K / 16;
-    if (nsteps > 0) {
+    if (cnt > 0) {
       Frag f[DEPTH];
-      const int last = nsteps - 1;
 #pragma unroll
-      for (int dd = 0; dd < DEPTH; ++dd) load(f[dd], 16 * min(dd, last));
-      const int full = nsteps / DEPTH * DEPTH;
+      for (int dd = 0; dd < DEPTH; ++dd) load(f[dd], next());
+      const int full = cnt / DEPTH * DEPTH;
       for (int s = 0; s < full; s += DEPTH) {
 #pragma unroll
         for (int dd = 0; dd < DEPTH; ++dd) {
           mma(f[dd]);
-          load(f[dd], 16 * min(s + dd + DEPTH, last));
+          load(f[dd], next());
         }
       }
 #pragma unroll
       for (int dd = 0; dd < DEPTH - 1; ++dd)
-        if (full + dd < nsteps) mma(f[dd]);
+        if (full + dd < cnt) mma(f[dd]);
     }
-    if (16 * nsteps < K) {  // the last, partial K step: elements past K are zero
+    // the last, partial K step (elements past K are zero)
+    if (16 * nsteps < K && (!masked || ((any >> nsteps) & 1ull))) {
       Frag f;
 #pragma unroll
       for (int jj = 0; jj < 4; ++jj) {
@@ -202,7 +238,7 @@ __device__ __forceinline__ void mvtail_body(const typename KMvTail<T>::Args& a, 
     // stage 3: y^T[L][m] += sum_{R in block rb} PhiR[L][S][R] t2^T[R][m]
 #pragma unroll
     for (int q = 0; q < SM; ++q) {
-      if (s0 + q < a.S) {
+      if (mq[q] != 0) {
 #pragma unroll
         for (int lb = 0; lb < LB; ++lb) {
           const T* prow = a.PhiR + ((int64_t)min(lb * 16 + r, a.l2 - 1) * a.S + s0 + q) * r2;
@@ -282,6 +318,32 @@ __device__ __forceinline__ void mvtail_lb(const typename KMvTail<T>::Args& a, co
 // S groups of 2 in both precisions (the item order, hence the summation order, is the same wherever the
 // kernel runs); K steps prefetched 2 deep in FP64 (244 VGPRs: 2 waves per SIMD) and 3 deep in FP32
 // (164 VGPRs: 3 waves per SIMD).
+// Tile masks of a permuted operator core Apm[(m, S)][k] (nm S rows of K values, K <= kTileMaskMaxK):
+// out[mb S + S'] has bit j set when some row m of block mb (rows 16 mb .. 16 mb + 15 below nm) of S'
+// holds a nonzero (or a NaN) at some k in [16 j, 16 j + 16). One wave per (mb, S'), lane j takes step j.
+template <typename T>
+struct KTileMask {
+  struct Args {
+    const T* Apm;
+    uint64_t* out;
+    int nm, S, K;
+  };
+  static constexpr int kThreads = 64;
+  static __device__ void run(const Args& a, const Blk& bk) {
+    const int j = threadIdx.x, mb = bk.x, sp = bk.y;
+    bool nz = false;
+    if (16 * j < a.K) {
+      const int k1 = min(16 * j + 16, a.K);
+      for (int m = 16 * mb; m < min(16 * mb + 16, a.nm); ++m) {
+        const T* row = a.Apm + ((int64_t)m * a.S + sp) * a.K;
+        for (int k = 16 * j; k < k1; ++k) nz = nz || !(row[k] == T(0));
+      }
+    }
+    const uint64_t bits = __ballot(nz);
+    if (j == 0) a.out[(int64_t)mb * a.S + sp] = bits;
+  }
+};
+
 template <typename T>
 __device__ __forceinline__ void KMvTail<T>::run(const Args& a, const Blk& bk) {
   if (a.skip && *a.skip) return;

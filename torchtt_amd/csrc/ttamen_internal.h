@@ -267,6 +267,14 @@ int lu_max_order();
 // (kernels_matvec.h) where the shape allows, else as three GEMMs on Apm. cnt (optional): cnt_cap
 // zeroed counters the fused kernel may use (it leaves them at zero); without them it takes
 // counters from the workspace and clears them first.
+// Tile masks of a permuted operator core Apm[(m, S)][k] (nm S rows of K = s n values): word mb S + S'
+// has bit j set when rows 16 mb .. 16 mb + 15 (below nm) of S' hold a nonzero at some k in [16 j, 16 j + 16).
+// Defined for K <= kTileMaskMaxK (64 steps); the local operator skips the steps whose bit is clear.
+constexpr int kTileMaskMaxK = 1024;
+inline size_t tile_mask_words(int64_t nm, int64_t S) { return (size_t)((nm + 15) / 16) * S; }
+void op_tile_masks(const double* Apm, int nm, int S, int K, uint64_t* out, hipStream_t st);
+void op_tile_masks(const float* Apm, int nm, int S, int K, uint64_t* out, hipStream_t st);
+
 struct LocalOp {
   const double* PhiL = nullptr;
   const double* Apm = nullptr;
@@ -276,6 +284,7 @@ struct LocalOp {
   const double* Apk = nullptr;
   unsigned* cnt = nullptr;
   int cnt_cap = 0;
+  const uint64_t* tmask = nullptr;  // tile masks of Apm (op_tile_masks), or null (K above kTileMaskMaxK)
 };
 // Optional first pass of the GMRES orthogonalisation fused into the local operator (one vector, the
 // default form, FP64/FP32): each workgroup of the stages-2/3 kernel writes its share of Q_i . y (i < kc,
@@ -302,6 +311,7 @@ struct LocalOpF32 {
   const float* Apk = nullptr;
   unsigned* cnt = nullptr;
   int cnt_cap = 0;
+  const uint64_t* tmask = nullptr;  // tile masks of Apm, or null
 };
 // Elements of the operator core Apm (MS x KS, MS = n S, KS = s n) packed in MFMA fragment order.
 size_t frag_a_elems(int MS, int KS);
