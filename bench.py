@@ -190,7 +190,10 @@ def roofline(prof, precision):
     return {"kernel": kern, "bound": "mfma", "achieved": ach, "peak": peak, "unit": "TFLOP/s",
             "frac": ach / peak, "traffic": traffic, "traffic_source": src,
             "applications": int(n), "avg_us": avg_s * 1e6,
-            "flops_per_application": fl / max(n, 1), "compulsory_bytes_per_application": by / max(n, 1)}
+            "flops_per_application": fl / max(n, 1),
+            "flops_note": "the operator-core contraction counted over the core's nonzero 16x16 tiles (the steps "
+                          "the kernel runs), the two interface contractions in full",
+            "compulsory_bytes_per_application": by / max(n, 1)}
 
 
 def base_record(args, value, total, config, dtype):

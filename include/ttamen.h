@@ -45,14 +45,14 @@ typedef struct ttamen_stats {
   int64_t local_solves;      /* local problems solved */
   int64_t direct_solves;     /* of which by the dense direct solver */
   int64_t matvecs;           /* local-operator applications */
-  double matvec_flops;       /* algorithmic flops of those applications */
+  double matvec_flops;       /* flops of those applications as dense contractions */
   double matvec_bytes;       /* compulsory bytes of those applications */
   double max_res_last;       /* max local residual of the last sweep */
   double elapsed_s;          /* wall time inside the call */
   int64_t max_rank;          /* largest TT rank reached */
   int64_t refinements;       /* FP64 refinement steps (precision 1 or 2) */
   int64_t matvecs_f32;       /* FP32 / BF16 local-operator applications (precision 1 or 2) */
-  double matvec_flops_f32;   /* their algorithmic flops */
+  double matvec_flops_f32;   /* their flops as dense contractions */
 } ttamen_stats;
 
 /*
@@ -536,12 +536,16 @@ int ttamen_gemm_f64(int64_t M, int64_t N, int64_t K, int64_t batch, double alpha
  * of precision 1 (only the Arnoldi steps that ran are counted). ttamen_profile_read fills
  * out4[0..7] with {FP64 applications, their event time in ms, algorithmic flops,
  * compulsory bytes, then the same four for FP32} accumulated since ttamen_profile_enable.
+ * The flops count the operator-core contraction over the core's nonzero 16 x 16 tiles only
+ * (the tiles the local operator multiplies), the other two contractions in full.
  * Process-global and lock-protected (batched solves serialise their timed launches).
  */
 int ttamen_profile_enable(int enable);
 /* Orthogonalisation of the local GMRES while profiling is enabled (no counterpart in the
  * reference): out3 = Arnoldi steps timed (HIP events around the CGS dots, update and close
- * launches of a step), their total ms, and their algorithmic bytes ((2k+7) N w for step k). */
+ * launches of a step; with an FP64 basis the first pass's dots run inside the local operator and
+ * are not timed), their total ms, and their algorithmic bytes ((2k+7) N w for step k; (k+5) N w
+ * without the first pass's dots). */
 int ttamen_profile_read_ortho(double* out3);
 int ttamen_profile_read(double* out4);
 

@@ -351,6 +351,11 @@ class Solver {
       if (tmask_[k])
         op_tile_masks(Apm_[k], (int)P.N[k], (int)P.rA[k + 1], (int)(P.rA[k] * P.N[k]), tmask_[k], st_);
     }
+    tdens_.assign(d, 1.0);
+    if (g_prof.enabled && !current_recorder()) {  // profiled solves only: a host read of the masks
+      for (int k = 0; k < d; ++k)
+        if (tmask_[k]) tdens_[k] = tile_density(tmask_[k], P.N[k], P.rA[k + 1], P.rA[k] * P.N[k]);
+    }
     if (O.precision >= 1)
       for (int k = 0; k < d; ++k) {
         permute_core_A_f32(A_[k], Apm32_[k], (int)P.rA[k], (int)P.N[k], (int)P.rA[k + 1], st_);
@@ -456,6 +461,26 @@ class Solver {
   std::vector<float*> Apm32_, Apk32_, A32_;
   std::vector<double*> Apk_;
   std::vector<uint64_t*> tmask_;
+  std::vector<double> tdens_;  // share of the stage-2 operator work in nonzero tiles (profiled solves)
+
+  // Share of the nm x (S K) operator entries that lie in tiles the masks mark nonzero.
+  double tile_density(const uint64_t* mask, int64_t nm, int64_t S, int64_t K) {
+    std::vector<uint64_t> h(tile_mask_words(nm, S));
+    TT_HIP(hipStreamSynchronize(st_));
+    TT_HIP(hipMemcpy(h.data(), mask, h.size() * sizeof(uint64_t), hipMemcpyDeviceToHost));
+    double live = 0.0;
+    for (int64_t mb = 0; mb * 16 < nm; ++mb)
+      for (int64_t sp = 0; sp < S; ++sp)
+        for (int64_t j = 0; 16 * j < K; ++j)
+          if ((h[mb * S + sp] >> j) & 1ull)
+            live += (double)std::min<int64_t>(16, nm - 16 * mb) * std::min<int64_t>(16, K - 16 * j);
+    return live / ((double)nm * S * K);
+  }
+  // Flops of one local-operator application as profiled: stage 2 counts the nonzero operator tiles only.
+  static double profiled_flops(const LocalOp& op) {
+    const double n = op.n, s = op.s, S = op.S, r1 = op.r1, r2 = op.r2, l1 = op.l1, l2 = op.l2;
+    return 2 * n * s * l1 * r1 * r2 + 2 * n * n * s * S * l1 * r2 * op.tdens + 2 * n * S * l1 * l2 * r2;
+  }
   static constexpr int kMvCounters = 1024;  // zeroed counters of the fused local operator
   unsigned* mv_cnt_ = nullptr;
   double* spare_ = nullptr;
@@ -496,7 +521,7 @@ class Solver {
       TT_HIP(hipEventRecord(e1, st_));
       g_prof.pending.push_back({e0, e1});
       const double n = op.n, s = op.s, S = op.S, r1 = op.r1, r2 = op.r2, l1 = op.l1, l2 = op.l2;
-      g_prof.flops += nvec * (2 * n * s * l1 * r1 * r2 + 2 * n * n * s * S * l1 * r2 + 2 * n * S * l1 * l2 * r2);
+      g_prof.flops += nvec * profiled_flops(op);
       g_prof.bytes += 8.0 * (nvec * (r1 * n * r2 + l1 * n * l2) + s * l1 * r1 + s * n * n * S + S * l2 * r2);
       g_prof.count += nvec;
       if (g_prof.pending.size() >= 256) g_prof.flush();
@@ -573,7 +598,7 @@ class Solver {
         const int64_t r1 = rx_[k], r2 = rx_[k + 1];
         LocalOp op;
         op.PhiL = Phiz_[k]; op.Apm = Apm_[k]; op.Apk = Apk_[k]; op.PhiR = Phiz_[k + 1];
-        op.tmask = tmask_[k];
+        op.tmask = tmask_[k]; op.tdens = tdens_[k];
         op.l1 = (int)rzphi_[k]; op.s = (int)s; op.r1 = (int)r1; op.n = (int)n; op.S = (int)S;
         op.l2 = (int)rzphi_[k + 1]; op.r2 = (int)r2;
         const int64_t rows = rzphi_[k], cols = n * rzphi_[k + 1];
@@ -686,7 +711,8 @@ class Solver {
                  (int)r2, sc(NRMSC()), ws_, st_);
     norm2(rhs, Nl, sc(TMP(T_NRHS)), rs_, st_);
     LocalOp op;
-    op.PhiL = Phi_[k]; op.Apm = Apm_[k]; op.Apk = Apk_[k]; op.PhiR = Phi_[k + 1]; op.tmask = tmask_[k];
+    op.PhiL = Phi_[k]; op.Apm = Apm_[k]; op.Apk = Apk_[k]; op.PhiR = Phi_[k + 1];
+    op.tmask = tmask_[k]; op.tdens = tdens_[k];
     op.l1 = (int)r1; op.s = (int)s; op.r1 = (int)r1; op.n = (int)n; op.S = (int)S;
     op.l2 = (int)r2; op.r2 = (int)r2;
     const bool use_full = Nl < O.max_full;
@@ -817,7 +843,7 @@ class Solver {
       const int64_t rzk = rzphi_[k], rzk1 = rzphi_[k + 1];
       LocalOp zop;
       zop.PhiL = Phiz_[k]; zop.Apm = Apm_[k]; zop.Apk = Apk_[k]; zop.PhiR = Phiz_[k + 1];
-      zop.tmask = tmask_[k];
+      zop.tmask = tmask_[k]; zop.tdens = tdens_[k];
       zop.l1 = (int)rzk; zop.s = (int)s; zop.r1 = (int)r1; zop.n = (int)n; zop.S = (int)S;
       zop.l2 = (int)rzk1; zop.r2 = (int)r2;
       const int64_t zrows = rzk * n, zcols = rzk1;
@@ -854,7 +880,7 @@ class Solver {
         const int64_t radd = rzphi_[k + 1];
         LocalOp eop;
         eop.PhiL = Phi_[k]; eop.Apm = Apm_[k]; eop.Apk = Apk_[k]; eop.PhiR = Phiz_[k + 1];
-        eop.tmask = tmask_[k];
+        eop.tmask = tmask_[k]; eop.tdens = tdens_[k];
         eop.l1 = (int)r1; eop.s = (int)s; eop.r1 = (int)r1; eop.n = (int)n; eop.S = (int)S;
         eop.l2 = (int)radd; eop.r2 = (int)r2;
         const int64_t rows = r1 * n;
@@ -983,8 +1009,7 @@ class Solver {
         local_matvec_f32(op32, xin, yout, 1, ws_, st_, skip);
         TT_HIP(hipEventRecord(e1, st_));
         const double n = op.n, s = op.s, S = op.S, r1 = op.r1, r2 = op.r2, l1 = op.l1, l2 = op.l2;
-        g_prof.tagged.push_back({(const void*)this, e0, e1, cyc, step,
-                                 2 * n * s * l1 * r1 * r2 + 2 * n * n * s * S * l1 * r2 + 2 * n * S * l1 * l2 * r2,
+        g_prof.tagged.push_back({(const void*)this, e0, e1, cyc, step, profiled_flops(op),
                                  4.0 * (r1 * n * r2 + l1 * n * l2 + s * l1 * r1 + s * n * n * S + S * l2 * r2)});
         return 0;
       };

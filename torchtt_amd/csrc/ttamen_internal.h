@@ -285,6 +285,7 @@ struct LocalOp {
   unsigned* cnt = nullptr;
   int cnt_cap = 0;
   const uint64_t* tmask = nullptr;  // tile masks of Apm (op_tile_masks), or null (K above kTileMaskMaxK)
+  double tdens = 1.0;  // share of the operator entries in nonzero tiles (host bookkeeping of profiled flops)
 };
 // Optional first pass of the GMRES orthogonalisation fused into the local operator (one vector, the
 // default form, FP64/FP32): each workgroup of the stages-2/3 kernel writes its share of Q_i . y (i < kc,
