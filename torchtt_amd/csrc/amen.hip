@@ -281,7 +281,10 @@ class Solver {
   int TMP(int i) const { return 1 + 3 * (d - 1) + i; }
   enum { T_NRHS = 0, T_DNORM, T_RESOLD, T_RESNEW, T_CUR, T_MAXRES, T_DX, T_SOLN, T_ONE, T_R32, T_TAUN, T_CAND };
   static constexpr int kMaxRefine = 12;      // FP64 refinement steps per local solve (FP32 mode)
-  static constexpr double kInnerTolFloor = 1e-5;  // smallest relative tolerance asked of FP32 GMRES
+  // Smallest relative tolerance asked of FP32 GMRES. Measured on the 256-system batch and the d=10 n=64
+  // solve: 1e-5, 1e-6, 3e-7 and 1e-4 give the same solutions to the stated accuracy and about the same
+  // GMRES iteration counts; 1e-6 needs the fewest refinement steps for its time (batch 4.64 -> 4.56 s).
+  static constexpr double kInnerTolFloor = 1e-6;
   // BF16 products carry 8 significant bits: each refinement step gains about two digits at
   // best, so the inner tolerance stops at 1e-3 and more steps are allowed.
   static constexpr int kMaxRefineBf16 = 40;
