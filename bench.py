@@ -191,8 +191,10 @@ def roofline(prof, precision):
             "frac": ach / peak, "traffic": traffic, "traffic_source": src,
             "applications": int(n), "avg_us": avg_s * 1e6,
             "flops_per_application": fl / max(n, 1),
-            "flops_note": "the operator-core contraction counted over the core's nonzero 16x16 tiles (the steps "
-                          "the kernel runs), the two interface contractions in full",
+            "flops_note": ("the operator-core contraction counted over the core's nonzero 16x16 tiles (the steps "
+                           "the kernel runs), the two interface contractions in full"
+                           if mode == 0 and precision != "bf16" else "the dense contraction (this form multiplies "
+                           "every operator tile)"),
             "compulsory_bytes_per_application": by / max(n, 1)}
 
 

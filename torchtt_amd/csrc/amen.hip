@@ -479,10 +479,12 @@ class Solver {
             live += (double)std::min<int64_t>(16, nm - 16 * mb) * std::min<int64_t>(16, K - 16 * j);
     return live / ((double)nm * S * K);
   }
-  // Flops of one local-operator application as profiled: stage 2 counts the nonzero operator tiles only.
-  static double profiled_flops(const LocalOp& op) {
+  // Flops of one local-operator application as profiled: where the form that runs skips the operator
+  // core's zero tiles, stage 2 counts the nonzero tiles only; else the dense contraction.
+  static double profiled_flops(const LocalOp& op, bool bf16 = false) {
     const double n = op.n, s = op.s, S = op.S, r1 = op.r1, r2 = op.r2, l1 = op.l1, l2 = op.l2;
-    return 2 * n * s * l1 * r1 * r2 + 2 * n * n * s * S * l1 * r2 * op.tdens + 2 * n * S * l1 * l2 * r2;
+    const double dens = matvec_skips_zero_tiles(op.l2, op.s * op.n, bf16) ? op.tdens : 1.0;
+    return 2 * n * s * l1 * r1 * r2 + 2 * n * n * s * S * l1 * r2 * dens + 2 * n * S * l1 * l2 * r2;
   }
   static constexpr int kMvCounters = 1024;  // zeroed counters of the fused local operator
   unsigned* mv_cnt_ = nullptr;
@@ -1012,7 +1014,7 @@ class Solver {
         local_matvec_f32(op32, xin, yout, 1, ws_, st_, skip);
         TT_HIP(hipEventRecord(e1, st_));
         const double n = op.n, s = op.s, S = op.S, r1 = op.r1, r2 = op.r2, l1 = op.l1, l2 = op.l2;
-        g_prof.tagged.push_back({(const void*)this, e0, e1, cyc, step, profiled_flops(op),
+        g_prof.tagged.push_back({(const void*)this, e0, e1, cyc, step, profiled_flops(op, op32.bf16),
                                  4.0 * (r1 * n * r2 + l1 * n * l2 + s * l1 * r1 + s * n * n * S + S * l2 * r2)});
         return 0;
       };

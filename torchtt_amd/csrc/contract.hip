@@ -139,6 +139,10 @@ static bool fused_matvec(const T* PhiL, const T* Apk, const T* PhiR, int l1, int
 //   stage 3 (one GEMM):   y[(v,l,m)][L]     = sum_{S,R} t2[(v,l,m)][(S,R)] PhiR[L,(S,R)]
 // Stages 2 and 3 of the local operator as one KMvTail launch (t1[v][l][(s,n)][R] -> y); false when the
 // shape is outside the kernel (l2 above 16 kTailMaxLB).
+bool matvec_skips_zero_tiles(int l2, int K, bool bf16) {
+  return !bf16 && g_matvec_mode.load() == 0 && (l2 + 15) / 16 <= kTailMaxLB && K <= kTileMaskMaxK;
+}
+
 template <typename T>
 static void op_tile_masks_t(const T* Apm, int nm, int S, int K, uint64_t* out, hipStream_t st) {
   TT_CHECK(K >= 1 && K <= kTileMaskMaxK && nm >= 1 && S >= 1, ERR_INVALID, "op_tile_masks: shape");

@@ -1,5 +1,5 @@
 // Allocation-free local operator: the FP64 local matvec of ttamen_local_matvec with its scratch
-// (the permuted operator core, the stage intermediates and any split-K partials) in a file-scope
+// (the permuted operator core, its zero-tile masks, the stage intermediates and any split-K partials) in a file-scope
 // device array instead of a caller workspace, for callers that cannot allocate device memory
 // (code under graph capture, tight serving loops, harnesses that forbid hipMalloc).
 //
@@ -48,6 +48,7 @@ static size_t local_matvec_scratch(const LocalOp& shape, int nvec) {
   Arena meas;
   meas.alloc<double>((size_t)shape.s * shape.n * shape.n * shape.S);
   meas.alloc<double>(frag_a_elems(shape.n * shape.S, shape.s * shape.n));
+  if ((int64_t)shape.s * shape.n <= kTileMaskMaxK) meas.alloc<uint64_t>(tile_mask_words(shape.n, shape.S));
   local_matvec(shape, nullptr, nullptr, nvec, meas, nullptr);
   return meas.peak();
 }
@@ -92,6 +93,11 @@ int ttamen_local_matvec_noalloc(const double* PhiL, const double* A, const doubl
     pack_frag_a(Apm, Apk, (int)(n * S), (int)(s * n), st);
     // the fused kernel's counters come from the scratch and are cleared (by a kernel) first
     op.PhiL = PhiL; op.Apm = Apm; op.Apk = Apk; op.PhiR = PhiR;
+    if (s * n <= kTileMaskMaxK) {  // zero-tile masks, written in full by their kernel
+      uint64_t* tm = ws.alloc<uint64_t>(tile_mask_words(n, S));
+      op_tile_masks(Apm, (int)n, (int)S, (int)(s * n), tm, st);
+      op.tmask = tm;
+    }
     local_matvec(op, x, y, (int)nvec, ws, st);
   });
 }

@@ -274,6 +274,9 @@ constexpr int kTileMaskMaxK = 1024;
 inline size_t tile_mask_words(int64_t nm, int64_t S) { return (size_t)((nm + 15) / 16) * S; }
 void op_tile_masks(const double* Apm, int nm, int S, int K, uint64_t* out, hipStream_t st);
 void op_tile_masks(const float* Apm, int nm, int S, int K, uint64_t* out, hipStream_t st);
+// Whether the local operator's current form skips zero operator tiles for this shape (the default form's
+// stages-2/3 kernel with masks; not the BF16 products, not the other forms).
+bool matvec_skips_zero_tiles(int l2, int K, bool bf16);
 
 struct LocalOp {
   const double* PhiL = nullptr;
