@@ -130,7 +130,7 @@ static void gemm_impl(const GemmDesc& g, Arena& ws, hipStream_t st, bool bf16 = 
     return (int)((g.K + kchunk - 1) / kchunk);
   };
   const bool tunable = !ws.measuring() && current_recorder() == nullptr && g.skip == nullptr && g.K > 0 &&
-                       g_tune_mode.load() == 1 && !gemm_tuning_off();
+                       g_tune_mode.load() == 1 && !gemm_tuning_off() && g.split_out == nullptr;
   if (tunable) {
     const uint64_t key = bucket_key(g, bf16 ? 2 : (sizeof(T) == 8 ? 0 : 1));
     int choice = -1;
@@ -210,9 +210,19 @@ static void gemm_impl(const GemmDesc& g, Arena& ws, hipStream_t st, bool bf16 = 
     }
   }
   int splits = split_plan(BM, BN, &p.kchunk);
+  if (g.split_out) {
+    TT_CHECK(g.alpha == 1.0 && g.beta == 0.0 && g.scale_ptr == nullptr && g.K > 0, ERR_INVALID,
+             "gemm: kept split-K partials need alpha 1, beta 0, no scale");
+    g.split_out->partial = nullptr;
+    g.split_out->splits = splits;
+  }
   size_t mk = ws.mark();
   const size_t partial_elems = splits > 1 ? (size_t)splits * p.B * g.M * g.N : 0;
   if (partial_elems) p.partial = ws.alloc<T>(partial_elems);
+  if (g.split_out && partial_elems) {  // the consumer adds them: they stay allocated
+    g.split_out->partial = p.partial;
+    mk = ws.mark();
+  }
   if (!ws.measuring()) {
     if (g.K == 0) {
       // C = beta * C: run the kernel with an empty K range.
@@ -222,7 +232,7 @@ static void gemm_impl(const GemmDesc& g, Arena& ws, hipStream_t st, bool bf16 = 
     else if (BM == 32) launch_tile<T, 32, 64>(p, splits, st, bf16);
     else if (BN == 32) launch_tile<T, 64, 32>(p, splits, st, bf16);
     else launch_tile<T, 64, 64>(p, splits, st, bf16);
-    if (splits > 1) {
+    if (splits > 1 && !g.split_out) {
       int64_t total = (int64_t)g.M * g.N * p.B;
       int blocks = (int)std::min<int64_t>((total + 255) / 256, 4096);
       launch<KGemmReduce<T>>(dim3(blocks), dim3(256), 0, st, {p, splits});

@@ -28,6 +28,8 @@ constexpr int kQrPasses = 4;
 // G (n x n, row-major, symmetric) + shift*I = R^T R, R upper triangular written
 // row-major to Rout.
 //   * If skip_in && *skip_in: an earlier pass already finished; only propagate the flag.
+//   * With splits > 1, G is the sum of the Gram GEMM's split-K partials Gp[z * n * n + t], z = 0 .. splits-1,
+//     added here in slice order (the GEMM's own reduction, which this replaces, adds them the same way).
 //   * delta = max |G - I|. If delta < kNearIdentity the input is already orthonormal to
 //     that level: no shift, and this pass is the last (*final_out = 1).
 //   * Otherwise shift = shift_c * u * trace(G) when shift_c > 0 (shifted Cholesky-QR).
@@ -38,7 +40,7 @@ constexpr double kNearIdentity = 1e-6;
 struct KCholesky {
   struct Args {
     const double* G; int n; double shift_c; double dead_rel; int last_pass; double* Rout;
-    int* dead; int* ndead; const int* skip_in; int* final_out;
+    int* dead; int* ndead; const int* skip_in; int* final_out; const double* Gp; int splits;
   };
   static constexpr int kThreads = 1024;
   static __device__ void run(const Args& a, const Blk&);
@@ -61,8 +63,18 @@ __device__ void KCholesky::run(const Args& a, const Blk&) {
   const int nn = n * n;
   for (int t0 = tid; t0 < nn; t0 += 8 * nt) {  // 8 independent loads in flight per thread
     double v[8];
+    if (a.splits > 1) {  // sum of the slices, in slice order
 #pragma unroll
-    for (int u = 0; u < 8; ++u) v[u] = (t0 + u * nt < nn) ? G[t0 + u * nt] : 0.0;
+      for (int u = 0; u < 8; ++u) v[u] = 0.0;
+      for (int z = 0; z < a.splits; ++z) {
+        const double* Pz = a.Gp + (int64_t)z * nn;
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] += (t0 + u * nt < nn) ? Pz[t0 + u * nt] : 0.0;
+      }
+    } else {
+#pragma unroll
+      for (int u = 0; u < 8; ++u) v[u] = (t0 + u * nt < nn) ? G[t0 + u * nt] : 0.0;
+    }
 #pragma unroll
     for (int u = 0; u < 8; ++u) {
       const int t = t0 + u * nt;
@@ -539,20 +551,26 @@ int qr(const double* A, int64_t sr, int64_t sc, int64_t m, int64_t n, double* Q,
     const double* src = (pass == 0) ? A : Q;
     int64_t ssr = (pass == 0) ? sr : n, ssc = (pass == 0) ? sc : 1;
     const int* skip = (pass == 0) ? nullptr : flags + pass;
-    GemmDesc g;  // G = src^T src
+    GemmDesc g;  // G = src^T src (split-K partials added by the Cholesky kernel)
     g.M = (int)n; g.N = (int)n; g.K = (int)m;
     g.a.p = src; g.a.r = ssc; g.a.c = ssr;
     g.b.p = src; g.b.r = ssr; g.b.c = ssc;
     g.c.p = G; g.c.r = n; g.c.c = 1;
     g.skip = skip;
+    SplitOut gs;
+    g.split_out = &gs;
+    const size_t pmk = ws.mark();  // the partials live until the Cholesky below has been launched
     gemm_f64(g, ws, st);
     if (!ws.measuring()) {
       const double shift_c = (pass < 2) ? 11.0 * ((double)m * n + (double)n * (n + 1)) : 0.0;
       const double dead_rel = (pass < 2) ? 0.0 : 1e-26;
       launch<KCholesky>(dim3(1), dim3(1024), chol_lds, st,
                         {G, (int)n, shift_c, dead_rel, (int)(pass == kQrPasses - 1), Rk, dead, ndead,
-                         skip, flags + pass + 1});
+                         skip, flags + pass + 1, (const double*)gs.partial, gs.splits});
       TT_LAUNCH_CHECK();
+    }
+    ws.release(pmk);
+    if (!ws.measuring()) {
       launch<KTrsmRows>(dim3(trsm_blocks), dim3(kTrsmThreads), trsm_lds, st,
                         {src, ssr, ssc, m, (int)n, Rk, Q, skip});
       TT_LAUNCH_CHECK();

@@ -120,6 +120,12 @@ struct OutRef {
 
 enum class DType : int { F64 = 0, F32 = 1, BF16 = 2 };
 
+// Partial products of a split-K GEMM left to its consumer (GemmDesc::split_out).
+struct SplitOut {
+  void* partial = nullptr;  // splits x B x M x N, row-major per slice, in the GEMM's type
+  int splits = 1;           // 1: K was not split and C holds the product
+};
+
 struct GemmDesc {
   int M = 0, N = 0, K = 0;
   int B1 = 1, B2 = 1;
@@ -129,6 +135,10 @@ struct GemmDesc {
   const double* scale_ptr = nullptr;
   int scale_mode = 0;
   const int* skip = nullptr;
+  // Optional: when the GEMM splits K, leave the slices' partial products (allocated from the workspace
+  // and kept there) for the consumer to add in slice order instead of reducing them into C; alpha = 1,
+  // beta = 0 and no scale. Receives them and their count (1: C was written).
+  SplitOut* split_out = nullptr;
 };
 
 // Tile autotuner of the single-system GEMMs (outside cohorts): mode 0 uses the fixed rule, mode 1
