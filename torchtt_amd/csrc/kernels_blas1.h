@@ -300,14 +300,15 @@ struct KSdiv {
 };
 
 // Inner tolerance of a mixed-precision refinement step, times ||r32||:
-//   S[dst] = min(0.5, max(floor, 0.5 * real_tol * S[nrhs] / S[rn])) * S[r32]
+//   S[dst] = min(0.5, max(floor, real_tol * S[nrhs] / S[rn])) * S[r32]
+// (the step's GMRES aims at the target itself; the FP64 residual decides whether another step runs)
 struct KTauScale {
   struct Args { double* S; int dst, nrhs, rn, r32; double real_tol, floor; };
   static constexpr int kThreads = 64;
   static constexpr int kMega = 38;
   static __device__ __forceinline__ void run(const Args& p, const Blk&) {
     if (threadIdx.x != 0) return;
-    const double t = fmin(0.5, fmax(p.floor, 0.5 * p.real_tol * p.S[p.nrhs] / p.S[p.rn]));
+    const double t = fmin(0.5, fmax(p.floor, p.real_tol * p.S[p.nrhs] / p.S[p.rn]));
     p.S[p.dst] = t * p.S[p.r32];
   }
 };
