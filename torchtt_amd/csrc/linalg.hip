@@ -510,8 +510,12 @@ __device__ void KQrSmall::run(const Args& a, const Blk&) {
 
 // ---------------------------------------------------------------- QR driver
 int qr(const double* A, int64_t sr, int64_t sc, int64_t m, int64_t n, double* Q, double* R,
-       Arena& ws, hipStream_t st) {
+       Arena& ws, hipStream_t st, SplitOut* r_split) {
   TT_CHECK(m >= 1 && n >= 1, ERR_INVALID, "qr: empty matrix");
+  if (r_split) {
+    r_split->partial = nullptr;
+    r_split->splits = 1;
+  }
   if (m < n) {
     if (!ws.measuring()) {
       eye(Q, m, m, st);
@@ -580,15 +584,16 @@ int qr(const double* A, int64_t sr, int64_t sc, int64_t m, int64_t n, double* Q,
     launch<KCompleteDead>(dim3(1), dim3(1024), 0, st, {Q, m, (int)n, dead, ndead});
     TT_LAUNCH_CHECK();
   }
+  ws.release(mk);  // what the passes used; partials kept for the caller are allocated after it
   {
     GemmDesc g;  // R = Q^T A
     g.M = (int)n; g.N = (int)n; g.K = (int)m;
     g.a.p = Q; g.a.r = 1; g.a.c = n;
     g.b.p = A; g.b.r = sr; g.b.c = sc;
     g.c.p = R; g.c.r = n; g.c.c = 1;
+    g.split_out = r_split;
     gemm_f64(g, ws, st);
   }
-  ws.release(mk);
   return (int)n;
 }
 
@@ -608,11 +613,12 @@ __device__ __forceinline__ void rr_pair(int t, int i, int K, int& p, int& q) {
 // One-sided Jacobi on the columns of a k x k matrix M (element (i,j) at M[i*sr + j*sc]).
 // Outputs: s (k, descending), Ur (k x k row-major; column r = r-th left singular
 // vector), rank_of[j] (sorted position of original column j), log of rotations
-// (c, s) per (sweep, round, pair), *nsweeps.
+// (c, s) per (sweep, round, pair), *nsweeps. With splits > 1, M is the sum of split-K
+// partials Mp[z * k * k + i * sr + j * sc], z = 0 .. splits-1, added here in slice order.
 struct KJacobi {
   struct Args {
     const double* M; int64_t sr, sc; int k; double* s_out; double* Ur; int* rank_of;
-    double2* log; int* nsweeps;
+    double2* log; int* nsweeps; const double* Mp; int splits;
   };
   static constexpr int kThreads = 1024;
   static __device__ void run(const Args& a, const Blk&);
@@ -632,7 +638,14 @@ __device__ void KJacobi::run(const Args& a, const Blk&) {
   const int npairs = K / 2;
   for (int t = tid; t < k * k; t += nt) {
     int j = t / k, i = t % k;
-    W[t] = M[(int64_t)i * sr + (int64_t)j * sc];
+    const int64_t off = (int64_t)i * sr + (int64_t)j * sc;
+    if (a.splits > 1) {
+      double v = 0.0;
+      for (int z = 0; z < a.splits; ++z) v += a.Mp[(int64_t)z * k * k + off];
+      W[t] = v;
+    } else {
+      W[t] = M[off];
+    }
   }
   const double tol = (double)k * kUnitRoundoff;
   int sw = 0;
@@ -807,8 +820,10 @@ __device__ void KJacobiV::run(const Args& a, const Blk& bk) {
 }
 
 static void jacobi_svd_small(const double* M, int64_t sr, int64_t sc, int k, double* s,
-                             double* Ur, double* V, int64_t ldv, Arena& ws, hipStream_t st) {
+                             double* Ur, double* V, int64_t ldv, Arena& ws, hipStream_t st,
+                             const double* Mp = nullptr, int splits = 1) {
   if (k > kMaxSmall) {
+    TT_CHECK(splits == 1, ERR_INVALID, "svd: split-K partials above the LDS size");
     svd_large(M, sr, sc, k, s, Ur, V, ldv, ws, st);
     return;
   }
@@ -819,7 +834,7 @@ static void jacobi_svd_small(const double* M, int64_t sr, int64_t sc, int k, dou
   double2* log = ws.alloc<double2>((size_t)kJacobiMaxSweeps * std::max(1, K - 1) * (K / 2 + 1));
   if (!ws.measuring()) {
     size_t lds = ((size_t)k * k + k) * sizeof(double);
-    launch<KJacobi>(dim3(1), dim3(1024), lds, st, {M, sr, sc, k, s, Ur, rank_of, log, nsw});
+    launch<KJacobi>(dim3(1), dim3(1024), lds, st, {M, sr, sc, k, s, Ur, rank_of, log, nsw, Mp, splits});
     TT_LAUNCH_CHECK();
     launch<KJacobiV>(dim3(k), dim3(64), (size_t)K * sizeof(double), st,
                      {k, log, nsw, rank_of, V, ldv});
@@ -837,11 +852,12 @@ int svd(const double* Mx, int64_t sr, int64_t sc, int64_t m, int64_t n, double* 
     double* Q = ws.alloc<double>(m * n);
     double* R = ws.alloc<double>(n * n);
     double* Ur = ws.alloc<double>(n * n);
-    qr(Mx, sr, sc, m, n, Q, R, ws, st);
+    SplitOut rs;  // R's split-K partials, added by the Jacobi kernel as it loads R
+    qr(Mx, sr, sc, m, n, Q, R, ws, st, n <= kMaxSmall ? &rs : nullptr);
     // One-sided Jacobi on the columns of R^T (the rows of R): R^T = V S Ur^T, so its left vectors are
     // R's right ones and the rotations give R's left ones. Measured against Jacobi on R's columns: fewer
     // rotations to converge (d=10 n=64 solve 309 -> 305 ms, 256-system batch 1% faster).
-    jacobi_svd_small(R, 1, n, (int)n, s, V, Ur, n, ws, st);
+    jacobi_svd_small(R, 1, n, (int)n, s, V, Ur, n, ws, st, (const double*)rs.partial, rs.splits);
     GemmDesc g;  // U = Q Ur
     g.M = (int)m; g.N = (int)n; g.K = (int)n;
     g.a = rm(Q, n); g.b = rm(Ur, n); g.c = orm(U, n);

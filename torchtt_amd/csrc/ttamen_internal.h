@@ -249,11 +249,13 @@ void svd_large(const double* M, int64_t sr, int64_t sc, int k, double* s, double
                Arena& ws, hipStream_t st);
 
 // Thin QR of a tall m x n (m >= n) matrix given by strides, via shifted
-// Cholesky-QR with three passes and triangular solves; Q is m x n row-major
+// Cholesky-QR with up to four passes and triangular solves; Q is m x n row-major
 // (ld n), R = Q^T A is n x n row-major. If m < n: Q = I_m, R = A (m x n).
+// With r_split, R's split-K partials may be left to the caller (SplitOut: R is then
+// their sum in slice order, not written; splits = 1 when R was written).
 // Returns the number of columns of Q (min(m, n)).
 int qr(const double* A, int64_t sr, int64_t sc, int64_t m, int64_t n, double* Q, double* R,
-       Arena& ws, hipStream_t st);
+       Arena& ws, hipStream_t st, SplitOut* r_split = nullptr);
 // Thin SVD of an m x n matrix (strided): U (m x k row-major, ld k), s (k), V (n x k
 // row-major, ld k), k = min(m, n). Singular values descending.
 int svd(const double* M, int64_t sr, int64_t sc, int64_t m, int64_t n, double* U, double* s,
