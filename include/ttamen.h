@@ -504,6 +504,15 @@ int ttamen_device_info(char* buf, size_t len);
 int ttamen_qr(const double* A, int64_t m, int64_t n, double* Q, double* R, void* workspace,
               size_t workspace_bytes, void* stream);
 /*
+ * QR of a row-major m x (r + p) matrix A = [U | Z] whose first r columns are orthonormal and whose
+ * last p (1..4) columns are new (FP64): Q = [U | Qz] m x (r + p), R = Q^T A, Q^T Q = I (dead columns
+ * of Qz completed against all of Q). No counterpart in the reference as an entry point: it is how the
+ * solver performs the enrichment QR of cpp/amen_solve.h:454 (torch QR of [u, z]), exposed for tests.
+ * Requires r + p <= m and r + p <= 140; synchronous.
+ */
+int ttamen_qr_extend(const double* A, int64_t m, int64_t r, int64_t p, double* Q, double* R, void* workspace,
+                     size_t workspace_bytes, void* stream);
+/*
  * Thin SVD of a row-major m x n matrix (FP64): U m x k, s k (descending), V n x k,
  * A = U diag(s) V^T. Replaces at::linalg_svd in cpp/amen_solve.h:231,383,430 and
  * cpp/compression.h:23.

@@ -491,6 +491,47 @@ def test_qr(kind, m, n):
     assert np.linalg.norm(Qh @ Rh - A) <= 1e-14 * np.sqrt(m * n) * 10 * np.linalg.norm(A) + 1e-300
 
 
+@pytest.mark.parametrize("case", ["rand", "in_span", "zero_and_duplicate", "nearly_in_span"])
+@pytest.mark.parametrize("m,r,p", [(3328, 52, 4), (300, 40, 3), (44, 40, 4), (64, 1, 1)])
+def test_qr_extend(case, m, r, p):
+    """QR of [U | Z] with U orthonormal (the solver's enrichment QR): Q keeps U as its first columns, is
+    orthonormal, and Q R = A; Z columns in span(U), zero or repeated are completed against all of Q."""
+    L, lib = _lib()
+    _say("qr_extend", case, m, r, p)
+    g = np.random.default_rng(m + r + p)
+    U, _ = np.linalg.qr(g.standard_normal((m, r)))
+    Z = g.standard_normal((m, p))
+    if case == "in_span":
+        Z[:, 0] = U @ g.standard_normal(r)
+    elif case == "zero_and_duplicate":
+        Z[:, 0] = 0.0
+        if p > 1:
+            Z[:, p - 1] = Z[:, 0 if p == 2 else 1]
+    elif case == "nearly_in_span":
+        Z[:, 0] = U @ g.standard_normal(r) + 1e-12 * g.standard_normal(m)
+    A = np.ascontiguousarray(np.concatenate([U, Z], axis=1))
+    n = r + p
+    Ad = torch.from_numpy(A).to(DEV)
+    Q = torch.empty(m, n, dtype=torch.float64, device=DEV)
+    R = torch.empty(n, n, dtype=torch.float64, device=DEV)
+    ws = _ws(lib.ttamen_dense_workspace_size(m, n))
+    L.check("qr_extend", lib.ttamen_qr_extend(_p(Ad), m, r, p, _p(Q), _p(R), _p(ws), ws.numel(), _stream()))
+    Qh, Rh = Q.cpu().numpy(), R.cpu().numpy()
+    assert np.array_equal(Qh[:, :r], U)
+    assert np.linalg.norm(Qh.T @ Qh - np.eye(n)) < 1e-13 * np.sqrt(n) * 10
+    assert np.linalg.norm(Qh @ Rh - A) <= 1e-14 * np.sqrt(m * n) * 10 * np.linalg.norm(A)
+
+
+def test_qr_extend_refuses_shapes():
+    L, lib = _lib()
+    m, r, p = 40, 38, 3  # r + p > m
+    A = torch.zeros(m, r + p, dtype=torch.float64, device=DEV)
+    Q = torch.empty(m, r + p, dtype=torch.float64, device=DEV)
+    R = torch.empty(r + p, r + p, dtype=torch.float64, device=DEV)
+    ws = _ws(lib.ttamen_dense_workspace_size(m, r + p))
+    assert lib.ttamen_qr_extend(_p(A), m, r, p, _p(Q), _p(R), _p(ws), ws.numel(), _stream()) == 1
+
+
 @pytest.mark.parametrize("kind,m,n", [("rand", 200, 30), ("rand", 5376, 84), ("rand", 4, 256), ("rand", 30, 30),
                                       ("graded", 300, 40), ("zero_cols", 60, 9), ("rand", 1, 7), ("rand", 9, 1),
                                       ("rand", 1000, 136), ("rand", 4, 256), ("rand", 256, 4), ("zero_cols", 128, 4),

@@ -150,6 +150,7 @@ _SIGNATURES = {
     "ttamen_phi_bck_rhs": (ctypes.c_int, [c_dp] * 4 + [c_i64] * 5 + [c_dp, c_size, c_dp]),
     "ttamen_phi_workspace_size": (c_size, [c_i64] * 7),
     "ttamen_qr": (ctypes.c_int, [c_dp, c_i64, c_i64, c_dp, c_dp, c_dp, c_size, c_dp]),
+    "ttamen_qr_extend": (ctypes.c_int, [c_dp, c_i64, c_i64, c_i64, c_dp, c_dp, c_dp, c_size, c_dp]),
     "ttamen_svd": (ctypes.c_int, [c_dp, c_i64, c_i64, c_dp, c_dp, c_dp, c_dp, c_size, c_dp]),
     "ttamen_dense_workspace_size": (c_size, [c_i64, c_i64]),
     "ttamen_dense_solve": (ctypes.c_int, [c_dp, c_dp, c_dp, c_i64, c_dp, c_size, c_dp]),

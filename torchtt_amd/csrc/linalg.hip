@@ -391,6 +391,7 @@ __device__ __forceinline__ void block_sum_vec(const double (&v)[K], double* sh, 
 struct KQrSmall {
   struct Args {
     const double* A; int64_t sr, sc, m; int n; double* Q; double* R; int* dead; int* ndead;
+    int complete;  // 0: leave dead columns (flagged in dead) for the caller to complete
   };
   static constexpr int kThreads = 256;
   static __device__ void run(const Args& a, const Blk&);
@@ -483,7 +484,7 @@ __device__ void KQrSmall::run(const Args& a, const Blk&) {
     if (s_last) break;
   }
   // orthonormal completion of dead columns (as KCompleteDead, with this block)
-  if (*a.ndead > 0) KCompleteDead::run({a.Q, m, n, a.dead, a.ndead}, Blk{0, 0, 0, 1, 1, 1});
+  if (a.complete && *a.ndead > 0) KCompleteDead::run({a.Q, m, n, a.dead, a.ndead}, Blk{0, 0, 0, 1, 1, 1});
   __syncthreads();
   // R = Q^T A
   double rr[NC * NC];
@@ -527,7 +528,7 @@ int qr(const double* A, int64_t sr, int64_t sc, int64_t m, int64_t n, double* Q,
     size_t mk = ws.mark();
     int* dead = ws.alloc<int>(n + 1);
     if (!ws.measuring()) {
-      launch<KQrSmall>(dim3(1), dim3(256), 0, st, {A, sr, sc, m, (int)n, Q, R, dead, dead + n});
+      launch<KQrSmall>(dim3(1), dim3(256), 0, st, {A, sr, sc, m, (int)n, Q, R, dead, dead + n, 1});
       TT_LAUNCH_CHECK();
     }
     ws.release(mk);
@@ -595,6 +596,58 @@ int qr(const double* A, int64_t sr, int64_t sc, int64_t m, int64_t n, double* Q,
     gemm_f64(g, ws, st);
   }
   return (int)n;
+}
+
+// ---------------------------------------------------------------- QR of an extended orthonormal basis
+// A = [U | Z], U orthonormal. Z's component outside span(U) is found by one block Gram-Schmidt pass and a
+// QR of the remainder (KQrSmall, completing its own dead columns); since that QR divides by the remainder's
+// pivots, its rounding can carry back components along U of size u times the remainder's condition, so its
+// columns are projected against U once more and orthonormalised again (this time leaving dead columns
+// flagged), and the dead ones are completed against all of Q. R = Q^T A.
+bool qr_extend(const double* A, int64_t lda, int64_t m, int64_t r, int64_t p, double* Q, double* R, Arena& ws,
+               hipStream_t st) {
+  const int64_t n = r + p;
+  if (r < 1 || p < 1 || p > kQrSmallCols || n > m || n > kMaxSmall) return false;
+  size_t mk = ws.mark();
+  double* Z1 = ws.alloc<double>(m * p);
+  double* W = ws.alloc<double>(r * p);
+  double* Qz = ws.alloc<double>(m * p);
+  double* Rz = ws.alloc<double>(p * p);
+  int* dead = ws.alloc<int>(n + 1);
+  int* dead1 = ws.alloc<int>(p + 1);
+  auto project = [&](double* X) {  // X -= U (U^T X)
+    GemmDesc g;
+    g.M = (int)r; g.N = (int)p; g.K = (int)m;
+    g.a = rmT(A, lda); g.b = rm(X, p); g.c = orm(W, p);
+    gemm_f64(g, ws, st);
+    GemmDesc h;
+    h.M = (int)m; h.N = (int)p; h.K = (int)r;
+    h.a = rm(A, lda); h.b = rm(W, p); h.c = orm(X, p);
+    h.alpha = -1.0; h.beta = 1.0;
+    gemm_f64(h, ws, st);
+  };
+  if (!ws.measuring()) copy2d(A + r, lda, 1, Z1, p, m, p, st);  // Z
+  project(Z1);
+  if (!ws.measuring()) {
+    launch<KQrSmall>(dim3(1), dim3(256), 0, st, {Z1, p, 1, m, (int)p, Qz, Rz, dead1, dead1 + p, 1});
+    TT_LAUNCH_CHECK();
+  }
+  project(Qz);
+  if (!ws.measuring()) {
+    zero_words(dead, r, st);  // U's columns are live
+    launch<KQrSmall>(dim3(1), dim3(256), 0, st, {Qz, p, 1, m, (int)p, Z1, Rz, dead + r, dead + n, 0});
+    TT_LAUNCH_CHECK();
+    copy2d(A, lda, 1, Q, n, m, r, st);      // Q = [U | Qz]
+    copy2d(Z1, p, 1, Q + r, n, m, p, st);
+    launch<KCompleteDead>(dim3(1), dim3(1024), 0, st, {Q, m, (int)n, dead, dead + n});
+    TT_LAUNCH_CHECK();
+  }
+  GemmDesc g;  // R = Q^T A
+  g.M = (int)n; g.N = (int)n; g.K = (int)m;
+  g.a = rmT(Q, n); g.b = rm(A, lda); g.c = orm(R, n);
+  gemm_f64(g, ws, st);
+  ws.release(mk);
+  return true;
 }
 
 // ---------------------------------------------------------------- Jacobi SVD

@@ -256,6 +256,13 @@ void svd_large(const double* M, int64_t sr, int64_t sc, int k, double* s, double
 // Returns the number of columns of Q (min(m, n)).
 int qr(const double* A, int64_t sr, int64_t sc, int64_t m, int64_t n, double* Q, double* R,
        Arena& ws, hipStream_t st, SplitOut* r_split = nullptr);
+// QR of A = [U | Z] (m x (r + p), row-major, ld lda) whose first r columns U are orthonormal and whose
+// last p columns Z are few (p <= 4): Q = [U | Qz] (m x (r + p), ld r + p), Qz from a block Gram-Schmidt
+// pass of Z against U, a QR of the remainder, a second pass of those columns and a second QR, any dead column
+// of Qz completed against all of Q's live columns; R = Q^T A ((r + p) x (r + p)). Returns false, doing nothing, when the shape is outside that
+// (p > 4, r + p > m or above the LDS-resident QR's 140 columns).
+bool qr_extend(const double* A, int64_t lda, int64_t m, int64_t r, int64_t p, double* Q, double* R, Arena& ws,
+               hipStream_t st);
 // Thin SVD of an m x n matrix (strided): U (m x k row-major, ld k), s (k), V (n x k
 // row-major, ld k), k = min(m, n). Singular values descending.
 int svd(const double* M, int64_t sr, int64_t sc, int64_t m, int64_t n, double* U, double* s,
