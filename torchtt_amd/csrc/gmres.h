@@ -69,6 +69,12 @@ class HostSync {
     post_to_host(host_ + s, dflag, sizeof(int), st);
     if (!current_recorder()) TT_HIP(hipEventRecord(ev_[s], st));
   }
+  // Slot of post idx for a kernel to store the flag into itself (pinned host memory the device writes);
+  // mark(idx) then records the event that wait(idx) waits for.
+  int* slot(int idx) { return host_ + idx % NS; }
+  void mark(int idx, hipStream_t st) {
+    if (!current_recorder()) TT_HIP(hipEventRecord(ev_[idx % NS], st));
+  }
   bool wait(int idx, hipStream_t st) {
     if (idx < 0) return false;
     int s = idx % NS;

@@ -145,15 +145,17 @@ GmresResult gmres_run(GmresDev g, int64_t N, T* x, const T* rhs, const double* n
       }
       if (nb1 == 0) launch<KCgsDots<T>>(dim3(nb), dim3(kGB), 0, st, {Q, N, k + 1, w, P1, done});
       launch<KCgsUpdate<T>>(dim3(nb), dim3(kGB), 0, st, {Q, N, k + 1, w, P1, Hcol, Pn1, P2, done, nb1});
+      // the close also stores the done flag into the host mailbox slot of this step, so the stream carries
+      // no copy of it (hs.mark records the event that hs.wait waits for)
       launch<KGmresClose<T>>(dim3(nb), dim3(kGB), 0, st,
                              {Q, N, k + 1, w, qn, Pw, Pn1, P2, Hcol, g, k, done, g.istate + GM_REORTHS,
-                              Pn1 + kGmresMaxBlocks, nb1});
+                              Pn1 + kGmresMaxBlocks, nb1, hs.slot(k)});
       TT_LAUNCH_CHECK();
       if (prof) {
         TT_HIP(hipEventRecord(tag.e1, st));
         tags.push_back(tag);
       }
-      hs.post(done, k, st);
+      hs.mark(k, st);
       if (cohort) {
         // Inside a cohort a host read suspends the member until its queue has run: poll
         // every gmres_poll iterations (iterations enqueued after convergence are no-ops).

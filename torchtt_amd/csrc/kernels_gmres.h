@@ -200,11 +200,15 @@ struct KGmresClose {
     const double* P2; double* Hcol; GmresDev g; int k; const int* done_in; int* reorth_count;
     double* Pex;  // per-block partials of ||w''||^2 (exact-norm path)
     int nb1;      // partials in Pw (0: one per block of this launch)
+    int* host_done;  // pinned host word that receives the done flag once the step is closed, or null
   };
   static constexpr int kThreads = kGB;
   static constexpr int kMega = std::is_same<T, double>::value ? 8 : 9;
   static __device__ __forceinline__ void run(const Args& a, const Blk& bk) {
-    if (*a.done_in) return;
+    if (*a.done_in) {
+      if (bk.x == 0 && threadIdx.x == 0 && a.host_done) *(volatile int*)a.host_done = 1;
+      return;
+    }
     __shared__ double h2[kGmresMaxCols + 1];
     __shared__ double nn[2];
     const int kc = a.kc;
@@ -310,6 +314,7 @@ struct KGmresClose {
       } else if (k + 1 >= g.m) {
         g.istate[GM_DONE] = 1;  // cycle exhausted (flag stays 0)
       }
+      if (a.host_done) *(volatile int*)a.host_done = g.istate[GM_DONE];
     }
     __syncthreads();
     for (int i = threadIdx.x; i <= k + 1; i += kGB) h[i] = hc[i];
