@@ -30,12 +30,15 @@ constexpr int kQrPasses = 4;
 //   * If skip_in && *skip_in: an earlier pass already finished; only propagate the flag.
 //   * With splits > 1, G is the sum of the Gram GEMM's split-K partials Gp[z * n * n + t], z = 0 .. splits-1,
 //     added here in slice order (the GEMM's own reduction, which this replaces, adds them the same way).
-//   * delta = max |G - I|. If delta < kNearIdentity the input is already orthonormal to
-//     that level: no shift, and this pass is the last (*final_out = 1).
+//   * delta = max |G - I|. If n delta < kNearIdentity the pass is unshifted and the last one
+//     (*final_out = 1): ||G - I||_2 <= n delta, so the input's kappa^2 <= (1 + n delta) /
+//     (1 - n delta) < 1.23, and a plain Cholesky-QR pass, whose orthogonality error is of
+//     order u kappa^2, leaves Q orthonormal to working precision as from an input with
+//     delta -> 0.
 //   * Otherwise shift = shift_c * u * trace(G) when shift_c > 0 (shifted Cholesky-QR).
 //   * A pivot <= dead_rel * max(diag) marks the column dead: R[j][j] = 1 and the rest of
 //     row j is zero; dead[j] records it and *ndead counts them.
-constexpr double kNearIdentity = 1e-6;
+constexpr double kNearIdentity = 0.1;  // bound on n max|G - I|
 
 struct KCholesky {
   struct Args {
@@ -97,7 +100,7 @@ __device__ void KCholesky::run(const Args& a, const Blk&) {
       tr += W[j * n + j];
       md = fmax(md, W[j * n + j]);
     }
-    const int near_id = delta < kNearIdentity;
+    const int near_id = delta * n < kNearIdentity;
     double sh = (shift_c > 0.0 && !near_id) ? shift_c * kUnitRoundoff * tr : 0.0;
     s_shift = sh;
     s_maxd = md + sh;
@@ -436,7 +439,7 @@ __device__ void KQrSmall::run(const Args& a, const Blk&) {
       for (int i = 0; i < n; ++i)
         for (int j = 0; j < n; ++j) delta = fmax(delta, fabs(G[i][j] - (i == j ? 1.0 : 0.0)));
       for (int j = 0; j < n; ++j) { tr += G[j][j]; md = fmax(md, G[j][j]); }
-      const int near_id = delta < kNearIdentity;
+      const int near_id = delta * n < kNearIdentity;
       const double shift_c = (pass < 2) ? 11.0 * ((double)m * n + (double)n * (n + 1)) : 0.0;
       const double dead_rel = (pass < 2) ? 0.0 : 1e-26;
       const double shv = (shift_c > 0.0 && !near_id) ? shift_c * kUnitRoundoff * tr : 0.0;
@@ -547,7 +550,7 @@ int qr(const double* A, int64_t sr, int64_t sc, int64_t m, int64_t n, double* Q,
 
   // Passes: two shifted Cholesky-QR passes (each shrinks the condition number by about
   // sqrt(11 (m n + n (n+1)) u)), then plain passes; a pass whose input Gram matrix is
-  // within 1e-6 of the identity is unshifted and the last one. Later passes see the flag
+  // within kNearIdentity / n of the identity is unshifted and the last one. Later passes see the flag
   // and return at once. Numerically rank-deficient inputs (singular values from 1 down
   // to below u, as AMEn's local solutions have) can leave the third pass's input with a
   // condition number near 1e10, which one plain pass does not orthonormalise (Q^T Q off
