@@ -62,14 +62,20 @@ __device__ void KLuPanel::run(const Args& a, const Blk&) {
     }
     if ((tid & 63) == 0) { shv[tid >> 6] = v; shi[tid >> 6] = idx; }
     __syncthreads();
-    if (tid == 0) {
-      double bv = shv[0];
-      int bi = shi[0];
-      for (int w = 1; w < (nt >> 6); ++w)
-        if (shv[w] > bv || (shv[w] == bv && shi[w] < bi)) { bv = shv[w]; bi = shi[w]; }
-      if (bi >= rows) bi = jj;
-      s_p = bi;
-      piv[j0 + jj] = j0 + bi;
+    if (tid < 16) {  // the waves' candidates: (largest value, then smallest index) in any order
+      double bv = (tid < (nt >> 6)) ? shv[tid] : -1.0;
+      int bi = (tid < (nt >> 6)) ? shi[tid] : rows;
+#pragma unroll
+      for (int o = 8; o > 0; o >>= 1) {
+        const double ov = __shfl_xor(bv, o, 16);
+        const int oi = __shfl_xor(bi, o, 16);
+        if (ov > bv || (ov == bv && oi < bi)) { bv = ov; bi = oi; }
+      }
+      if (tid == 0) {
+        if (bi >= rows) bi = jj;
+        s_p = bi;
+        piv[j0 + jj] = j0 + bi;
+      }
     }
     __syncthreads();
     const int p = s_p;
@@ -85,10 +91,24 @@ __device__ void KLuPanel::run(const Args& a, const Blk&) {
     for (int i = jj + 1 + tid; i < rows; i += nt) P[jj * rows + i] /= d;
     __syncthreads();
     // rank-1 update of the panel's trailing part: lanes along the rows (conflict-free), waves
-    // over the columns
-    for (int c = jj + 1 + (tid >> 6); c < jb; c += nt >> 6) {
-      const double u = P[c * rows + jj];
-      for (int i = jj + 1 + (tid & 63); i < rows; i += 64) P[c * rows + i] -= P[jj * rows + i] * u;
+    // over the columns. A lane takes 8 rows at a time: their multipliers once for all of the
+    // wave's columns, and each column's 8 entries with their loads in flight together.
+    for (int i0 = jj + 1 + (tid & 63); i0 < rows; i0 += 64 * 8) {
+      double l[8];
+#pragma unroll
+      for (int k = 0; k < 8; ++k) l[k] = (i0 + 64 * k < rows) ? P[jj * rows + i0 + 64 * k] : 0.0;
+      for (int c = jj + 1 + (tid >> 6); c < jb; c += nt >> 6) {
+        const double u = P[c * rows + jj];
+        double x[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) x[k] = (i0 + 64 * k < rows) ? P[c * rows + i0 + 64 * k] : 0.0;
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+          if (i0 + 64 * k < rows) {
+            x[k] -= l[k] * u;
+            P[c * rows + i0 + 64 * k] = x[k];
+          }
+      }
     }
     __syncthreads();
   }
