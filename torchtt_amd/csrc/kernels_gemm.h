@@ -411,6 +411,23 @@ struct KGemmReduce {
   static __device__ __forceinline__ void run(const Args& a, const Blk& bk);
 };
 
+// sum_{z < splits} P[z * stride] in slice order from 0.0 (the split-K reduction's order), with up to
+// 8 slices' loads in flight: a loop that waits for each slice's load before issuing the next pays a
+// memory round trip per slice (the partials were written by workgroups of every XCD).
+template <typename T>
+__device__ __forceinline__ double sum_slices(const T* P, int64_t stride, int splits) {
+  double s = 0.0;
+  for (int z = 0; z < splits; z += 8) {
+    double v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v[u] = (z + u < splits) ? (double)P[(int64_t)(z + u) * stride] : 0.0;
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      if (z + u < splits) s += v[u];
+  }
+  return s;
+}
+
 template <typename T>
 __device__ __forceinline__ void KGemmReduce<T>::run(const Args& a, const Blk& bk) {
   const GemmParams<T> p = a.p;
@@ -423,8 +440,7 @@ __device__ __forceinline__ void KGemmReduce<T>::run(const Args& a, const Blk& bk
        t += (int64_t)bk.gx * blockDim.x) {
     int64_t bidx = t / MN, r = t - bidx * MN;
     int i = (int)(r / p.N), j = (int)(r - (int64_t)i * p.N);
-    double s = 0.0;
-    for (int z = 0; z < splits; ++z) s += (double)p.partial[((int64_t)z * p.B + bidx) * MN + r];
+    const double s = sum_slices(p.partial + bidx * MN + r, (int64_t)p.B * MN, splits);
     int b1 = (int)(bidx / p.B2), b2 = (int)(bidx - (int64_t)b1 * p.B2);
     T* cp = p.c + b1 * p.cb1 + b2 * p.cb2 + (int64_t)i * p.cr + (int64_t)j * p.cc;
     double val = al * s;

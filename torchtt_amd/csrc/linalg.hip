@@ -15,6 +15,7 @@
 //   parallel ordering, Rutishauser's rotation formula. The rotations are logged and
 //   applied to the identity by a second, row-parallel kernel.
 #include "ttamen_internal.h"
+#include "kernels_gemm.h"  // sum_slices
 #include <algorithm>
 #include <cmath>
 
@@ -696,9 +697,7 @@ __device__ void KJacobi::run(const Args& a, const Blk&) {
     int j = t / k, i = t % k;
     const int64_t off = (int64_t)i * sr + (int64_t)j * sc;
     if (a.splits > 1) {
-      double v = 0.0;
-      for (int z = 0; z < a.splits; ++z) v += a.Mp[(int64_t)z * k * k + off];
-      W[t] = v;
+      W[t] = sum_slices(a.Mp + off, (int64_t)k * k, a.splits);
     } else {
       W[t] = M[off];
     }
