@@ -108,7 +108,9 @@ static void gemm_impl(const GemmDesc& g, Arena& ws, hipStream_t st, bool bf16 = 
   const bool small_grid = tiles64 < 1024 && current_recorder() == nullptr;
   int BM = (g.M <= 32 || small_grid) ? 32 : 64;
   int BN = (g.N <= 32 || small_grid) ? 32 : 64;
-  // Split K when the output grid is too small to fill the 256 CUs and K is long. The plan
+  // Split K when the output grid is too small to fill the 256 CUs and K is long, into slices at
+  // least 64 deep (four steps of the K loop, which waits on each step's loads: a long slice on a
+  // few workgroups is latency-bound). The plan
   // depends on the shape only (the tile count of the cohort's rule: 64, or 32 for an extent of
   // at most 32), never on the tile that runs, so a GEMM sums its K slices in the same order
   // alone, inside a cohort and under the autotuner: each result is bitwise the same.
@@ -121,7 +123,7 @@ static void gemm_impl(const GemmDesc& g, Arena& ws, hipStream_t st, bool bf16 = 
       return 1;
     }
     if (tiles < 256 && g.K >= 512) {
-      splits = std::min<int64_t>((512 + tiles - 1) / tiles, g.K / 256);
+      splits = std::min<int64_t>((512 + tiles - 1) / tiles, g.K / 64);
       splits = std::max(1, std::min(splits, 64));
     }
     int64_t kchunk = (g.K + splits - 1) / splits;

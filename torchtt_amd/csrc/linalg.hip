@@ -15,7 +15,6 @@
 //   parallel ordering, Rutishauser's rotation formula. The rotations are logged and
 //   applied to the identity by a second, row-parallel kernel.
 #include "ttamen_internal.h"
-#include "kernels_gemm.h"  // sum_slices
 #include <algorithm>
 #include <cmath>
 
@@ -67,13 +66,29 @@ __device__ void KCholesky::run(const Args& a, const Blk&) {
   const int nn = n * n;
   for (int t0 = tid; t0 < nn; t0 += 8 * nt) {  // 8 independent loads in flight per thread
     double v[8];
-    if (a.splits > 1) {  // sum of the slices, in slice order
+    if (a.splits > 1) {  // sum of the slices, in slice order: four elements x four slices in flight per round
 #pragma unroll
       for (int u = 0; u < 8; ++u) v[u] = 0.0;
-      for (int z = 0; z < a.splits; ++z) {
-        const double* Pz = a.Gp + (int64_t)z * nn;
 #pragma unroll
-        for (int u = 0; u < 8; ++u) v[u] += (t0 + u * nt < nn) ? Pz[t0 + u * nt] : 0.0;
+      for (int h = 0; h < 8; h += 4) {
+        if (h * nt >= nn) break;  // uniform: no thread has an element in this half
+        for (int z = 0; z < a.splits; z += 4) {
+          double q[4][4];
+#pragma unroll
+          for (int zz = 0; zz < 4; ++zz) {
+            const double* Pz = a.Gp + (int64_t)(z + zz) * nn;
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+              const int t = t0 + (h + u) * nt;
+              q[zz][u] = (z + zz < a.splits && t < nn) ? Pz[t] : 0.0;
+            }
+          }
+#pragma unroll
+          for (int zz = 0; zz < 4; ++zz)
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+              if (z + zz < a.splits) v[h + u] += q[zz][u];
+        }
       }
     } else {
 #pragma unroll
@@ -693,13 +708,38 @@ __device__ void KJacobi::run(const Args& a, const Blk&) {
   const int g = tid >> 4, gl = tid & 15;  // 64 groups of 16 lanes
   const int K = (k % 2) ? k + 1 : k;
   const int npairs = K / 2;
-  for (int t = tid; t < k * k; t += nt) {
-    int j = t / k, i = t % k;
-    const int64_t off = (int64_t)i * sr + (int64_t)j * sc;
-    if (a.splits > 1) {
-      W[t] = sum_slices(a.Mp + off, (int64_t)k * k, a.splits);
-    } else {
-      W[t] = M[off];
+  if (a.splits > 1) {  // sum of the slices, in slice order: four elements x four slices in flight per round
+    const int64_t kk = (int64_t)k * k;
+    for (int t0 = tid; t0 < k * k; t0 += 4 * nt) {
+      int64_t off[4];
+      double v[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int t = t0 + u * nt, j = t / k, i = t - j * k;
+        off[u] = (int64_t)i * sr + (int64_t)j * sc;
+        v[u] = 0.0;
+      }
+      for (int z = 0; z < a.splits; z += 4) {
+        double q[4][4];
+#pragma unroll
+        for (int zz = 0; zz < 4; ++zz)
+#pragma unroll
+          for (int u = 0; u < 4; ++u)
+            q[zz][u] = (z + zz < a.splits && t0 + u * nt < k * k) ? a.Mp[(int64_t)(z + zz) * kk + off[u]] : 0.0;
+#pragma unroll
+        for (int zz = 0; zz < 4; ++zz)
+#pragma unroll
+          for (int u = 0; u < 4; ++u)
+            if (z + zz < a.splits) v[u] += q[zz][u];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        if (t0 + u * nt < k * k) W[t0 + u * nt] = v[u];
+    }
+  } else {
+    for (int t = tid; t < k * k; t += nt) {
+      int j = t / k, i = t % k;
+      W[t] = M[(int64_t)i * sr + (int64_t)j * sc];
     }
   }
   const double tol = (double)k * kUnitRoundoff;
