@@ -50,7 +50,9 @@ def _rel(a, b):
                                                (300, 20, 1000, 1, "tt"), (1, 1, 1, 1, "nn"),
                                                (64, 48, 96, 2, "tn"), (64, 48, 96, 2, "nt"),
                                                (64, 48, 96, 2, "tt"), (130, 66, 34, 3, "nn"),
-                                               (130, 67, 34, 2, "tt"), (129, 66, 35, 2, "nt")])
+                                               (130, 67, 34, 2, "tt"), (129, 66, 35, 2, "nt"),
+                                               # split K: 63 slices of 80 with a ragged last one of 40
+                                               (40, 33, 5000, 2, "nt")])
 def test_gemm_f64(M, N, K, batch, trans):
     L, lib = _lib()
     _say("gemm", M, N, K, batch, trans)
