@@ -536,11 +536,12 @@ class Solver {
   // A-interface update of core k (reference amen_solve.h:38-68): forward Pn[L,S,R] = sum Phi[l,s,r]
   // xL[l,M,L] A[s,M,N,S] xR[r,N,R], backward its mirror. With precision 3 the operands are rounded to
   // FP32 and contracted on the FP32 matrix cores; the interface is kept in FP64.
+  // With inv_scale, the result is multiplied by 1 / *inv_scale (in the last GEMM's epilogue on the FP64 path).
   void phi_update(bool fwd, int k, const double* Phi, const double* xL, const double* xR, double* out, int l, int s,
-                  int r, int n, int S, int L, int R) {
+                  int r, int n, int S, int L, int R, const double* inv_scale = nullptr) {
     if (O.precision != 3) {
-      if (fwd) phi_fwd_A(Phi, xL, Apm_[k], xR, out, l, s, r, n, S, L, R, ws_, st_);
-      else phi_bck_A(Phi, xL, A_[k], xR, out, l, s, r, n, S, L, R, ws_, st_);
+      if (fwd) phi_fwd_A(Phi, xL, Apm_[k], xR, out, l, s, r, n, S, L, R, ws_, st_, inv_scale);
+      else phi_bck_A(Phi, xL, A_[k], xR, out, l, s, r, n, S, L, R, ws_, st_, inv_scale);
       return;
     }
     const size_t mk = ws_.mark();
@@ -558,6 +559,7 @@ class Solver {
     if (fwd) phi_fwd_A_f32(P32, L32, Apm32_[k], R32, O32, l, s, r, n, S, L, R, ws_, st_);
     else phi_bck_A_f32(P32, L32, A32_[k], R32, O32, l, s, r, n, S, L, R, ws_, st_);
     cast_f32_f64(O32, out, nOut, st_);
+    if (inv_scale) scale_inv(out, nOut, inv_scale, st_);
     ws_.release(mk);
   }
 
@@ -678,12 +680,11 @@ class Solver {
     nrmsc_update(S_, NRMSC(), NORMA(k - 1), NORMX(k - 1), NORMB(k - 1), 1, st_);
     if (!last && P.rzcap > 0) {
       const int64_t rzk = rz_[k], rzk1 = rz_[k + 1];
+      // the z interfaces take the x interfaces' norms, divided out in their last GEMM
       phi_update(false, k, Phiz_[k + 1], z_[k], x_[k], Phiz_[k], (int)rzk, (int)s, (int)r1, (int)n,
-                 (int)S, (int)rzk1, (int)r2);
-      scale_inv(Phiz_[k], rzk * s * r1, sc(NORMA(k - 1)), st_);
+                 (int)S, (int)rzk1, (int)r2, sc(NORMA(k - 1)));
       phi_bck_rhs(Phizb_[k + 1], b_[k], z_[k], Phizb_[k], (int)rbk, (int)rzk, (int)n, (int)rbk1,
-                  (int)rzk1, ws_, st_);
-      scale_inv(Phizb_[k], rbk * rzk, sc(NORMB(k - 1)), st_);
+                  (int)rzk1, ws_, st_, sc(NORMB(k - 1)));
       rzphi_[k] = rzk;
       rzphi_[k + 1] = rzk1;
     }
@@ -945,12 +946,11 @@ class Solver {
       nrmsc_update(S_, NRMSC(), NORMA(k), NORMX(k), NORMB(k), 1, st_);
       if (!last && P.rzcap > 0) {
         const int64_t rzk = rz_[k], rzk1 = rz_[k + 1];
+        // the z interfaces take the x interfaces' norms, divided out in their last GEMM
         phi_update(true, k, Phiz_[k], z_[k], x_[k], Phiz_[k + 1], (int)rzk, (int)s, (int)r1,
-                   (int)n, (int)S, (int)rzk1, (int)r);
-        scale_inv(Phiz_[k + 1], rzk1 * S * r, sc(NORMA(k)), st_);
+                   (int)n, (int)S, (int)rzk1, (int)r, sc(NORMA(k)));
         phi_fwd_rhs(Phizb_[k], b_[k], z_[k], Phizb_[k + 1], (int)rbk, (int)rzk, (int)n,
-                    (int)rbk1, (int)rzk1, ws_, st_);
-        scale_inv(Phizb_[k + 1], rbk1 * rzk1, sc(NORMB(k)), st_);
+                    (int)rbk1, (int)rzk1, ws_, st_, sc(NORMB(k)));
         rzphi_[k + 1] = rzk1;
       }
     } else {

@@ -300,7 +300,8 @@ static void phi_fwd_half_t(const T* Phi, const T* Apm, const T* xR, T* t2, int l
 
 template <typename T>
 static void phi_fwd_A_t(const T* Phi, const T* xL, const T* Apm, const T* xR, T* Pn, int l, int s, int r, int nm,
-                        int nk, int S, int L, int R, Arena& ws, hipStream_t st, GemmFn gemm) {
+                        int nk, int S, int L, int R, Arena& ws, hipStream_t st, GemmFn gemm,
+                        const double* inv_scale = nullptr) {
   size_t mk = ws.mark();
   T* t2 = ws.alloc<T>((size_t)l * nm * S * R);
   phi_fwd_half_t<T>(Phi, Apm, xR, t2, l, s, r, nm, nk, S, R, ws, st, gemm);
@@ -310,6 +311,7 @@ static void phi_fwd_A_t(const T* Phi, const T* xL, const T* Apm, const T* xR, T*
     g.a.p = xL; g.a.r = 1; g.a.c = L;
     g.b.p = t2; g.b.r = (int64_t)S * R; g.b.c = 1;
     g.c.p = Pn; g.c.r = (int64_t)S * R; g.c.c = 1;
+    if (inv_scale) { g.scale_ptr = inv_scale; g.scale_mode = 2; }
     gemm(g, ws, st);
   }
   ws.release(mk);
@@ -321,7 +323,8 @@ static void phi_fwd_A_t(const T* Phi, const T* xL, const T* Apm, const T* xR, T*
 //   Pp[l][s][r]     = sum_{(M,L)} xL[l][(M,L)] u2[r][s][(M,L)]    (batch s)
 template <typename T>
 static void phi_bck_A_t(const T* Phi, const T* xL, const T* A, const T* xR, T* Pp, int l, int s, int r, int nm,
-                        int nk, int S, int L, int R, Arena& ws, hipStream_t st, GemmFn gemm) {
+                        int nk, int S, int L, int R, Arena& ws, hipStream_t st, GemmFn gemm,
+                        const double* inv_scale = nullptr) {
   size_t mk = ws.mark();
   T* u1 = ws.alloc<T>((size_t)r * nk * S * L);
   T* u2 = ws.alloc<T>((size_t)r * s * nm * L);
@@ -347,6 +350,7 @@ static void phi_bck_A_t(const T* Phi, const T* xL, const T* A, const T* xR, T* P
     g.a.p = xL; g.a.r = (int64_t)nm * L; g.a.c = 1;
     g.b.p = u2; g.b.r = 1; g.b.c = (int64_t)s * nm * L; g.b.b2 = (int64_t)nm * L;
     g.c.p = Pp; g.c.r = (int64_t)s * r; g.c.c = 1; g.c.b2 = r;
+    if (inv_scale) { g.scale_ptr = inv_scale; g.scale_mode = 2; }
     gemm(g, ws, st);
   }
   ws.release(mk);
@@ -358,23 +362,23 @@ void phi_fwd_half_rect(const double* Phi, const double* Apm, const double* xR, d
 }
 void phi_fwd_A_rect(const double* Phi, const double* xL, const double* Apm, const double* xR,
                     double* Pn, int l, int s, int r, int nm, int nk, int S, int L, int R,
-                    Arena& ws, hipStream_t st) {
-  phi_fwd_A_t<double>(Phi, xL, Apm, xR, Pn, l, s, r, nm, nk, S, L, R, ws, st, gemm_f64);
+                    Arena& ws, hipStream_t st, const double* inv_scale) {
+  phi_fwd_A_t<double>(Phi, xL, Apm, xR, Pn, l, s, r, nm, nk, S, L, R, ws, st, gemm_f64, inv_scale);
 }
 void phi_bck_A_rect(const double* Phi, const double* xL, const double* A, const double* xR,
                     double* Pp, int l, int s, int r, int nm, int nk, int S, int L, int R,
-                    Arena& ws, hipStream_t st) {
-  phi_bck_A_t<double>(Phi, xL, A, xR, Pp, l, s, r, nm, nk, S, L, R, ws, st, gemm_f64);
+                    Arena& ws, hipStream_t st, const double* inv_scale) {
+  phi_bck_A_t<double>(Phi, xL, A, xR, Pp, l, s, r, nm, nk, S, L, R, ws, st, gemm_f64, inv_scale);
 }
 void phi_fwd_A(const double* Phi, const double* xL, const double* Apm, const double* xR,
                double* Pn, int l, int s, int r, int n, int S, int L, int R, Arena& ws,
-               hipStream_t st) {
-  phi_fwd_A_rect(Phi, xL, Apm, xR, Pn, l, s, r, n, n, S, L, R, ws, st);
+               hipStream_t st, const double* inv_scale) {
+  phi_fwd_A_rect(Phi, xL, Apm, xR, Pn, l, s, r, n, n, S, L, R, ws, st, inv_scale);
 }
 void phi_bck_A(const double* Phi, const double* xL, const double* A, const double* xR,
                double* Pp, int l, int s, int r, int n, int S, int L, int R, Arena& ws,
-               hipStream_t st) {
-  phi_bck_A_rect(Phi, xL, A, xR, Pp, l, s, r, n, n, S, L, R, ws, st);
+               hipStream_t st, const double* inv_scale) {
+  phi_bck_A_rect(Phi, xL, A, xR, Pp, l, s, r, n, n, S, L, R, ws, st, inv_scale);
 }
 // FP32 operands and result, FP32 accumulation on the matrix cores.
 void phi_fwd_A_f32(const float* Phi, const float* xL, const float* Apm, const float* xR, float* Pn, int l, int s,
@@ -388,7 +392,7 @@ void phi_bck_A_f32(const float* Phi, const float* xL, const float* A, const floa
 
 // Phib forward (reference amen_solve.h:93-98, 'br,bnB,rnR->BR').
 void phi_fwd_rhs(const double* Pb, const double* bc, const double* x, double* Pn, int b, int r,
-                 int n, int B, int R, Arena& ws, hipStream_t st) {
+                 int n, int B, int R, Arena& ws, hipStream_t st, const double* inv_scale) {
   size_t mk = ws.mark();
   double* u = ws.alloc<double>((size_t)r * n * B);
   {
@@ -405,6 +409,7 @@ void phi_fwd_rhs(const double* Pb, const double* bc, const double* x, double* Pn
     g.a.p = u; g.a.r = 1; g.a.c = B;
     g.b.p = x; g.b.r = R; g.b.c = 1;
     g.c.p = Pn; g.c.r = R; g.c.c = 1;
+    if (inv_scale) { g.scale_ptr = inv_scale; g.scale_mode = 2; }
     gemm_f64(g, ws, st);
   }
   ws.release(mk);
@@ -412,7 +417,7 @@ void phi_fwd_rhs(const double* Pb, const double* bc, const double* x, double* Pn
 
 // Phib backward (reference amen_solve.h:78-83, 'BR,bnB,rnR->br').
 void phi_bck_rhs(const double* Pb, const double* bc, const double* x, double* Pp, int b, int r,
-                 int n, int B, int R, Arena& ws, hipStream_t st) {
+                 int n, int B, int R, Arena& ws, hipStream_t st, const double* inv_scale) {
   size_t mk = ws.mark();
   double* u = ws.alloc<double>((size_t)b * n * R);
   {
@@ -429,6 +434,7 @@ void phi_bck_rhs(const double* Pb, const double* bc, const double* x, double* Pp
     g.a.p = u; g.a.r = (int64_t)n * R; g.a.c = 1;
     g.b.p = x; g.b.r = 1; g.b.c = (int64_t)n * R;
     g.c.p = Pp; g.c.r = r; g.c.c = 1;
+    if (inv_scale) { g.scale_ptr = inv_scale; g.scale_mode = 2; }
     gemm_f64(g, ws, st);
   }
   ws.release(mk);

@@ -348,23 +348,24 @@ size_t local_matvec_fused_bytes(int l1, int s, int r1, int n, int S, int l2, int
 void local_matvec_f32(const LocalOpF32& op, const float* x, float* y, int nvec, Arena& ws,
                       hipStream_t st, const int* skip = nullptr, MvDots* dots = nullptr);
 
-// Interface updates (FP64).
+// Interface updates (FP64). With inv_scale, the result is multiplied by 1 / *inv_scale (a device
+// scalar) in the last contraction's epilogue, exactly as a separate scale_inv would round it.
 // fwd: Pn[L,S,R] = sum Phi[l,s,r] xL[l,M,L] A[s,M,N,S] xR[r,N,R]
 void phi_fwd_A(const double* Phi, const double* xL, const double* Apm, const double* xR,
                double* Pn, int l, int s, int r, int n, int S, int L, int R, Arena& ws,
-               hipStream_t st);
+               hipStream_t st, const double* inv_scale = nullptr);
 // bck: Pp[l,s,r] = sum xL[l,M,L] Phi[L,S,R] A[s,M,N,S] xR[r,N,R]
 void phi_bck_A(const double* Phi, const double* xL, const double* A, const double* xR,
                double* Pp, int l, int s, int r, int n, int S, int L, int R, Arena& ws,
-               hipStream_t st);
+               hipStream_t st, const double* inv_scale = nullptr);
 // The same for a rectangular operator core (M of size nm on the xL side, N of size nk on the
 // xR side; Apm from permute_core_A_rect, A unpermuted).
 void phi_fwd_A_rect(const double* Phi, const double* xL, const double* Apm, const double* xR,
                     double* Pn, int l, int s, int r, int nm, int nk, int S, int L, int R,
-                    Arena& ws, hipStream_t st);
+                    Arena& ws, hipStream_t st, const double* inv_scale = nullptr);
 void phi_bck_A_rect(const double* Phi, const double* xL, const double* A, const double* xR,
                     double* Pp, int l, int s, int r, int nm, int nk, int S, int L, int R,
-                    Arena& ws, hipStream_t st);
+                    Arena& ws, hipStream_t st, const double* inv_scale = nullptr);
 // The A-interface updates with FP32 operands and result (FP32 accumulation on the matrix cores).
 void phi_fwd_A_f32(const float* Phi, const float* xL, const float* Apm, const float* xR, float* Pn, int l, int s,
                    int r, int n, int S, int L, int R, Arena& ws, hipStream_t st);
@@ -374,12 +375,12 @@ void phi_bck_A_f32(const float* Phi, const float* xL, const float* A, const floa
 // A[s,M,N,S] xR[r,N,R] (l x nm x S x R; Apm from permute_core_A_rect).
 void phi_fwd_half_rect(const double* Phi, const double* Apm, const double* xR, double* t2, int l, int s,
                        int r, int nm, int nk, int S, int R, Arena& ws, hipStream_t st);
-// fwd rhs: Pn[B,R] = sum Pb[b,r] bc[b,n,B] x[r,n,R]
+// fwd rhs: Pn[B,R] = sum Pb[b,r] bc[b,n,B] x[r,n,R]   (times 1 / *inv_scale, if given)
 void phi_fwd_rhs(const double* Pb, const double* bc, const double* x, double* Pn, int b, int r,
-                 int n, int B, int R, Arena& ws, hipStream_t st);
-// bck rhs: Pp[b,r] = sum bc[b,n,B] Pb[B,R] x[r,n,R]
+                 int n, int B, int R, Arena& ws, hipStream_t st, const double* inv_scale = nullptr);
+// bck rhs: Pp[b,r] = sum bc[b,n,B] Pb[B,R] x[r,n,R]   (times 1 / *inv_scale, if given)
 void phi_bck_rhs(const double* Pb, const double* bc, const double* x, double* Pp, int b, int r,
-                 int n, int B, int R, Arena& ws, hipStream_t st);
+                 int n, int B, int R, Arena& ws, hipStream_t st, const double* inv_scale = nullptr);
 // rhs[r,n,R] = scale * sum PL[b,r] bc[b,n,B] PR[B,R]   (scale = *scale_ptr, if given)
 void assemble_rhs(const double* PL, const double* bc, const double* PR, double* out, int b,
                   int r, int n, int B, int R, const double* scale_ptr, Arena& ws,
