@@ -675,8 +675,7 @@ class Solver {
                 ws_, st_);
     norm2(Phi_[k], r1 * s * r1, sc(NORMA(k - 1)), rs_, st_, true);
     scale_inv(Phi_[k], r1 * s * r1, sc(NORMA(k - 1)), st_);
-    norm2(Phib_[k], rbk * r1, sc(NORMB(k - 1)), rs_, st_, true);
-    scale_inv(Phib_[k], rbk * r1, sc(NORMB(k - 1)), st_);
+    norm2_scale_inv(Phib_[k], rbk * r1, sc(NORMB(k - 1)), rs_, st_, true);
     nrmsc_update(S_, NRMSC(), NORMA(k - 1), NORMX(k - 1), NORMB(k - 1), 1, st_);
     if (!last && P.rzcap > 0) {
       const int64_t rzk = rz_[k], rzk1 = rz_[k + 1];
@@ -941,8 +940,7 @@ class Solver {
                   (int)r, ws_, st_);
       norm2(Phi_[k + 1], r * S * r, sc(NORMA(k)), rs_, st_, true);
       scale_inv(Phi_[k + 1], r * S * r, sc(NORMA(k)), st_);
-      norm2(Phib_[k + 1], rbk1 * r, sc(NORMB(k)), rs_, st_, true);
-      scale_inv(Phib_[k + 1], rbk1 * r, sc(NORMB(k)), st_);
+      norm2_scale_inv(Phib_[k + 1], rbk1 * r, sc(NORMB(k)), rs_, st_, true);
       nrmsc_update(S_, NRMSC(), NORMA(k), NORMX(k), NORMB(k), 1, st_);
       if (!last && P.rzcap > 0) {
         const int64_t rzk = rz_[k], rzk1 = rz_[k + 1];

@@ -26,6 +26,13 @@ void norm2(const double* x, int64_t n, double* out_slot, RedScratch rs, hipStrea
   TT_LAUNCH_CHECK();
 }
 
+void norm2_scale_inv(double* x, int64_t n, double* out_slot, RedScratch rs, hipStream_t st, bool guard) {
+  launch<KSumsq<double>>(dim3(grid_for(n)), dim3(256), 0, st,
+                         {x, nullptr, n, rs.partial, rs.counter, out_slot, nullptr, guard ? 1 : 0, nullptr,
+                          nullptr, x});
+  TT_LAUNCH_CHECK();
+}
+
 void norm2_f32(const float* x, int64_t n, double* out_slot, RedScratch rs, hipStream_t st,
                const int* skip) {
   launch<KSumsq<float>>(dim3(grid_for(n)), dim3(256), 0, st,

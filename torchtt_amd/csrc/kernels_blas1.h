@@ -35,6 +35,7 @@ struct KSumsq {
   struct Args {
     const T* x; const T* y; int64_t n; double* partial; unsigned* counter; double* out;
     const double* den; int guard; double* mul; const int* skip;
+    double* scale_x;  // if set: the last block multiplies scale_x[0 .. n) by 1 / result (a small vector)
   };
   static constexpr int kThreads = 256;
   static constexpr int kMega = std::is_same<T, double>::value ? 17 : 18;
@@ -90,6 +91,12 @@ struct KSumsq {
       *p.out = r;
       if (p.mul) *p.mul = (*p.mul) * r;
       *p.counter = 0u;
+      sh[0] = r;
+    }
+    if (p.scale_x) {
+      __syncthreads();
+      const double inv = 1.0 / sh[0];
+      for (int64_t i = threadIdx.x; i < p.n; i += 256) p.scale_x[i] *= inv;
     }
   }
 };

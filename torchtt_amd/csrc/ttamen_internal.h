@@ -174,6 +174,8 @@ constexpr int kRedBlocks = 256;
 // mul_slot (optional): *mul_slot *= result (after the guard).
 void norm2(const double* x, int64_t n, double* out_slot, RedScratch rs, hipStream_t st,
            bool guard = false, double* mul_slot = nullptr, const int* skip = nullptr);
+// out_slot = ||x||_2 as norm2 does, then x *= 1 / out_slot (by the kernel's last block: for small x).
+void norm2_scale_inv(double* x, int64_t n, double* out_slot, RedScratch rs, hipStream_t st, bool guard);
 void norm2_f32(const float* x, int64_t n, double* out_slot, RedScratch rs, hipStream_t st,
                const int* skip = nullptr);
 // out_slot = ||x - y||_2 / (den_slot ? *den_slot : 1)
