@@ -380,17 +380,23 @@ struct KGmresY {
   static constexpr int kMega = 15;
   static __device__ __forceinline__ void run(const Args& a, const Blk&);
 };
+// y is kept in LDS during the back substitution and written out at the end: with no global store
+// inside the loop, H's loads do not wait behind the previous row's store of y.
 __device__ __forceinline__ void KGmresY::run(const Args& a, const Blk&) {
-  if (threadIdx.x != 0) return;
+  __shared__ double ys[kGmresMaxCols];
   const GmresDev g = a.g;
   const int kc = g.istate[GM_KCOLS];
   const int ld = g.m + 1;
-  for (int i = kc - 1; i >= 0; --i) {
-    double s = g.beta[i];
-    for (int j = i + 1; j < kc; ++j) s -= g.H[(int64_t)j * ld + i] * g.y[j];
-    const double d = g.H[(int64_t)i * ld + i];
-    g.y[i] = (d != 0.0) ? s / d : 0.0;
+  if (threadIdx.x == 0) {
+    for (int i = kc - 1; i >= 0; --i) {
+      double s = g.beta[i];
+      for (int j = i + 1; j < kc; ++j) s -= g.H[(int64_t)j * ld + i] * ys[j];
+      const double d = g.H[(int64_t)i * ld + i];
+      ys[i] = (d != 0.0) ? s / d : 0.0;
+    }
   }
+  __syncthreads();
+  for (int i = threadIdx.x; i < kc; i += blockDim.x) g.y[i] = ys[i];
 }
 
 // x += sum_{i<kc} y_i Q_i
