@@ -524,6 +524,30 @@ def test_qr_extend(case, m, r, p):
     assert np.linalg.norm(Qh @ Rh - A) <= 1e-14 * np.sqrt(m * n) * 10 * np.linalg.norm(A)
 
 
+@pytest.mark.parametrize("pert", [0.0, 1e-3, 1e-2])
+@pytest.mark.parametrize("m,n", [(3328, 52), (600, 80)])
+def test_qr_nearly_orthonormal_input(m, n, pert):
+    """Inputs already orthonormal up to `pert`: a Cholesky-QR pass whose Gram G has n max|G - I| < 0.1 is
+    unshifted and the last (0 and 1e-3 here at n = 52; 1e-2 and the n = 80 cases with 1e-3 take the shifted
+    passes first). Q must come out orthonormal to working precision and Q R = A either way; for an orthonormal
+    input R is the identity up to rounding."""
+    L, lib = _lib()
+    _say("qr nearly orthonormal", m, n, pert)
+    rng = np.random.default_rng(m + n)
+    U, _ = np.linalg.qr(rng.standard_normal((m, n)))
+    A = U + pert * rng.standard_normal((m, n)) / np.sqrt(m)
+    Ad = torch.from_numpy(A).to(DEV)
+    Q = torch.empty(m, n, dtype=torch.float64, device=DEV)
+    R = torch.empty(n, n, dtype=torch.float64, device=DEV)
+    ws = _ws(lib.ttamen_dense_workspace_size(m, n))
+    L.check("qr", lib.ttamen_qr(_p(Ad), m, n, _p(Q), _p(R), _p(ws), ws.numel(), _stream()))
+    Qh, Rh = Q.cpu().numpy(), R.cpu().numpy()
+    assert np.linalg.norm(Qh.T @ Qh - np.eye(n)) < 1e-13 * np.sqrt(n)
+    assert np.linalg.norm(Qh @ Rh - A) <= 1e-14 * np.sqrt(m * n) * 10 * np.linalg.norm(A)
+    if pert == 0.0:
+        assert np.max(np.abs(np.abs(Rh) - np.eye(n))) < 1e-13
+
+
 def test_qr_extend_refuses_shapes():
     L, lib = _lib()
     m, r, p = 40, 38, 3  # r + p > m
